@@ -1,0 +1,222 @@
+/* slimrt: C ABI of the MI355X per-pixel ray tracer.
+ *
+ * This is the boundary the reference's GPU path crosses (src/slim/renderer/renderer_GPU.h): scene upload once,
+ * per-frame uploads of what moves, one render call into the canvas' pixel and depth buffers.  Plain C: no
+ * torch or HIP types in any signature; streams are passed as void*.
+ *
+ * Every function returns SLIMRT_OK (0) or a negative SLIMRT_E_* code; slimrt_last_error(ctx) then holds a
+ * message.  Calls on one context must not overlap (the reference is single-threaded too,
+ * src/slim/renderer/renderer.h:24-35); distinct contexts are independent.
+ *
+ * Struct layouts use 32-bit integers (the layout of the reference's asset files, where u32 is 4 bytes).
+ */
+#ifndef SLIMRT_H
+#define SLIMRT_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define SLIMRT_OK 0
+#define SLIMRT_E_ARG (-1)        /* invalid argument */
+#define SLIMRT_E_DEVICE (-2)     /* no usable GPU, or a HIP runtime error */
+#define SLIMRT_E_MEMORY (-3)     /* device allocation failed */
+#define SLIMRT_E_STATE (-4)      /* call out of order (e.g. render before a scene upload) */
+#define SLIMRT_E_OVERFLOW (-5)   /* a traversal stack would have overflowed; results of that frame are invalid */
+
+/* slimrt_create flags */
+#define SLIMRT_DEVICE_GPU 0u     /* render with the gfx950 kernels (fails with SLIMRT_E_DEVICE without a GPU) */
+#define SLIMRT_DEVICE_CPU 1u     /* render on host threads with the same per-pixel code (the CPU side of use_GPU) */
+
+/* render_mode values: RenderMode (src/slim/core/base.h:517-524) */
+#define SLIMRT_MODE_NORMALS 0u
+#define SLIMRT_MODE_NORMAL_MAP 1u
+#define SLIMRT_MODE_BEAUTY 2u
+#define SLIMRT_MODE_DEPTH 3u
+#define SLIMRT_MODE_MIP_LEVEL 4u
+#define SLIMRT_MODE_UVS 5u
+
+/* output values */
+#define SLIMRT_OUTPUT_CANVAS 0u    /* Canvas::setPixel with opacity -1: RGBA = clamp(c)^2, 1 (src/slim/draw/canvas.h:128-169) */
+#define SLIMRT_OUTPUT_RADIANCE 1u  /* the colour renderPixel returns (tone-mapped, unclamped), alpha 1 */
+
+/* render flags */
+#define SLIMRT_RAY_CONES 1u        /* keep the primary ray's cone scaling through hits (see DESIGN: ray cones) */
+
+typedef struct slimrt_context slimrt_context;
+
+/* Geometry (src/slim/core/transform.h:59-65) with its Transform (transform.h:5-10); orientation is the
+ * quaternion (axis xyz, amount) of OrientationUsingQuaternion. */
+typedef struct {
+    float orientation[4];
+    float position[3];
+    float scale[3];
+    uint32_t type;           /* GeometryType (base.h:399-410) */
+    uint32_t material_id;
+    uint32_t id;             /* mesh index for GeometryType_Mesh */
+    uint32_t flags;          /* GEOMETRY_IS_VISIBLE | _SHADOWING | _TRANSPARENT (base.h:148-150) */
+} slimrt_geometry;
+
+/* Material (src/slim/scene/material.h:93-117) */
+typedef struct {
+    float albedo[3];
+    float roughness;
+    float emission[3];
+    float ior;
+    float reflectivity[3];
+    float metalness;
+    float uv_repeat[2];
+    float normal_magnitude;
+    uint32_t flags;          /* MATERIAL_* (base.h:142-146) */
+    uint32_t texture_count;
+    uint32_t texture_ids[2];
+    uint32_t brdf;           /* BRDFType (base.h:624-629) */
+} slimrt_material;
+
+/* Light (src/slim/scene/material.h:17-25) */
+typedef struct {
+    float color[3];
+    float intensity;
+    float position_or_direction[3];
+    uint32_t flags;          /* Light_IsDirectional = 1, Light_IsShadowing = 2 */
+} slimrt_light;
+
+/* BVHNode (src/slim/scene/bvh.h:5-15) in the 32-byte asset layout. */
+typedef struct {
+    float min[3];
+    uint32_t first_index;
+    float max[3];
+    uint16_t leaf_count;
+    uint8_t depth;
+    uint8_t flags;
+} slimrt_bvh_node;
+
+/* Triangle (src/slim/scene/mesh.h:19-24), 128 bytes. */
+typedef struct {
+    float local_to_tangent[9];   /* rows X, Y, Z */
+    float position[3], normal[3], n1[3], n2[3], n3[3];
+    float uv1[2], uv2[2], uv3[2];
+    float uv_coverage, padding;
+} slimrt_triangle;
+
+/* Mesh (src/slim/scene/mesh.h:26-45): object-space AABB, BVH and leaf-ordered triangles. */
+typedef struct {
+    float aabb_min[3], aabb_max[3];
+    const slimrt_bvh_node *nodes;
+    uint32_t node_count, height;
+    const slimrt_triangle *triangles;
+    uint32_t triangle_count, normals_count, uvs_count;
+} slimrt_mesh;
+
+/* TextureMip (src/slim/core/texture.h:13-16): (width+1)*(height+1) texel quads of 12 bytes each. */
+typedef struct {
+    uint32_t width, height;
+    const uint8_t *texel_quads;
+} slimrt_texture_mip;
+
+/* Texture (src/slim/core/texture.h:44-46): mip 0 size, ImageFlags bits (base.h:1283-1295), mips. */
+typedef struct {
+    uint32_t width, height, flags, mip_count;
+    const slimrt_texture_mip *mips;
+} slimrt_texture;
+
+/* Everything initDataOnGPU uploads (src/slim/renderer/renderer_GPU.h:80-181). */
+typedef struct {
+    const slimrt_geometry *geometries;  uint32_t geometry_count;
+    const slimrt_material *materials;   uint32_t material_count;
+    const slimrt_light *lights;         uint32_t light_count;
+    const slimrt_mesh *meshes;          uint32_t mesh_count;
+    const slimrt_texture *textures;     uint32_t texture_count;
+    const slimrt_bvh_node *scene_nodes; uint32_t scene_node_count, scene_height;
+    const uint32_t *scene_leaf_ids;     /* bvh_leaf_geometry_indices, geometry_count entries */
+} slimrt_scene_desc;
+
+/* CameraRayProjection (src/slim/scene/camera.h:59-64) + RayTracerSettings (src/slim/renderer/ray_tracer.h:10-17)
+ * + canvas dimensions (CanvasData, src/slim/draw/canvas.h:11-18). */
+typedef struct {
+    float inverted_camera_rotation[9];
+    float start[3], right[3], down[3], camera_position[3];
+    float C_start[2];
+    float squared_distance_to_projection_plane;
+    float sample_size;
+    uint32_t width, height;     /* canvas pixels */
+    uint32_t ssaa;              /* 1: 2x2 samples per pixel, stored as pixel quads (canvas.h:147) */
+    uint32_t max_depth;
+    uint32_t render_mode;
+    int32_t skybox_color, skybox_radiance, skybox_irradiance;   /* texture ids, -1 for none */
+    uint32_t output;            /* SLIMRT_OUTPUT_* */
+    uint32_t flags;             /* SLIMRT_RAY_CONES */
+    float mip_level_colors[9][3];
+} slimrt_render_params;
+
+/* Pixel (src/slim/core/base.h:1091-1094) */
+typedef struct { float r, g, b, opacity; } slimrt_pixel;
+
+/* ---- context ------------------------------------------------------------------------------------------ */
+/* Replaces the one-time GPU setup of RayTracingRenderer's constructor (renderer.h:36-61). */
+int slimrt_create(slimrt_context **out, uint32_t flags);
+int slimrt_destroy(slimrt_context *ctx);
+const char *slimrt_last_error(const slimrt_context *ctx);
+/* Launch subsequent work on this HIP stream (hipStream_t passed as void*); NULL selects the context's own. */
+int slimrt_set_stream(slimrt_context *ctx, void *hip_stream);
+
+/* ---- uploads ------------------------------------------------------------------------------------------ */
+/* initDataOnGPU (src/slim/renderer/renderer_GPU.h:80-181).  Meshes and textures are concatenated into flat
+ * device arrays; geometries may be re-uploaded per frame with the functions below. */
+int slimrt_upload_scene(slimrt_context *ctx, const slimrt_scene_desc *scene);
+/* uploadGeometries / uploadMaterials / uploadLights / uploadSceneBVH (renderer_GPU.h:70-78). */
+int slimrt_upload_geometries(slimrt_context *ctx, const slimrt_geometry *geometries, uint32_t count);
+int slimrt_upload_materials(slimrt_context *ctx, const slimrt_material *materials, uint32_t count);
+int slimrt_upload_lights(slimrt_context *ctx, const slimrt_light *lights, uint32_t count);
+int slimrt_upload_scene_bvh(slimrt_context *ctx, const slimrt_bvh_node *nodes, uint32_t node_count,
+                            const uint32_t *leaf_ids, uint32_t height);
+
+/* ---- rendering ---------------------------------------------------------------------------------------- */
+/* renderOnGPU (renderer_GPU.h:48-68): render a frame and copy it into host canvas buffers (pixels: one
+ * slimrt_pixel per sample; depths: one float per sample).  Synchronous, as the reference's download is. */
+int slimrt_render(slimrt_context *ctx, const slimrt_render_params *params, slimrt_pixel *pixels, float *depths);
+/* Same frame into device buffers, asynchronously on the context's stream (no reference counterpart: for
+ * callers that keep the canvas in HBM).  Check the frame's status with slimrt_synchronize. */
+int slimrt_render_device(slimrt_context *ctx, const slimrt_render_params *params, void *d_pixels, void *d_depths);
+/* Waits for the context's stream and reports the status of the frames since the last call
+ * (SLIMRT_E_OVERFLOW if a traversal stack overflowed). */
+int slimrt_synchronize(slimrt_context *ctx);
+/* Renders one frame with work counters and returns them: rays (SceneTracer::trace calls), sibling-pair
+ * visits, root-box tests, instance (leaf geometry) visits, triangle tests, mesh closest-hit attribute
+ * fetches, texel-quad taps, samples.  Slower than slimrt_render; for measurement only. */
+int slimrt_count_work(slimrt_context *ctx, const slimrt_render_params *params, uint64_t counters[8]);
+/* Kernel time of the last slimrt_render / slimrt_render_device in milliseconds (HIP events on the stream). */
+int slimrt_last_kernel_ms(slimrt_context *ctx, float *ms);
+
+/* ---- host-side scene derivation (CPU; what the reference does on the host before uploading) ------------- */
+/* Orientation<quat>::setRotation (src/slim/core/base.h:574-620): Euler x, y, z -> quaternion. */
+int slimrt_host_orientation(const float euler[3], float quat_out[4]);
+/* CameraRayProjection::reset (src/slim/scene/camera.h:69-83) for a Camera with OrientationUsing3x3Matrix
+ * Euler angles, position and focal length; fills the projection fields and width/height/ssaa of params. */
+int slimrt_host_projection(const float camera_euler[3], const float camera_position[3], float focal_length,
+                           uint32_t width, uint32_t height, uint32_t ssaa, slimrt_render_params *params);
+/* Scene::updateAABBs (src/slim/scene/scene.h:165-179): world AABBs (min xyz, max xyz per geometry). */
+int slimrt_host_world_aabbs(const slimrt_geometry *geometries, uint32_t count, const slimrt_mesh *meshes,
+                            uint32_t mesh_count, float *aabbs_out);
+/* BVHBuilder::build (src/slim/scene/bvh_builder.h:195-263) over N boxes (min xyz, max xyz).  nodes_out holds
+ * 2N - 1 nodes (at least 1); leaf_ids_out N entries. */
+int slimrt_host_build_bvh(const float *aabbs, uint32_t count, uint32_t max_leaf_size, slimrt_bvh_node *nodes_out,
+                          uint32_t *node_count_out, uint32_t *height_out, uint32_t *leaf_ids_out);
+/* BVHBuilder::buildMesh (src/slim/scene/bvh_builder.h:265-341): mesh BVH and leaf-ordered triangle records.
+ * normals/uvs may be NULL (counts 0).  nodes_out: 2T - 1 nodes; triangles_out: T records; aabb_out: 6 floats
+ * (componentwise min/max of the positions). */
+int slimrt_host_build_mesh(const float *positions, uint32_t vertex_count, const uint32_t *position_indices,
+                           const float *normals, uint32_t normal_count, const uint32_t *normal_indices,
+                           const float *uvs, uint32_t uv_count, const uint32_t *uv_indices,
+                           uint32_t triangle_count, uint32_t max_leaf_size,
+                           slimrt_bvh_node *nodes_out, uint32_t *node_count_out, uint32_t *height_out,
+                           slimrt_triangle *triangles_out, float aabb_out[6]);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif

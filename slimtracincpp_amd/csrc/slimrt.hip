@@ -1,0 +1,574 @@
+// slimrt: gfx950 kernels and the C ABI around them (include/slimrt.h).
+//
+// Device layout in HBM (one flat array per kind, concatenated over meshes / textures):
+//   geometries 64 B | materials 64 B | lights 32 B | scene BVH nodes 32 B (+ leaf ids) | mesh records 48 B |
+//   mesh BVH nodes 32 B | triangle hit records 64 B | triangle attribute records 64 B | texture and mip records |
+//   texel quads 12 B.
+// The render kernel is a per-sample megakernel: each 256-thread workgroup owns a 16x16 tile of samples, each
+// wave an 8x8 sub-tile (coherent primary rays), and tiles are assigned so that neighbouring tiles run on the
+// same XCD (workgroups are dealt round-robin to the 8 XCDs; each XCD has its own L2).  Traversal stacks live
+// in LDS, one column per thread, sized from the uploaded BVH heights.
+#include <hip/hip_runtime.h>
+#include <string>
+#include <vector>
+#include <thread>
+#include <algorithm>
+#include <cstring>
+#include <cstdio>
+
+#include "slimrt_trace.h"
+#include "../../include/slimrt.h"
+
+using namespace slimrt;
+
+static_assert(sizeof(slimrt_bvh_node) == 32, "node layout");
+static_assert(sizeof(Node) == 32, "node layout");
+static_assert(sizeof(slimrt_triangle) == 128, "triangle layout");
+static_assert(sizeof(slimrt_geometry) == 56 && sizeof(slimrt_material) == 80 && sizeof(slimrt_light) == 32, "record layout");
+static_assert(sizeof(slimrt_mesh) == 64 && sizeof(slimrt_texture) == 24 && sizeof(slimrt_texture_mip) == 16, "record layout");
+static_assert(sizeof(RenderParams) == sizeof(slimrt_render_params), "render params layout");
+
+namespace {
+
+constexpr uint32_t kBlock = 256;      // 4 waves of 64
+constexpr uint32_t kTile = 16;        // 16x16 samples per workgroup, 8x8 per wave
+
+template <bool kCount>
+__global__ __launch_bounds__(kBlock) void render_kernel(SceneView s, RenderParams rp, float4 *__restrict__ pixels,
+                                                       float *__restrict__ depths, uint32_t scene_depth,
+                                                       uint32_t mesh_depth, uint32_t tiles_x, uint32_t tiles,
+                                                       uint32_t tiles_per_xcd, uint32_t *__restrict__ status,
+                                                       unsigned long long *__restrict__ counters) {
+    extern __shared__ uint32_t lds[];
+    const uint32_t xcd = blockIdx.x & 7u, slot = blockIdx.x >> 3;
+    const uint32_t tile = xcd * tiles_per_xcd + slot;
+    if (tile >= tiles) return;
+    const uint32_t t = threadIdx.x, wave = t >> 6, lane = t & 63u;
+    const uint32_t x = (tile % tiles_x) * kTile + (wave & 1u) * 8u + (lane & 7u);
+    const uint32_t y = (tile / tiles_x) * kTile + (wave >> 1) * 8u + (lane >> 3);
+    const uint32_t ws = rp.width << rp.ssaa, hs = rp.height << rp.ssaa;
+
+    Counters cnt{};
+    Tracer tr;
+    tr.scene_stack = Stack{lds + t, kBlock};
+    tr.mesh_stack = Stack{lds + scene_depth * kBlock + t, kBlock};
+    tr.scene_depth = scene_depth;
+    tr.mesh_depth = mesh_depth;
+    tr.overflow = 0;
+    tr.cnt = kCount ? &cnt : nullptr;
+    if (x < ws && y < hs) {
+        float px[4], depth;
+        render_sample(s, rp, x, y, tr, px, depth);
+        const uint32_t o = canvas_offset(rp, x, y);
+        pixels[o] = make_float4(px[0], px[1], px[2], px[3]);
+        depths[o] = depth;
+        if (tr.overflow) atomicOr(status, 1u);
+    }
+    if (kCount) {
+        const uint64_t *c = &cnt.rays;
+        for (int i = 0; i < 8; i++) {
+            uint64_t v = c[i];
+            for (int off = 32; off > 0; off >>= 1) v += __shfl_down(v, off, 64);
+            if (lane == 0 && v) atomicAdd(counters + i, (unsigned long long)v);
+        }
+    }
+}
+
+template <typename T>
+struct DevBuf {
+    T *ptr = nullptr;
+    size_t cap = 0;   // elements
+};
+
+}  // namespace
+
+struct slimrt_context {
+    bool cpu = false;
+    int device = 0;
+    hipStream_t stream = nullptr, own_stream = nullptr;
+    hipEvent_t ev0 = nullptr, ev1 = nullptr;
+    bool timed = false;
+    std::string err;
+
+    // host mirror of the device arrays (the CPU path renders from these)
+    std::vector<Geo> geos;
+    std::vector<Mat> mats;
+    std::vector<Light> lights;
+    std::vector<Node> snodes, mnodes;
+    std::vector<uint32_t> leaf_ids;
+    std::vector<MeshRec> meshes;
+    std::vector<TriHit> tri_hit;
+    std::vector<TriAttr> tri_attr;
+    std::vector<TexRec> textures;
+    std::vector<MipRec> mips;
+    std::vector<uint8_t> quads;
+    uint32_t scene_height = 0, mesh_height = 0, scene_flags = 0;
+    bool loaded = false;
+
+    DevBuf<Geo> d_geos;
+    DevBuf<Mat> d_mats;
+    DevBuf<Light> d_lights;
+    DevBuf<Node> d_snodes, d_mnodes;
+    DevBuf<uint32_t> d_leaf;
+    DevBuf<MeshRec> d_meshes;
+    DevBuf<TriHit> d_tri_hit;
+    DevBuf<TriAttr> d_tri_attr;
+    DevBuf<TexRec> d_tex;
+    DevBuf<MipRec> d_mips;
+    DevBuf<uint8_t> d_quads;
+    DevBuf<float4> d_pixels;
+    DevBuf<float> d_depths;
+    uint32_t *d_status = nullptr;
+    unsigned long long *d_counters = nullptr;
+};
+
+namespace {
+
+int fail(slimrt_context *c, int code, const std::string &msg) {
+    if (c) c->err = msg;
+    return code;
+}
+
+#define HIP_TRY(ctx, expr)                                                                         \
+    do {                                                                                           \
+        hipError_t e_ = (expr);                                                                    \
+        if (e_ != hipSuccess)                                                                      \
+            return fail(ctx, e_ == hipErrorOutOfMemory ? SLIMRT_E_MEMORY : SLIMRT_E_DEVICE,        \
+                        std::string(#expr) + ": " + hipGetErrorString(e_));                        \
+    } while (0)
+
+template <typename T>
+int dev_upload(slimrt_context *c, DevBuf<T> &b, const T *src, size_t n) {
+    if (n == 0) n = 1;   // keep a valid pointer for empty arrays
+    if (b.cap < n) {
+        if (b.ptr) HIP_TRY(c, hipFree(b.ptr));
+        b.ptr = nullptr;
+        b.cap = 0;
+        HIP_TRY(c, hipMalloc(&b.ptr, n * sizeof(T)));
+        b.cap = n;
+    }
+    if (src) HIP_TRY(c, hipMemcpyAsync(b.ptr, src, n * sizeof(T), hipMemcpyHostToDevice, c->stream));
+    return SLIMRT_OK;
+}
+
+template <typename T>
+void dev_free(DevBuf<T> &b) {
+    if (b.ptr) (void)hipFree(b.ptr);
+    b.ptr = nullptr;
+    b.cap = 0;
+}
+
+Geo to_geo(const slimrt_geometry &g) {
+    Geo o;
+    o.qx = g.orientation[0]; o.qy = g.orientation[1]; o.qz = g.orientation[2]; o.qw = g.orientation[3];
+    o.px = g.position[0]; o.py = g.position[1]; o.pz = g.position[2]; o.type = g.type;
+    o.sx = g.scale[0]; o.sy = g.scale[1]; o.sz = g.scale[2]; o.material_id = g.material_id;
+    o.isx = 1.0f / g.scale[0]; o.isy = 1.0f / g.scale[1]; o.isz = 1.0f / g.scale[2];   // Ray::localize (ray.h:30)
+    o.id_flags = (g.id << 8) | (g.flags & 0xFFu);
+    return o;
+}
+
+Mat to_mat(const slimrt_material &m) {
+    Mat o;
+    o.ar = m.albedo[0]; o.ag = m.albedo[1]; o.ab = m.albedo[2]; o.roughness = m.roughness;
+    o.er = m.emission[0]; o.eg = m.emission[1]; o.eb = m.emission[2]; o.ior = m.ior;
+    o.fr = m.reflectivity[0]; o.fg = m.reflectivity[1]; o.fb = m.reflectivity[2]; o.metalness = m.metalness;
+    o.uv_ru = m.uv_repeat[0]; o.uv_rv = m.uv_repeat[1]; o.normal_magnitude = m.normal_magnitude;
+    o.packed = (m.flags & 0xFFu) | ((m.texture_count & 0xFu) << 8) | ((m.brdf & 0xFu) << 12) |
+               ((m.texture_ids[0] & 0xFFu) << 16) | ((m.texture_ids[1] & 0xFFu) << 24);
+    return o;
+}
+
+Light to_light(const slimrt_light &l) {
+    Light o;
+    o.r = l.color[0]; o.g = l.color[1]; o.b = l.color[2]; o.intensity = l.intensity;
+    o.x = l.position_or_direction[0]; o.y = l.position_or_direction[1]; o.z = l.position_or_direction[2];
+    o.flags = l.flags;
+    return o;
+}
+
+Node to_node(const slimrt_bvh_node &n) {
+    Node o;
+    std::memcpy(&o, &n, sizeof o);   // identical 32-byte layout
+    return o;
+}
+
+SceneView host_view(const slimrt_context *c) {
+    SceneView v;
+    v.geos = c->geos.data(); v.mats = c->mats.data(); v.lights = c->lights.data();
+    v.snodes = c->snodes.data(); v.leaf_ids = c->leaf_ids.data(); v.meshes = c->meshes.data();
+    v.mnodes = c->mnodes.data(); v.tri_hit = c->tri_hit.data(); v.tri_attr = c->tri_attr.data();
+    v.textures = c->textures.data(); v.mips = c->mips.data(); v.quads = c->quads.data();
+    v.G = (uint32_t)c->geos.size(); v.M = (uint32_t)c->mats.size(); v.L = (uint32_t)c->lights.size();
+    v.NM = (uint32_t)c->meshes.size(); v.NT = (uint32_t)c->textures.size(); v.flags = c->scene_flags;
+    return v;
+}
+
+SceneView device_view(const slimrt_context *c) {
+    SceneView v = host_view(c);
+    v.geos = c->d_geos.ptr; v.mats = c->d_mats.ptr; v.lights = c->d_lights.ptr;
+    v.snodes = c->d_snodes.ptr; v.leaf_ids = c->d_leaf.ptr; v.meshes = c->d_meshes.ptr;
+    v.mnodes = c->d_mnodes.ptr; v.tri_hit = c->d_tri_hit.ptr; v.tri_attr = c->d_tri_attr.ptr;
+    v.textures = c->d_tex.ptr; v.mips = c->d_mips.ptr; v.quads = c->d_quads.ptr;
+    return v;
+}
+
+void update_scene_flags(slimrt_context *c) {   // Scene::Scene (scene.h:239-242)
+    c->scene_flags = 0;
+    for (const Geo &g : c->geos)
+        if (g.type == GT_Quad && g.material_id < c->mats.size() && (mat_flags(c->mats[g.material_id]) & MAT_EMISSIVE))
+            c->scene_flags = 1;
+}
+
+int validate_params(slimrt_context *c, const slimrt_render_params *p) {
+    if (!p) return fail(c, SLIMRT_E_ARG, "params is NULL");
+    if (!c->loaded) return fail(c, SLIMRT_E_STATE, "no scene uploaded");
+    if (p->width == 0 || p->height == 0 || p->width > 65535 || p->height > 65535)
+        return fail(c, SLIMRT_E_ARG, "canvas dimensions out of range");
+    if (p->render_mode > 5 || p->output > 1 || p->max_depth > 255) return fail(c, SLIMRT_E_ARG, "invalid settings");
+    const int32_t nt = (int32_t)c->textures.size();
+    if (p->skybox_color >= nt || p->skybox_radiance >= nt || p->skybox_irradiance >= nt)
+        return fail(c, SLIMRT_E_ARG, "skybox texture id out of range");
+    return SLIMRT_OK;
+}
+
+int launch(slimrt_context *c, const slimrt_render_params *p, float4 *pixels, float *depths, bool count) {
+    RenderParams rp;
+    std::memcpy(&rp, p, sizeof rp);
+    rp.ssaa = rp.ssaa ? 1u : 0u;
+    const uint32_t ws = rp.width << rp.ssaa, hs = rp.height << rp.ssaa;
+    const uint32_t tiles_x = (ws + kTile - 1) / kTile, tiles_y = (hs + kTile - 1) / kTile;
+    const uint32_t tiles = tiles_x * tiles_y;
+    const uint32_t per_xcd = (tiles + 7) / 8;
+    const uint32_t scene_depth = c->scene_height + 1, mesh_depth = c->mesh_height + 1;
+    const size_t lds = (size_t)(scene_depth + mesh_depth) * kBlock * sizeof(uint32_t);
+    if (lds > 160 * 1024) return fail(c, SLIMRT_E_ARG, "BVH too deep for the LDS traversal stacks");
+    SceneView v = device_view(c);
+    if (c->timed) HIP_TRY(c, hipEventRecord(c->ev0, c->stream));
+    if (count) {
+        hipLaunchKernelGGL(render_kernel<true>, dim3(per_xcd * 8), dim3(kBlock), lds, c->stream, v, rp, pixels, depths,
+                           scene_depth, mesh_depth, tiles_x, tiles, per_xcd, c->d_status, c->d_counters);
+    } else {
+        hipLaunchKernelGGL(render_kernel<false>, dim3(per_xcd * 8), dim3(kBlock), lds, c->stream, v, rp, pixels, depths,
+                           scene_depth, mesh_depth, tiles_x, tiles, per_xcd, c->d_status, c->d_counters);
+    }
+    HIP_TRY(c, hipGetLastError());
+    if (c->timed) HIP_TRY(c, hipEventRecord(c->ev1, c->stream));
+    return SLIMRT_OK;
+}
+
+int cpu_render(slimrt_context *c, const slimrt_render_params *p, slimrt_pixel *pixels, float *depths, uint64_t *counters) {
+    RenderParams rp;
+    std::memcpy(&rp, p, sizeof rp);
+    rp.ssaa = rp.ssaa ? 1u : 0u;
+    const uint32_t ws = rp.width << rp.ssaa, hs = rp.height << rp.ssaa;
+    const SceneView v = host_view(c);
+    const uint32_t sd = c->scene_height + 1, md = c->mesh_height + 1;
+    unsigned nthreads = std::max(1u, std::min(64u, std::thread::hardware_concurrency()));
+    std::vector<Counters> per(nthreads);
+    std::vector<uint32_t> overflow(nthreads, 0);
+    auto work = [&](unsigned ti) {
+        std::vector<uint32_t> stack(sd + md);
+        Tracer tr;
+        tr.scene_stack = Stack{stack.data(), 1};
+        tr.mesh_stack = Stack{stack.data() + sd, 1};
+        tr.scene_depth = sd;
+        tr.mesh_depth = md;
+        tr.overflow = 0;
+        per[ti] = Counters{};
+        tr.cnt = counters ? &per[ti] : nullptr;
+        for (uint32_t y = ti; y < hs; y += nthreads)
+            for (uint32_t x = 0; x < ws; x++) {
+                float px[4], depth;
+                render_sample(v, rp, x, y, tr, px, depth);
+                if (pixels) {
+                    const uint32_t o = canvas_offset(rp, x, y);
+                    pixels[o] = slimrt_pixel{px[0], px[1], px[2], px[3]};
+                    depths[o] = depth;
+                }
+            }
+        overflow[ti] = tr.overflow;
+    };
+    std::vector<std::thread> pool;
+    for (unsigned i = 0; i < nthreads; i++) pool.emplace_back(work, i);
+    for (auto &th : pool) th.join();
+    if (counters) {
+        std::memset(counters, 0, 8 * sizeof(uint64_t));
+        for (auto &k : per) {
+            const uint64_t *s = &k.rays;
+            for (int i = 0; i < 8; i++) counters[i] += s[i];
+        }
+    }
+    for (uint32_t o : overflow)
+        if (o) return fail(c, SLIMRT_E_OVERFLOW, "traversal stack overflow");
+    return SLIMRT_OK;
+}
+
+int ensure_outputs(slimrt_context *c, size_t n) {
+    if (c->d_pixels.cap < n) {
+        dev_free(c->d_pixels);
+        dev_free(c->d_depths);
+        HIP_TRY(c, hipMalloc(&c->d_pixels.ptr, n * sizeof(float4)));
+        c->d_pixels.cap = n;
+        HIP_TRY(c, hipMalloc(&c->d_depths.ptr, n * sizeof(float)));
+        c->d_depths.cap = n;
+    }
+    return SLIMRT_OK;
+}
+
+int check_status(slimrt_context *c) {
+    uint32_t st = 0;
+    HIP_TRY(c, hipMemcpyAsync(&st, c->d_status, sizeof st, hipMemcpyDeviceToHost, c->stream));
+    HIP_TRY(c, hipStreamSynchronize(c->stream));
+    if (st) {
+        HIP_TRY(c, hipMemsetAsync(c->d_status, 0, sizeof st, c->stream));
+        HIP_TRY(c, hipStreamSynchronize(c->stream));
+        return fail(c, SLIMRT_E_OVERFLOW, "traversal stack overflow");
+    }
+    return SLIMRT_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int slimrt_create(slimrt_context **out, uint32_t flags) {
+    if (!out) return SLIMRT_E_ARG;
+    *out = nullptr;
+    auto *c = new slimrt_context();
+    c->cpu = (flags & SLIMRT_DEVICE_CPU) != 0;
+    if (!c->cpu) {
+        int n = 0;
+        hipError_t e = hipGetDeviceCount(&n);
+        if (e != hipSuccess || n == 0) {
+            std::string msg = std::string("no HIP device available: ") + (e != hipSuccess ? hipGetErrorString(e) : "0 devices");
+            delete c;
+            fprintf(stderr, "slimrt_create: %s\n", msg.c_str());
+            return SLIMRT_E_DEVICE;
+        }
+        (void)hipGetDevice(&c->device);
+        if (hipStreamCreateWithFlags(&c->own_stream, hipStreamNonBlocking) != hipSuccess ||
+            hipEventCreate(&c->ev0) != hipSuccess || hipEventCreate(&c->ev1) != hipSuccess ||
+            hipMalloc(&c->d_status, sizeof(uint32_t)) != hipSuccess ||
+            hipMalloc(&c->d_counters, 8 * sizeof(unsigned long long)) != hipSuccess ||
+            hipMemset(c->d_status, 0, sizeof(uint32_t)) != hipSuccess) {
+            delete c;
+            return SLIMRT_E_DEVICE;
+        }
+        c->stream = c->own_stream;
+    }
+    *out = c;
+    return SLIMRT_OK;
+}
+
+int slimrt_destroy(slimrt_context *c) {
+    if (!c) return SLIMRT_E_ARG;
+    if (!c->cpu) {
+        (void)hipStreamSynchronize(c->stream);
+        dev_free(c->d_geos); dev_free(c->d_mats); dev_free(c->d_lights); dev_free(c->d_snodes); dev_free(c->d_mnodes);
+        dev_free(c->d_leaf); dev_free(c->d_meshes); dev_free(c->d_tri_hit); dev_free(c->d_tri_attr); dev_free(c->d_tex);
+        dev_free(c->d_mips); dev_free(c->d_quads); dev_free(c->d_pixels); dev_free(c->d_depths);
+        if (c->d_status) (void)hipFree(c->d_status);
+        if (c->d_counters) (void)hipFree(c->d_counters);
+        if (c->ev0) (void)hipEventDestroy(c->ev0);
+        if (c->ev1) (void)hipEventDestroy(c->ev1);
+        if (c->own_stream) (void)hipStreamDestroy(c->own_stream);
+    }
+    delete c;
+    return SLIMRT_OK;
+}
+
+const char *slimrt_last_error(const slimrt_context *c) { return c ? c->err.c_str() : "null context"; }
+
+int slimrt_set_stream(slimrt_context *c, void *stream) {
+    if (!c) return SLIMRT_E_ARG;
+    if (c->cpu) return SLIMRT_OK;
+    c->stream = stream ? (hipStream_t)stream : c->own_stream;
+    return SLIMRT_OK;
+}
+
+int slimrt_upload_geometries(slimrt_context *c, const slimrt_geometry *g, uint32_t count) {
+    if (!c || (!g && count)) return SLIMRT_E_ARG;
+    if (c->loaded && count != c->geos.size()) return fail(c, SLIMRT_E_ARG, "geometry count differs from the scene's");
+    c->geos.resize(count);
+    for (uint32_t i = 0; i < count; i++) {
+        if (g[i].type == GT_Mesh && g[i].id >= c->meshes.size() && c->loaded)
+            return fail(c, SLIMRT_E_ARG, "geometry references a missing mesh");
+        c->geos[i] = to_geo(g[i]);
+    }
+    update_scene_flags(c);
+    return c->cpu ? SLIMRT_OK : dev_upload(c, c->d_geos, c->geos.data(), c->geos.size());
+}
+
+int slimrt_upload_materials(slimrt_context *c, const slimrt_material *m, uint32_t count) {
+    if (!c || (!m && count)) return SLIMRT_E_ARG;
+    c->mats.resize(count);
+    for (uint32_t i = 0; i < count; i++) c->mats[i] = to_mat(m[i]);
+    update_scene_flags(c);
+    return c->cpu ? SLIMRT_OK : dev_upload(c, c->d_mats, c->mats.data(), c->mats.size());
+}
+
+int slimrt_upload_lights(slimrt_context *c, const slimrt_light *l, uint32_t count) {
+    if (!c || (!l && count)) return SLIMRT_E_ARG;
+    c->lights.resize(count);
+    for (uint32_t i = 0; i < count; i++) c->lights[i] = to_light(l[i]);
+    return c->cpu ? SLIMRT_OK : dev_upload(c, c->d_lights, c->lights.data(), c->lights.size());
+}
+
+int slimrt_upload_scene_bvh(slimrt_context *c, const slimrt_bvh_node *nodes, uint32_t node_count, const uint32_t *leaf_ids,
+                            uint32_t height) {
+    if (!c || !nodes || !leaf_ids || node_count == 0) return SLIMRT_E_ARG;
+    const uint32_t G = (uint32_t)c->geos.size();
+    c->snodes.resize(node_count);
+    for (uint32_t i = 0; i < node_count; i++) {
+        c->snodes[i] = to_node(nodes[i]);
+        const uint32_t lc = nodes[i].leaf_count;
+        // children always follow their parent (BVHBuilder::splitNode appends them), which also rules out cycles
+        const bool bad = lc ? (nodes[i].first_index + lc > G)
+                            : (nodes[i].first_index <= i || nodes[i].first_index + 1 >= node_count);
+        if (bad) return fail(c, SLIMRT_E_ARG, "scene BVH node index out of range");
+    }
+    c->leaf_ids.assign(leaf_ids, leaf_ids + G);
+    for (uint32_t id : c->leaf_ids)
+        if (id >= G) return fail(c, SLIMRT_E_ARG, "scene BVH leaf id out of range");
+    c->scene_height = height;
+    if (c->cpu) return SLIMRT_OK;
+    int rc = dev_upload(c, c->d_snodes, c->snodes.data(), c->snodes.size());
+    return rc ? rc : dev_upload(c, c->d_leaf, c->leaf_ids.data(), c->leaf_ids.size());
+}
+
+int slimrt_upload_scene(slimrt_context *c, const slimrt_scene_desc *d) {
+    if (!c || !d) return SLIMRT_E_ARG;
+    if (d->geometry_count == 0 || !d->geometries || !d->scene_nodes || !d->scene_leaf_ids)
+        return fail(c, SLIMRT_E_ARG, "a scene needs geometries and a scene BVH");
+    c->loaded = false;
+    // meshes: concatenate nodes and split triangle records
+    c->meshes.clear(); c->mnodes.clear(); c->tri_hit.clear(); c->tri_attr.clear();
+    c->mesh_height = 0;
+    for (uint32_t k = 0; k < d->mesh_count; k++) {
+        const slimrt_mesh &m = d->meshes[k];
+        if (!m.nodes || !m.triangles || m.node_count == 0 || m.triangle_count == 0)
+            return fail(c, SLIMRT_E_ARG, "mesh without nodes or triangles");
+        MeshRec r;
+        r.minx = m.aabb_min[0]; r.miny = m.aabb_min[1]; r.minz = m.aabb_min[2];
+        r.maxx = m.aabb_max[0]; r.maxy = m.aabb_max[1]; r.maxz = m.aabb_max[2];
+        r.node_base = (uint32_t)c->mnodes.size();
+        r.tri_base = (uint32_t)c->tri_hit.size();
+        r.node_count = m.node_count; r.tri_count = m.triangle_count;
+        r.normals_count = m.normals_count; r.uvs_count = m.uvs_count;
+        c->meshes.push_back(r);
+        for (uint32_t i = 0; i < m.node_count; i++) {
+            const slimrt_bvh_node &n = m.nodes[i];
+            const bool bad = n.leaf_count ? (n.first_index + n.leaf_count > m.triangle_count)
+                                          : (n.first_index <= i || n.first_index + 1 >= m.node_count);
+            if (bad) return fail(c, SLIMRT_E_ARG, "mesh BVH node index out of range");
+            c->mnodes.push_back(to_node(n));
+        }
+        for (uint32_t i = 0; i < m.triangle_count; i++) {
+            const slimrt_triangle &t = m.triangles[i];
+            const float *L = t.local_to_tangent;
+            c->tri_hit.push_back(TriHit{L[0], L[1], L[2], t.position[0], L[3], L[4], L[5], t.position[1],
+                                        L[6], L[7], L[8], t.position[2], t.normal[0], t.normal[1], t.normal[2], t.uv_coverage});
+            c->tri_attr.push_back(TriAttr{t.n1[0], t.n1[1], t.n1[2], t.uv1[0], t.n2[0], t.n2[1], t.n2[2], t.uv1[1],
+                                          t.n3[0], t.n3[1], t.n3[2], t.uv2[0], t.uv2[1], t.uv3[0], t.uv3[1], 0.0f});
+        }
+        c->mesh_height = std::max(c->mesh_height, m.height);
+    }
+    // textures: flatten mips and texel quads
+    c->textures.clear(); c->mips.clear(); c->quads.clear();
+    for (uint32_t k = 0; k < d->texture_count; k++) {
+        const slimrt_texture &t = d->textures[k];
+        if (!t.mips || t.mip_count == 0) return fail(c, SLIMRT_E_ARG, "texture without mips");
+        c->textures.push_back(TexRec{t.width, t.height, t.flags, t.mip_count, (uint32_t)c->mips.size(), 0, 0, 0});
+        for (uint32_t m = 0; m < t.mip_count; m++) {
+            const slimrt_texture_mip &mm = t.mips[m];
+            if (!mm.texel_quads || mm.width == 0 || mm.height == 0) return fail(c, SLIMRT_E_ARG, "empty texture mip");
+            const size_t nq = (size_t)(mm.width + 1) * (mm.height + 1);
+            c->mips.push_back(MipRec{mm.width, mm.height, (uint32_t)(c->quads.size() / 12), 0});
+            c->quads.insert(c->quads.end(), mm.texel_quads, mm.texel_quads + 12 * nq);
+        }
+    }
+    for (uint32_t i = 0; i < d->material_count; i++) {
+        const slimrt_material &m = d->materials[i];
+        for (uint32_t s = 0; s < std::min<uint32_t>(m.texture_count, 2); s++)
+            if (m.texture_ids[s] >= d->texture_count) return fail(c, SLIMRT_E_ARG, "material texture id out of range");
+    }
+    for (uint32_t i = 0; i < d->geometry_count; i++) {
+        const slimrt_geometry &g = d->geometries[i];
+        if (g.material_id >= d->material_count) return fail(c, SLIMRT_E_ARG, "geometry material id out of range");
+        if (g.type == GT_Mesh && g.id >= d->mesh_count) return fail(c, SLIMRT_E_ARG, "geometry mesh id out of range");
+    }
+    int rc;
+    c->geos.clear();
+    if ((rc = slimrt_upload_materials(c, d->materials, d->material_count))) return rc;
+    if ((rc = slimrt_upload_geometries(c, d->geometries, d->geometry_count))) return rc;
+    if ((rc = slimrt_upload_lights(c, d->lights, d->light_count))) return rc;
+    if ((rc = slimrt_upload_scene_bvh(c, d->scene_nodes, d->scene_node_count, d->scene_leaf_ids, d->scene_height))) return rc;
+    if (!c->cpu) {
+        if ((rc = dev_upload(c, c->d_meshes, c->meshes.data(), c->meshes.size()))) return rc;
+        if ((rc = dev_upload(c, c->d_mnodes, c->mnodes.data(), c->mnodes.size()))) return rc;
+        if ((rc = dev_upload(c, c->d_tri_hit, c->tri_hit.data(), c->tri_hit.size()))) return rc;
+        if ((rc = dev_upload(c, c->d_tri_attr, c->tri_attr.data(), c->tri_attr.size()))) return rc;
+        if ((rc = dev_upload(c, c->d_tex, c->textures.data(), c->textures.size()))) return rc;
+        if ((rc = dev_upload(c, c->d_mips, c->mips.data(), c->mips.size()))) return rc;
+        if ((rc = dev_upload(c, c->d_quads, c->quads.data(), c->quads.size()))) return rc;
+        HIP_TRY(c, hipStreamSynchronize(c->stream));
+    }
+    c->loaded = true;
+    return SLIMRT_OK;
+}
+
+int slimrt_render_device(slimrt_context *c, const slimrt_render_params *p, void *d_pixels, void *d_depths) {
+    if (!c) return SLIMRT_E_ARG;
+    int rc = validate_params(c, p);
+    if (rc) return rc;
+    if (c->cpu) return fail(c, SLIMRT_E_STATE, "render_device needs a GPU context");
+    if (!d_pixels || !d_depths) return fail(c, SLIMRT_E_ARG, "output buffers are NULL");
+    c->timed = true;
+    return launch(c, p, (float4 *)d_pixels, (float *)d_depths, false);
+}
+
+int slimrt_synchronize(slimrt_context *c) {
+    if (!c) return SLIMRT_E_ARG;
+    if (c->cpu) return SLIMRT_OK;
+    return check_status(c);
+}
+
+int slimrt_render(slimrt_context *c, const slimrt_render_params *p, slimrt_pixel *pixels, float *depths) {
+    if (!c) return SLIMRT_E_ARG;
+    int rc = validate_params(c, p);
+    if (rc) return rc;
+    if (!pixels || !depths) return fail(c, SLIMRT_E_ARG, "output buffers are NULL");
+    if (c->cpu) return cpu_render(c, p, pixels, depths, nullptr);
+    const size_t n = (size_t)p->width * p->height * (p->ssaa ? 4 : 1);
+    if ((rc = ensure_outputs(c, n))) return rc;
+    c->timed = true;
+    if ((rc = launch(c, p, c->d_pixels.ptr, c->d_depths.ptr, false))) return rc;
+    HIP_TRY(c, hipMemcpyAsync(pixels, c->d_pixels.ptr, n * sizeof(float4), hipMemcpyDeviceToHost, c->stream));
+    HIP_TRY(c, hipMemcpyAsync(depths, c->d_depths.ptr, n * sizeof(float), hipMemcpyDeviceToHost, c->stream));
+    return check_status(c);
+}
+
+int slimrt_count_work(slimrt_context *c, const slimrt_render_params *p, uint64_t counters[8]) {
+    if (!c || !counters) return SLIMRT_E_ARG;
+    int rc = validate_params(c, p);
+    if (rc) return rc;
+    if (c->cpu) return cpu_render(c, p, nullptr, nullptr, counters);
+    const size_t n = (size_t)p->width * p->height * (p->ssaa ? 4 : 1);
+    if ((rc = ensure_outputs(c, n))) return rc;
+    HIP_TRY(c, hipMemsetAsync(c->d_counters, 0, 8 * sizeof(unsigned long long), c->stream));
+    c->timed = false;
+    if ((rc = launch(c, p, c->d_pixels.ptr, c->d_depths.ptr, true))) return rc;
+    HIP_TRY(c, hipMemcpyAsync(counters, c->d_counters, 8 * sizeof(uint64_t), hipMemcpyDeviceToHost, c->stream));
+    return check_status(c);
+}
+
+int slimrt_last_kernel_ms(slimrt_context *c, float *ms) {
+    if (!c || !ms) return SLIMRT_E_ARG;
+    if (c->cpu || !c->timed) return fail(c, SLIMRT_E_STATE, "no timed GPU frame");
+    HIP_TRY(c, hipEventSynchronize(c->ev1));
+    HIP_TRY(c, hipEventElapsedTime(ms, c->ev0, c->ev1));
+    return SLIMRT_OK;
+}
+
+}  // extern "C"

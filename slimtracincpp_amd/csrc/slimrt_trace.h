@@ -1,0 +1,601 @@
+// Two-level BVH traversal, shading and the per-pixel integrator (__host__ __device__).
+//
+// Traversal visits sibling pairs as the reference does (SceneTracer::trace, src/slim/scene/scene_tracer.h:26-98;
+// MeshTracer::trace, src/slim/scene/mesh_tracer.h:51-134): both children of a pair are slab-tested, leaves are
+// intersected in place, the nearer of two hit inner children is entered and the other pushed.  The traversal
+// stacks live wherever the caller puts them: on the GPU in LDS, one column per thread (element i of a thread's
+// stack at base[i * stride]), on the CPU in a local array (stride 1).
+#pragma once
+
+#include "slimrt_core.h"
+
+namespace slimrt {
+
+struct Stack {
+    uint32_t *base;
+    uint32_t stride;
+    SLIM_HD uint32_t &operator[](uint32_t i) const { return base[i * stride]; }
+};
+
+struct Counters {   // work counters, accumulated only when the caller asks for them
+    uint64_t rays, pairs, roots, instances, triangles, mesh_hits, taps, pixels;
+};
+
+struct Tracer {
+    Stack scene_stack, mesh_stack;
+    uint32_t scene_depth, mesh_depth;   // capacities
+    uint32_t overflow;                  // set when a push would exceed a capacity
+    Ray aux_ray;                        // last localized ray (SceneTracer::aux_ray)
+    Hit aux_hit;                        // SceneTracer::aux_hit
+    Counters *cnt;                      // null: no counting
+};
+
+#define SLIM_COUNT(tr, field) do { if ((tr).cnt) (tr).cnt->field++; } while (0)
+
+SLIM_HD Node load_node(const Node *nodes, uint32_t i) { return nodes[i]; }
+
+// MeshTracer::hitTriangles (mesh_tracer.h:23-49); triangle indices are relative to the mesh.
+SLIM_HD bool hit_triangles(const SceneView &s, const MeshRec &m, uint32_t first, uint32_t n, float closest,
+                           const Ray &r, Hit &hit, bool any_hit, Tracer &tr) {
+    bool found = false;
+    closest = fmin2(closest, hit.distance);
+    Hit th = hit;
+    for (uint32_t i = 0; i < n; i++) {
+        const TriHit t = s.tri_hit[m.tri_base + first + i];
+        SLIM_COUNT(tr, triangles);
+        v3 tp{t.px, t.py, t.pz};
+        if (hits_plane(r, tp, v3{t.nx, t.ny, t.nz}, th)) {
+            v3 d = th.position - tp;
+            float ux = t.xx * d.x + t.yx * d.y + t.zx * d.z;
+            float uy = t.xy * d.x + t.yy * d.y + t.zy * d.z;
+            if (ux < 0 || uy < 0 || (ux + uy) > 1 || th.distance >= closest) continue;
+            closest = th.distance;
+            hit = th;
+            hit.uv = v2{ux, uy};
+            hit.uv_coverage = t.uv_coverage;
+            hit.id = first + i;
+            found = true;
+            if (any_hit) break;
+        }
+    }
+    return found;
+}
+
+// MeshTracer::trace (mesh_tracer.h:51-134)
+SLIM_HD bool mesh_trace(const SceneView &s, uint32_t mesh_id, const Ray &r, Hit &hit, bool any_hit, Tracer &tr) {
+    const MeshRec m = s.meshes[mesh_id];
+    const Node *nodes = s.mnodes + m.node_base;
+    float ln, rn, lf, rf;
+    bool found = false;
+    SLIM_COUNT(tr, roots);
+    const Node root = load_node(nodes, 0);
+    if (!(hits_node(r, root, ln, lf) && ln < hit.distance)) return false;
+    if (root.leaf_count_depth & 0xFFFFu) return hit_triangles(s, m, 0, m.tri_count, lf, r, hit, any_hit, tr);
+    uint32_t pair = root.first_index, top = 0;
+    for (;;) {
+        SLIM_COUNT(tr, pairs);
+        const Node L = load_node(nodes, pair), R = load_node(nodes, pair + 1);
+        bool hl = hits_node(r, L, ln, lf) && ln < hit.distance;
+        bool hr = hits_node(r, R, rn, rf) && rn < hit.distance;
+        bool go_l = false, go_r = false;
+        if (hl) {
+            uint32_t lc = L.leaf_count_depth & 0xFFFFu;
+            if (lc) {
+                if (hit_triangles(s, m, L.first_index, lc, lf, r, hit, any_hit, tr)) {
+                    found = true;
+                    if (any_hit) break;
+                }
+            } else go_l = true;
+        }
+        if (hr) {
+            uint32_t rc = R.leaf_count_depth & 0xFFFFu;
+            if (rc) {
+                if (hit_triangles(s, m, R.first_index, rc, rf, r, hit, any_hit, tr)) {
+                    found = true;
+                    if (any_hit) break;
+                }
+            } else go_r = true;
+        }
+        if (go_l) {
+            if (go_r) {
+                uint32_t near_pair = L.first_index, far_pair = R.first_index;
+                if (!any_hit && ln > rn) { near_pair = R.first_index; far_pair = L.first_index; }
+                if (top < tr.mesh_depth) tr.mesh_stack[top++] = far_pair; else tr.overflow = 1;
+                pair = near_pair;
+            } else pair = L.first_index;
+        } else if (go_r) {
+            pair = R.first_index;
+        } else {
+            if (top == 0) break;
+            pair = tr.mesh_stack[--top];
+        }
+    }
+    if (found && !any_hit && (m.normals_count | m.uvs_count)) {
+        const TriAttr t = s.tri_attr[m.tri_base + hit.id];
+        float a = hit.uv.u, b = hit.uv.v, c = 1 - a - b;
+        SLIM_COUNT(tr, mesh_hits);
+        if (m.uvs_count) {
+            hit.uv.u = t.uv3u * a + (t.uv2u * b + t.uv1u * c);
+            hit.uv.v = t.uv3v * a + (t.uv2v * b + t.uv1v * c);
+        }
+        if (m.normals_count) {
+            hit.normal.x = t.n3x * a + (t.n2x * b + t.n1x * c);
+            hit.normal.y = t.n3y * a + (t.n2y * b + t.n1y * c);
+            hit.normal.z = t.n3z * a + (t.n2z * b + t.n1z * c);
+        }
+    }
+    return found;
+}
+
+// SceneTracer::hitGeometryInLocalSpace (scene_tracer.h:138-165)
+SLIM_HD bool hit_geometry_local(const SceneView &s, const Geo &g, const Ray &r, Hit &hit, bool any_hit, Tracer &tr) {
+    SLIM_COUNT(tr, instances);
+    ray_localize(tr.aux_ray, r.origin, r.direction, g);
+    float n, f;
+    bool box_hit;
+    if (g.type == GT_Mesh) {
+        const MeshRec &m = s.meshes[geo_id(g)];
+        box_hit = hits_box_bounds(tr.aux_ray, m.minx, m.miny, m.minz, m.maxx, m.maxy, m.maxz, n, f);
+    } else {
+        float mx = g.type == GT_Tet ? kTetMax : 1.0f;
+        float ylo = -mx, yhi = mx;
+        if (g.type == GT_Quad) { ylo = -kEps; yhi = kEps; }
+        box_hit = hits_box_bounds(tr.aux_ray, -mx, ylo, -mx, mx, yhi, mx, n, f);
+    }
+    if (!box_hit) return false;
+    const bool transparent = (geo_flags(g) & GEO_TRANSPARENT) != 0;
+    switch (g.type) {
+        case GT_Quad: return hits_quad(tr.aux_ray, hit, transparent);
+        case GT_Box: return hits_box(tr.aux_ray, hit, transparent);
+        case GT_Sphere: return hits_sphere(tr.aux_ray, hit, transparent, nullptr);
+        case GT_Tet: return hits_tet(tr.aux_ray, hit, transparent);
+        case GT_Mesh: return mesh_trace(s, geo_id(g), tr.aux_ray, hit, any_hit, tr);
+        default: return false;
+    }
+}
+
+// SceneTracer::hitGeometries (scene_tracer.h:111-136); returns a geometry index or -1.
+SLIM_HD int hit_geometries(const SceneView &s, uint32_t first_leaf, uint32_t n, float closest, const Ray &r, Hit &hit,
+                           bool any_hit, Tracer &tr) {
+    int hit_geo = -1;
+    const uint32_t vis = any_hit ? GEO_SHADOWING : GEO_VISIBLE;
+    tr.aux_hit.distance = fmin2(closest + kEps, hit.distance);
+    for (uint32_t i = 0; i < n; i++) {
+        const uint32_t gi = s.leaf_ids[first_leaf + i];
+        const Geo g = s.geos[gi];
+        if (!(geo_flags(g) & vis)) continue;
+        if (hit_geometry_local(s, g, r, tr.aux_hit, any_hit, tr)) {
+            if (any_hit) return (int)gi;
+            if (tr.aux_hit.distance < hit.distance) {
+                hit_geo = (int)gi;
+                hit = tr.aux_hit;
+                hit.NdotRd = -(dot(hit.normal, tr.aux_ray.direction));
+            }
+        }
+    }
+    return hit_geo;
+}
+
+// SceneTracer::trace (scene_tracer.h:26-98); returns a geometry index or -1.  Offsets the ray origin.
+SLIM_HD int scene_trace(const SceneView &s, Ray &r, Hit &hit, bool any_hit, float max_distance, Tracer &tr) {
+    SLIM_COUNT(tr, rays);
+    ray_reset(r, scale_add(r.direction, kTraceOffset, r.origin), r.direction);
+    hit.distance = max_distance;
+    float ln, rn, lf, rf;
+    SLIM_COUNT(tr, roots);
+    const Node root = load_node(s.snodes, 0);
+    if (!(hits_node(r, root, ln, lf) && ln < hit.distance)) return -1;
+    if (root.leaf_count_depth & 0xFFFFu) return hit_geometries(s, 0, s.G, lf, r, hit, any_hit, tr);
+    int closest = -1;
+    uint32_t pair = root.first_index, top = 0;
+    for (;;) {
+        SLIM_COUNT(tr, pairs);
+        const Node L = load_node(s.snodes, pair), R = load_node(s.snodes, pair + 1);
+        bool hl = hits_node(r, L, ln, lf) && ln < hit.distance;
+        bool hr = hits_node(r, R, rn, rf) && rn < hit.distance;
+        bool go_l = false, go_r = false;
+        if (hl) {
+            uint32_t lc = L.leaf_count_depth & 0xFFFFu;
+            if (lc) {
+                int g = hit_geometries(s, L.first_index, lc, lf, r, hit, any_hit, tr);
+                if (g >= 0) { closest = g; if (any_hit) break; }
+            } else go_l = true;
+        }
+        if (hr) {
+            uint32_t rc = R.leaf_count_depth & 0xFFFFu;
+            if (rc) {
+                int g = hit_geometries(s, R.first_index, rc, rf, r, hit, any_hit, tr);
+                if (g >= 0) { closest = g; if (any_hit) break; }
+            } else go_r = true;
+        }
+        if (go_l) {
+            if (go_r) {
+                uint32_t near_pair = L.first_index, far_pair = R.first_index;
+                if (!any_hit && ln > rn) { near_pair = R.first_index; far_pair = L.first_index; }
+                if (top < tr.scene_depth) tr.scene_stack[top++] = far_pair; else tr.overflow = 1;
+                pair = near_pair;
+            } else pair = L.first_index;
+        } else if (go_r) {
+            pair = R.first_index;
+        } else {
+            if (top == 0) break;
+            pair = tr.scene_stack[--top];
+        }
+    }
+    return closest;
+}
+
+// ------------------------------------------------------------------ shading (src/slim/scene/material.h, renderer/surface_shader.h)
+SLIM_HD float ggx_D(float roughness, float NdotH) {   // material.h:27-36
+    float a = roughness * roughness;
+    float denom = NdotH * NdotH * (a - 1.0f) + 1.0f;
+    return a / (kPi * denom * denom);
+}
+SLIM_HD float ggx_G(float roughness, float NdotL, float NdotV) {   // material.h:38-55
+    float a = roughness * roughness, k = a * 0.5f, omk = 1.0f - k;
+    float denom = NdotV * omk + k;
+    float result = NdotV / fmaxf(denom, kEps);
+    denom = NdotL * omk + k;
+    result *= NdotL / fmaxf(denom, kEps);
+    return result;
+}
+SLIM_HD color schlick(float HdotL, color F0) {   // material.h:57-59
+    float p = powf(1.0f - HdotL, 5.0f);
+    return F0 + one_minus(F0) * p;
+}
+SLIM_HD float tone_map(float x) {   // toneMappedBaked, base.h:777-786
+    x = x - 0.004f;
+    if (x < 0.0f) x = 0.0f;
+    float x2s = x * x * 6.2f;
+    return (x2s + x * 0.5f) / (x2s + x * 1.7f + 0.06f);
+}
+
+struct Surface {
+    int geo;
+    Mat mat;
+    color F, albedo_from_map, Fs, Fd;
+    v3 P, N, V, L, R, RF;
+    float Ld, Ld2, NdotL, NdotV;
+    bool refracted;
+};
+
+SLIM_HD color sample_slot(const SceneView &s, const Mat &m, int slot, v2 uv, float coverage) {   // surface_shader.h:9-11
+    return (s.NT && mat_tex_count(m) > (uint32_t)slot) ? tex_sample(s, mat_tex(m, slot), uv.u, uv.v, coverage) : cs(0);
+}
+
+SLIM_HD v3 rotate_normal(v3 normal, color smp, float magnitude) {   // material.h:83-86
+    v3 N = normalized(scale_add(v3{smp.r, smp.b, smp.g}, 2.0f, vs(-1.0f)));
+    quat q = axis_angle(normalized(v3{N.z, 0, -N.x}), acosf(N.y) * magnitude);
+    return qrot(q, normal);
+}
+
+SLIM_HD void radiance_fraction(Surface &sf) {   // surface_shader.h:109-145
+    const Mat &m = sf.mat;
+    sf.Fs = cs(0);
+    sf.Fd = color{m.ar, m.ag, m.ab} * sf.albedo_from_map;
+    const uint32_t brdf = mat_brdf(m);
+    if (brdf == BRDF_CookTorrance) {
+        sf.Fd = sf.Fd * ((1.0f - m.metalness) * kOneOverPi);
+        if (sf.NdotV > 0.0f && m.roughness > 0.0f) {
+            v3 H = normalized(sf.L + sf.V);
+            float NdotH = clamp01(dot(sf.N, H));
+            float HdotL = clamp01(dot(H, sf.L));
+            sf.F = schlick(HdotL, color{m.fr, m.fg, m.fb});
+            float D = ggx_D(m.roughness, NdotH), G = ggx_G(m.roughness, sf.NdotL, sf.NdotV);
+            sf.Fs = sf.F * (D * G / (4.0f * sf.NdotL * sf.NdotV));
+            sf.Fd = sf.Fd * one_minus(sf.F);
+        }
+    } else {
+        sf.Fd = sf.Fd * (m.roughness * kOneOverPi);
+        if (brdf != BRDF_Lambert) {
+            float spec, exponent;
+            if (brdf == BRDF_Phong) { exponent = 4.0f; spec = clamp01(dot(sf.R, sf.L)); }
+            else { exponent = 16.0f; spec = clamp01(dot(sf.N, normalized(sf.L + sf.V))); }
+            if (spec > 0.0f) sf.Fs = color{m.fr, m.fg, m.fb} * (powf(spec, exponent) * (1.0f - m.roughness));
+        }
+    }
+}
+
+SLIM_HD bool facing_light(Surface &sf, const Light &l) {   // surface_shader.h:147-160
+    if (l.flags & LIGHT_DIRECTIONAL) {
+        sf.Ld = INFINITY;
+        sf.Ld2 = 1.0f;
+        sf.L = -v3{l.x, l.y, l.z};
+    } else {
+        sf.L = v3{l.x, l.y, l.z} - sf.P;
+        sf.Ld2 = sqlen(sf.L);
+        sf.Ld = sqrtf(sf.Ld2);
+        sf.L = divs(sf.L, sf.Ld);
+    }
+    sf.NdotL = clamp01(dot(sf.L, sf.N));
+    return sf.NdotL > 0.0f;
+}
+
+SLIM_HD void shade_from_light(Surface &sf, const Light &l, const SceneView &s, Tracer &tr, color &c) {   // surface_shader.h:43-59
+    if (facing_light(sf, l)) {
+        bool lit = true;
+        if (l.flags & LIGHT_SHADOWING) {
+            Ray shadow;
+            shadow.origin = sf.P;
+            shadow.direction = sf.L;
+            Hit sh;
+            lit = scene_trace(s, shadow, sh, true, sf.Ld, tr) < 0;
+        }
+        if (lit) {
+            radiance_fraction(sf);
+            c = cmul_add(sf.Fs + sf.Fd, color{l.r, l.g, l.b} * (sf.NdotL * l.intensity / sf.Ld2), c);
+        }
+    }
+}
+
+SLIM_HD void prepare_for_shading(Surface &sf, const SceneView &s, const Ray &r, Hit &hit, Tracer &tr) {   // surface_shader.h:61-107
+    const Geo g = s.geos[sf.geo];
+    sf.mat = s.mats[g.material_id];
+    const Mat &m = sf.mat;
+    hit.uv.u *= m.uv_ru;
+    hit.uv.v *= m.uv_rv;
+    if (hit.from_behind) hit.normal = -hit.normal;
+    float cone = hit.distance * hit.scaling_factor;
+    cone *= cone;
+    cone *= cone;
+    hit.uv_coverage *= cone / (m.uv_ru * m.uv_rv * hit.NdotRd * fabsf(dot(v3{1.0f - hit.normal.x, 1.0f - hit.normal.y, 1.0f - hit.normal.z}, v3{g.sx, g.sy, g.sz})));
+    hit.position = ray_at(r, hit.distance);
+    hit.normal = normalized(qrot(geo_quat(g), v3{g.sx, g.sy, g.sz} * hit.normal));
+    const uint32_t mf = mat_flags(m);
+    if (mat_tex_count(m) && (mf & MAT_NORMAL_MAP)) {
+        if (s.NT && mat_tex_count(m) > 1) SLIM_COUNT(tr, taps);
+        hit.normal = rotate_normal(hit.normal, sample_slot(s, m, 1, hit.uv, hit.uv_coverage), m.normal_magnitude);
+    }
+    sf.P = hit.position;
+    sf.N = hit.normal;
+    sf.R = sf.RF = reflected(r.direction, sf.N);
+    sf.V = -r.direction;
+    sf.NdotV = clamp01(dot(sf.N, sf.V));
+    const bool albedo_map = mat_tex_count(m) && (mf & MAT_ALBEDO_MAP);
+    if (albedo_map && s.NT) SLIM_COUNT(tr, taps);
+    sf.albedo_from_map = albedo_map ? sample_slot(s, m, 0, hit.uv, hit.uv_coverage) : cs(1.0f);
+    sf.refracted = (mf & MAT_REFRACTIVE) != 0;
+    if (sf.refracted) {
+        float ior = hit.from_behind ? (m.ior / kIorAir) : (kIorAir / m.ior);
+        float c = ior * ior * (1.0f - (sf.NdotV * sf.NdotV));
+        sf.refracted = c < 1.0f;
+        if (sf.refracted) sf.RF = normalized(scale_add(sf.N, ior * sf.NdotV - sqrtf(1 - c), r.direction * ior));
+    }
+}
+
+// Irradiance vector of an emissive quad (surface_shader.h:162-186)
+SLIM_HD v3 area_light_vector(const Surface &sf, const Geo &g, v3 *quad_v) {
+    const float sx = g.sx, sz = g.sz;
+    if (sx == 0 || sz == 0) return vs(0.0f);
+    quat q = geo_quat(g);
+    v3 U = qrot(q, v3{sx < 0 ? -sx : sx, 0.0f, 0.0f});
+    v3 W = qrot(q, v3{0.0f, 0.0f, sz < 0 ? -sz : sz});
+    v3 p{g.px, g.py, g.pz};
+    quad_v[0] = (p - U) - W;
+    quad_v[1] = (p + U) - W;
+    quad_v[2] = (p + U) + W;
+    quad_v[3] = (p - U) + W;
+    v3 u1 = normalized(quad_v[0] - sf.P), u2 = normalized(quad_v[1] - sf.P);
+    v3 u3 = normalized(quad_v[2] - sf.P), u4 = normalized(quad_v[3] - sf.P);
+    return ((cross(u1, u2) * (acosf(dot(u1, u2)) * 0.5f) + cross(u2, u3) * (acosf(dot(u2, u3)) * 0.5f)) +
+            cross(u3, u4) * (acosf(dot(u3, u4)) * 0.5f)) + cross(u4, u1) * (acosf(dot(u4, u1)) * 0.5f);
+}
+
+// SurfaceShader::shadeFromEmissiveQuads (surface_shader.h:188-264): heuristic occlusion over all geometries.
+SLIM_HD void shade_from_emissive_quads(Surface &sf, const SceneView &s, color &c) {
+    v3 quad_v[4];
+    for (uint32_t i = 0; i < s.G; i++) {
+        const Geo eq = s.geos[i];
+        if ((int)i == sf.geo || eq.type != GT_Quad || !(mat_flags(s.mats[eq.material_id]) & MAT_EMISSIVE)) continue;
+        sf.L = v3{eq.px, eq.py, eq.pz} - sf.P;
+        sf.NdotL = dot(sf.N, sf.L);
+        if (sf.NdotL <= 0.0f) continue;
+        sf.Ld2 = sqlen(sf.L);
+        sf.Ld = sqrtf(sf.Ld2);
+        float Ld_rcp = 1.0f / sf.Ld;
+        sf.L = sf.L * Ld_rcp;
+        sf.NdotL *= Ld_rcp;
+        v3 Ro = scale_add(sf.L, kTraceOffset, sf.P);
+        float emission_intensity = dot(sf.N, area_light_vector(sf, eq, quad_v));
+        if (emission_intensity > 0) {
+            bool skip = true;
+            for (int k = 0; k < 4; k++)
+                if (dot(sf.N, quad_v[k] - sf.P) >= 0) { skip = false; break; }
+            if (skip) continue;
+            float shaded = 1.0f, sq_dist;
+            for (uint32_t j = 0; j < s.G; j++) {
+                const Geo sg = s.geos[j];
+                if (j == i || (int)j == sf.geo) continue;
+                Ray sr;
+                ray_localize(sr, Ro, sf.L, sg);
+                Hit sh;
+                sh.distance = INFINITY;
+                sr.direction = normalized(sr.direction);   // the reciprocal stays that of the unnormalised direction
+                float d = 1.0f;
+                const bool transparent = (geo_flags(sg) & GEO_TRANSPARENT) != 0;
+                if (sg.type == GT_Sphere) {
+                    if (hits_sphere(sr, sh, transparent, &sq_dist))
+                        d -= (1.0f - sqrtf(sq_dist)) / (sh.distance * emission_intensity * 3.0f);
+                } else if (sg.type == GT_Quad) {
+                    if (hits_quad(sr, sh, transparent))
+                        d -= 3.0f * (1.0f - fmax2(fabsf(sh.position.x), fabsf(sh.position.z))) / (sh.distance * emission_intensity);
+                }
+                shaded = fmin2(shaded, d);
+            }
+            if (shaded > 0.0f) {
+                radiance_fraction(sf);
+                const Mat em = s.mats[eq.material_id];
+                c = cmul_add(sf.Fd + sf.Fs, color{em.er, em.eg, em.eb} * (emission_intensity * shaded * 7.0f), c);
+            }
+        }
+    }
+}
+
+SLIM_HD float depth_at(const Projection &p, v3 pos) {   // camera.h:66
+    v3 d = pos - v3{p.camera_position[0], p.camera_position[1], p.camera_position[2]};
+    return p.inv_rot[2] * d.x + p.inv_rot[5] * d.y + p.inv_rot[8] * d.z;
+}
+
+// Point-light glow: SceneTracer::hitLight + SphereTracer (scene_tracer.h:100-109, ray.h:314-346).
+// Mutates max_distance exactly as the reference does.  On a hit returns true and the density integral.
+SLIM_HD bool light_glow(const Light &l, const Ray &r, float &max_distance, float &density) {
+    if (l.flags & LIGHT_DIRECTIONAL) return false;
+    const float radius = l.intensity * kLightRadiusFactor;
+    const float oor = kLightInvRadiusFactor / l.intensity;
+    const float t_max = max_distance * oor;
+    const v3 rc = (v3{l.x, l.y, l.z} - r.origin) * oor;
+    const float b = dot(r.direction, rc);
+    const float c = sqlen(rc) - 1;
+    float h = b * b - c;
+    if (h >= 0) {
+        h = sqrtf(h);
+        const float t_near = b - h, t_far = b + h;
+        if (t_far > 0 && t_near < t_max) {
+            max_distance = t_max * radius;
+            const float tn = fmax2(t_near, 0), tf = fmin2(t_far, t_max);
+            const float tn2 = tn * tn, tf2 = tf * tf;
+            density = (c * tn - b * tn2 + tn * tn2 / 3.0f - (c * tf - b * tf2 + tf * tf2 / 3.0f)) * (3.0f / 4.0f);
+            return true;
+        }
+    }
+    return false;
+}
+
+// pow(density, 8.0f) * 4 evaluated in double as on the reference's host (ray_tracer.h:101)
+SLIM_HD float glow_factor(float density) {
+    double x = (double)density, x2 = x * x, x4 = x2 * x2;
+    return (float)(x4 * x4 * 4.0);
+}
+
+SLIM_HD void primary_ray(const Projection &p, uint32_t x, uint32_t y, v3 &dir, float &scaling) {   // renderer_GPU.h:36-43
+    float a = (float)(int32_t)x * p.sample_size + p.c_start[0];
+    float b = (float)(-(int32_t)y) * p.sample_size + p.c_start[1];
+    scaling = 1.0f / sqrtf(p.sq_dist + (a * a + b * b));
+    dir = (v3{p.start[0], p.start[1], p.start[2]} + v3{p.down[0], p.down[1], p.down[2]} * (float)(int32_t)y) +
+          v3{p.right[0], p.right[1], p.right[2]} * (float)(int32_t)x;
+}
+
+// renderPixelBeauty (src/slim/renderer/ray_tracer.h:19-108)
+SLIM_HD void render_beauty(const SceneView &s, const RenderParams &rp, v3 direction, float scaling, Tracer &tr,
+                           color &out, float &depth) {
+    Surface sf;
+    Ray ray;
+    Hit hit;
+    hit.scaling_factor = scaling;
+    color col = cs(0), cur, next_tp = cs(0), tp = cs(1.0f);
+    depth = INFINITY;
+    const Projection &p = rp.proj;
+    ray_reset(ray, v3{p.camera_position[0], p.camera_position[1], p.camera_position[2]}, normalized(direction));
+    uint32_t depth_left = rp.max_depth;
+    while (depth_left) {
+        cur = cs(0);
+        sf.geo = scene_trace(s, ray, hit, false, INFINITY, tr);
+        if (sf.geo >= 0) {
+            hit.scaling_factor = rp.ray_cones ? scaling : 0.0f;
+            const Geo g = s.geos[sf.geo];
+            const Mat m = s.mats[g.material_id];
+            if ((mat_flags(m) & MAT_EMISSIVE) && g.type == GT_Quad && !hit.from_behind) {
+                depth_left = 0;
+                cur = color{m.er, m.eg, m.eb};
+            } else {
+                prepare_for_shading(sf, s, ray, hit, tr);
+                if (depth_left == rp.max_depth) depth = depth_at(p, hit.position);
+                for (uint32_t i = 0; i < s.L; i++) shade_from_light(sf, s.lights[i], s, tr, cur);
+                if (s.flags & 1u) shade_from_emissive_quads(sf, s, cur);
+                if (rp.sky_irradiance >= 0 && rp.sky_radiance >= 0) {
+                    sf.L = sf.N;
+                    sf.NdotL = 1.0f;
+                    if (tr.cnt) tr.cnt->taps += 2;
+                    color D = tex_sample_cube(s, (uint32_t)rp.sky_irradiance, sf.N.x, sf.N.y, sf.N.z);
+                    color S = tex_sample_cube(s, (uint32_t)rp.sky_radiance, sf.R.x, sf.R.y, sf.R.z);
+                    radiance_fraction(sf);
+                    cur = cmul_add(D, sf.Fd, cmul_add(sf.Fs, S, cur));
+                }
+                if ((mat_flags(sf.mat) & (MAT_REFLECTIVE | MAT_REFRACTIVE)) && --depth_left) {
+                    sf.F = schlick(clamp01(dot(sf.N, sf.R)), color{sf.mat.fr, sf.mat.fg, sf.mat.fb});
+                    next_tp = sf.refracted ? one_minus(sf.F) : sf.F;
+                    ray_reset(ray, hit.position, sf.RF);
+                } else depth_left = 0;
+            }
+        } else {
+            depth_left = 0;
+            if (rp.sky_color >= 0) SLIM_COUNT(tr, taps);
+            if (rp.sky_color >= 0) cur = tex_sample_cube(s, (uint32_t)rp.sky_color, ray.direction.x, ray.direction.y, ray.direction.z);
+        }
+        for (uint32_t i = 0; i < s.L; i++) {
+            const Light l = s.lights[i];
+            float density;
+            if (light_glow(l, ray, hit.distance, density)) cur = cscale_add(color{l.r, l.g, l.b}, glow_factor(density), cur);
+        }
+        col = cmul_add(cur, tp, col);
+        tp = tp * next_tp;
+    }
+    out = color{tone_map(col.r), tone_map(col.g), tone_map(col.b)};
+}
+
+// renderPixelDebugMode (src/slim/renderer/ray_tracer.h:110-142)
+SLIM_HD void render_debug(const SceneView &s, const RenderParams &rp, v3 direction, float scaling, Tracer &tr,
+                          color &out, float &depth) {
+    Surface sf;
+    Ray ray;
+    Hit hit;
+    out = cs(0);
+    depth = INFINITY;
+    const Projection &p = rp.proj;
+    ray_reset(ray, v3{p.camera_position[0], p.camera_position[1], p.camera_position[2]}, normalized(direction));
+    sf.geo = scene_trace(s, ray, hit, false, INFINITY, tr);
+    if (sf.geo >= 0) {
+        hit.scaling_factor = rp.ray_cones ? scaling : 0.0f;
+        prepare_for_shading(sf, s, ray, hit, tr);
+        depth = depth_at(p, hit.position);
+        switch (rp.render_mode) {
+            case RM_UVs: out = color{hit.uv.u, hit.uv.v, 1.0f}; break;
+            case RM_Depth: out = cs(4.0f / hit.distance); break;
+            case RM_Normals: out = color{hit.normal.x * 0.5f + 0.5f, hit.normal.y * 0.5f + 0.5f, hit.normal.z * 0.5f + 0.5f}; break;
+            case RM_NormalMap: {
+                if (s.NT && mat_tex_count(sf.mat) > 1) SLIM_COUNT(tr, taps);
+                color n = sample_slot(s, sf.mat, 1, hit.uv, hit.uv_coverage);
+                out = color{n.r * 0.5f + 0.5f, n.g * 0.5f + 0.5f, n.b * 0.5f + 0.5f};
+            } break;
+            case RM_MipLevel:
+                if (s.NT) {
+                    const TexRec t0 = s.textures[0];
+                    uint32_t lvl = mip_level(hit.uv_coverage * (float)((uint64_t)t0.width * t0.height), t0.mip_count);
+                    lvl = lvl > 8 ? 8 : lvl;
+                    out = color{rp.mip_colors[lvl][0], rp.mip_colors[lvl][1], rp.mip_colors[lvl][2]};
+                } else out = cs(0.5f);
+                break;
+            default: break;
+        }
+    }
+}
+
+// One sample: primary ray, integrator, and the value the canvas or the radiance buffer receives.
+SLIM_HD void render_sample(const SceneView &s, const RenderParams &rp, uint32_t x, uint32_t y, Tracer &tr,
+                           float out_px[4], float &out_depth) {
+    v3 dir;
+    float scaling;
+    primary_ray(rp.proj, x, y, dir, scaling);
+    color c;
+    if (rp.render_mode == RM_Beauty) render_beauty(s, rp, dir, scaling, tr, c, out_depth);
+    else render_debug(s, rp, dir, scaling, tr, c, out_depth);
+    if (tr.cnt) tr.cnt->pixels++;
+    if (rp.output == 0) {   // Canvas::setPixel with opacity -1 (src/slim/draw/canvas.h:128-169)
+        color k{clamp01(c.r), clamp01(c.g), clamp01(c.b)};
+        out_px[0] = k.r * (k.r * 1.0f);
+        out_px[1] = k.g * (k.g * 1.0f);
+        out_px[2] = k.b * (k.b * 1.0f);
+    } else {
+        out_px[0] = c.r;
+        out_px[1] = c.g;
+        out_px[2] = c.b;
+    }
+    out_px[3] = 1.0f;
+}
+
+// Canvas offsets: samples are stored row-major, or as 2x2 quads per pixel under SSAA (canvas.h:147).
+SLIM_HD uint32_t canvas_offset(const RenderParams &rp, uint32_t x, uint32_t y) {
+    return rp.ssaa ? ((rp.width * (y >> 1) + (x >> 1)) * 4 + (2 * (y & 1)) + (x & 1)) : (rp.width * y + x);
+}
+
+}  // namespace slimrt

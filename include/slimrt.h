@@ -42,6 +42,8 @@ extern "C" {
 /* output values */
 #define SLIMRT_OUTPUT_CANVAS 0u    /* Canvas::setPixel with opacity -1: RGBA = clamp(c)^2, 1 (src/slim/draw/canvas.h:128-169) */
 #define SLIMRT_OUTPUT_RADIANCE 1u  /* the colour renderPixel returns (tone-mapped, unclamped), alpha 1 */
+#define SLIMRT_OUTPUT_HITS 2u      /* primary closest hit per sample: int32 geometry index (-1 miss), int32 triangle id
+                                      (-1 unless a mesh), float distance, 0 - in the four 32-bit lanes of the pixel */
 
 /* render flags */
 #define SLIMRT_RAY_CONES 1u        /* keep the primary ray's cone scaling through hits (see DESIGN: ray cones) */
@@ -84,11 +86,12 @@ typedef struct {
     uint32_t flags;          /* Light_IsDirectional = 1, Light_IsShadowing = 2 */
 } slimrt_light;
 
-/* BVHNode (src/slim/scene/bvh.h:5-15) in the 32-byte asset layout. */
+/* BVHNode (src/slim/scene/bvh.h:5-15) in the 32-byte layout of the reference's asset files:
+ * AABB (min, max), first_index (child pair, or first leaf item), leaf_count (0 for inner nodes), depth, flags. */
 typedef struct {
     float min[3];
-    uint32_t first_index;
     float max[3];
+    uint32_t first_index;
     uint16_t leaf_count;
     uint8_t depth;
     uint8_t flags;

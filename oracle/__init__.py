@@ -58,6 +58,7 @@ def lib():
         L.oracle_intersect.restype = i
         L.oracle_intersect.argtypes = [i, vp, vp, i, f, vp]
         L.oracle_build_bvh.argtypes = [vp, u32, u32, vp, vp, vp, vp]
+        L.oracle_set_scene_leaf_size.argtypes = [vp, u32]
         _lib = L
     return _lib
 
@@ -131,6 +132,9 @@ class OracleScene:
         lib().oracle_primary_hits(self.h, y0, y1, _p(g), _p(t), _p(d))
         shape = (y1 - y0, self.samples_w)
         return g.reshape(shape), t.reshape(shape), d.reshape(shape)
+
+    def set_scene_leaf_size(self, max_leaf: int):
+        lib().oracle_set_scene_leaf_size(self.h, max_leaf)
 
     def get(self, name: str) -> np.ndarray:
         n = lib().oracle_get(self.h, name.encode(), None, 0)

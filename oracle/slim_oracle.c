@@ -1316,14 +1316,14 @@ static void update_aabbs(Scene *s) {                                            
     }
 }
 
-static void update_scene_bvh(Scene *s) {                                                                      /* scene.h:181-191 */
+static void update_scene_bvh(Scene *s, uint32_t max_leaf) {                                                   /* scene.h:181-191 */
     Builder b;
     builder_init(&b, s->G);
     for (uint32_t i = 0; i < s->G; i++) {
         b.nodes[i].box = s->geo_aabbs[i];
         b.nodes[i].first_index = b.node_ids[i] = i;
     }
-    bvh_build(&b, s->snodes, &s->snode_count, &s->sheight, s->G, 1);
+    bvh_build(&b, s->snodes, &s->snode_count, &s->sheight, s->G, max_leaf);
     for (uint32_t i = 0; i < s->G; i++) s->leaf_ids[i] = b.leaf_ids[i];
     builder_free(&b);
 }
@@ -1441,7 +1441,7 @@ void *oracle_scene_load(const char *path, int ray_cones) {
     s->snodes = (Node *)calloc(2 * (size_t)s->G + 1, sizeof(Node));
     s->leaf_ids = (uint32_t *)calloc(s->G, 4);
     update_aabbs(s);
-    update_scene_bvh(s);
+    update_scene_bvh(s, 1);
     projection_reset(s, cam);
     return s;
 }
@@ -1581,6 +1581,10 @@ size_t oracle_get(void *h, const char *name, void *out, size_t cap) {
     }
     return 0;
 }
+
+/* Rebuilds the scene BVH with a different leaf size; max_leaf >= G puts every geometry in the root leaf, which
+ * turns SceneTracer::trace into a linear scan over all geometries (the brute-force check of traversal). */
+void oracle_set_scene_leaf_size(void *h, uint32_t max_leaf) { update_scene_bvh((Scene *)h, max_leaf); }
 
 /* ---------------------------------------------------------------- unit-level entry points for closed-form tests */
 float oracle_tone_map(float x) { return tone_map(x); }

@@ -187,9 +187,11 @@ Light to_light(const slimrt_light &l) {
     return o;
 }
 
-Node to_node(const slimrt_bvh_node &n) {
+Node to_node(const slimrt_bvh_node &n) {   // asset layout -> device layout (bounds interleaved with indices)
     Node o;
-    std::memcpy(&o, &n, sizeof o);   // identical 32-byte layout
+    o.minx = n.min[0]; o.miny = n.min[1]; o.minz = n.min[2]; o.first_index = n.first_index;
+    o.maxx = n.max[0]; o.maxy = n.max[1]; o.maxz = n.max[2];
+    o.leaf_count_depth = (uint32_t)n.leaf_count | ((uint32_t)n.depth << 16) | ((uint32_t)n.flags << 24);
     return o;
 }
 
@@ -225,7 +227,7 @@ int validate_params(slimrt_context *c, const slimrt_render_params *p) {
     if (!c->loaded) return fail(c, SLIMRT_E_STATE, "no scene uploaded");
     if (p->width == 0 || p->height == 0 || p->width > 65535 || p->height > 65535)
         return fail(c, SLIMRT_E_ARG, "canvas dimensions out of range");
-    if (p->render_mode > 5 || p->output > 1 || p->max_depth > 255) return fail(c, SLIMRT_E_ARG, "invalid settings");
+    if (p->render_mode > 5 || p->output > 2 || p->max_depth > 255) return fail(c, SLIMRT_E_ARG, "invalid settings");
     const int32_t nt = (int32_t)c->textures.size();
     if (p->skybox_color >= nt || p->skybox_radiance >= nt || p->skybox_irradiance >= nt)
         return fail(c, SLIMRT_E_ARG, "skybox texture id out of range");

@@ -576,6 +576,21 @@ SLIM_HD void render_sample(const SceneView &s, const RenderParams &rp, uint32_t 
     v3 dir;
     float scaling;
     primary_ray(rp.proj, x, y, dir, scaling);
+    if (rp.output == 2) {   // primary closest hit: geometry index, leaf-ordered triangle id (meshes), distance
+        Ray ray;
+        Hit hit;
+        const Projection &p = rp.proj;
+        ray_reset(ray, v3{p.camera_position[0], p.camera_position[1], p.camera_position[2]}, normalized(dir));
+        const int g = scene_trace(s, ray, hit, false, INFINITY, tr);
+        const int tri = (g >= 0 && s.geos[g].type == GT_Mesh) ? (int)hit.id : -1;
+        out_px[0] = __builtin_bit_cast(float, g);
+        out_px[1] = __builtin_bit_cast(float, tri);
+        out_px[2] = g >= 0 ? hit.distance : INFINITY;
+        out_px[3] = 0.0f;
+        out_depth = out_px[2];
+        if (tr.cnt) tr.cnt->pixels++;
+        return;
+    }
     color c;
     if (rp.render_mode == RM_Beauty) render_beauty(s, rp, dir, scaling, tr, c, out_depth);
     else render_debug(s, rp, dir, scaling, tr, c, out_depth);

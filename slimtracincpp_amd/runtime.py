@@ -20,7 +20,7 @@ LIB_PATH = os.path.join(HERE, "libslimrt.so")
 u32, i32, f32, vp = C.c_uint32, C.c_int32, C.c_float, C.c_void_p
 
 SLIMRT_DEVICE_GPU, SLIMRT_DEVICE_CPU = 0, 1
-OUTPUT_CANVAS, OUTPUT_RADIANCE = 0, 1
+OUTPUT_CANVAS, OUTPUT_RADIANCE, OUTPUT_HITS = 0, 1, 2
 RAY_CONES = 1
 ERRORS = {-1: "SLIMRT_E_ARG", -2: "SLIMRT_E_DEVICE", -3: "SLIMRT_E_MEMORY", -4: "SLIMRT_E_STATE", -5: "SLIMRT_E_OVERFLOW"}
 
@@ -32,7 +32,7 @@ MATERIAL = np.dtype([("albedo", "<f4", 3), ("roughness", "<f4"), ("emission", "<
                      ("normal_magnitude", "<f4"), ("flags", "<u4"), ("texture_count", "<u4"),
                      ("texture_ids", "<u4", 2), ("brdf", "<u4")])
 LIGHT = np.dtype([("color", "<f4", 3), ("intensity", "<f4"), ("position", "<f4", 3), ("flags", "<u4")])
-BVH_NODE = np.dtype([("min", "<f4", 3), ("first_index", "<u4"), ("max", "<f4", 3), ("leaf_count", "<u2"),
+BVH_NODE = np.dtype([("min", "<f4", 3), ("max", "<f4", 3), ("first_index", "<u4"), ("leaf_count", "<u2"),
                      ("depth", "u1"), ("flags", "u1")])
 TRIANGLE = np.dtype([("local_to_tangent", "<f4", 9), ("position", "<f4", 3), ("normal", "<f4", 3),
                      ("n1", "<f4", 3), ("n2", "<f4", 3), ("n3", "<f4", 3), ("uv1", "<f4", 2), ("uv2", "<f4", 2),
@@ -314,6 +314,14 @@ class Renderer:
         if p.ssaa:
             return px, dp
         return px.reshape(hs, ws, 4), dp.reshape(hs, ws)
+
+    def primary_hits(self, **kw):
+        """Primary closest hit per sample: geometry index, leaf-ordered triangle id, distance (OUTPUT_HITS)."""
+        px, _ = self.render(output=OUTPUT_HITS, **kw)
+        flat = px.reshape(-1, 4)
+        shape = px.shape[:-1]
+        return (flat[:, 0].view(np.int32).reshape(shape), flat[:, 1].view(np.int32).reshape(shape),
+                flat[:, 2].copy().reshape(shape))
 
     def render_device(self, params: RenderParams, pixels_ptr: int, depths_ptr: int):
         _check(lib().slimrt_render_device(self.ctx, C.byref(params), C.c_void_p(pixels_ptr), C.c_void_p(depths_ptr)),

@@ -1,0 +1,67 @@
+"""Writes the golden vectors in this directory by running the reference's own CPU renderer (oracle/_ref/slimref,
+compiled from the reference sources) on seeded scenes.  Needs the reference sources (build: make -C oracle ref).
+
+Each ref_<name>.npz holds the scene spec bytes it was computed from ("spec"; for the large-mesh case only their
+SHA-256, "spec__sha256", and the test regenerates the spec with the seeded generator), the reference's outputs
+(pre-canvas colour, depth, primary hit geometry / triangle / distance) and what it derived on the host (geometry
+quaternions and world AABBs, scene BVH nodes and leaf order, camera projection, per-mesh BVH nodes and triangle
+records, the latter two as SHA-256 digests when large).
+"""
+import hashlib
+import os
+import sys
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+ROOT = os.path.dirname(os.path.dirname(HERE))
+sys.path.insert(0, ROOT)
+
+from slimtracincpp_amd import scenes  # noqa: E402
+from slimtracincpp_amd.specfile import spec_bytes  # noqa: E402
+import oracle  # noqa: E402
+
+CASES = {
+    "implicit_s0_320x180_beauty": lambda: scenes.implicit_scene(0),
+    "implicit_t1_160x90_normals": lambda: scenes.implicit_scene(1, 160, 90, transparent=True, render_mode=scenes.MODE_NORMALS),
+    "implicit_t1_160x90_uvs": lambda: scenes.implicit_scene(1, 160, 90, transparent=True, render_mode=scenes.MODE_UVS),
+    "implicit_t1_160x90_depth": lambda: scenes.implicit_scene(1, 160, 90, transparent=True, render_mode=scenes.MODE_DEPTH),
+    "implicit_t1_160x90_beauty_d3": lambda: scenes.implicit_scene(1, 160, 90, max_depth=3, transparent=True),
+    "mesh_pbr_s0_160x90_l3_beauty": lambda: scenes.mesh_pbr_scene(0, 160, 90, level=3, texture_size=64, sky_size=16,
+                                                                  irradiance_size=8),
+    "mesh_pbr_s0_160x90_l3_uvs": lambda: scenes.mesh_pbr_scene(0, 160, 90, level=3, texture_size=64, sky_size=16,
+                                                               irradiance_size=8, render_mode=scenes.MODE_UVS),
+    "mesh_pbr_s1_96x54_l6_beauty": lambda: scenes.mesh_pbr_scene(1, 96, 54, level=6, texture_size=64, sky_size=16,
+                                                                 irradiance_size=8),
+}
+
+DIGEST_ABOVE = 4096   # arrays larger than this many elements are stored as SHA-256 digests
+
+
+def digest(a: np.ndarray) -> str:
+    return hashlib.sha256(np.ascontiguousarray(a).tobytes()).hexdigest()
+
+
+def main():
+    for name, make in CASES.items():
+        spec = make()
+        sb = spec_bytes(spec)
+        ref = oracle.run_reference(spec)
+        if len(sb) > 1 << 20:
+            out = {"spec__sha256": np.array(hashlib.sha256(sb).hexdigest())}
+        else:
+            out = {"spec": np.frombuffer(sb, np.uint8)}
+        for k, v in ref.items():
+            if k in ("timing", "rows"):
+                continue
+            if v.size > DIGEST_ABOVE and not k.startswith(("color", "depth", "hit_")):
+                out[k + "__sha256"] = np.array(digest(v))
+            else:
+                out[k] = v
+        path = os.path.join(HERE, f"ref_{name}.npz")
+        np.savez_compressed(path, **out)
+        print(f"{name}: {os.path.getsize(path) / 1024:.0f} KiB")
+
+
+if __name__ == "__main__":
+    main()

@@ -89,3 +89,100 @@ def test_mesh_pbr_small(gpu_available, mesh_small, mode):
     ok, err = _close(px[..., :3], col, atol=BEAUTY_ATOL if mode == scenes.MODE_BEAUTY else 0, rtol=0 if mode == scenes.MODE_BEAUTY else DEBUG_RTOL)
     print(f"mesh mode {mode}: max err {err:.3g}")
     assert ok, err
+
+
+@pytest.fixture(scope="module")
+def mesh_1080p():
+    spec = scenes.mesh_pbr_scene(0, 1920, 1080)
+    return spec, runtime.Renderer(spec, "gpu"), oracle.OracleScene(spec)
+
+
+def test_mesh_pbr_1080p_primary_hits(gpu_available, mesh_1080p):
+    """The benchmark workload at full size: primary hit ids (geometry, triangle) on >= 99.99% of samples."""
+    spec, r, o = mesh_1080p
+    g, t, d = r.primary_hits()
+    g2, t2, d2 = o.primary_hits()
+    agree = (g == g2) & (t == t2)
+    print(f"1080p hit-id agreement {agree.mean():.6f}, mismatches {int((~agree).sum())}")
+    assert agree.mean() >= 0.9999
+    both = agree & np.isfinite(d) & np.isfinite(d2)
+    assert np.all(np.abs(d[both] - d2[both]) <= DEBUG_RTOL * np.abs(d2[both]))
+
+
+def test_mesh_pbr_1080p_beauty(gpu_available, mesh_1080p):
+    spec, r, o = mesh_1080p
+    px, depth = r.render()
+    col, dep, cnt = o.render()
+    ok, err = _close(px[..., :3], col, atol=BEAUTY_ATOL)
+    exact = float(np.mean(px[..., :3].view(np.uint32) == col.view(np.uint32)))
+    print(f"1080p beauty max err {err:.3g}, bit-exact fraction {exact:.6f}")
+    assert ok, err
+    ok, err = _close(depth, dep, rtol=DEBUG_RTOL)
+    assert ok, err
+    assert r.count_work() == cnt
+
+
+@pytest.mark.parametrize("mode", [scenes.MODE_NORMALS, scenes.MODE_UVS, scenes.MODE_DEPTH])
+def test_mesh_pbr_1080p_debug_modes(gpu_available, mesh_1080p, mode):
+    spec, r, o = mesh_1080p
+    px, _ = r.render(mode=mode)
+    col, _, _ = o.render(mode=mode)
+    ok, err = _close(px[..., :3], col, rtol=DEBUG_RTOL)
+    assert ok, err
+
+
+def test_frames_are_deterministic(gpu_available, mesh_small):
+    spec, r, o = mesh_small
+    a, da = r.render()
+    b, db = r.render()
+    assert np.array_equal(a.view(np.uint32), b.view(np.uint32)) and np.array_equal(da.view(np.uint32), db.view(np.uint32))
+
+
+def test_ray_cone_mode(gpu_available):
+    spec = scenes.mesh_pbr_scene(0, 160, 90, level=3, texture_size=256, sky_size=16, irradiance_size=8)
+    r = runtime.Renderer(spec, "gpu")
+    o = oracle.OracleScene(spec, ray_cones=True)
+    for mode in (scenes.MODE_BEAUTY, scenes.MODE_MIP_LEVEL):
+        px, _ = r.render(mode=mode, ray_cones=True)
+        col, _, _ = o.render(mode=mode)
+        ok, err = _close(px[..., :3], col, atol=BEAUTY_ATOL)
+        assert ok, (mode, err)
+
+
+def test_ssaa_on_gpu(gpu_available):
+    spec = scenes.implicit_scene(0, 64, 36)
+    spec.ssaa = True
+    r = runtime.Renderer(spec, "gpu")
+    c = runtime.Renderer(spec, "cpu")
+    a, _ = r.render()
+    b, _ = c.render()
+    ok, err = _close(a, b, atol=BEAUTY_ATOL)
+    assert ok, err
+
+
+def test_stack_overflow_reported_on_gpu(gpu_available):
+    spec = scenes.mesh_pbr_scene(0, 32, 18, level=3, texture_size=16, sky_size=8, irradiance_size=4)
+    sb = runtime.build_scene(spec)
+    sb.meshes[0].height = 1
+    sb.mesh_descs[0].height = 1
+    r = runtime.Renderer(sb, "gpu")
+    with pytest.raises(runtime.SlimrtError) as e:
+        r.render()
+    assert e.value.code == -5
+
+
+def test_moving_geometries_rebuild(gpu_available):
+    """Per-frame transform updates (Scene::updateAABBs + updateBVH, then uploadGeometries + uploadSceneBVH)."""
+    spec = scenes.implicit_scene(0, 96, 54)
+    r = runtime.Renderer(spec, "gpu")
+    rng = np.random.default_rng(3)
+    for frame in range(3):
+        gf = spec.geo_f.copy()
+        gf[1:, 3:6] += rng.uniform(-0.5, 0.5, (gf.shape[0] - 1, 3)).astype(np.float32)
+        gf[1:, 0:3] += np.float32(0.3 * frame)
+        r.update_geometries(gf)
+        moved = scenes.SceneSpec(**{**spec.__dict__, "geo_f": gf})
+        col, _, _ = oracle.OracleScene(moved).render()
+        px, _ = r.render()
+        ok, err = _close(px[..., :3], col, atol=BEAUTY_ATOL)
+        assert ok, (frame, err)

@@ -1,0 +1,62 @@
+"""The CPU oracle against golden vectors written by the reference's own CPU renderer (tests/golden/make_golden.py),
+and - where the reference sources are present - against the compiled reference itself on fresh seeded scenes.
+Everything here is bit-exact: same inputs, same operation order, no FMA contraction on either side."""
+import hashlib
+import os
+import tempfile
+
+import numpy as np
+import pytest
+
+import oracle
+from slimtracincpp_amd import scenes
+from tests.helpers import bits_equal, golden_cases, golden_spec_path, load_golden
+
+DERIVED = ["geo_quat", "geo_aabb", "sbvh_f", "sbvh_u", "sbvh_meta", "sbvh_leaf", "proj"]
+
+
+@pytest.mark.parametrize("name", golden_cases())
+def test_oracle_matches_reference_golden(name, tmp_path):
+    g = load_golden(name)
+    o = oracle.OracleScene(golden_spec_path(name, g, str(tmp_path)))
+    for k in list(g):
+        if k.startswith(("spec", "color", "depth", "hit_")):
+            continue
+        base = k.replace("__sha256", "")
+        mine = o.get(base)
+        if k.endswith("__sha256"):
+            assert hashlib.sha256(mine.tobytes()).hexdigest() == str(g[k]), base
+        else:
+            assert bits_equal(mine, g[k]), base
+    col, dep, _ = o.render()
+    assert bits_equal(col.reshape(-1), g["color"]), "colour"
+    assert bits_equal(dep.reshape(-1), g["depth"]), "depth"
+    geo, tri, t = o.primary_hits()
+    assert np.array_equal(geo.reshape(-1), g["hit_geo"])
+    assert np.array_equal(tri.reshape(-1), g["hit_tri"])
+    assert bits_equal(t.reshape(-1), g["hit_t"])
+
+
+needs_ref = pytest.mark.skipif(not (os.path.exists(oracle.REF_PATH) and os.path.isdir("/root/reference/src")),
+                               reason="compiled reference not available")
+
+
+@needs_ref
+@pytest.mark.parametrize("make", [
+    lambda: scenes.implicit_scene(11, 96, 64, max_depth=3, transparent=True),
+    lambda: scenes.implicit_scene(12, 64, 48, transparent=True, render_mode=scenes.MODE_NORMALS),
+    lambda: scenes.mesh_pbr_scene(5, 64, 40, level=2, texture_size=32, sky_size=8, irradiance_size=4),
+    lambda: scenes.mesh_pbr_scene(6, 64, 40, level=3, texture_size=32, sky_size=8, irradiance_size=4,
+                                  leaf_size=2, render_mode=scenes.MODE_MIP_LEVEL),
+], ids=["implicit_d3", "implicit_normals", "mesh_l2", "mesh_l3_leaf2_mip"])
+def test_oracle_matches_live_reference(make):
+    spec = make()
+    ref = oracle.run_reference(spec)
+    o = oracle.OracleScene(spec)
+    for k in DERIVED + [k for k in ref if k.startswith("mesh")]:
+        assert bits_equal(o.get(k), ref[k]), k
+    col, dep, _ = o.render()
+    assert bits_equal(col.reshape(-1), ref["color"])
+    assert bits_equal(dep.reshape(-1), ref["depth"])
+    geo, tri, t = o.primary_hits()
+    assert np.array_equal(geo.reshape(-1), ref["hit_geo"]) and np.array_equal(tri.reshape(-1), ref["hit_tri"])

@@ -82,7 +82,8 @@ def main():
     n = args.width * args.height
     pixels = torch.empty((n, 4), dtype=torch.float32, device="cuda")
     depths = torch.empty((n,), dtype=torch.float32, device="cuda")
-    stream = torch.cuda.current_stream()
+    stream = torch.cuda.Stream()             # a real (non-NULL) stream shared by torch's events and the kernels
+    torch.cuda.set_stream(stream)
     r.set_stream(stream.cuda_stream)
 
     cnt = r.count_work(p)

@@ -1,0 +1,82 @@
+"""The C++ application-layer API (include/slimtracin/slimtracin.hpp), exercised by tests/cpp/app_api_test.cpp the
+way the reference's example apps use theirs, against the oracle's canvas (clamp(colour)^2, opacity 1)."""
+import os
+import subprocess
+import tempfile
+
+import numpy as np
+import pytest
+
+import oracle
+from slimtracincpp_amd import scenes
+from slimtracincpp_amd.specfile import write_spec
+from tests.helpers import GOLDEN
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+EXE = os.path.join(ROOT, "tests", "cpp", "app_api_test")
+
+
+def _exe():
+    if not os.path.exists(EXE):
+        subprocess.run(["make", "-s", "-C", os.path.join(ROOT, "slimtracincpp_amd", "csrc"), "app_test"], check=True)
+    return EXE
+
+
+def _run(kind, inp, device, n):
+    with tempfile.TemporaryDirectory() as td:
+        out = os.path.join(td, "canvas.bin")
+        subprocess.run([_exe(), kind, inp, out, device], check=True, timeout=600)
+        raw = np.fromfile(out, np.float32)
+    return raw[:4 * n].reshape(n, 4), raw[4 * n:5 * n]
+
+
+def _canvas(col):
+    c = np.clip(col.reshape(-1, 3), 0, 1).astype(np.float32)
+    return (c * (c * np.float32(1.0))).astype(np.float32)
+
+
+SPECS = {
+    "implicit": lambda: scenes.implicit_scene(0, 96, 54),
+    "implicit_t": lambda: scenes.implicit_scene(3, 96, 54, max_depth=3, transparent=True, render_mode=scenes.MODE_UVS),
+    "mesh": lambda: scenes.mesh_pbr_scene(0, 96, 54, level=3, texture_size=64, sky_size=16, irradiance_size=8),
+}
+
+
+def _spec_case(name, device, tol):
+    spec = SPECS[name]()
+    with tempfile.TemporaryDirectory() as td:
+        path = write_spec(spec, os.path.join(td, "s.spec"))
+        n = spec.width * spec.height
+        px, dep = _run("spec", path, device, n)
+    col, odep, _ = oracle.OracleScene(spec).render()
+    want = _canvas(col)
+    if tol == 0:
+        assert np.array_equal(px[:, :3].view(np.uint32), want.view(np.uint32))
+        assert np.array_equal(dep.view(np.uint32), odep.reshape(-1).view(np.uint32))
+    else:
+        assert np.max(np.abs(px[:, :3] - want)) <= tol
+    assert np.all(px[:, 3] == 1.0)
+
+
+@pytest.mark.parametrize("name", list(SPECS))
+def test_app_api_cpu_equals_oracle(name):
+    _spec_case(name, "cpu", 0)
+
+
+def test_app_api_demo_scene_cpu():
+    px, dep = _run("demo", os.path.join(GOLDEN, "monkey.mesh"), "cpu", 320 * 180)
+    assert np.all(np.isfinite(px)) and px[:, :3].max() > 0.05
+    assert np.isfinite(dep).mean() > 0.5
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("name", list(SPECS))
+def test_app_api_gpu_equals_oracle(gpu_available, name):
+    _spec_case(name, "gpu", 2e-4)
+
+
+@pytest.mark.gpu
+def test_app_api_demo_scene_gpu_equals_cpu(gpu_available):
+    a, da = _run("demo", os.path.join(GOLDEN, "monkey.mesh"), "cpu", 320 * 180)
+    b, db = _run("demo", os.path.join(GOLDEN, "monkey.mesh"), "gpu", 320 * 180)
+    assert np.max(np.abs(a - b)) <= 2e-4

@@ -59,6 +59,8 @@ def lib():
         L.oracle_intersect.argtypes = [i, vp, vp, i, f, vp]
         L.oracle_build_bvh.argtypes = [vp, u32, u32, vp, vp, vp, vp]
         L.oracle_set_scene_leaf_size.argtypes = [vp, u32]
+        L.oracle_path_samples.restype = i
+        L.oracle_path_samples.argtypes = [vp, i, i, i, vp, vp, vp]
         _lib = L
     return _lib
 
@@ -132,6 +134,14 @@ class OracleScene:
         lib().oracle_primary_hits(self.h, y0, y1, _p(g), _p(t), _p(d))
         shape = (y1 - y0, self.samples_w)
         return g.reshape(shape), t.reshape(shape), d.reshape(shape)
+
+    def path_samples(self, x: int, y: int, max_depth: int = -1):
+        """Radiance-cache training samples of sample (x, y)'s beauty path: raw (n, 14), L (n, 3), vertex (n)."""
+        raw = np.zeros((16, 14), np.float32)
+        L = np.zeros((16, 3), np.float32)
+        v = np.zeros(16, np.int32)
+        n = lib().oracle_path_samples(self.h, x, y, max_depth, _p(raw), _p(L), _p(v))
+        return raw[:n], L[:n], v[:n]
 
     def set_scene_leaf_size(self, max_leaf: int):
         lib().oracle_set_scene_leaf_size(self.h, max_leaf)

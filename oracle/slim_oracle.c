@@ -1177,8 +1177,22 @@ static void shade_from_emissive_quads(Surface *sf, const Scene *s, color *c) {  
 
 static float depth_at(const Scene *s, v3 p) { return m3_mulv(s->proj.inv_rot, vsub(p, s->proj.camera_position)).z; } /* camera.h:66 */
 
+/* Per-iteration record of one path, for the radiance cache's training targets (no counterpart in the reference):
+ * the integrator below accumulates colour += throughput_i * current_i, throughput_{i+1} = throughput_i * next_i,
+ * so the radiance leaving hit i is L_i = current_i + next_i * L_{i+1}.  Shaded hits give one sample each: the
+ * 14-float hit record (position, shading normal, view direction, roughness, metalness, albedo times the albedo
+ * map) and L_i. */
+#define PATH_MAX_ITER 16
+typedef struct {
+    int n_iter;
+    color cur[PATH_MAX_ITER], next[PATH_MAX_ITER];
+    int shaded[PATH_MAX_ITER];
+    float raw[PATH_MAX_ITER][14];
+} PathRec;
+
 /* renderPixelBeauty (src/slim/renderer/ray_tracer.h:19-108) */
-static void render_beauty(const Scene *s, Tracer *tr, int max_depth, v3 direction, float scaling, color *out, float *depth) {
+static void render_beauty_rec(const Scene *s, Tracer *tr, int max_depth, v3 direction, float scaling, color *out,
+                              float *depth, PathRec *rec) {
     Surface sf;
     memset(&sf, 0, sizeof sf);
     Ray ray;
@@ -1189,8 +1203,10 @@ static void render_beauty(const Scene *s, Tracer *tr, int max_depth, v3 directio
     *depth = INFINITY;
     ray_reset(&ray, s->proj.camera_position, vnorm(direction));
     uint32_t depth_left = (uint32_t)max_depth;
+    if (rec) rec->n_iter = 0;
     while (depth_left) {
         cur = cs(0);
+        int shaded = 0, bounced = 0;
         sf.geo = scene_trace(tr, s, &ray, &hit, 0, INFINITY);
         if (s->ray_cones) hit.scaling_factor = scaling;
         if (sf.geo >= 0) {
@@ -1202,6 +1218,15 @@ static void render_beauty(const Scene *s, Tracer *tr, int max_depth, v3 directio
             } else {
                 prepare_for_shading(&sf, s, &ray, &hit, tr->cnt);
                 if (depth_left == (uint32_t)max_depth) *depth = depth_at(s, hit.position);
+                shaded = 1;
+                if (rec && rec->n_iter < PATH_MAX_ITER) {
+                    float *r = rec->raw[rec->n_iter];
+                    r[0] = sf.P.x; r[1] = sf.P.y; r[2] = sf.P.z; r[3] = sf.N.x; r[4] = sf.N.y; r[5] = sf.N.z;
+                    r[6] = sf.V.x; r[7] = sf.V.y; r[8] = sf.V.z; r[9] = sf.mat->roughness; r[10] = sf.mat->metalness;
+                    r[11] = sf.mat->albedo.r * sf.albedo_from_map.r;
+                    r[12] = sf.mat->albedo.g * sf.albedo_from_map.g;
+                    r[13] = sf.mat->albedo.b * sf.albedo_from_map.b;
+                }
                 for (uint32_t i = 0; i < s->L; i++) shade_from_light(&sf, &s->lights[i], s, tr, &cur);
                 if (s->scene_flags & 1) shade_from_emissive_quads(&sf, s, &cur);
                 if (s->sky_irradiance >= 0 && s->sky_radiance >= 0) {
@@ -1216,6 +1241,7 @@ static void render_beauty(const Scene *s, Tracer *tr, int max_depth, v3 directio
                     sf.F = schlick(clamp01(vdot(sf.N, sf.R)), sf.mat->reflectivity);
                     next_tp = sf.refracted ? c1minus(sf.F) : sf.F;
                     ray_reset(&ray, hit.position, sf.RF);
+                    bounced = 1;
                 } else depth_left = 0;
             }
         } else {
@@ -1228,12 +1254,22 @@ static void render_beauty(const Scene *s, Tracer *tr, int max_depth, v3 directio
                 double p = pow((double)integrate_density(tr), 8.0) * 4;
                 cur = cscaleadd(s->lights[i].color, (float)p, cur);
             }
+        if (rec && rec->n_iter < PATH_MAX_ITER) {
+            rec->cur[rec->n_iter] = cur;
+            rec->next[rec->n_iter] = bounced ? next_tp : cs(0);
+            rec->shaded[rec->n_iter] = shaded;
+            rec->n_iter++;
+        }
         col = cmuladd(cur, tp, col);
         tp = cmul(tp, next_tp);
     }
     out->r = tone_map(col.r);
     out->g = tone_map(col.g);
     out->b = tone_map(col.b);
+}
+
+static void render_beauty(const Scene *s, Tracer *tr, int max_depth, v3 direction, float scaling, color *out, float *depth) {
+    render_beauty_rec(s, tr, max_depth, direction, scaling, out, depth, NULL);
 }
 
 static color mip_color(int level) {   /* RayTracingRenderer mip_level_colors (renderer.h:50-58), Color(ColorID) (base.h:805-858) */
@@ -1493,6 +1529,41 @@ int oracle_render(void *h, int y0, int y1, int mode, int max_depth, float *color
     if (counters) memcpy(counters, &cnt, sizeof cnt);
     free(tr->stack); free(tr->mesh_stack); free(tr);
     return 0;
+}
+
+/* Training samples of sample (x, y)'s beauty path (at most 16): raw (14 floats each), L (3 floats each) and the
+ * path vertex (1 = primary hit) of every shaded hit, in path order.  Returns the number of samples. */
+int oracle_path_samples(void *h, int x, int y, int max_depth, float *raw, float *L, int32_t *vertex) {
+    Scene *s = (Scene *)h;
+    if (max_depth < 0) max_depth = s->max_depth;
+    Counters cnt;
+    memset(&cnt, 0, sizeof cnt);
+    Tracer *tr = (Tracer *)calloc(1, sizeof(Tracer));
+    tr->stack = (uint32_t *)calloc(s->G + 2, 4);
+    tr->mesh_stack = (uint32_t *)calloc(max_mesh_height(s) + 2, 4);
+    tr->cnt = &cnt;
+    PathRec *rec = (PathRec *)calloc(1, sizeof(PathRec));
+    v3 dir;
+    float scaling, depth;
+    color c;
+    primary_ray(s, x, y, &dir, &scaling);
+    render_beauty_rec(s, tr, max_depth, dir, scaling, &c, &depth, rec);
+    color Li = cs(0);
+    int n = 0;
+    for (int i = rec->n_iter - 1; i >= 0; i--) {
+        Li = C3(rec->cur[i].r + rec->next[i].r * Li.r, rec->cur[i].g + rec->next[i].g * Li.g,
+                rec->cur[i].b + rec->next[i].b * Li.b);
+        rec->cur[i] = Li;   /* now L_i */
+    }
+    for (int i = 0; i < rec->n_iter; i++) {
+        if (!rec->shaded[i]) continue;
+        memcpy(raw + 14 * n, rec->raw[i], 14 * sizeof(float));
+        L[3 * n] = rec->cur[i].r; L[3 * n + 1] = rec->cur[i].g; L[3 * n + 2] = rec->cur[i].b;
+        vertex[n] = i + 1;
+        n++;
+    }
+    free(rec); free(tr->stack); free(tr->mesh_stack); free(tr);
+    return n;
 }
 
 /* Primary closest hit per sample: geometry index (-1 miss), leaf-ordered triangle id (-1 if not a mesh), t. */

@@ -101,9 +101,13 @@ __global__ void pack_kernel(const float *__restrict__ p, uint2 *__restrict__ fwd
 
 // ---------------------------------------------------------------- inference
 // 256 threads = 4 waves; each wave runs 16-query tiles through the network with the weights staged in LDS.
+// With n_dev, the query count is min(*n_dev, n) (a queue filled on the device).
 __global__ __launch_bounds__(256) void infer_kernel(const uint2 *__restrict__ frag_fwd, const float *__restrict__ p,
-                                                    const float *__restrict__ raw, uint32_t n, float3 center,
+                                                    const float *__restrict__ raw, uint32_t n,
+                                                    const uint32_t *__restrict__ n_dev, float3 center,
                                                     float3 inv_half, float *__restrict__ out) {
+    if (n_dev) n = min(*n_dev, n);
+    if (n == 0) return;
     __shared__ uint2 sw[FWD_FRAGS * 64];
     __shared__ float sb[4 * NH + NOUTP];
     for (int i = threadIdx.x; i < FWD_FRAGS * 64; i += 256) sw[i] = frag_fwd[i];
@@ -158,14 +162,45 @@ __device__ __forceinline__ float sum16(float v) {   // sum over the 16 lanes tha
     return v;
 }
 
+// Batch slot j of a step maps to sample map(j) of the buffer.  Without a device count the batch is samples
+// 0..n-1.  With one (count = min(*n_dev, cap)), slot offset + j of the frame's `total` slots takes sample
+// (offset + j) * count / total when the buffer holds at least `total` samples (a frame's steps spread over the
+// whole buffer), else sample (offset + j) mod count (every step full, samples repeated).  An empty buffer
+// gives an empty batch: zero gradient.
+struct SampleMap {
+    uint32_t n_eff, count, offset, total;
+    int mode;   // 0: identity, 1: stride, 2: wrap
+    __device__ uint32_t operator()(uint32_t j) const {
+        return mode == 1 ? (uint32_t)(((uint64_t)(offset + j) * count) / total)
+                         : mode == 2 ? (offset + j) % count : j;
+    }
+};
+
+__device__ __forceinline__ SampleMap sample_map(uint32_t n, const uint32_t *n_dev, uint32_t cap, uint32_t offset,
+                                                uint32_t total) {
+    SampleMap m{n, n, 0, n, 0};
+    if (n_dev) {
+        const uint32_t count = min(*n_dev, cap);
+        m.count = count;
+        m.offset = offset;
+        m.total = total;
+        m.mode = count >= total ? 1 : 2;
+        m.n_eff = count ? n : 0u;
+    }
+    return m;
+}
+
 __global__ __launch_bounds__(64) void train_kernel(const uint2 *__restrict__ frag_fwd, const uint2 *__restrict__ frag_bwd,
                                                    const float *__restrict__ p, const float *__restrict__ raw,
-                                                   const float *__restrict__ targets, uint32_t n, float inv_count,
-                                                   float3 center, float3 inv_half, float *__restrict__ partial,
-                                                   float *__restrict__ partial_loss) {
+                                                   const float *__restrict__ targets, uint32_t n,
+                                                   const uint32_t *__restrict__ n_dev, uint32_t cap, uint32_t offset,
+                                                   uint32_t total, float3 center, float3 inv_half,
+                                                   float *__restrict__ partial, float *__restrict__ partial_loss) {
     __shared__ TrainLds s;
     const int lane = threadIdx.x;
     const uint32_t base = blockIdx.x * 64;
+    const SampleMap map = sample_map(n, n_dev, cap, offset, total);
+    const float inv_count = map.n_eff ? 1.0f / (3.0f * (float)map.n_eff) : 0.0f;
     float *g = partial + (size_t)blockIdx.x * NPARAMS;
     float loss = 0.0f;
     bf16x4 dz4[4];                       // layer-4 output gradient per tile (B operand of dX4)
@@ -173,8 +208,8 @@ __global__ __launch_bounds__(64) void train_kernel(const uint2 *__restrict__ fra
     // ---- forward, 4 tiles
     for (int t = 0; t < 4; t++) {
         const int sq = 16 * t + (lane & 15);         // sample within the wave
-        const uint32_t q = base + sq;
-        const bool valid = q < n;
+        const bool valid = base + sq < map.n_eff;
+        const uint32_t q = valid ? map(base + sq) : 0u;
         bf16x4 x[6], h[4];
         encode(raw, q, valid, center, inv_half, lane, x);
         for (int kc = 0; kc < 6; kc++)
@@ -282,7 +317,7 @@ __global__ __launch_bounds__(64) void train_kernel(const uint2 *__restrict__ fra
     }
     // loss partial (sum over this wave's samples, unnormalised)
     for (int off = 32; off > 0; off >>= 1) loss += __shfl_down(loss, off, 64);
-    if (lane == 0) partial_loss[blockIdx.x] = loss;
+    if (lane == 0) partial_loss[blockIdx.x] = loss * inv_count;
 }
 
 // Sum of the per-wave partials, then (optionally) one Adam step on the float32 master weights.
@@ -364,14 +399,16 @@ int ensure_partials(slimrt_nrc *c, uint32_t blocks) {
     return 0;
 }
 
+// One gradient evaluation (and optionally one Adam step) on a batch of n samples; see SampleMap for n_dev.
 int train(slimrt_nrc *c, const float *raw, const float *targets, uint32_t n, bool adam, float lr, float *grad_host,
-          float *loss_host) {
+          float *loss_host, const uint32_t *n_dev = nullptr, uint32_t cap = 0, uint32_t offset = 0, uint32_t total = 0,
+          float *loss_dev = nullptr) {
     if (!c || !raw || !targets || n == 0) return -1;
     const uint32_t blocks = (n + 63) / 64;
     int rc = ensure_partials(c, blocks);
     if (rc) return rc;
     hipLaunchKernelGGL(train_kernel, dim3(blocks), dim3(64), 0, c->stream, c->d_fwd, c->d_bwd, c->d_p, raw, targets, n,
-                       1.0f / (3.0f * (float)n), c->center, c->inv_half, c->d_partial, c->d_ploss);
+                       n_dev, cap, offset, total, c->center, c->inv_half, c->d_partial, c->d_ploss);
     NRC_TRY(c, hipGetLastError());
     float c1 = 1.0f, c2 = 1.0f;
     if (adam) {
@@ -383,15 +420,13 @@ int train(slimrt_nrc *c, const float *raw, const float *targets, uint32_t n, boo
                        c->d_p, c->d_m, c->d_v, grad_host ? c->d_grad : nullptr, adam ? 1 : 0, lr, c1, c2);
     NRC_TRY(c, hipGetLastError());
     if (adam && (rc = pack(c))) return rc;
-    if (loss_host) {
-        hipLaunchKernelGGL(loss_sum_kernel, dim3(1), dim3(64), 0, c->stream, c->d_ploss, blocks, c->d_loss);
-        NRC_TRY(c, hipMemcpyAsync(loss_host, c->d_loss, sizeof(float), hipMemcpyDeviceToHost, c->stream));
+    if (loss_host || loss_dev) {
+        hipLaunchKernelGGL(loss_sum_kernel, dim3(1), dim3(64), 0, c->stream, c->d_ploss, blocks, loss_dev ? loss_dev : c->d_loss);
+        NRC_TRY(c, hipGetLastError());
     }
+    if (loss_host) NRC_TRY(c, hipMemcpyAsync(loss_host, c->d_loss, sizeof(float), hipMemcpyDeviceToHost, c->stream));
     if (grad_host) NRC_TRY(c, hipMemcpyAsync(grad_host, c->d_grad, NPARAMS * sizeof(float), hipMemcpyDeviceToHost, c->stream));
-    if (loss_host || grad_host) {
-        NRC_TRY(c, hipStreamSynchronize(c->stream));
-        if (loss_host) *loss_host /= (3.0f * (float)n);
-    }
+    if (loss_host || grad_host) NRC_TRY(c, hipStreamSynchronize(c->stream));
     return 0;
 }
 }  // namespace
@@ -474,9 +509,33 @@ int slimrt_nrc_infer(slimrt_nrc *c, const float *raw, uint32_t n, float *out) {
     const uint32_t tiles = (n + 15) / 16;
     uint32_t blocks = (tiles + 3) / 4;
     if (blocks > 2048) blocks = 2048;
-    hipLaunchKernelGGL(infer_kernel, dim3(blocks), dim3(256), 0, c->stream, c->d_fwd, c->d_p, raw, n, c->center,
-                       c->inv_half, out);
+    hipLaunchKernelGGL(infer_kernel, dim3(blocks), dim3(256), 0, c->stream, c->d_fwd, c->d_p, raw, n,
+                       (const uint32_t *)nullptr, c->center, c->inv_half, out);
     NRC_TRY(c, hipGetLastError());
+    return 0;
+}
+
+int slimrt_nrc_infer_queue(slimrt_nrc *c, const float *raw, const uint32_t *d_count, uint32_t cap, float *out) {
+    if (!c || !raw || !d_count || !out || cap == 0) return nrc_fail(c, -1, "infer_queue: NULL buffer or zero capacity");
+    const uint32_t tiles = (cap + 15) / 16;
+    uint32_t blocks = (tiles + 3) / 4;
+    if (blocks > 2048) blocks = 2048;
+    hipLaunchKernelGGL(infer_kernel, dim3(blocks), dim3(256), 0, c->stream, c->d_fwd, c->d_p, raw, cap, d_count,
+                       c->center, c->inv_half, out);
+    NRC_TRY(c, hipGetLastError());
+    return 0;
+}
+
+int slimrt_nrc_train_queue(slimrt_nrc *c, const float *raw, const float *targets, const uint32_t *d_count, uint32_t cap,
+                           uint32_t steps, uint32_t batch, float lr, float *d_losses) {
+    if (!c || !raw || !targets || !d_count || cap == 0) return nrc_fail(c, -1, "train_queue: NULL buffer or zero capacity");
+    if (steps == 0 || batch == 0 || batch % 64 || (uint64_t)steps * batch > 0xFFFFFFFFull)
+        return nrc_fail(c, -1, "train_queue: batch must be a positive multiple of 64");
+    for (uint32_t st = 0; st < steps; st++) {
+        int rc = train(c, raw, targets, batch, true, lr, nullptr, nullptr, d_count, cap, st * batch, steps * batch,
+                       d_losses ? d_losses + st : nullptr);
+        if (rc) return rc;
+    }
     return 0;
 }
 

@@ -16,8 +16,9 @@
 #include <cstring>
 #include <cstdio>
 
-#include "slimrt_trace.h"
+#include "slimrt_nrc_trace.h"
 #include "../../include/slimrt.h"
+#include "../../include/slimrt_nrc.h"
 
 using namespace slimrt;
 
@@ -74,6 +75,101 @@ __global__ __launch_bounds__(kBlock) void render_kernel(SceneView s, RenderParam
     }
 }
 
+// ---------------------------------------------------------------- rendering with the neural radiance cache
+// The frame kernel has the render kernel's tiling; one in train_one_in waves (chosen by a hash of the wave's
+// position and the frame) traces full paths for training, so training and cached pixels never share a wave.
+__global__ __launch_bounds__(kBlock) void nrc_frame_kernel(SceneView s, RenderParams rp, NrcFrame nf,
+                                                           float4 *__restrict__ pixels, float *__restrict__ depths,
+                                                           uint32_t scene_depth, uint32_t mesh_depth, uint32_t tiles_x,
+                                                           uint32_t tiles, uint32_t tiles_per_xcd,
+                                                           uint32_t *__restrict__ status) {
+    extern __shared__ uint32_t lds[];
+    const uint32_t xcd = blockIdx.x & 7u, slot = blockIdx.x >> 3;
+    const uint32_t tile = xcd * tiles_per_xcd + slot;
+    if (tile >= tiles) return;
+    const uint32_t t = threadIdx.x, wave = t >> 6, lane = t & 63u;
+    const uint32_t x = (tile % tiles_x) * kTile + (wave & 1u) * 8u + (lane & 7u);
+    const uint32_t y = (tile / tiles_x) * kTile + (wave >> 1) * 8u + (lane >> 3);
+    const uint32_t ws = rp.width << rp.ssaa, hs = rp.height << rp.ssaa;
+    uint32_t h = (tile * 4u + wave) * 0x9E3779B1u + nf.frame * 0x85EBCA77u;
+    h ^= h >> 15;
+    h *= 0x2C1B3C6Du;
+    h ^= h >> 12;
+    const bool training = nf.train_one_in && (h % nf.train_one_in) == 0;
+
+    Tracer tr;
+    tr.scene_stack = Stack{lds + t, kBlock};
+    tr.mesh_stack = Stack{lds + scene_depth * kBlock + t, kBlock};
+    tr.scene_depth = scene_depth;
+    tr.mesh_depth = mesh_depth;
+    tr.overflow = 0;
+    tr.cnt = nullptr;
+    if (x < ws && y < hs) {
+        v3 dir;
+        float scaling, depth;
+        primary_ray(rp.proj, x, y, dir, scaling);
+        const uint32_t o = canvas_offset(rp, x, y);
+        color col;
+        if (render_beauty_nrc(s, rp, dir, scaling, tr, nf, training, o, col, depth)) {
+            float px[4];
+            output_value(rp, color{tone_map(col.r), tone_map(col.g), tone_map(col.b)}, px);
+            pixels[o] = make_float4(px[0], px[1], px[2], px[3]);
+        }
+        depths[o] = depth;
+        if (tr.overflow) atomicOr(status, 1u);
+        if (training) (void)wave_append(true, nf.counters + NC_TRAIN_PIXELS);
+    }
+}
+
+// Pixel = colour before the cache vertex + throughput * max(cache, 0), tone-mapped (ray_tracer.h:107).
+__global__ void nrc_composite_kernel(RenderParams rp, const uint32_t *__restrict__ q_count, uint32_t q_cap,
+                                     const uint32_t *__restrict__ q_pixel, const float *__restrict__ q_prefix,
+                                     const float *__restrict__ q_tp, const float *__restrict__ radiance,
+                                     float4 *__restrict__ pixels) {
+    const uint32_t n = min(*q_count, q_cap);
+    for (uint32_t q = blockIdx.x * blockDim.x + threadIdx.x; q < n; q += gridDim.x * blockDim.x) {
+        const color L{fmaxf(radiance[3 * q], 0.0f), fmaxf(radiance[3 * q + 1], 0.0f), fmaxf(radiance[3 * q + 2], 0.0f)};
+        const color tp{q_tp[3 * q], q_tp[3 * q + 1], q_tp[3 * q + 2]};
+        const color col = cmul_add(L, tp, color{q_prefix[3 * q], q_prefix[3 * q + 1], q_prefix[3 * q + 2]});
+        float px[4];
+        output_value(rp, color{tone_map(col.r), tone_map(col.g), tone_map(col.b)}, px);
+        pixels[q_pixel[q]] = make_float4(px[0], px[1], px[2], px[3]);
+    }
+}
+
+// Clears the frame counters and step losses (a kernel rather than a memset, so the frame can be captured).
+__global__ void nrc_reset_kernel(uint32_t *__restrict__ counters, float *__restrict__ losses, uint32_t nloss) {
+    const uint32_t i = threadIdx.x;
+    if (i < NC_COUNT) counters[i] = 0u;
+    if (i < nloss) losses[i] = 0.0f;
+}
+
+__global__ void nrc_gate_clear_kernel(float *__restrict__ cells, uint32_t n) {
+    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) cells[i] = 0.0f;
+}
+
+// Folds the frame's per-cell sums into the moving averages (weight 1/16) and clears the sums.
+__global__ void nrc_gate_fold_kernel(float *__restrict__ cells, uint32_t ncells) {
+    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < ncells; i += gridDim.x * blockDim.x) {
+        float *c = cells + (size_t)i * kGateFloats;
+        const float cnt = c[2];
+        if (cnt > 0.0f) {
+            const float m1 = c[0] / cnt, m2 = c[1] / cnt;
+            if (c[5] == 0.0f) {
+                c[3] = m1;
+                c[4] = m2;
+            } else {
+                c[3] += (m1 - c[3]) * (1.0f / 16.0f);
+                c[4] += (m2 - c[4]) * (1.0f / 16.0f);
+            }
+            c[5] = fminf(c[5] + 1.0f, 1.0e6f);
+            c[0] = 0.0f;
+            c[1] = 0.0f;
+            c[2] = 0.0f;
+        }
+    }
+}
+
 template <typename T>
 struct DevBuf {
     T *ptr = nullptr;
@@ -120,6 +216,17 @@ struct slimrt_context {
     DevBuf<float> d_depths;
     uint32_t *d_status = nullptr;
     unsigned long long *d_counters = nullptr;
+
+    // rendering with the cache: queues, training samples, gate table, frame counters
+    float *q_raw = nullptr, *q_prefix = nullptr, *q_tp = nullptr, *q_out = nullptr;
+    uint32_t *q_pixel = nullptr, q_cap = 0;
+    float *t_raw = nullptr, *t_target = nullptr, *t_next = nullptr;
+    uint32_t *t_link = nullptr, t_cap = 0;
+    uint32_t *nrc_counters = nullptr;
+    float *nrc_losses = nullptr;
+    uint32_t nrc_last_steps = 0;
+    float *cells = nullptr;
+    uint32_t cells_log2 = 0;
 };
 
 namespace {
@@ -318,6 +425,60 @@ int ensure_outputs(slimrt_context *c, size_t n) {
     return SLIMRT_OK;
 }
 
+constexpr uint32_t kMaxTrainSteps = 64;
+
+void free_nrc(slimrt_context *c) {
+    for (void *p : {(void *)c->q_raw, (void *)c->q_prefix, (void *)c->q_tp, (void *)c->q_out, (void *)c->q_pixel,
+                    (void *)c->t_raw, (void *)c->t_target, (void *)c->t_next, (void *)c->t_link,
+                    (void *)c->nrc_counters, (void *)c->nrc_losses, (void *)c->cells})
+        if (p) (void)hipFree(p);
+    c->q_raw = c->q_prefix = c->q_tp = c->q_out = c->t_raw = c->t_target = c->t_next = c->nrc_losses = c->cells = nullptr;
+    c->q_pixel = c->nrc_counters = c->t_link = nullptr;
+    c->q_cap = c->t_cap = c->cells_log2 = 0;
+}
+
+int ensure_nrc(slimrt_context *c, uint32_t q_cap, uint32_t t_cap, uint32_t cells_log2, bool &cells_new) {
+    cells_new = false;
+    if (!c->nrc_counters) {
+        HIP_TRY(c, hipMalloc(&c->nrc_counters, NC_COUNT * sizeof(uint32_t)));
+        HIP_TRY(c, hipMalloc(&c->nrc_losses, kMaxTrainSteps * sizeof(float)));
+    }
+    if (c->q_cap < q_cap) {
+        for (void *p : {(void *)c->q_raw, (void *)c->q_prefix, (void *)c->q_tp, (void *)c->q_out, (void *)c->q_pixel})
+            if (p) HIP_TRY(c, hipFree(p));
+        c->q_raw = c->q_prefix = c->q_tp = c->q_out = nullptr;
+        c->q_pixel = nullptr;
+        c->q_cap = 0;
+        HIP_TRY(c, hipMalloc(&c->q_raw, (size_t)q_cap * kNrcRaw * sizeof(float)));
+        HIP_TRY(c, hipMalloc(&c->q_prefix, (size_t)q_cap * 3 * sizeof(float)));
+        HIP_TRY(c, hipMalloc(&c->q_tp, (size_t)q_cap * 3 * sizeof(float)));
+        HIP_TRY(c, hipMalloc(&c->q_out, (size_t)q_cap * 3 * sizeof(float)));
+        HIP_TRY(c, hipMalloc(&c->q_pixel, (size_t)q_cap * sizeof(uint32_t)));
+        c->q_cap = q_cap;
+    }
+    if (c->t_cap < t_cap) {
+        for (void *p : {(void *)c->t_raw, (void *)c->t_target, (void *)c->t_next, (void *)c->t_link})
+            if (p) HIP_TRY(c, hipFree(p));
+        c->t_raw = c->t_target = c->t_next = nullptr;
+        c->t_link = nullptr;
+        c->t_cap = 0;
+        HIP_TRY(c, hipMalloc(&c->t_raw, (size_t)t_cap * kNrcRaw * sizeof(float)));
+        HIP_TRY(c, hipMalloc(&c->t_target, (size_t)t_cap * 3 * sizeof(float)));
+        HIP_TRY(c, hipMalloc(&c->t_next, (size_t)t_cap * 3 * sizeof(float)));
+        HIP_TRY(c, hipMalloc(&c->t_link, (size_t)t_cap * sizeof(uint32_t)));
+        c->t_cap = t_cap;
+    }
+    if (c->cells_log2 != cells_log2) {
+        if (c->cells) HIP_TRY(c, hipFree(c->cells));
+        c->cells = nullptr;
+        c->cells_log2 = 0;
+        HIP_TRY(c, hipMalloc(&c->cells, ((size_t)kGateFloats << cells_log2) * sizeof(float)));
+        c->cells_log2 = cells_log2;
+        cells_new = true;
+    }
+    return SLIMRT_OK;
+}
+
 int check_status(slimrt_context *c) {
     uint32_t st = 0;
     HIP_TRY(c, hipMemcpyAsync(&st, c->d_status, sizeof st, hipMemcpyDeviceToHost, c->stream));
@@ -370,6 +531,7 @@ int slimrt_destroy(slimrt_context *c) {
         dev_free(c->d_geos); dev_free(c->d_mats); dev_free(c->d_lights); dev_free(c->d_snodes); dev_free(c->d_mnodes);
         dev_free(c->d_leaf); dev_free(c->d_meshes); dev_free(c->d_tri_hit); dev_free(c->d_tri_attr); dev_free(c->d_tex);
         dev_free(c->d_mips); dev_free(c->d_quads); dev_free(c->d_pixels); dev_free(c->d_depths);
+        free_nrc(c);
         if (c->d_status) (void)hipFree(c->d_status);
         if (c->d_counters) (void)hipFree(c->d_counters);
         if (c->ev0) (void)hipEventDestroy(c->ev0);
@@ -571,6 +733,135 @@ int slimrt_last_kernel_ms(slimrt_context *c, float *ms) {
     HIP_TRY(c, hipEventSynchronize(c->ev1));
     HIP_TRY(c, hipEventElapsedTime(ms, c->ev0, c->ev1));
     return SLIMRT_OK;
+}
+
+int slimrt_render_nrc(slimrt_context *c, const slimrt_render_params *p, slimrt_nrc *nrc,
+                      const slimrt_nrc_render_params *np, void *d_pixels, void *d_depths) {
+    if (!c) return SLIMRT_E_ARG;
+    int rc = validate_params(c, p);
+    if (rc) return rc;
+    if (c->cpu) return fail(c, SLIMRT_E_STATE, "render_nrc needs a GPU context");
+    if (!nrc || !np || !d_pixels || !d_depths) return fail(c, SLIMRT_E_ARG, "NULL cache, settings or output buffer");
+    if (p->render_mode != SLIMRT_MODE_BEAUTY || p->output > SLIMRT_OUTPUT_RADIANCE)
+        return fail(c, SLIMRT_E_ARG, "the cache renders beauty frames to the canvas or the radiance buffer");
+    if (p->max_depth == 0 || p->max_depth > kNrcMaxDepth) return fail(c, SLIMRT_E_ARG, "max_depth must be 1..8 with the cache");
+    if (np->gate_cells_log2 < 10 || np->gate_cells_log2 > 24 || !(np->gate_cell_size > 0.0f))
+        return fail(c, SLIMRT_E_ARG, "gate table: cells_log2 10..24 and a positive cell size");
+    const bool train = np->train_one_in != 0 && np->train_steps != 0;
+    if (train && (np->train_steps > kMaxTrainSteps || np->train_batch == 0 || np->train_batch % 64 ||
+                  np->train_batch > (1u << 22) || !(np->learning_rate > 0.0f)))
+        return fail(c, SLIMRT_E_ARG, "training: 1..64 steps, batch a multiple of 64 up to 2^22, positive learning rate");
+    RenderParams rp;
+    std::memcpy(&rp, p, sizeof rp);
+    rp.ssaa = rp.ssaa ? 1u : 0u;
+    const uint32_t ws = rp.width << rp.ssaa, hs = rp.height << rp.ssaa;
+    const uint32_t samples = ws * hs;
+    const uint32_t tiles_x = (ws + kTile - 1) / kTile, tiles_y = (hs + kTile - 1) / kTile;
+    const uint32_t tiles = tiles_x * tiles_y, per_xcd = (tiles + 7) / 8;
+    const uint32_t scene_depth = c->scene_height + 1, mesh_depth = c->mesh_height + 1;
+    const size_t lds = (size_t)(scene_depth + mesh_depth) * kBlock * sizeof(uint32_t);
+    if (lds > 160 * 1024) return fail(c, SLIMRT_E_ARG, "BVH too deep for the LDS traversal stacks");
+    // training waves are a hashed 1-in-N subset: twice the expected share (at least 64 waves) bounds the samples
+    uint64_t t_cap = 0;
+    if (np->train_one_in) {
+        const uint64_t waves = ((uint64_t)tiles * 4 + np->train_one_in - 1) / np->train_one_in;
+        t_cap = std::min<uint64_t>((uint64_t)tiles * 4, 2 * waves + 64) * 64 * p->max_depth;
+        t_cap = std::min<uint64_t>(t_cap, kLinkNone - 1);
+    }
+    bool cells_new;
+    if ((rc = ensure_nrc(c, samples, (uint32_t)std::max<uint64_t>(t_cap, 64), np->gate_cells_log2, cells_new))) return rc;
+    if ((rc = slimrt_nrc_set_stream(nrc, c->stream))) return fail(c, SLIMRT_E_STATE, "cache stream");
+
+    NrcFrame nf;
+    nf.q_raw = c->q_raw; nf.q_pixel = c->q_pixel; nf.q_prefix = c->q_prefix; nf.q_tp = c->q_tp; nf.q_cap = samples;
+    nf.t_raw = c->t_raw; nf.t_target = c->t_target; nf.t_next = c->t_next; nf.t_link = c->t_link;
+    nf.t_cap = (uint32_t)t_cap;
+    nf.counters = c->nrc_counters;
+    nf.cells = c->cells;
+    nf.cell_mask = (1u << np->gate_cells_log2) - 1u;
+    nf.cell_inv_size = 1.0f / np->gate_cell_size;
+    nf.gate_threshold = np->gate_threshold;
+    nf.gate_min_frames = np->gate_min_frames;
+    nf.cache_vertex = np->cache_depth + 1;
+    nf.train_one_in = np->train_one_in;
+    nf.frame = np->frame;
+    const uint32_t ncells = 1u << np->gate_cells_log2;
+
+    c->timed = true;
+    HIP_TRY(c, hipEventRecord(c->ev0, c->stream));
+    if (cells_new || (np->flags & SLIMRT_NRC_RESET_GATE)) {
+        hipLaunchKernelGGL(nrc_gate_clear_kernel, dim3(1024), dim3(256), 0, c->stream, c->cells, ncells * kGateFloats);
+        HIP_TRY(c, hipGetLastError());
+    }
+    hipLaunchKernelGGL(nrc_reset_kernel, dim3(1), dim3(64), 0, c->stream, c->nrc_counters, c->nrc_losses, kMaxTrainSteps);
+    HIP_TRY(c, hipGetLastError());
+    hipLaunchKernelGGL(nrc_frame_kernel, dim3(per_xcd * 8), dim3(kBlock), lds, c->stream, device_view(c), rp, nf,
+                       (float4 *)d_pixels, (float *)d_depths, scene_depth, mesh_depth, tiles_x, tiles, per_xcd, c->d_status);
+    HIP_TRY(c, hipGetLastError());
+    if (np->cache_depth + 1 <= p->max_depth) {
+        if (slimrt_nrc_infer_queue(nrc, c->q_raw, c->nrc_counters + NC_QUERIES, samples, c->q_out))
+            return fail(c, SLIMRT_E_DEVICE, std::string("cache inference: ") + slimrt_nrc_last_error(nrc));
+        const uint32_t blocks = std::min<uint32_t>((samples + 255) / 256, 4096);
+        hipLaunchKernelGGL(nrc_composite_kernel, dim3(blocks), dim3(256), 0, c->stream, rp,
+                           (const uint32_t *)(c->nrc_counters + NC_QUERIES), samples, (const uint32_t *)c->q_pixel,
+                           (const float *)c->q_prefix, (const float *)c->q_tp, (const float *)c->q_out, (float4 *)d_pixels);
+        HIP_TRY(c, hipGetLastError());
+    }
+    c->nrc_last_steps = 0;
+    if (train) {
+        if (slimrt_nrc_train_queue(nrc, c->t_raw, c->t_target, c->nrc_counters + NC_SAMPLES, (uint32_t)t_cap,
+                                   np->train_steps, np->train_batch, np->learning_rate, c->nrc_losses))
+            return fail(c, SLIMRT_E_DEVICE, std::string("cache training: ") + slimrt_nrc_last_error(nrc));
+        c->nrc_last_steps = np->train_steps;
+    }
+    if (np->train_one_in) {
+        hipLaunchKernelGGL(nrc_gate_fold_kernel, dim3(std::min<uint32_t>((ncells + 255) / 256, 4096)), dim3(256), 0,
+                           c->stream, c->cells, ncells);
+        HIP_TRY(c, hipGetLastError());
+    }
+    HIP_TRY(c, hipEventRecord(c->ev1, c->stream));
+    return SLIMRT_OK;
+}
+
+int slimrt_nrc_frame_samples(slimrt_context *c, float *raw, float *targets, uint32_t *pixel, uint32_t *vertex,
+                             uint32_t cap, uint32_t *count) {
+    if (!c || !count || (cap && (!raw || !targets || !pixel || !vertex))) return SLIMRT_E_ARG;
+    if (c->cpu || !c->nrc_counters) return fail(c, SLIMRT_E_STATE, "no frame rendered with the cache");
+    uint32_t cnt[NC_COUNT];
+    HIP_TRY(c, hipMemcpyAsync(cnt, c->nrc_counters, sizeof cnt, hipMemcpyDeviceToHost, c->stream));
+    HIP_TRY(c, hipStreamSynchronize(c->stream));
+    const uint32_t n = std::min(cnt[NC_SAMPLES], c->t_cap);
+    *count = n;
+    const uint32_t m = std::min(n, cap);
+    if (m) {
+        std::vector<float> next((size_t)m * 3);
+        HIP_TRY(c, hipMemcpyAsync(raw, c->t_raw, (size_t)m * kNrcRaw * sizeof(float), hipMemcpyDeviceToHost, c->stream));
+        HIP_TRY(c, hipMemcpyAsync(targets, c->t_target, (size_t)m * 3 * sizeof(float), hipMemcpyDeviceToHost, c->stream));
+        HIP_TRY(c, hipMemcpyAsync(pixel, c->t_link, (size_t)m * sizeof(uint32_t), hipMemcpyDeviceToHost, c->stream));
+        HIP_TRY(c, hipMemcpyAsync(next.data(), c->t_next, next.size() * sizeof(float), hipMemcpyDeviceToHost, c->stream));
+        HIP_TRY(c, hipStreamSynchronize(c->stream));
+        for (uint32_t i = 0; i < m; i++) std::memcpy(&vertex[i], &next[3 * (size_t)i], 4);
+    }
+    return check_status(c);
+}
+
+int slimrt_nrc_frame_stats(slimrt_context *c, uint32_t stats[8]) {
+    if (!c || !stats) return SLIMRT_E_ARG;
+    if (c->cpu || !c->nrc_counters) return fail(c, SLIMRT_E_STATE, "no frame rendered with the cache");
+    uint32_t cnt[NC_COUNT];
+    float losses[kMaxTrainSteps];
+    HIP_TRY(c, hipMemcpyAsync(cnt, c->nrc_counters, sizeof cnt, hipMemcpyDeviceToHost, c->stream));
+    HIP_TRY(c, hipMemcpyAsync(losses, c->nrc_losses, sizeof losses, hipMemcpyDeviceToHost, c->stream));
+    HIP_TRY(c, hipStreamSynchronize(c->stream));
+    stats[0] = std::min(cnt[NC_QUERIES], c->q_cap);
+    stats[1] = std::min(cnt[NC_SAMPLES], c->t_cap);
+    stats[2] = cnt[NC_FALLBACKS];
+    stats[3] = cnt[NC_TRAIN_PIXELS];
+    const float first = c->nrc_last_steps ? losses[0] : 0.0f, last = c->nrc_last_steps ? losses[c->nrc_last_steps - 1] : 0.0f;
+    std::memcpy(&stats[4], &first, 4);
+    std::memcpy(&stats[5], &last, 4);
+    stats[6] = stats[7] = 0;
+    return check_status(c);
 }
 
 }  // extern "C"

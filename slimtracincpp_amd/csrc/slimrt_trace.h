@@ -570,6 +570,21 @@ SLIM_HD void render_debug(const SceneView &s, const RenderParams &rp, v3 directi
     }
 }
 
+// The value the canvas or the radiance buffer receives for a tone-mapped sample colour.
+SLIM_HD void output_value(const RenderParams &rp, color c, float out_px[4]) {
+    if (rp.output == 0) {   // Canvas::setPixel with opacity -1 (src/slim/draw/canvas.h:128-169)
+        color k{clamp01(c.r), clamp01(c.g), clamp01(c.b)};
+        out_px[0] = k.r * (k.r * 1.0f);
+        out_px[1] = k.g * (k.g * 1.0f);
+        out_px[2] = k.b * (k.b * 1.0f);
+    } else {
+        out_px[0] = c.r;
+        out_px[1] = c.g;
+        out_px[2] = c.b;
+    }
+    out_px[3] = 1.0f;
+}
+
 // One sample: primary ray, integrator, and the value the canvas or the radiance buffer receives.
 SLIM_HD void render_sample(const SceneView &s, const RenderParams &rp, uint32_t x, uint32_t y, Tracer &tr,
                            float out_px[4], float &out_depth) {
@@ -595,17 +610,7 @@ SLIM_HD void render_sample(const SceneView &s, const RenderParams &rp, uint32_t 
     if (rp.render_mode == RM_Beauty) render_beauty(s, rp, dir, scaling, tr, c, out_depth);
     else render_debug(s, rp, dir, scaling, tr, c, out_depth);
     if (tr.cnt) tr.cnt->pixels++;
-    if (rp.output == 0) {   // Canvas::setPixel with opacity -1 (src/slim/draw/canvas.h:128-169)
-        color k{clamp01(c.r), clamp01(c.g), clamp01(c.b)};
-        out_px[0] = k.r * (k.r * 1.0f);
-        out_px[1] = k.g * (k.g * 1.0f);
-        out_px[2] = k.b * (k.b * 1.0f);
-    } else {
-        out_px[0] = c.r;
-        out_px[1] = c.g;
-        out_px[2] = c.b;
-    }
-    out_px[3] = 1.0f;
+    output_value(rp, c, out_px);
 }
 
 // Canvas offsets: samples are stored row-major, or as 2x2 quads per pixel under SSAA (canvas.h:147).

@@ -3,7 +3,8 @@
 The cache has no counterpart in the reference; this is a plain restatement of the network the HIP kernels
 implement (slimtracincpp_amd/csrc/nrc.hip), used to check their forward pass, gradients and Adam step:
 
-  encoding   position normalised to the scene box, 12 frequency bands of sin/cos per axis (72 values), then
+  encoding   position p normalised to the scene box, 12 frequency bands per axis (72 values): sin and cos of
+             2^b pi p, computed as one turn times frac(2^(b-1) p) (the reduction is exact in float32), then
              normal (3), view direction (3), roughness, metalness, albedo (3): 83 values, zero-padded to 96
   network    96 -> 64 -> 64 -> 64 -> 64 -> 3 (padded to 16), ReLU after each hidden layer, linear output
   numerics   inputs, weights and hidden activations rounded to bfloat16 (round to nearest even), products
@@ -25,7 +26,6 @@ import numpy as np
 N_IN, N_HIDDEN, N_OUT, N_OUT_PAD, N_BANDS = 96, 64, 3, 16, 12
 N_ENCODED = 3 * N_BANDS * 2 + 11   # 83
 LAYERS = [(N_IN, N_HIDDEN), (N_HIDDEN, N_HIDDEN), (N_HIDDEN, N_HIDDEN), (N_HIDDEN, N_HIDDEN), (N_HIDDEN, N_OUT_PAD)]
-PI = np.float32(3.14159265)
 
 
 def param_offsets():
@@ -74,10 +74,11 @@ def encode(raw: np.ndarray, center: np.ndarray, inv_half: np.ndarray) -> np.ndar
     p = ((raw[:, 0:3] - np.asarray(center, np.float32)) * np.asarray(inv_half, np.float32)).astype(np.float32)
     for d in range(3):
         for b in range(N_BANDS):
-            scale = np.float32(float(1 << b)) * PI
-            arg = (scale * p[:, d]).astype(np.float32)
-            f[:, 24 * d + 2 * b] = np.sin(arg.astype(np.float64)).astype(np.float32)
-            f[:, 24 * d + 2 * b + 1] = np.cos(arg.astype(np.float64)).astype(np.float32)
+            x = (p[:, d] * np.float32(2.0 ** (b - 1))).astype(np.float32)     # exact
+            turns = (x - np.floor(x)).astype(np.float32)                       # exact
+            arg = 2.0 * np.pi * turns.astype(np.float64)
+            f[:, 24 * d + 2 * b] = np.sin(arg).astype(np.float32)
+            f[:, 24 * d + 2 * b + 1] = np.cos(arg).astype(np.float32)
     f[:, 72:83] = raw[:, 3:14]
     return f
 

@@ -1,4 +1,4 @@
-// Neural radiance cache on gfx950: fused BF16 MFMA inference, fused forward/backward with per-wave partial
+// Neural radiance cache on gfx950: fused BF16 MFMA inference, fused forward/backward with per-workgroup partial
 // gradients, and a reduce + Adam kernel (include/slimrt_nrc.h).
 //
 // Matrix products use v_mfma_f32_16x16x16_bf16 with weights as the A operand and activations as the B operand
@@ -28,12 +28,31 @@ constexpr int W0 = 0, B0 = W0 + NIN * NH, W1 = B0 + NH, B1 = W1 + NH * NH, W2 = 
 static_assert(NPARAMS == (int)SLIMRT_NRC_PARAMS, "parameter count");
 constexpr int FWD_FRAGS = 24 + 3 * 16 + 4;   // layer 0: 4 out tiles x 6 in chunks; 1-3: 4x4; 4: 1x4
 constexpr int BWD_FRAGS = 3 * 16 + 4;        // layers 1-3: 4 in tiles x 4 out chunks; 4: 4 in tiles x 1
-constexpr float kPi = 3.14159265f;
 
 __host__ __device__ inline int layer_w(int li) { return li == 0 ? W0 : li == 1 ? W1 : li == 2 ? W2 : li == 3 ? W3 : W4; }
 __host__ __device__ inline int layer_b(int li) { return li == 0 ? B0 : li == 1 ? B1 : li == 2 ? B2 : li == 3 ? B3 : B4; }
 __host__ __device__ inline int layer_fo(int li) { return li == 4 ? NOUTP : NH; }
 __host__ __device__ inline int fwd_base(int li) { return li == 0 ? 0 : 24 + 16 * (li - 1); }
+
+// Gradient partials are stored in "slot" order, as the MFMA accumulators hold them: weight tile t (numbered as
+// the forward fragments: layer 0 tile mt*6+nt, layers 1-3 24+16(li-1)+mt*4+nt, layer 4 72+nt) occupies slots
+// 256t .. 256t+255, lane l's four outputs j at 256t + 4l + j; the 272 biases follow in parameter order.
+constexpr int kWeightSlots = 76 * 256;
+static_assert(kWeightSlots + 4 * NH + NOUTP == NPARAMS, "every parameter has one slot");
+__host__ __device__ inline int bias_slot(int li, int o) { return kWeightSlots + (li < 4 ? li * NH + o : 4 * NH + o); }
+__host__ __device__ inline int slot_param(int s) {
+    if (s >= kWeightSlots) {
+        const int b = s - kWeightSlots;
+        return b < 4 * NH ? layer_b(b / NH) + b % NH : B4 + (b - 4 * NH);
+    }
+    const int tile = s >> 8, lane = (s >> 2) & 63, j = s & 3;
+    int li, mt, nt;
+    if (tile < 24) { li = 0; mt = tile / 6; nt = tile % 6; }
+    else if (tile < 72) { li = 1 + (tile - 24) / 16; mt = ((tile - 24) % 16) / 4; nt = (tile - 24) % 4; }
+    else { li = 4; mt = 0; nt = tile - 72; }
+    const int o = 16 * mt + 4 * (lane >> 4) + j, i = 16 * nt + (lane & 15);
+    return layer_w(li) + i * layer_fo(li) + o;
+}
 __host__ __device__ inline int bwd_base(int li) { return 16 * (li - 1); }
 
 __device__ __forceinline__ unsigned short to_bf16(float x) {   // round to nearest even
@@ -49,11 +68,15 @@ __device__ __forceinline__ f32x4 mfma(bf16x4 a, bf16x4 b, f32x4 c) {
 __device__ __forceinline__ bf16x4 as_bf16x4(uint2 v) { return __builtin_bit_cast(bf16x4, v); }
 
 // Encoded input feature f of a query (frequency bands of the normalised position, then the raw attributes).
+// Band b of coordinate p is sin and cos of 2^b * pi * p, evaluated as one turn times the fractional part of
+// 2^(b-1) * p: the scaling by a power of two and the fractional part are exact in float32, so the hardware
+// sine and cosine (which take turns) see a reduced argument in [0, 1].
 __device__ __forceinline__ float feature(int f, const float *raw, const float *pn) {
     if (f < 72) {
         const int d = f / 24, r = f % 24, b = r >> 1;
-        const float arg = ((float)(1 << b) * kPi) * pn[d];
-        return (r & 1) ? cosf(arg) : sinf(arg);
+        const float x = pn[d] * __builtin_bit_cast(float, (uint32_t)(126 + b) << 23);   // * 2^(b-1)
+        const float turns = x - floorf(x);
+        return (r & 1) ? __builtin_amdgcn_cosf(turns) : __builtin_amdgcn_sinf(turns);
     }
     if (f < 83) return raw[3 + (f - 72)];
     return 0.0f;
@@ -141,24 +164,41 @@ __global__ __launch_bounds__(256) void infer_kernel(const uint2 *__restrict__ fr
     }
 }
 
-// ---------------------------------------------------------------- training: forward + backward, per-wave partials
-// One wave per workgroup, 64 samples (4 tiles of 16).  LDS: each layer's input X_l and two output-gradient
-// buffers, all bf16 [feature][sample].
+// ---------------------------------------------------------------- training: forward + backward, per-workgroup partials
+// One workgroup of four waves per 64 samples (4 tiles of 16).  LDS: the weight fragments and biases, each
+// layer's input X_l and two output-gradient buffers, all bf16 [feature][sample].
+// One row per feature, 64 samples plus 8 halves of padding: rows 36 dwords apart spread a wave's 8-byte
+// operand reads (16 rows x 4 sample groups) over all 64 LDS banks.
+constexpr int kRow = 72;
 struct TrainLds {
-    unsigned short x0[NIN * 64];
-    unsigned short x[4][NH * 64];   // inputs of layers 1..4 (outputs of hidden layers 0..3)
-    unsigned short dz[2][NH * 64];
+    uint2 wf[FWD_FRAGS * 64];          // forward and backward weight fragments, staged once per wave
+    uint2 wb[BWD_FRAGS * 64];
+    float bias[4 * NH + NOUTP];
+    float bpart[2][4][NH];             // per-wave bias-gradient sums (double-buffered by layer parity)
+    float b4part[4][NOUTP];
+    float loss[4];
+    unsigned short x0[NIN * kRow];
+    unsigned short x[4][NH * kRow];   // inputs of layers 1..4 (outputs of hidden layers 0..3)
+    unsigned short dz[2][NH * kRow];
 };
 
 __device__ __forceinline__ bf16x4 lds_bf16x4(const unsigned short *base) {   // 8 contiguous bytes
     return __builtin_bit_cast(bf16x4, *(const uint2 *)base);
 }
 
-__device__ __forceinline__ float sum16(float v) {   // sum over the 16 lanes that share lane/16
-    v += __shfl_xor(v, 1, 64);
-    v += __shfl_xor(v, 2, 64);
-    v += __shfl_xor(v, 4, 64);
-    v += __shfl_xor(v, 8, 64);
+template <int CTRL>
+__device__ __forceinline__ float dpp(float v) {
+    return __builtin_bit_cast(float, __builtin_amdgcn_update_dpp(0, __builtin_bit_cast(int, v), CTRL, 0xF, 0xF, false));
+}
+
+// Sum over the 16 lanes that share lane/16, as the butterfly v += v[lane ^ k] for k = 1, 2, 4, 8 would add:
+// quad_perm [1,0,3,2] and [2,3,0,1] pair lanes within quads, then row_half_mirror and row_mirror bring in the
+// other quad and the other half-row (whose lanes all hold the same partial sums by then).
+__device__ __forceinline__ float sum16(float v) {
+    v += dpp<0xB1>(v);
+    v += dpp<0x4E>(v);
+    v += dpp<0x141>(v);
+    v += dpp<0x140>(v);
     return v;
 }
 
@@ -190,144 +230,190 @@ __device__ __forceinline__ SampleMap sample_map(uint32_t n, const uint32_t *n_de
     return m;
 }
 
-__global__ __launch_bounds__(64) void train_kernel(const uint2 *__restrict__ frag_fwd, const uint2 *__restrict__ frag_bwd,
-                                                   const float *__restrict__ p, const float *__restrict__ raw,
-                                                   const float *__restrict__ targets, uint32_t n,
-                                                   const uint32_t *__restrict__ n_dev, uint32_t cap, uint32_t offset,
-                                                   uint32_t total, float3 center, float3 inv_half,
-                                                   float *__restrict__ partial, float *__restrict__ partial_loss) {
+// Hidden layer LI of the forward pass for one 16-sample tile: in (B operands) -> out (next layer's B operands),
+// with the bf16 activations also stored to LDS for the backward pass.  Every index is a compile-time constant,
+// so the operand arrays stay in registers.
+template <int LI>
+__device__ __forceinline__ void train_fwd_layer(TrainLds &s, int lane, int sq, const bf16x4 *in, bf16x4 *out) {
+    constexpr int KCN = LI == 0 ? 6 : 4;
+    f32x4 acc[4];
+#pragma unroll
+    for (int mt = 0; mt < 4; mt++) {
+        acc[mt] = f32x4{0, 0, 0, 0};
+#pragma unroll
+        for (int kc = 0; kc < KCN; kc++)
+            acc[mt] = mfma(as_bf16x4(s.wf[(fwd_base(LI) + mt * KCN + kc) * 64 + lane]), in[kc], acc[mt]);
+    }
+#pragma unroll
+    for (int mt = 0; mt < 4; mt++)
+#pragma unroll
+        for (int j = 0; j < 4; j++) {
+            const int o = 16 * mt + 4 * (lane >> 4) + j;
+            const float z = acc[mt][j] + s.bias[LI * NH + o];
+            const unsigned short hb = to_bf16(z > 0.0f ? z : 0.0f);
+            out[mt][j] = (short)hb;
+            s.x[LI][o * kRow + sq] = hb;
+        }
+}
+
+// Backward step of layer LI for the workgroup's 64 samples: its weight gradient (dW^T = dZ X^T, 16x16 tiles dealt
+// round-robin to the four waves) and, for LI > 0, the output gradient of layer LI-1 (W dZ masked by that layer's
+// ReLU; wave w takes samples 16w..16w+15) with its bias gradient (per-wave sums combined in wave order).
+template <int LI>
+__device__ __forceinline__ void train_bwd_layer(TrainLds &s, int wave, int lane, float *g) {
+    constexpr int FO = LI == 4 ? NOUTP : NH, FI = LI == 0 ? NIN : NH, CUR = (4 - LI) & 1;
+    constexpr int NT = FI / 16, TILES = (FO / 16) * NT;
+    const unsigned short *X = LI == 0 ? s.x0 : s.x[LI == 0 ? 0 : LI - 1];
+    const unsigned short *DZ = s.dz[CUR];
+#pragma unroll
+    for (int r = 0; r < TILES / 4; r++) {
+        const int k = wave + 4 * r, mt = k / NT, nt = k % NT;
+        f32x4 acc = f32x4{0, 0, 0, 0};
+#pragma unroll
+        for (int kk = 0; kk < 4; kk++) {
+            const int so = 16 * kk + 4 * (lane >> 4);
+            acc = mfma(lds_bf16x4(DZ + (16 * mt + (lane & 15)) * kRow + so),
+                       lds_bf16x4(X + (16 * nt + (lane & 15)) * kRow + so), acc);
+        }
+        constexpr int base = LI == 0 ? 0 : LI < 4 ? 24 + 16 * (LI - 1) : 72;
+        *(f32x4 *)(g + (base + k) * 256 + lane * 4) = acc;
+    }
+    if constexpr (LI > 0) {
+        constexpr int KCN = FO / 16;
+        unsigned short *DZN = s.dz[CUR ^ 1];
+        const int sq = 16 * wave + (lane & 15);
+        bf16x4 b[KCN];
+#pragma unroll
+        for (int kc = 0; kc < KCN; kc++)
+#pragma unroll
+            for (int j = 0; j < 4; j++) b[kc][j] = (short)DZ[(16 * kc + 4 * (lane >> 4) + j) * kRow + sq];
+#pragma unroll
+        for (int mt = 0; mt < 4; mt++) {
+            f32x4 acc = f32x4{0, 0, 0, 0};
+#pragma unroll
+            for (int kc = 0; kc < KCN; kc++) {
+                const int frag = LI == 4 ? 48 + mt : bwd_base(LI) + mt * 4 + kc;
+                acc = mfma(as_bf16x4(s.wb[frag * 64 + lane]), b[kc], acc);
+            }
+#pragma unroll
+            for (int j = 0; j < 4; j++) {
+                const int i = 16 * mt + 4 * (lane >> 4) + j;
+                const float dzv = from_bf16(X[i * kRow + sq]) > 0.0f ? acc[j] : 0.0f;
+                DZN[i * kRow + sq] = to_bf16(dzv);
+                const float sum = sum16(dzv);
+                if ((lane & 15) == 0) s.bpart[LI & 1][wave][i] = sum;
+            }
+        }
+    }
+    __syncthreads();
+    if constexpr (LI > 0) {
+        if (wave == 0) {
+            const float (*bp)[NH] = s.bpart[LI & 1];
+            g[bias_slot(LI - 1, lane)] = ((bp[0][lane] + bp[1][lane]) + bp[2][lane]) + bp[3][lane];
+        }
+    }
+}
+
+// One workgroup of four waves trains on 64 samples; wave w runs samples 16w..16w+15 through the forward pass.
+__global__ __launch_bounds__(256) void train_kernel(const uint2 *__restrict__ frag_fwd, const uint2 *__restrict__ frag_bwd,
+                                                    const float *__restrict__ p, const float *__restrict__ raw,
+                                                    const float *__restrict__ targets, uint32_t n,
+                                                    const uint32_t *__restrict__ n_dev, uint32_t cap, uint32_t offset,
+                                                    uint32_t total, float3 center, float3 inv_half,
+                                                    float *__restrict__ partial, float *__restrict__ partial_loss) {
     __shared__ TrainLds s;
-    const int lane = threadIdx.x;
+    const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
     const uint32_t base = blockIdx.x * 64;
     const SampleMap map = sample_map(n, n_dev, cap, offset, total);
     const float inv_count = map.n_eff ? 1.0f / (3.0f * (float)map.n_eff) : 0.0f;
+    for (int i = tid; i < FWD_FRAGS * 64; i += 256) s.wf[i] = frag_fwd[i];
+    for (int i = tid; i < BWD_FRAGS * 64; i += 256) s.wb[i] = frag_bwd[i];
+    for (int i = tid; i < 4 * NH + NOUTP; i += 256) s.bias[i] = p[layer_b(i / NH) + (i % NH)];
+    __syncthreads();
     float *g = partial + (size_t)blockIdx.x * NPARAMS;
-    float loss = 0.0f;
-    bf16x4 dz4[4];                       // layer-4 output gradient per tile (B operand of dX4)
 
-    // ---- forward, 4 tiles
-    for (int t = 0; t < 4; t++) {
-        const int sq = 16 * t + (lane & 15);         // sample within the wave
+    // ---- forward: this wave's tile of 16 samples
+    {
+        const int sq = 16 * wave + (lane & 15);
         const bool valid = base + sq < map.n_eff;
         const uint32_t q = valid ? map(base + sq) : 0u;
-        bf16x4 x[6], h[4];
+        bf16x4 x[6], h0[4], h1[4];
         encode(raw, q, valid, center, inv_half, lane, x);
+#pragma unroll
         for (int kc = 0; kc < 6; kc++)
-            for (int j = 0; j < 4; j++) s.x0[(16 * kc + 4 * (lane >> 4) + j) * 64 + sq] = (unsigned short)x[kc][j];
-        for (int li = 0; li < 4; li++) {
-            const int kcn = li == 0 ? 6 : 4;
-            f32x4 acc[4];
-            for (int mt = 0; mt < 4; mt++) {
-                acc[mt] = f32x4{0, 0, 0, 0};
-                for (int kc = 0; kc < kcn; kc++)
-                    acc[mt] = mfma(as_bf16x4(frag_fwd[(fwd_base(li) + mt * kcn + kc) * 64 + lane]), li == 0 ? x[kc] : h[kc], acc[mt]);
-            }
-            for (int mt = 0; mt < 4; mt++)
-                for (int j = 0; j < 4; j++) {
-                    const int o = 16 * mt + 4 * (lane >> 4) + j;
-                    float z = acc[mt][j] + p[layer_b(li) + o];
-                    unsigned short hb = to_bf16(z > 0.0f ? z : 0.0f);
-                    h[mt][j] = (short)hb;
-                    s.x[li][o * 64 + sq] = hb;
-                }
-        }
+#pragma unroll
+            for (int j = 0; j < 4; j++) s.x0[(16 * kc + 4 * (lane >> 4) + j) * kRow + sq] = (unsigned short)x[kc][j];
+        train_fwd_layer<0>(s, lane, sq, x, h0);
+        train_fwd_layer<1>(s, lane, sq, h0, h1);
+        train_fwd_layer<2>(s, lane, sq, h1, h0);
+        train_fwd_layer<3>(s, lane, sq, h0, h1);
         f32x4 y = f32x4{0, 0, 0, 0};
-        for (int kc = 0; kc < 4; kc++) y = mfma(as_bf16x4(frag_fwd[(72 + kc) * 64 + lane]), h[kc], y);
+#pragma unroll
+        for (int kc = 0; kc < 4; kc++) y = mfma(as_bf16x4(s.wf[(72 + kc) * 64 + lane]), h1[kc], y);
         // output gradient of the relative L2 loss; lanes 0..15 hold outputs j = 0..3 of sample lane
         f32x4 d = f32x4{0, 0, 0, 0};
+        float loss = 0.0f;
         if (lane < 16 && valid) {
+#pragma unroll
             for (int c = 0; c < NOUT; c++) {
-                const float yc = y[c] + p[B4 + c], tc = targets[(size_t)q * NOUT + c];
+                const float yc = y[c] + s.bias[4 * NH + c], tc = targets[(size_t)q * NOUT + c];
                 const float den = yc * yc + 0.01f;
                 const float e = yc - tc;
                 loss += e * e / den;
                 d[c] = 2.0f * e / den * inv_count;
             }
         }
+#pragma unroll
         for (int j = 0; j < 4; j++) {
             const int o = 4 * (lane >> 4) + j;
-            const unsigned short hb = to_bf16(d[j]);
-            dz4[t][j] = (short)hb;
-            s.dz[0][o * 64 + sq] = hb;                // rows 0..15 of buffer 0
+            s.dz[0][o * kRow + sq] = to_bf16(d[j]);   // rows 0..15 of buffer 0
+            const float sum = sum16(d[j]);             // bias 4 gradient: fp32 sums of the output gradient
+            if ((lane & 15) == 0) s.b4part[wave][o] = sum;
         }
-        // bias 4 gradient: fp32 sums of the output gradient
-        for (int j = 0; j < 4; j++) {
-            float v = sum16(d[j]);
-            if ((lane & 15) == 0) {
-                const int o = 4 * (lane >> 4) + j;
-                if (t == 0) g[B4 + o] = v; else g[B4 + o] += v;
-            }
-        }
+        for (int off = 32; off > 0; off >>= 1) loss += __shfl_down(loss, off, 64);
+        if (lane == 0) s.loss[wave] = loss;
     }
     __syncthreads();
+    if (wave == 0 && lane < NOUTP)
+        g[bias_slot(4, lane)] = ((s.b4part[0][lane] + s.b4part[1][lane]) + s.b4part[2][lane]) + s.b4part[3][lane];
+    if (tid == 0) partial_loss[blockIdx.x] = (((s.loss[0] + s.loss[1]) + s.loss[2]) + s.loss[3]) * inv_count;
 
-    // ---- backward, layer by layer.  cur = buffer holding dZ of the layer being processed.
-    int cur = 0;
-    for (int li = 4; li >= 0; li--) {
-        const int fo = layer_fo(li), fi = li == 0 ? NIN : NH;
-        const unsigned short *X = li == 0 ? s.x0 : s.x[li - 1];
-        const unsigned short *DZ = s.dz[cur];
-        // weight gradient dW^T[out][in] = sum over 64 samples of dZ[out][s] X[in][s]
-        for (int mt = 0; mt < fo / 16; mt++)
-            for (int nt = 0; nt < fi / 16; nt++) {
-                f32x4 acc = f32x4{0, 0, 0, 0};
-                for (int kk = 0; kk < 4; kk++) {
-                    const int so = 16 * kk + 4 * (lane >> 4);
-                    acc = mfma(lds_bf16x4(DZ + (16 * mt + (lane & 15)) * 64 + so),
-                               lds_bf16x4(X + (16 * nt + (lane & 15)) * 64 + so), acc);
-                }
-                for (int j = 0; j < 4; j++) {
-                    const int o = 16 * mt + 4 * (lane >> 4) + j, i = 16 * nt + (lane & 15);
-                    g[layer_w(li) + i * fo + o] = acc[j];
-                }
-            }
-        if (li == 0) break;
-        // input gradient dX = W dZ per tile, masked by the ReLU of the previous layer -> dZ of layer li-1
-        const int nxt = cur ^ 1;
-        float bias_acc[4][4];
-        for (int t = 0; t < 4; t++) {
-            const int sq = 16 * t + (lane & 15);
-            bf16x4 b[4];
-            const int kcn = fo / 16;
-            for (int kc = 0; kc < kcn; kc++) {
-                if (li == 4) b[kc] = dz4[t];
-                else
-                    for (int j = 0; j < 4; j++) b[kc][j] = (short)DZ[(16 * kc + 4 * (lane >> 4) + j) * 64 + sq];
-            }
-            for (int mt = 0; mt < 4; mt++) {
-                f32x4 acc = f32x4{0, 0, 0, 0};
-                for (int kc = 0; kc < kcn; kc++) {
-                    const int frag = li == 4 ? 48 + mt : bwd_base(li) + mt * 4 + kc;
-                    acc = mfma(as_bf16x4(frag_bwd[frag * 64 + lane]), b[kc], acc);
-                }
-                for (int j = 0; j < 4; j++) {
-                    const int i = 16 * mt + 4 * (lane >> 4) + j;
-                    const float dzv = from_bf16(X[i * 64 + sq]) > 0.0f ? acc[j] : 0.0f;
-                    s.dz[nxt][i * 64 + sq] = to_bf16(dzv);
-                    const float sum = sum16(dzv);
-                    bias_acc[mt][j] = t == 0 ? sum : bias_acc[mt][j] + sum;
-                }
-            }
-        }
-        if ((lane & 15) == 0)
-            for (int mt = 0; mt < 4; mt++)
-                for (int j = 0; j < 4; j++) g[layer_b(li - 1) + 16 * mt + 4 * (lane >> 4) + j] = bias_acc[mt][j];
-        __syncthreads();
-        cur = nxt;
-    }
-    // loss partial (sum over this wave's samples, unnormalised)
-    for (int off = 32; off > 0; off >>= 1) loss += __shfl_down(loss, off, 64);
-    if (lane == 0) partial_loss[blockIdx.x] = loss * inv_count;
+    // ---- backward, layer by layer
+    train_bwd_layer<4>(s, wave, lane, g);
+    train_bwd_layer<3>(s, wave, lane, g);
+    train_bwd_layer<2>(s, wave, lane, g);
+    train_bwd_layer<1>(s, wave, lane, g);
+    train_bwd_layer<0>(s, wave, lane, g);
 }
 
-// Sum of the per-wave partials, then (optionally) one Adam step on the float32 master weights.
-__global__ void reduce_adam_kernel(const float *__restrict__ partial, uint32_t nblocks, float *__restrict__ p,
-                                   float *__restrict__ m, float *__restrict__ v, float *__restrict__ grad_out,
-                                   int do_adam, float lr, float c1, float c2) {
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= NPARAMS) return;
-    float gsum = 0.0f;
-    for (uint32_t b = 0; b < nblocks; b++) gsum += partial[(size_t)b * NPARAMS + i];
+// Sum of the per-workgroup partials, then (optionally) one Adam step on the float32 master weights.  A workgroup of
+// 4 waves owns 64 slots; wave w sums rows w, w+4, ... and the first wave combines the four sums (fixed order:
+// the result does not depend on scheduling).
+__global__ __launch_bounds__(256) void reduce_adam_kernel(const float *__restrict__ partial, uint32_t nblocks,
+                                                          float *__restrict__ p, float *__restrict__ m,
+                                                          float *__restrict__ v, float *__restrict__ grad_out,
+                                                          int do_adam, float lr, float c1, float c2) {
+    __shared__ float part[4][64];
+    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const int s = blockIdx.x * 64 + lane;
+    float acc = 0.0f;
+    if (s < NPARAMS) {
+        uint32_t b = wave;
+        for (; b + 12 < nblocks; b += 16) {   // four independent loads in flight
+            const float a0 = partial[(size_t)b * NPARAMS + s], a1 = partial[(size_t)(b + 4) * NPARAMS + s];
+            const float a2 = partial[(size_t)(b + 8) * NPARAMS + s], a3 = partial[(size_t)(b + 12) * NPARAMS + s];
+            acc += a0;
+            acc += a1;
+            acc += a2;
+            acc += a3;
+        }
+        for (; b < nblocks; b += 4) acc += partial[(size_t)b * NPARAMS + s];
+    }
+    part[wave][lane] = acc;
+    __syncthreads();
+    if (wave != 0 || s >= NPARAMS) return;
+    float gsum = ((part[0][lane] + part[1][lane]) + part[2][lane]) + part[3][lane];
+    const int i = slot_param(s);
     const bool pad = (i >= W4 && i < B4 && (i - W4) % NOUTP >= NOUT) || (i >= B4 + NOUT);
     if (pad) gsum = 0.0f;
     if (grad_out) grad_out[i] = gsum;
@@ -341,12 +427,12 @@ __global__ void reduce_adam_kernel(const float *__restrict__ partial, uint32_t n
     p[i] = p[i] - lr * mh / (sqrtf(vh) + 1e-8f);
 }
 
-__global__ void loss_sum_kernel(const float *__restrict__ partial_loss, uint32_t nblocks, float *__restrict__ out) {
-    if (threadIdx.x == 0 && blockIdx.x == 0) {
-        float s = 0.0f;
-        for (uint32_t b = 0; b < nblocks; b++) s += partial_loss[b];
-        *out = s;
-    }
+__global__ __launch_bounds__(64) void loss_sum_kernel(const float *__restrict__ partial_loss, uint32_t nblocks,
+                                                      float *__restrict__ out) {
+    float s = 0.0f;
+    for (uint32_t b = threadIdx.x; b < nblocks; b += 64) s += partial_loss[b];
+    for (int off = 32; off > 0; off >>= 1) s += __shfl_down(s, off, 64);
+    if (threadIdx.x == 0) *out = s;
 }
 
 uint64_t splitmix(uint64_t &x) {
@@ -407,7 +493,7 @@ int train(slimrt_nrc *c, const float *raw, const float *targets, uint32_t n, boo
     const uint32_t blocks = (n + 63) / 64;
     int rc = ensure_partials(c, blocks);
     if (rc) return rc;
-    hipLaunchKernelGGL(train_kernel, dim3(blocks), dim3(64), 0, c->stream, c->d_fwd, c->d_bwd, c->d_p, raw, targets, n,
+    hipLaunchKernelGGL(train_kernel, dim3(blocks), dim3(256), 0, c->stream, c->d_fwd, c->d_bwd, c->d_p, raw, targets, n,
                        n_dev, cap, offset, total, c->center, c->inv_half, c->d_partial, c->d_ploss);
     NRC_TRY(c, hipGetLastError());
     float c1 = 1.0f, c2 = 1.0f;
@@ -416,7 +502,7 @@ int train(slimrt_nrc *c, const float *raw, const float *targets, uint32_t n, boo
         c1 = (float)(1.0 - pow(0.9, c->step));
         c2 = (float)(1.0 - pow(0.99, c->step));
     }
-    hipLaunchKernelGGL(reduce_adam_kernel, dim3((NPARAMS + 255) / 256), dim3(256), 0, c->stream, c->d_partial, blocks,
+    hipLaunchKernelGGL(reduce_adam_kernel, dim3((NPARAMS + 63) / 64), dim3(256), 0, c->stream, c->d_partial, blocks,
                        c->d_p, c->d_m, c->d_v, grad_host ? c->d_grad : nullptr, adam ? 1 : 0, lr, c1, c2);
     NRC_TRY(c, hipGetLastError());
     if (adam && (rc = pack(c))) return rc;

@@ -126,6 +126,8 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-neural-cache", action="store_true")
     ap.add_argument("--nrc-frames", type=int, default=300)
+    ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
+                    help="also run a plain device-to-device copy of this many MiB (HBM counter calibration)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU only; --gpus must be 1"}))
@@ -146,6 +148,13 @@ def main():
     torch.cuda.set_stream(stream)
     r.set_stream(stream.cuda_stream)
 
+    if args.calibrate_copy:
+        src = torch.ones(args.calibrate_copy << 18, dtype=torch.float32, device="cuda")
+        dst = torch.empty_like(src)
+        for _ in range(3):   # an elementwise kernel: reads and writes exactly MIB MiB (a blit copy is not counted)
+            torch.mul(src, 2.0, out=dst)
+        torch.cuda.synchronize()
+        del src, dst
     cnt = r.count_work(p)
     for _ in range(args.warmup):
         r.render_device(p, pixels.data_ptr(), depths.data_ptr())

@@ -16,8 +16,7 @@
 #include <cstring>
 #include <cstdio>
 
-#include "slimrt_nrc_trace.h"
-#include "../../include/slimrt.h"
+#include "slimrt_context.h"
 #include "../../include/slimrt_nrc.h"
 
 using namespace slimrt;
@@ -31,8 +30,6 @@ static_assert(sizeof(RenderParams) == sizeof(slimrt_render_params), "render para
 
 namespace {
 
-constexpr uint32_t kBlock = 256;      // 4 waves of 64
-constexpr uint32_t kTile = 16;        // 16x16 samples per workgroup, 8x8 per wave
 
 template <bool kCount>
 __global__ __launch_bounds__(kBlock) void render_kernel(SceneView s, RenderParams rp, float4 *__restrict__ pixels,
@@ -170,176 +167,9 @@ __global__ void nrc_gate_fold_kernel(float *__restrict__ cells, uint32_t ncells)
     }
 }
 
-template <typename T>
-struct DevBuf {
-    T *ptr = nullptr;
-    size_t cap = 0;   // elements
-};
-
 }  // namespace
 
-struct slimrt_context {
-    bool cpu = false;
-    int device = 0;
-    hipStream_t stream = nullptr, own_stream = nullptr;
-    hipEvent_t ev0 = nullptr, ev1 = nullptr;
-    bool timed = false;
-    std::string err;
-
-    // host mirror of the device arrays (the CPU path renders from these)
-    std::vector<Geo> geos;
-    std::vector<Mat> mats;
-    std::vector<Light> lights;
-    std::vector<Node> snodes, mnodes;
-    std::vector<uint32_t> leaf_ids;
-    std::vector<MeshRec> meshes;
-    std::vector<TriHit> tri_hit;
-    std::vector<TriAttr> tri_attr;
-    std::vector<TexRec> textures;
-    std::vector<MipRec> mips;
-    std::vector<uint8_t> quads;
-    uint32_t scene_height = 0, mesh_height = 0, scene_flags = 0;
-    bool loaded = false;
-
-    DevBuf<Geo> d_geos;
-    DevBuf<Mat> d_mats;
-    DevBuf<Light> d_lights;
-    DevBuf<Node> d_snodes, d_mnodes;
-    DevBuf<uint32_t> d_leaf;
-    DevBuf<MeshRec> d_meshes;
-    DevBuf<TriHit> d_tri_hit;
-    DevBuf<TriAttr> d_tri_attr;
-    DevBuf<TexRec> d_tex;
-    DevBuf<MipRec> d_mips;
-    DevBuf<uint8_t> d_quads;
-    DevBuf<float4> d_pixels;
-    DevBuf<float> d_depths;
-    uint32_t *d_status = nullptr;
-    unsigned long long *d_counters = nullptr;
-
-    // rendering with the cache: queues, training samples, gate table, frame counters
-    float *q_raw = nullptr, *q_prefix = nullptr, *q_tp = nullptr, *q_out = nullptr;
-    uint32_t *q_pixel = nullptr, q_cap = 0;
-    float *t_raw = nullptr, *t_target = nullptr, *t_next = nullptr;
-    uint32_t *t_link = nullptr, t_cap = 0;
-    uint32_t *nrc_counters = nullptr;
-    float *nrc_losses = nullptr;
-    uint32_t nrc_last_steps = 0;
-    float *cells = nullptr;
-    uint32_t cells_log2 = 0;
-};
-
 namespace {
-
-int fail(slimrt_context *c, int code, const std::string &msg) {
-    if (c) c->err = msg;
-    return code;
-}
-
-#define HIP_TRY(ctx, expr)                                                                         \
-    do {                                                                                           \
-        hipError_t e_ = (expr);                                                                    \
-        if (e_ != hipSuccess)                                                                      \
-            return fail(ctx, e_ == hipErrorOutOfMemory ? SLIMRT_E_MEMORY : SLIMRT_E_DEVICE,        \
-                        std::string(#expr) + ": " + hipGetErrorString(e_));                        \
-    } while (0)
-
-template <typename T>
-int dev_upload(slimrt_context *c, DevBuf<T> &b, const T *src, size_t n) {
-    if (n == 0) n = 1;   // keep a valid pointer for empty arrays
-    if (b.cap < n) {
-        if (b.ptr) HIP_TRY(c, hipFree(b.ptr));
-        b.ptr = nullptr;
-        b.cap = 0;
-        HIP_TRY(c, hipMalloc(&b.ptr, n * sizeof(T)));
-        b.cap = n;
-    }
-    if (src) HIP_TRY(c, hipMemcpyAsync(b.ptr, src, n * sizeof(T), hipMemcpyHostToDevice, c->stream));
-    return SLIMRT_OK;
-}
-
-template <typename T>
-void dev_free(DevBuf<T> &b) {
-    if (b.ptr) (void)hipFree(b.ptr);
-    b.ptr = nullptr;
-    b.cap = 0;
-}
-
-Geo to_geo(const slimrt_geometry &g) {
-    Geo o;
-    o.qx = g.orientation[0]; o.qy = g.orientation[1]; o.qz = g.orientation[2]; o.qw = g.orientation[3];
-    o.px = g.position[0]; o.py = g.position[1]; o.pz = g.position[2]; o.type = g.type;
-    o.sx = g.scale[0]; o.sy = g.scale[1]; o.sz = g.scale[2]; o.material_id = g.material_id;
-    o.isx = 1.0f / g.scale[0]; o.isy = 1.0f / g.scale[1]; o.isz = 1.0f / g.scale[2];   // Ray::localize (ray.h:30)
-    o.id_flags = (g.id << 8) | (g.flags & 0xFFu);
-    return o;
-}
-
-Mat to_mat(const slimrt_material &m) {
-    Mat o;
-    o.ar = m.albedo[0]; o.ag = m.albedo[1]; o.ab = m.albedo[2]; o.roughness = m.roughness;
-    o.er = m.emission[0]; o.eg = m.emission[1]; o.eb = m.emission[2]; o.ior = m.ior;
-    o.fr = m.reflectivity[0]; o.fg = m.reflectivity[1]; o.fb = m.reflectivity[2]; o.metalness = m.metalness;
-    o.uv_ru = m.uv_repeat[0]; o.uv_rv = m.uv_repeat[1]; o.normal_magnitude = m.normal_magnitude;
-    o.packed = (m.flags & 0xFFu) | ((m.texture_count & 0xFu) << 8) | ((m.brdf & 0xFu) << 12) |
-               ((m.texture_ids[0] & 0xFFu) << 16) | ((m.texture_ids[1] & 0xFFu) << 24);
-    return o;
-}
-
-Light to_light(const slimrt_light &l) {
-    Light o;
-    o.r = l.color[0]; o.g = l.color[1]; o.b = l.color[2]; o.intensity = l.intensity;
-    o.x = l.position_or_direction[0]; o.y = l.position_or_direction[1]; o.z = l.position_or_direction[2];
-    o.flags = l.flags;
-    return o;
-}
-
-Node to_node(const slimrt_bvh_node &n) {   // asset layout -> device layout (bounds interleaved with indices)
-    Node o;
-    o.minx = n.min[0]; o.miny = n.min[1]; o.minz = n.min[2]; o.first_index = n.first_index;
-    o.maxx = n.max[0]; o.maxy = n.max[1]; o.maxz = n.max[2];
-    o.leaf_count_depth = (uint32_t)n.leaf_count | ((uint32_t)n.depth << 16) | ((uint32_t)n.flags << 24);
-    return o;
-}
-
-SceneView host_view(const slimrt_context *c) {
-    SceneView v;
-    v.geos = c->geos.data(); v.mats = c->mats.data(); v.lights = c->lights.data();
-    v.snodes = c->snodes.data(); v.leaf_ids = c->leaf_ids.data(); v.meshes = c->meshes.data();
-    v.mnodes = c->mnodes.data(); v.tri_hit = c->tri_hit.data(); v.tri_attr = c->tri_attr.data();
-    v.textures = c->textures.data(); v.mips = c->mips.data(); v.quads = c->quads.data();
-    v.G = (uint32_t)c->geos.size(); v.M = (uint32_t)c->mats.size(); v.L = (uint32_t)c->lights.size();
-    v.NM = (uint32_t)c->meshes.size(); v.NT = (uint32_t)c->textures.size(); v.flags = c->scene_flags;
-    return v;
-}
-
-SceneView device_view(const slimrt_context *c) {
-    SceneView v = host_view(c);
-    v.geos = c->d_geos.ptr; v.mats = c->d_mats.ptr; v.lights = c->d_lights.ptr;
-    v.snodes = c->d_snodes.ptr; v.leaf_ids = c->d_leaf.ptr; v.meshes = c->d_meshes.ptr;
-    v.mnodes = c->d_mnodes.ptr; v.tri_hit = c->d_tri_hit.ptr; v.tri_attr = c->d_tri_attr.ptr;
-    v.textures = c->d_tex.ptr; v.mips = c->d_mips.ptr; v.quads = c->d_quads.ptr;
-    return v;
-}
-
-void update_scene_flags(slimrt_context *c) {   // Scene::Scene (scene.h:239-242)
-    c->scene_flags = 0;
-    for (const Geo &g : c->geos)
-        if (g.type == GT_Quad && g.material_id < c->mats.size() && (mat_flags(c->mats[g.material_id]) & MAT_EMISSIVE))
-            c->scene_flags = 1;
-}
-
-int validate_params(slimrt_context *c, const slimrt_render_params *p) {
-    if (!p) return fail(c, SLIMRT_E_ARG, "params is NULL");
-    if (!c->loaded) return fail(c, SLIMRT_E_STATE, "no scene uploaded");
-    if (p->width == 0 || p->height == 0 || p->width > 65535 || p->height > 65535)
-        return fail(c, SLIMRT_E_ARG, "canvas dimensions out of range");
-    if (p->render_mode > 5 || p->output > 2 || p->max_depth > 255) return fail(c, SLIMRT_E_ARG, "invalid settings");
-    const int32_t nt = (int32_t)c->textures.size();
-    if (p->skybox_color >= nt || p->skybox_radiance >= nt || p->skybox_irradiance >= nt)
-        return fail(c, SLIMRT_E_ARG, "skybox texture id out of range");
-    return SLIMRT_OK;
-}
 
 int launch(slimrt_context *c, const slimrt_render_params *p, float4 *pixels, float *depths, bool count) {
     RenderParams rp;
@@ -479,18 +309,6 @@ int ensure_nrc(slimrt_context *c, uint32_t q_cap, uint32_t t_cap, uint32_t cells
     return SLIMRT_OK;
 }
 
-int check_status(slimrt_context *c) {
-    uint32_t st = 0;
-    HIP_TRY(c, hipMemcpyAsync(&st, c->d_status, sizeof st, hipMemcpyDeviceToHost, c->stream));
-    HIP_TRY(c, hipStreamSynchronize(c->stream));
-    if (st) {
-        HIP_TRY(c, hipMemsetAsync(c->d_status, 0, sizeof st, c->stream));
-        HIP_TRY(c, hipStreamSynchronize(c->stream));
-        return fail(c, SLIMRT_E_OVERFLOW, "traversal stack overflow");
-    }
-    return SLIMRT_OK;
-}
-
 }  // namespace
 
 extern "C" {
@@ -532,6 +350,7 @@ int slimrt_destroy(slimrt_context *c) {
         dev_free(c->d_leaf); dev_free(c->d_meshes); dev_free(c->d_tri_hit); dev_free(c->d_tri_attr); dev_free(c->d_tex);
         dev_free(c->d_mips); dev_free(c->d_quads); dev_free(c->d_pixels); dev_free(c->d_depths);
         free_nrc(c);
+        dev_free(c->d_geo_stage);
         if (c->d_status) (void)hipFree(c->d_status);
         if (c->d_counters) (void)hipFree(c->d_counters);
         if (c->ev0) (void)hipEventDestroy(c->ev0);

@@ -69,6 +69,13 @@ class RenderParams(C.Structure):
                 ("output", u32), ("flags", u32), ("mip_level_colors", (f32 * 3) * 9)]
 
 
+class ViewsConfig(C.Structure):
+    _fields_ = [("tile_shape", u32), ("blocks_per_cu", u32), ("tiles_per_grab", u32), ("persistent", u32)]
+
+
+SCENE_REFIT, SCENE_REBUILD = 0, 1
+
+
 # RayTracingRenderer's mip_level_colors (src/slim/renderer/renderer.h:50-58) as Color(ColorID) values
 MIP_LEVEL_COLORS = [(1.0, 0.5, 0.5), (0.75, 0.75, 0.0), (0.5, 1.0, 0.5), (0.75, 0.0, 0.75), (0.0, 0.75, 0.75),
                     (0.5, 0.5, 1.0), (0.75, 0.75, 0.75), (0.5, 0.5, 0.5), (0.25, 0.25, 0.25)]
@@ -98,6 +105,12 @@ def lib():
             "slimrt_host_world_aabbs": [vp, u32, vp, u32, vp],
             "slimrt_host_build_bvh": [vp, u32, u32, vp, P(u32), P(u32), vp],
             "slimrt_host_build_mesh": [vp, u32, vp, vp, u32, vp, vp, u32, vp, u32, u32, vp, P(u32), P(u32), vp, vp],
+            "slimrt_update_scene_device": [vp, vp, u32, u32],
+            "slimrt_download_scene_bvh": [vp, vp, u32, vp, u32, P(u32)],
+            "slimrt_views_create": [vp, vp, u32, vp, vp, P(vp)], "slimrt_views_destroy": [vp],
+            "slimrt_views_configure": [vp, P(ViewsConfig)], "slimrt_views_get_config": [vp, P(ViewsConfig)],
+            "slimrt_views_frame": [vp, vp, u32, u32, u32], "slimrt_views_last_ms": [vp, P(f32)],
+            "slimrt_views_autotune": [vp, u32, P(ViewsConfig), P(f32)],
         }
         for name, args in sig.items():
             fn = getattr(L, name)
@@ -105,6 +118,8 @@ def lib():
             fn.restype = C.c_int
         L.slimrt_last_error.argtypes = [vp]
         L.slimrt_last_error.restype = C.c_char_p
+        L.slimrt_views_last_error.argtypes = [vp]
+        L.slimrt_views_last_error.restype = C.c_char_p
         _lib = L
     return _lib
 
@@ -345,6 +360,20 @@ class Renderer:
         names = ("rays", "pairs", "roots", "instances", "triangles", "mesh_hits", "taps", "pixels")
         return dict(zip(names, (int(v) for v in c)))
 
+    def update_scene_device(self, geo_f: np.ndarray, mode: int = SCENE_REFIT):
+        """New transforms for every geometry; world AABBs and the scene BVH (refit or rebuild) on the device."""
+        g = geometries_from_spec(self.scene.spec, geo_f)
+        _check(lib().slimrt_update_scene_device(self.ctx, _ptr(g), len(g), mode), self.ctx)
+        self._last_geometries = g   # the copy is asynchronous: keep the source alive until the next call
+
+    def download_scene_bvh(self):
+        n_cap = 2 * len(self.scene.geometries) + 1
+        nodes = np.zeros(n_cap, BVH_NODE)
+        leaf = np.zeros(len(self.scene.geometries), np.uint32)
+        n = u32()
+        _check(lib().slimrt_download_scene_bvh(self.ctx, _ptr(nodes), n_cap, _ptr(leaf), len(leaf), C.byref(n)), self.ctx)
+        return nodes[:n.value], leaf
+
     def update_geometries(self, geo_f: np.ndarray):
         """Per-frame transform update: host AABBs + SAH rebuild, then uploadGeometries + uploadSceneBVH."""
         sc = self.scene
@@ -352,3 +381,60 @@ class Renderer:
         _check(lib().slimrt_upload_geometries(self.ctx, _ptr(sc.geometries), len(sc.geometries)), self.ctx)
         _check(lib().slimrt_upload_scene_bvh(self.ctx, _ptr(sc.scene_nodes), len(sc.scene_nodes),
                                              _ptr(sc.leaf_ids), sc.scene_height), self.ctx)
+
+
+class Views:
+    """Many camera views of a Renderer's scene traced in one launch (include/slimrt_views.h).
+    pixels/depths: one (device pointer) pair per view, e.g. torch tensors' data_ptr()."""
+
+    def __init__(self, renderer: Renderer, params: list, pixels_ptrs: list, depths_ptrs: list):
+        self.r = renderer
+        self.params = (RenderParams * len(params))(*params)
+        self._px = (vp * len(params))(*[vp(p) for p in pixels_ptrs])
+        self._dp = (vp * len(params))(*[vp(p) for p in depths_ptrs])
+        self.h = vp()
+        _check(lib().slimrt_views_create(renderer.ctx, C.cast(self.params, vp), len(params), C.cast(self._px, vp),
+                                         C.cast(self._dp, vp), C.byref(self.h)), renderer.ctx)
+        self._geos = None
+
+    def _check(self, rc):
+        if rc != 0:
+            raise SlimrtError(rc, lib().slimrt_views_last_error(self.h).decode())
+
+    def close(self):
+        if self.h:
+            lib().slimrt_views_destroy(self.h)
+            self.h = vp()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def configure(self, tile_shape=0, blocks_per_cu=2, tiles_per_grab=1, persistent=0):
+        cfg = ViewsConfig(tile_shape, blocks_per_cu, tiles_per_grab, persistent)
+        self._check(lib().slimrt_views_configure(self.h, C.byref(cfg)))
+
+    def config(self) -> dict:
+        cfg = ViewsConfig()
+        self._check(lib().slimrt_views_get_config(self.h, C.byref(cfg)))
+        return {k: getattr(cfg, k) for k, _ in ViewsConfig._fields_}
+
+    def frame(self, geo_f: np.ndarray | None = None, mode: int = SCENE_REFIT, graph: bool = False):
+        if geo_f is None:
+            self._check(lib().slimrt_views_frame(self.h, None, 0, mode, int(graph)))
+        else:
+            self._geos = geometries_from_spec(self.r.scene.spec, geo_f)
+            self._check(lib().slimrt_views_frame(self.h, _ptr(self._geos), len(self._geos), mode, int(graph)))
+
+    def last_ms(self) -> float:
+        ms = f32()
+        self._check(lib().slimrt_views_last_ms(self.h, C.byref(ms)))
+        return ms.value
+
+    def autotune(self, reps: int = 3):
+        cfg = ViewsConfig()
+        ms = f32()
+        self._check(lib().slimrt_views_autotune(self.h, reps, C.byref(cfg), C.byref(ms)))
+        return {k: getattr(cfg, k) for k, _ in ViewsConfig._fields_}, ms.value

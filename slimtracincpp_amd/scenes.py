@@ -569,3 +569,44 @@ def mesh_pbr_scene(seed: int = 0, width: int = 1920, height: int = 1080, max_dep
         mat_f=_stack(mats_f), mat_u=_stack(mats_u),
         light_f=np.zeros((0, 7), np.float32), light_u=np.zeros(0, np.uint32),
         meshes=[mesh], textures=tex)
+
+
+# ---------------------------------------------------------------- batched views (many cameras, moving instances)
+VIEW_RESOLUTIONS = ((128, 128), (320, 180), (640, 360), (1280, 720), (1920, 1080))
+
+
+def instance_transforms(spec: SceneSpec, rng: PCG32) -> np.ndarray:
+    """A new transform for every mesh instance of `spec` (the floor and other primitives keep theirs): Euler
+    angles U[0, 2pi)^3, scale U[0.7, 1.5], position x, z U[-3.5, 3.5] resting on the floor (y = 1.15 s).
+    Draw order per instance: scale, x, z, euler xyz."""
+    geo_f = spec.geo_f.copy()
+    for i in range(len(geo_f)):
+        if spec.geo_u[i][0] != GEO_MESH:
+            continue
+        s = rng.uniform(0.7, 1.5)
+        px, pz = rng.uniform(-3.5, 3.5), rng.uniform(-3.5, 3.5)
+        eul = (rng.uniform(0, 2 * math.pi), rng.uniform(0, 2 * math.pi), rng.uniform(0, 2 * math.pi))
+        geo_f[i, 0:3] = eul
+        geo_f[i, 3:6] = (px, s * 1.15, pz)
+        geo_f[i, 6:9] = (s, s, s)
+    return geo_f.astype(np.float32)
+
+
+def batched_views(count: int = 48, seed: int = 0, resolutions=VIEW_RESOLUTIONS):
+    """`count` cameras on spheres of radius U[6, 12] around the origin (uniform direction, upper hemisphere
+    kept above the floor), looking at the origin with a vertical FOV U[40, 90] degrees, at resolutions drawn
+    uniformly from `resolutions`.  Returns [(width, height, euler, position, focal_length)].
+    Draw order per view: resolution index, radius, direction (z = U[0.05, 1] height, azimuth U[0, 2pi)), FOV."""
+    rng = PCG32(seed)
+    out = []
+    for _ in range(count):
+        w, h = resolutions[min(int(rng.uniform() * len(resolutions)), len(resolutions) - 1)]
+        r = rng.uniform(6.0, 12.0)
+        up = rng.uniform(0.05, 1.0)
+        az = rng.uniform(0.0, 2 * math.pi)
+        horiz = math.sqrt(max(0.0, 1.0 - up * up))
+        pos = np.array([r * horiz * math.cos(az), r * up + 0.5, r * horiz * math.sin(az)], np.float32)
+        fov = rng.uniform(40.0, 90.0)
+        focal = np.float32(1.0 / math.tan(math.radians(fov) / 2))
+        out.append((w, h, look_at_euler(pos, (0.0, 0.5, 0.0)), pos, focal))
+    return out

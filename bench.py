@@ -19,6 +19,11 @@ Reported:
                for training; confidence gate over the last ~16 frames), each timed with HIP events around its
                own launches; PSNR and MAE of every cached frame against its plain frame.  A second pass puts the
                cache at the primary hit, where it replaces the most work.
+  batched_views 48 cameras (resolutions drawn from 128^2 .. 1920x1080) over the mesh + PBR scene; every frame
+               the 3 instances get new random transforms.  Graph form: the frame (transform copy, device scene
+               BVH rebuild, every view's tiles in one launch, launch shape from the autotuner) replayed as one
+               HIP graph; baseline: the host rebuild + upload and one launch per view.  Per-frame GPU latency
+               p50/p99 (HIP events) and aggregate Mrays/s (rays counted on the first frames' transforms).
 """
 from __future__ import annotations
 
@@ -36,6 +41,17 @@ sys.path.insert(0, ROOT)
 HBM_PEAK_GBS = 8000.0
 BYTES_PER = {"pairs": 64, "instances": 64, "triangles": 64, "mesh_hits": 64, "taps": 12}
 OUT_BYTES_PER_SAMPLE = 20
+
+
+def measured_traffic(width: int, height: int):
+    """HBM bytes per launch of the render kernel on this workload, from the committed rocprofv3 --pmc summary
+    (FETCH_SIZE + WRITE_SIZE, each scaled by the factor the same counter reads on a copy kernel of known size)."""
+    import re
+    path = os.path.join(ROOT, "profiles", f"render_kernel_mesh_pbr_{width}x{height}_traffic.md")
+    if not os.path.exists(path):
+        return None
+    m = re.search(r"traffic_bytes_per_launch:\s*([0-9]+)", open(path).read())
+    return int(m.group(1)) if m else None
 
 
 def algorithmic_bytes(cnt: dict) -> int:
@@ -64,22 +80,28 @@ def cpu_baseline(spec, rays_per_frame: int, threads: int) -> dict:
                       f"process wall {wall:.1f}s incl. its mesh BVH build"}
 
 
-def orbit_camera(frame: int, frames: int, radius: float = 8.246, height: float = 2.0):
-    """Seeded orbit around the origin starting at the scene's own camera position (0, 2, -8)."""
+def orbit_camera(frame: int, frames: int, radius: float = 8.246, height: float = 2.0, target=(0.0, 0.0, 0.0)):
+    """Orbit around `target` starting on its -z side (for the plain scene: its own camera position (0, 2, -8))."""
     import math
     import numpy as np
     from slimtracincpp_amd import scenes
     a = -math.pi / 2 + 2 * math.pi * frame / frames
-    pos = np.array([radius * math.cos(a), height, radius * math.sin(a)], np.float32)
-    return scenes.look_at_euler(pos, (0.0, 0.0, 0.0)), pos, scenes.FOV60_FOCAL
+    pos = np.array([target[0] + radius * math.cos(a), height, target[2] + radius * math.sin(a)], np.float32)
+    return scenes.look_at_euler(pos, target), pos, scenes.FOV60_FOCAL
 
 
-def neural_cache_leg(frames: int, width: int, height: int, stream, cache_depth: int) -> dict:
+def neural_cache_leg(frames: int, width: int, height: int, stream, cache_depth: int, spec=None) -> dict:
+    """spec None: the plain scene at width x height, depth 8, orbit radius 8.25; else that scene at depth 8 with
+    an orbit of radius 6.6 at height 2.4 around (0, 1, 0.4)."""
     import numpy as np
     import torch
     from slimtracincpp_amd import scenes, runtime
     from slimtracincpp_amd.nrc import CachedFrames
-    spec = scenes.implicit_scene(0, width, height, max_depth=8)
+    if spec is None:
+        spec = scenes.implicit_scene(0, width, height, max_depth=8)
+        orbit = dict(radius=8.246, height=2.0, target=(0.0, 0.0, 0.0))
+    else:
+        orbit = dict(radius=6.6, height=2.4, target=(0.0, 1.0, 0.4))
     r = runtime.Renderer(spec, "gpu")
     r.set_stream(stream.cuda_stream)
     n = width * height
@@ -91,7 +113,8 @@ def neural_cache_leg(frames: int, width: int, height: int, stream, cache_depth: 
                       gate_cells_log2=20)
     ms_plain, ms_nrc, psnr, mae, queried, fallback = [], [], [], [], [], []
     for f in range(frames):
-        p = r.params(mode=scenes.MODE_BEAUTY, output=runtime.OUTPUT_RADIANCE, camera=orbit_camera(f, frames))
+        p = r.params(mode=scenes.MODE_BEAUTY, output=runtime.OUTPUT_RADIANCE, max_depth=8, size=(width, height),
+                     camera=orbit_camera(f, frames, **orbit))
         r.render_device(p, plain.data_ptr(), depths.data_ptr())
         ms_plain.append(r.last_kernel_ms())
         cf.render_device(p, cached.data_ptr(), depths.data_ptr())
@@ -116,6 +139,62 @@ def neural_cache_leg(frames: int, width: int, height: int, stream, cache_depth: 
             "psnr_db_per_frame": [round(float(v), 2) for v in psnr]}
 
 
+def batched_views_leg(frames: int, stream, spec, n_views: int = 48) -> dict:
+    import numpy as np
+    import torch
+    from slimtracincpp_amd import scenes, runtime
+    r = runtime.Renderer(spec, "gpu")
+    r.set_stream(stream.cuda_stream)
+    cams = scenes.batched_views(n_views, 0)
+    params, bufs = [], []
+    for w, h, eul, pos, focal in cams:
+        p = r.params(camera=(eul, pos, focal), size=(w, h))
+        params.append(p)
+        bufs.append((torch.empty((w * h, 4), dtype=torch.float32, device="cuda"),
+                     torch.empty(w * h, dtype=torch.float32, device="cuda")))
+    pixels = sum(p.width * p.height for p in params)
+    views = runtime.Views(r, params, [b[0].data_ptr() for b in bufs], [b[1].data_ptr() for b in bufs])
+    cfg, tuned_ms = views.autotune(reps=2)
+    # rays per frame: instrumented passes of every view on the first frames' transforms
+    rng = scenes.PCG32(7)
+    transforms = [scenes.instance_transforms(spec, rng) for _ in range(frames)]
+    rays = []
+    for geo_f in transforms[:2]:
+        r.update_scene_device(geo_f, runtime.SCENE_REBUILD)
+        rays.append(sum(r.count_work(p)["rays"] for p in params))
+    rays_frame = float(np.mean(rays))
+    graph_ms = []
+    for i, geo_f in enumerate(transforms):
+        views.frame(geo_f, runtime.SCENE_REBUILD, graph=True)
+        graph_ms.append(views.last_ms())
+    r.synchronize()
+    # baseline: host rebuild + upload, then one launch per view
+    base_ms, base_wall = [], []
+    ev0, ev1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    for geo_f in transforms:
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        ev0.record(stream)
+        r.update_geometries(geo_f)
+        for p, (px, dp) in zip(params, bufs):
+            r.render_device(p, px.data_ptr(), dp.data_ptr())
+        ev1.record(stream)
+        torch.cuda.synchronize()
+        base_wall.append((time.perf_counter() - t0) * 1e3)
+        base_ms.append(ev0.elapsed_time(ev1))
+    r.synchronize()
+    views.close()
+    g = np.array(graph_ms[1:])   # the first replay includes the capture
+    b = np.array(base_ms[1:])
+    return {"views": n_views, "frames": frames, "pixels_per_frame": pixels, "rays_per_frame": rays_frame,
+            "launch_config": cfg, "autotune_ms": tuned_ms,
+            "graph_ms_p50": float(np.percentile(g, 50)), "graph_ms_p99": float(np.percentile(g, 99)),
+            "graph_mrays_per_s": rays_frame / (float(np.mean(g)) * 1e-3) / 1e6,
+            "per_view_ms_p50": float(np.percentile(b, 50)), "per_view_ms_p99": float(np.percentile(b, 99)),
+            "per_view_wall_ms_p50": float(np.percentile(base_wall[1:], 50)),
+            "per_view_mrays_per_s": rays_frame / (float(np.mean(b)) * 1e-3) / 1e6}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -126,6 +205,8 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-neural-cache", action="store_true")
     ap.add_argument("--nrc-frames", type=int, default=300)
+    ap.add_argument("--no-views", action="store_true")
+    ap.add_argument("--view-frames", type=int, default=20)
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
                     help="also run a plain device-to-device copy of this many MiB (HBM counter calibration)")
     args = ap.parse_args()
@@ -204,13 +285,16 @@ def main():
                    "primary_mrays_per_s": n / (ms_step * 1e-3) / 1e6,
                    "wall_ms_per_step": wall_ms, "mode": "plain (full-depth trace)"},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": measured_traffic(args.width, args.height),
                      "kernel": "render_kernel<false>", "kernel_ms": k_ms, "algorithmic_bytes": bytes_frame,
                      "counters": cnt},
         "neural_cache": None,
     }
+    if not args.no_views:
+        result["batched_views"] = batched_views_leg(args.view_frames, stream, spec)
     if not args.no_neural_cache:
         legs = [neural_cache_leg(args.nrc_frames, args.width, args.height, stream, d) for d in (2, 0)]
+        mesh_legs = [neural_cache_leg(args.nrc_frames, args.width, args.height, stream, d, spec) for d in (2, 0)]
         result["neural_cache"] = {
             "workload": f"implicit scene seed 0 at {args.width}x{args.height}, bounce depth 8 (plain judge), "
                         f"{args.nrc_frames}-frame orbit at radius 8.25 and height 2 around the origin",
@@ -220,7 +304,10 @@ def main():
                     "std <= 0.1 x mean",
             "timing": "median HIP-event time of each form's launches per frame (cache form includes inference, "
                       "training and the gate update); PSNR/MAE over the second half of the orbit",
-            "after_second_bounce": legs[0], "at_primary_hit": legs[1]}
+            "after_second_bounce": legs[0], "at_primary_hit": legs[1],
+            "mesh_pbr_depth8": {"workload": "the mesh + PBR scene at bounce depth 8, orbit radius 6.6 at height 2.4 "
+                                            "around (0, 1, 0.4)",
+                                "after_second_bounce": mesh_legs[0], "at_primary_hit": mesh_legs[1]}}
     if not args.no_cpu_baseline:
         threads = min(16, os.cpu_count() or 1)
         result["cpu_baseline"] = cpu_baseline(spec, rays, threads)

@@ -184,7 +184,10 @@ int launch(slimrt_context *c, const slimrt_render_params *p, float4 *pixels, flo
     if (lds > 160 * 1024) return fail(c, SLIMRT_E_ARG, "BVH too deep for the LDS traversal stacks");
     SceneView v = device_view(c);
     if (c->timed) HIP_TRY(c, hipEventRecord(c->ev0, c->stream));
-    if (count) {
+    if (!count && (rp.flags & SLIMRT_WAVEFRONT) && rp.render_mode == RM_Beauty && rp.output != SLIMRT_OUTPUT_HITS) {
+        int rc = launch_wavefront(c, rp, pixels, depths, (rp.flags & SLIMRT_WAVEFRONT_UNSORTED) == 0);
+        if (rc) return rc;
+    } else if (count) {
         hipLaunchKernelGGL(render_kernel<true>, dim3(per_xcd * 8), dim3(kBlock), lds, c->stream, v, rp, pixels, depths,
                            scene_depth, mesh_depth, tiles_x, tiles, per_xcd, c->d_status, c->d_counters);
     } else {
@@ -351,6 +354,8 @@ int slimrt_destroy(slimrt_context *c) {
         dev_free(c->d_mips); dev_free(c->d_quads); dev_free(c->d_pixels); dev_free(c->d_depths);
         free_nrc(c);
         dev_free(c->d_geo_stage);
+        for (void *q : {c->wf_queue_a, c->wf_queue_b, c->wf_accum, (void *)c->wf_counters})
+            if (q) (void)hipFree(q);
         if (c->d_status) (void)hipFree(c->d_status);
         if (c->d_counters) (void)hipFree(c->d_counters);
         if (c->ev0) (void)hipEventDestroy(c->ev0);

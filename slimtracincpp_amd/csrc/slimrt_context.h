@@ -62,6 +62,8 @@ struct slimrt_context {
     DevBuf<float4> d_pixels;
     DevBuf<float> d_depths;
     DevBuf<slimrt_geometry> d_geo_stage;   // transform updates on their way to the scene update kernel
+    void *wf_queue_a = nullptr, *wf_queue_b = nullptr, *wf_accum = nullptr;   // wavefront ray queues, colours
+    uint32_t *wf_counters = nullptr, wf_cap = 0;
     uint32_t *d_status = nullptr;
     unsigned long long *d_counters = nullptr;
 
@@ -189,6 +191,9 @@ inline int validate_params(slimrt_context *c, const slimrt_render_params *p) {
         return fail(c, SLIMRT_E_ARG, "skybox texture id out of range");
     return SLIMRT_OK;
 }
+
+// Beauty frame with the bounces as a wavefront (slimrt_wavefront.hip).
+int launch_wavefront(slimrt_context *c, const RenderParams &rp, float4 *pixels, float *depths, bool sort);
 
 inline int check_status(slimrt_context *c) {
     uint32_t st = 0;

@@ -120,9 +120,12 @@ struct RenderParams {
     uint32_t max_depth, render_mode;
     int32_t sky_color, sky_radiance, sky_irradiance;
     uint32_t output;             // 0 canvas Pixel (clamp(c)^2, 1), 1 radiance (tone-mapped colour, 1)
-    uint32_t ray_cones;          // 0: reference data flow (scaling factor overwritten on hit); 1: keep it
+    uint32_t flags;              // bit 0: keep the pixel's ray-cone scaling factor through hits (else the
+                                 // reference data flow, which overwrites it on every hit); bits 1-2: wavefront
     float mip_colors[9][3];
 };
+
+SLIM_HD bool keeps_ray_cones(const RenderParams &rp) { return (rp.flags & 1u) != 0; }
 
 // ------------------------------------------------------------------ small algebra
 struct v3 { float x, y, z; };

@@ -115,7 +115,7 @@ __device__ __forceinline__ bool render_beauty_nrc(const SceneView &s, const Rend
         sf.geo = scene_trace(s, ray, hit, false, INFINITY, tr);
         bool bounced = false;
         if (sf.geo >= 0) {
-            hit.scaling_factor = rp.ray_cones ? scaling : 0.0f;
+            hit.scaling_factor = keeps_ray_cones(rp) ? scaling : 0.0f;
             const Geo g = s.geos[sf.geo];
             const Mat m = s.mats[g.material_id];
             if ((mat_flags(m) & MAT_EMISSIVE) && g.type == GT_Quad && !hit.from_behind) {

@@ -491,7 +491,7 @@ SLIM_HD void render_beauty(const SceneView &s, const RenderParams &rp, v3 direct
         cur = cs(0);
         sf.geo = scene_trace(s, ray, hit, false, INFINITY, tr);
         if (sf.geo >= 0) {
-            hit.scaling_factor = rp.ray_cones ? scaling : 0.0f;
+            hit.scaling_factor = keeps_ray_cones(rp) ? scaling : 0.0f;
             const Geo g = s.geos[sf.geo];
             const Mat m = s.mats[g.material_id];
             if ((mat_flags(m) & MAT_EMISSIVE) && g.type == GT_Quad && !hit.from_behind) {
@@ -545,7 +545,7 @@ SLIM_HD void render_debug(const SceneView &s, const RenderParams &rp, v3 directi
     ray_reset(ray, v3{p.camera_position[0], p.camera_position[1], p.camera_position[2]}, normalized(direction));
     sf.geo = scene_trace(s, ray, hit, false, INFINITY, tr);
     if (sf.geo >= 0) {
-        hit.scaling_factor = rp.ray_cones ? scaling : 0.0f;
+        hit.scaling_factor = keeps_ray_cones(rp) ? scaling : 0.0f;
         prepare_for_shading(sf, s, ray, hit, tr);
         depth = depth_at(p, hit.position);
         switch (rp.render_mode) {

@@ -1,0 +1,306 @@
+// Glass and mirror bounces as a wavefront pipeline (render params flag SLIMRT_WAVEFRONT).
+//
+// The per-sample megakernel runs every path to its end inside one thread, so the few paths that keep bouncing
+// through glass and mirrors hold their whole wave while the other lanes idle.  Here the first iteration of the
+// reference integrator (renderPixelBeauty's loop, src/slim/renderer/ray_tracer.h:42-105) runs for every sample
+// in the tile-shaped primary kernel; paths that bounce are appended to a ray queue - one atomic per wave,
+// lanes placed by ballot and popcount - and every further iteration is one launch over the compacted queue, so
+// every wave is full of live paths.  Between iterations the queue is sorted into the 8 direction octants
+// (counting sort: per-bin counts, prefix, scatter), which keeps a wave's rays heading the same way through the
+// BVHs.  A path's colour accumulates in a per-sample buffer in the integrator's own order, so every sample is
+// bit-identical to the megakernel's.
+//
+// A path in flight is 48 bytes: the bounced ray's origin and direction (as ray_reset leaves them), throughput,
+// the pixel's ray-cone scaling factor, the remaining depth and the canvas offset.
+#include <algorithm>
+#include <string>
+
+#include "slimrt_context.h"
+
+namespace {
+
+struct PathState {
+    float ox, oy, oz, dx, dy, dz;
+    float tr, tg, tb, scaling;
+    uint32_t depth_left, pixel;
+};
+static_assert(sizeof(PathState) == 48, "path state layout");
+
+enum : uint32_t { WF_COUNT_A = 0, WF_COUNT_B = 1, WF_BINS = 2, WF_CURSORS = 10, WF_COUNTERS = 18 };
+
+// One iteration of the integrator loop (the body of render_beauty, in its order); returns whether the path
+// continues, with ray, throughput and depth_left advanced to the bounce.
+__device__ __forceinline__ bool beauty_iteration(const SceneView &s, const RenderParams &rp, Ray &ray, float scaling,
+                                                 uint32_t &depth_left, color &tp, color &col, float &depth, Tracer &tr) {
+    Surface sf;
+    Hit hit;
+    hit.scaling_factor = scaling;
+    color cur = cs(0), next_tp = cs(0);
+    bool bounced = false;
+    sf.geo = scene_trace(s, ray, hit, false, INFINITY, tr);
+    if (sf.geo >= 0) {
+        hit.scaling_factor = keeps_ray_cones(rp) ? scaling : 0.0f;
+        const Geo g = s.geos[sf.geo];
+        const Mat m = s.mats[g.material_id];
+        if ((mat_flags(m) & MAT_EMISSIVE) && g.type == GT_Quad && !hit.from_behind) {
+            depth_left = 0;
+            cur = color{m.er, m.eg, m.eb};
+        } else {
+            prepare_for_shading(sf, s, ray, hit, tr);
+            if (depth_left == rp.max_depth) depth = depth_at(rp.proj, hit.position);
+            for (uint32_t i = 0; i < s.L; i++) shade_from_light(sf, s.lights[i], s, tr, cur);
+            if (s.flags & 1u) shade_from_emissive_quads(sf, s, cur);
+            if (rp.sky_irradiance >= 0 && rp.sky_radiance >= 0) {
+                sf.L = sf.N;
+                sf.NdotL = 1.0f;
+                color D = tex_sample_cube(s, (uint32_t)rp.sky_irradiance, sf.N.x, sf.N.y, sf.N.z);
+                color S = tex_sample_cube(s, (uint32_t)rp.sky_radiance, sf.R.x, sf.R.y, sf.R.z);
+                radiance_fraction(sf);
+                cur = cmul_add(D, sf.Fd, cmul_add(sf.Fs, S, cur));
+            }
+            if ((mat_flags(sf.mat) & (MAT_REFLECTIVE | MAT_REFRACTIVE)) && --depth_left) {
+                sf.F = schlick(clamp01(dot(sf.N, sf.R)), color{sf.mat.fr, sf.mat.fg, sf.mat.fb});
+                next_tp = sf.refracted ? one_minus(sf.F) : sf.F;
+                ray_reset(ray, hit.position, sf.RF);
+                bounced = true;
+            } else depth_left = 0;
+        }
+    } else {
+        depth_left = 0;
+        if (rp.sky_color >= 0) cur = tex_sample_cube(s, (uint32_t)rp.sky_color, ray.direction.x, ray.direction.y, ray.direction.z);
+    }
+    for (uint32_t i = 0; i < s.L; i++) {
+        const Light l = s.lights[i];
+        float density;
+        if (light_glow(l, ray, hit.distance, density)) cur = cscale_add(color{l.r, l.g, l.b}, glow_factor(density), cur);
+    }
+    col = cmul_add(cur, tp, col);
+    tp = tp * next_tp;
+    return bounced;
+}
+
+__device__ __forceinline__ void finish_sample(const RenderParams &rp, color col, float4 *pixels, uint32_t o) {
+    float px[4];
+    output_value(rp, color{tone_map(col.r), tone_map(col.g), tone_map(col.b)}, px);
+    pixels[o] = make_float4(px[0], px[1], px[2], px[3]);
+}
+
+__device__ __forceinline__ void push_path(PathState *q, uint32_t *count, uint32_t cap, bool cont, const Ray &ray,
+                                          color tp, float scaling, uint32_t depth_left, uint32_t pixel, color col,
+                                          float4 *accum, uint32_t *status) {
+    const uint32_t slot = wave_append(cont, count);
+    if (!cont) return;
+    if (slot >= cap) {   // cannot happen: one entry per sample at most
+        atomicOr(status, 2u);
+        return;
+    }
+    PathState st;
+    st.ox = ray.origin.x; st.oy = ray.origin.y; st.oz = ray.origin.z;
+    st.dx = ray.direction.x; st.dy = ray.direction.y; st.dz = ray.direction.z;
+    st.tr = tp.r; st.tg = tp.g; st.tb = tp.b; st.scaling = scaling;
+    st.depth_left = depth_left;
+    st.pixel = pixel;
+    q[slot] = st;
+    accum[pixel] = make_float4(col.r, col.g, col.b, 0.0f);
+}
+
+__global__ void wf_reset_kernel(uint32_t *__restrict__ counters) {
+    if (threadIdx.x < WF_COUNTERS) counters[threadIdx.x] = 0u;
+}
+
+// Iteration 1 for every sample, tiles as in the render kernel.
+__global__ __launch_bounds__(kBlock) void wf_primary_kernel(SceneView s, RenderParams rp, float4 *__restrict__ pixels,
+                                                            float *__restrict__ depths, float4 *__restrict__ accum,
+                                                            PathState *__restrict__ queue, uint32_t *__restrict__ counters,
+                                                            uint32_t cap, uint32_t scene_depth, uint32_t mesh_depth,
+                                                            uint32_t tiles_x, uint32_t tiles, uint32_t tiles_per_xcd,
+                                                            uint32_t *__restrict__ status) {
+    extern __shared__ uint32_t lds[];
+    const uint32_t xcd = blockIdx.x & 7u, slot = blockIdx.x >> 3;
+    const uint32_t tile = xcd * tiles_per_xcd + slot;
+    if (tile >= tiles) return;
+    const uint32_t t = threadIdx.x, wave = t >> 6, lane = t & 63u;
+    const uint32_t x = (tile % tiles_x) * kTile + (wave & 1u) * 8u + (lane & 7u);
+    const uint32_t y = (tile / tiles_x) * kTile + (wave >> 1) * 8u + (lane >> 3);
+    const uint32_t ws = rp.width << rp.ssaa, hs = rp.height << rp.ssaa;
+    Tracer tr;
+    tr.scene_stack = Stack{lds + t, kBlock};
+    tr.mesh_stack = Stack{lds + scene_depth * kBlock + t, kBlock};
+    tr.scene_depth = scene_depth;
+    tr.mesh_depth = mesh_depth;
+    tr.overflow = 0;
+    tr.cnt = nullptr;
+    const bool inside = x < ws && y < hs;
+    bool cont = false;
+    Ray ray;
+    color tp = cs(1.0f), col = cs(0);
+    float scaling = 0.0f, depth = INFINITY;
+    uint32_t depth_left = rp.max_depth, o = 0;
+    if (inside) {
+        v3 dir;
+        primary_ray(rp.proj, x, y, dir, scaling);
+        const Projection &p = rp.proj;
+        ray_reset(ray, v3{p.camera_position[0], p.camera_position[1], p.camera_position[2]}, normalized(dir));
+        if (depth_left) cont = beauty_iteration(s, rp, ray, scaling, depth_left, tp, col, depth, tr);
+        o = canvas_offset(rp, x, y);
+        depths[o] = depth;
+        if (!cont) finish_sample(rp, col, pixels, o);
+        if (tr.overflow) atomicOr(status, 1u);
+    }
+    push_path(queue, counters + WF_COUNT_A, cap, inside && cont, ray, tp, scaling, depth_left, o, col, accum, status);
+}
+
+// One further iteration over the queue `in` (count at in_count), appending continuing paths to `out`.
+__global__ __launch_bounds__(kBlock) void wf_bounce_kernel(SceneView s, RenderParams rp, float4 *__restrict__ pixels,
+                                                           float4 *__restrict__ accum, const PathState *__restrict__ in,
+                                                           const uint32_t *__restrict__ in_count, PathState *__restrict__ out,
+                                                           uint32_t *__restrict__ out_count, uint32_t cap,
+                                                           uint32_t scene_depth, uint32_t mesh_depth,
+                                                           uint32_t *__restrict__ status) {
+    extern __shared__ uint32_t lds[];
+    const uint32_t t = threadIdx.x;
+    Tracer tr;
+    tr.scene_stack = Stack{lds + t, kBlock};
+    tr.mesh_stack = Stack{lds + scene_depth * kBlock + t, kBlock};
+    tr.scene_depth = scene_depth;
+    tr.mesh_depth = mesh_depth;
+    tr.overflow = 0;
+    tr.cnt = nullptr;
+    const uint32_t n = min(*in_count, cap);
+    const uint32_t stride = gridDim.x * kBlock;
+    for (uint32_t base = blockIdx.x * kBlock; base < n; base += stride) {   // uniform trip count per workgroup
+        const uint32_t i = base + t;
+        const bool live = i < n;
+        bool cont = false;
+        Ray ray;
+        color tp = cs(0), col = cs(0);
+        float scaling = 0.0f, depth = 0.0f;
+        uint32_t depth_left = 0, pixel = 0;
+        if (live) {
+            const PathState st = in[i];
+            ray_reset(ray, v3{st.ox, st.oy, st.oz}, v3{st.dx, st.dy, st.dz});
+            tp = color{st.tr, st.tg, st.tb};
+            scaling = st.scaling;
+            depth_left = st.depth_left;
+            pixel = st.pixel;
+            const float4 a = accum[pixel];
+            col = color{a.x, a.y, a.z};
+            cont = beauty_iteration(s, rp, ray, scaling, depth_left, tp, col, depth, tr);
+            if (!cont) finish_sample(rp, col, pixels, pixel);
+        }
+        push_path(out, out_count, cap, live && cont, ray, tp, scaling, depth_left, pixel, col, accum, status);
+    }
+    if (tr.overflow) atomicOr(status, 1u);
+}
+
+__device__ __forceinline__ uint32_t octant(const PathState &st) {
+    return (st.dx < 0.0f ? 1u : 0u) | (st.dy < 0.0f ? 2u : 0u) | (st.dz < 0.0f ? 4u : 0u);
+}
+
+// Counting sort of the queue by direction octant: counts per bin (one atomic per wave and bin) ...
+__global__ __launch_bounds__(256) void wf_bin_count_kernel(const PathState *__restrict__ q, uint32_t *__restrict__ counters,
+                                                           uint32_t cap) {
+    const uint32_t n = min(counters[WF_COUNT_A], cap);
+    for (uint32_t base = blockIdx.x * 256; base < n; base += gridDim.x * 256) {
+        const uint32_t i = base + threadIdx.x;
+        const uint32_t key = i < n ? octant(q[i]) : 8u;
+#pragma unroll
+        for (uint32_t b = 0; b < 8; b++) (void)wave_append(key == b, counters + WF_BINS + b);
+    }
+}
+
+// ... exclusive prefix into the cursors; the sorted queue's count is the unsorted one's ...
+__global__ void wf_bin_scan_kernel(uint32_t *__restrict__ counters) {
+    if (threadIdx.x != 0) return;
+    uint32_t acc = 0;
+    for (uint32_t b = 0; b < 8; b++) {
+        counters[WF_CURSORS + b] = acc;
+        acc += counters[WF_BINS + b];
+        counters[WF_BINS + b] = 0u;
+    }
+    counters[WF_COUNT_B] = counters[WF_COUNT_A];
+}
+
+// ... and the scatter into queue B, which empties queue A for the next iteration's appends.
+__global__ __launch_bounds__(256) void wf_bin_scatter_kernel(const PathState *__restrict__ a, PathState *__restrict__ b,
+                                                             uint32_t *__restrict__ counters, uint32_t cap) {
+    const uint32_t n = min(counters[WF_COUNT_B], cap);
+    for (uint32_t base = blockIdx.x * 256; base < n; base += gridDim.x * 256) {
+        const uint32_t i = base + threadIdx.x;
+        PathState st;
+        uint32_t key = 8u;
+        if (i < n) {
+            st = a[i];
+            key = octant(st);
+        }
+#pragma unroll
+        for (uint32_t bin = 0; bin < 8; bin++) {
+            const uint32_t slot = wave_append(key == bin, counters + WF_CURSORS + bin);
+            if (key == bin && slot < cap) b[slot] = st;
+        }
+    }
+}
+
+__global__ void wf_clear_count_kernel(uint32_t *__restrict__ counters, uint32_t which) {
+    if (threadIdx.x == 0) counters[which] = 0u;
+}
+
+}  // namespace
+
+namespace slimrt {
+
+int launch_wavefront(slimrt_context *c, const RenderParams &rp, float4 *pixels, float *depths, bool sort) {
+    const uint32_t ws = rp.width << rp.ssaa, hs = rp.height << rp.ssaa;
+    const uint32_t samples = ws * hs;
+    if (c->wf_cap < samples) {
+        for (void *p : {(void *)c->wf_queue_a, (void *)c->wf_queue_b, (void *)c->wf_accum})
+            if (p) HIP_TRY(c, hipFree(p));
+        c->wf_queue_a = c->wf_queue_b = nullptr;
+        c->wf_accum = nullptr;
+        c->wf_cap = 0;
+        HIP_TRY(c, hipMalloc(&c->wf_queue_a, (size_t)samples * sizeof(PathState)));
+        HIP_TRY(c, hipMalloc(&c->wf_queue_b, (size_t)samples * sizeof(PathState)));
+        HIP_TRY(c, hipMalloc(&c->wf_accum, (size_t)samples * sizeof(float4)));
+        c->wf_cap = samples;
+    }
+    if (!c->wf_counters) HIP_TRY(c, hipMalloc(&c->wf_counters, WF_COUNTERS * sizeof(uint32_t)));
+    const uint32_t tiles_x = (ws + kTile - 1) / kTile, tiles_y = (hs + kTile - 1) / kTile;
+    const uint32_t tiles = tiles_x * tiles_y, per_xcd = (tiles + 7) / 8;
+    const uint32_t scene_depth = c->scene_height + 1, mesh_depth = c->mesh_height + 1;
+    const size_t lds = (size_t)(scene_depth + mesh_depth) * kBlock * sizeof(uint32_t);
+    if (lds > 160 * 1024) return fail(c, SLIMRT_E_ARG, "BVH too deep for the LDS traversal stacks");
+    int cus = 0;
+    HIP_TRY(c, hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, c->device));
+    const SceneView v = device_view(c);
+    PathState *qa = (PathState *)c->wf_queue_a, *qb = (PathState *)c->wf_queue_b;
+    float4 *accum = (float4 *)c->wf_accum;
+    uint32_t *cnt = c->wf_counters;
+    hipLaunchKernelGGL(wf_reset_kernel, dim3(1), dim3(64), 0, c->stream, cnt);
+    hipLaunchKernelGGL(wf_primary_kernel, dim3(per_xcd * 8), dim3(kBlock), lds, c->stream, v, rp, pixels, depths, accum,
+                       qa, cnt, samples, scene_depth, mesh_depth, tiles_x, tiles, per_xcd, c->d_status);
+    HIP_TRY(c, hipGetLastError());
+    const uint32_t bounce_blocks = (uint32_t)cus * 2, sort_blocks = (uint32_t)cus * 2;
+    uint32_t cur_count = WF_COUNT_A, next_count = WF_COUNT_B;   // unsorted: ping-pong between A and B
+    for (uint32_t it = 1; it < rp.max_depth; it++) {
+        if (sort) {   // A -> sorted B (A emptied), then B -> A
+            hipLaunchKernelGGL(wf_bin_count_kernel, dim3(sort_blocks), dim3(256), 0, c->stream, qa, cnt, samples);
+            hipLaunchKernelGGL(wf_bin_scan_kernel, dim3(1), dim3(64), 0, c->stream, cnt);
+            hipLaunchKernelGGL(wf_bin_scatter_kernel, dim3(sort_blocks), dim3(256), 0, c->stream, qa, qb, cnt, samples);
+            hipLaunchKernelGGL(wf_clear_count_kernel, dim3(1), dim3(64), 0, c->stream, cnt, (uint32_t)WF_COUNT_A);
+            hipLaunchKernelGGL(wf_bounce_kernel, dim3(bounce_blocks), dim3(kBlock), lds, c->stream, v, rp, pixels, accum,
+                               (const PathState *)qb, (const uint32_t *)(cnt + WF_COUNT_B), qa, cnt + WF_COUNT_A, samples,
+                               scene_depth, mesh_depth, c->d_status);
+        } else {
+            PathState *in = cur_count == WF_COUNT_A ? qa : qb, *out = cur_count == WF_COUNT_A ? qb : qa;
+            hipLaunchKernelGGL(wf_clear_count_kernel, dim3(1), dim3(64), 0, c->stream, cnt, next_count);
+            hipLaunchKernelGGL(wf_bounce_kernel, dim3(bounce_blocks), dim3(kBlock), lds, c->stream, v, rp, pixels, accum,
+                               (const PathState *)in, (const uint32_t *)(cnt + cur_count), out, cnt + next_count, samples,
+                               scene_depth, mesh_depth, c->d_status);
+            std::swap(cur_count, next_count);
+        }
+        HIP_TRY(c, hipGetLastError());
+    }
+    return SLIMRT_OK;
+}
+
+}  // namespace slimrt

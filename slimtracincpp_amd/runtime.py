@@ -74,6 +74,7 @@ class ViewsConfig(C.Structure):
 
 
 SCENE_REFIT, SCENE_REBUILD = 0, 1
+WAVEFRONT, WAVEFRONT_UNSORTED = 2, 4   # slimrt_render_params.flags, with RAY_CONES
 
 
 # RayTracingRenderer's mip_level_colors (src/slim/renderer/renderer.h:50-58) as Color(ColorID) values

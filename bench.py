@@ -238,11 +238,13 @@ def main():
         del src, dst
     cnt = r.count_work(p)
 
-    # Three forms of the same frame, bit-identical outputs (tests/test_wavefront.py): the per-sample megakernel
-    # and the bounces as a wavefront over compacted ray queues, unsorted or sorted by direction octant.  Each is
-    # timed over K back-to-back frames between synchronisations; the fastest is the reported path.
-    forms = {"megakernel": 0, "wavefront": runtime.WAVEFRONT | runtime.WAVEFRONT_UNSORTED,
-             "wavefront_sorted": runtime.WAVEFRONT}
+    # Five forms of the same frame, bit-identical outputs (tests/test_wavefront.py): the per-sample megakernel and
+    # the bounces as a wavefront over a compacted ray queue (run to the end in one launch, or re-compacted after
+    # every bounce; either sorted by direction octant or not).  Each is timed over K back-to-back frames between
+    # synchronisations; the fastest is the reported path.
+    forms = {"megakernel": 0, "wavefront": runtime.WAVEFRONT, "wavefront_sorted": runtime.WAVEFRONT | runtime.WAVEFRONT_SORT,
+             "wavefront_per_bounce": runtime.WAVEFRONT | runtime.WAVEFRONT_PER_BOUNCE,
+             "wavefront_per_bounce_sorted": runtime.WAVEFRONT | runtime.WAVEFRONT_PER_BOUNCE | runtime.WAVEFRONT_SORT}
     timings = {}
     for name, flags in forms.items():
         p.flags = flags
@@ -298,10 +300,14 @@ def main():
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": measured_traffic(args.width, args.height),
                      "kernel": {"megakernel": "render_kernel<false>",
-                                "wavefront": "the frame's launches: wf_primary_kernel + wf_bounce_kernel per bounce "
-                                             "(+ queue counter kernels)",
-                                "wavefront_sorted": "the frame's launches: wf_primary_kernel + per bounce octant "
-                                                    "sort (count, scan, scatter) + wf_bounce_kernel"}[form],
+                                "wavefront": "the frame's launches: wf_primary_kernel + one wf_bounce_kernel<true> "
+                                             "over the compacted bounce queue",
+                                "wavefront_sorted": "the frame's launches: wf_primary_kernel + octant sort (count, "
+                                                    "scan, scatter) + wf_bounce_kernel<true>",
+                                "wavefront_per_bounce": "the frame's launches: wf_primary_kernel + "
+                                                        "wf_bounce_kernel<false> per bounce",
+                                "wavefront_per_bounce_sorted": "the frame's launches: wf_primary_kernel + per bounce "
+                                                               "octant sort + wf_bounce_kernel<false>"}[form],
                      "kernel_ms": k_ms, "algorithmic_bytes": bytes_frame,
                      "counters": cnt},
         "neural_cache": None,

@@ -49,10 +49,12 @@ extern "C" {
 #define SLIMRT_RAY_CONES 1u        /* keep the primary ray's cone scaling through hits, for mip selection by ray cones
                                       (surface_shader.h:67-77); without it the hit record's copy in
                                       SceneTracer::hitGeometries (scene_tracer.h:129) zeroes it, as in the reference */
-#define SLIMRT_WAVEFRONT 2u        /* beauty frames: bounces as a queue-based wavefront (compacted ray queues sorted
-                                      by direction octant between bounces) instead of the per-sample loop; same
-                                      results bit for bit */
-#define SLIMRT_WAVEFRONT_UNSORTED 4u   /* with SLIMRT_WAVEFRONT: compact the queues but do not sort them */
+#define SLIMRT_WAVEFRONT 2u        /* beauty frames: after the first hit, the paths that bounce go to a compacted ray
+                                      queue and one launch runs each queued path to its end, instead of the
+                                      per-sample loop of one launch; same results bit for bit */
+#define SLIMRT_WAVEFRONT_PER_BOUNCE 4u /* with SLIMRT_WAVEFRONT: one launch per bounce, the queue compacted again
+                                          after every bounce */
+#define SLIMRT_WAVEFRONT_SORT 8u   /* with SLIMRT_WAVEFRONT: sort each queue by direction octant before its launch */
 
 typedef struct slimrt_context slimrt_context;
 
@@ -157,7 +159,7 @@ typedef struct {
     uint32_t render_mode;
     int32_t skybox_color, skybox_radiance, skybox_irradiance;   /* texture ids, -1 for none */
     uint32_t output;            /* SLIMRT_OUTPUT_* */
-    uint32_t flags;             /* SLIMRT_RAY_CONES | SLIMRT_WAVEFRONT | SLIMRT_WAVEFRONT_UNSORTED */
+    uint32_t flags;             /* SLIMRT_RAY_CONES | SLIMRT_WAVEFRONT (| _PER_BOUNCE | _SORT) */
     float mip_level_colors[9][3];
 } slimrt_render_params;
 

@@ -185,7 +185,8 @@ int launch(slimrt_context *c, const slimrt_render_params *p, float4 *pixels, flo
     SceneView v = device_view(c);
     if (c->timed) HIP_TRY(c, hipEventRecord(c->ev0, c->stream));
     if (!count && (rp.flags & SLIMRT_WAVEFRONT) && rp.render_mode == RM_Beauty && rp.output != SLIMRT_OUTPUT_HITS) {
-        int rc = launch_wavefront(c, rp, pixels, depths, (rp.flags & SLIMRT_WAVEFRONT_UNSORTED) == 0);
+        int rc = launch_wavefront(c, rp, pixels, depths, (rp.flags & SLIMRT_WAVEFRONT_PER_BOUNCE) != 0,
+                                  (rp.flags & SLIMRT_WAVEFRONT_SORT) != 0);
         if (rc) return rc;
     } else if (count) {
         hipLaunchKernelGGL(render_kernel<true>, dim3(per_xcd * 8), dim3(kBlock), lds, c->stream, v, rp, pixels, depths,

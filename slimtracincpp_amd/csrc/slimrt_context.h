@@ -193,7 +193,7 @@ inline int validate_params(slimrt_context *c, const slimrt_render_params *p) {
 }
 
 // Beauty frame with the bounces as a wavefront (slimrt_wavefront.hip).
-int launch_wavefront(slimrt_context *c, const RenderParams &rp, float4 *pixels, float *depths, bool sort);
+int launch_wavefront(slimrt_context *c, const RenderParams &rp, float4 *pixels, float *depths, bool per_bounce, bool sort);
 
 inline int check_status(slimrt_context *c) {
     uint32_t st = 0;

@@ -121,7 +121,7 @@ struct RenderParams {
     int32_t sky_color, sky_radiance, sky_irradiance;
     uint32_t output;             // 0 canvas Pixel (clamp(c)^2, 1), 1 radiance (tone-mapped colour, 1)
     uint32_t flags;              // bit 0: keep the pixel's ray-cone scaling factor through hits (else the
-                                 // reference data flow, which overwrites it on every hit); bits 1-2: wavefront
+                                 // reference data flow, which overwrites it on every hit); bits 1-3: wavefront
     float mip_colors[9][3];
 };
 

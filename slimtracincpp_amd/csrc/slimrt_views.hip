@@ -586,4 +586,35 @@ int slimrt_views_autotune(slimrt_views *v, uint32_t reps, slimrt_views_config *b
 
 const char *slimrt_views_last_error(const slimrt_views *v) { return v ? v->err.c_str() : "null views"; }
 
+int slimrt_render_autotune(slimrt_context *c, const slimrt_render_params *params, void *d_pixels, void *d_depths,
+                           uint32_t reps, uint32_t *best_flags, float *best_ms) {
+    if (!c || !params || !best_flags || reps == 0) return SLIMRT_E_ARG;
+    const uint32_t base = params->flags & ~(uint32_t)(SLIMRT_WAVEFRONT | SLIMRT_WAVEFRONT_PER_BOUNCE | SLIMRT_WAVEFRONT_SORT);
+    const uint32_t forms[5] = {0u, SLIMRT_WAVEFRONT, SLIMRT_WAVEFRONT | SLIMRT_WAVEFRONT_SORT,
+                               SLIMRT_WAVEFRONT | SLIMRT_WAVEFRONT_PER_BOUNCE,
+                               SLIMRT_WAVEFRONT | SLIMRT_WAVEFRONT_PER_BOUNCE | SLIMRT_WAVEFRONT_SORT};
+    slimrt_render_params p = *params;
+    float win_ms = INFINITY;
+    uint32_t win = base;
+    for (uint32_t f : forms) {
+        p.flags = base | f;
+        std::vector<float> t;
+        for (uint32_t r = 0; r <= reps; r++) {
+            int rc = slimrt_render_device(c, &p, d_pixels, d_depths);
+            if (rc) return rc;
+            float ms;
+            if ((rc = slimrt_last_kernel_ms(c, &ms))) return rc;
+            if (r) t.push_back(ms);
+        }
+        std::sort(t.begin(), t.end());
+        if (t[t.size() / 2] < win_ms) {
+            win_ms = t[t.size() / 2];
+            win = p.flags;
+        }
+    }
+    *best_flags = win;
+    if (best_ms) *best_ms = win_ms;
+    return check_status(c);
+}
+
 }  // extern "C"

@@ -3,12 +3,13 @@
 // The per-sample megakernel runs every path to its end inside one thread, so the few paths that keep bouncing
 // through glass and mirrors hold their whole wave while the other lanes idle.  Here the first iteration of the
 // reference integrator (renderPixelBeauty's loop, src/slim/renderer/ray_tracer.h:42-105) runs for every sample
-// in the tile-shaped primary kernel; paths that bounce are appended to a ray queue - one atomic per wave,
-// lanes placed by ballot and popcount - and every further iteration is one launch over the compacted queue, so
-// every wave is full of live paths.  Between iterations the queue is sorted into the 8 direction octants
-// (counting sort: per-bin counts, prefix, scatter), which keeps a wave's rays heading the same way through the
-// BVHs.  A path's colour accumulates in a per-sample buffer in the integrator's own order, so every sample is
-// bit-identical to the megakernel's.
+// in the tile-shaped primary kernel, which without the bounce loop needs fewer registers (3 waves per SIMD
+// instead of 2); paths that bounce are appended to a ray queue - one atomic per wave, lanes placed by ballot
+// and popcount - and the bounce kernel runs the compacted queue: by default each path to its end in one launch,
+// or (SLIMRT_WAVEFRONT_PER_BOUNCE) one iteration per launch with the survivors compacted again.  Optionally
+// (SLIMRT_WAVEFRONT_SORT) a queue is sorted into the 8 direction octants first (counting sort: per-bin counts,
+// prefix, scatter) so that a wave's rays head the same way through the BVHs.  A path's colour accumulates in a
+// per-sample buffer in the integrator's own order, so every sample is bit-identical to the megakernel's.
 //
 // A path in flight is 48 bytes: the bounced ray's origin and direction (as ray_reset leaves them), throughput,
 // the pixel's ray-cone scaling factor, the remaining depth and the canvas offset.
@@ -150,7 +151,9 @@ __global__ __launch_bounds__(kBlock) void wf_primary_kernel(SceneView s, RenderP
     push_path(queue, counters + WF_COUNT_A, cap, inside && cont, ray, tp, scaling, depth_left, o, col, accum, status);
 }
 
-// One further iteration over the queue `in` (count at in_count), appending continuing paths to `out`.
+// Further iterations over the queue `in` (count at in_count): with TO_END each path runs to its end; otherwise one
+// iteration, appending continuing paths to `out`.
+template <bool TO_END>
 __global__ __launch_bounds__(kBlock) void wf_bounce_kernel(SceneView s, RenderParams rp, float4 *__restrict__ pixels,
                                                            float4 *__restrict__ accum, const PathState *__restrict__ in,
                                                            const uint32_t *__restrict__ in_count, PathState *__restrict__ out,
@@ -186,9 +189,11 @@ __global__ __launch_bounds__(kBlock) void wf_bounce_kernel(SceneView s, RenderPa
             const float4 a = accum[pixel];
             col = color{a.x, a.y, a.z};
             cont = beauty_iteration(s, rp, ray, scaling, depth_left, tp, col, depth, tr);
+            if (TO_END)
+                while (cont) cont = beauty_iteration(s, rp, ray, scaling, depth_left, tp, col, depth, tr);
             if (!cont) finish_sample(rp, col, pixels, pixel);
         }
-        push_path(out, out_count, cap, live && cont, ray, tp, scaling, depth_left, pixel, col, accum, status);
+        if (!TO_END) push_path(out, out_count, cap, live && cont, ray, tp, scaling, depth_left, pixel, col, accum, status);
     }
     if (tr.overflow) atomicOr(status, 1u);
 }
@@ -249,7 +254,7 @@ __global__ void wf_clear_count_kernel(uint32_t *__restrict__ counters, uint32_t 
 
 namespace slimrt {
 
-int launch_wavefront(slimrt_context *c, const RenderParams &rp, float4 *pixels, float *depths, bool sort) {
+int launch_wavefront(slimrt_context *c, const RenderParams &rp, float4 *pixels, float *depths, bool per_bounce, bool sort) {
     const uint32_t ws = rp.width << rp.ssaa, hs = rp.height << rp.ssaa;
     const uint32_t samples = ws * hs;
     if (c->wf_cap < samples) {
@@ -280,22 +285,36 @@ int launch_wavefront(slimrt_context *c, const RenderParams &rp, float4 *pixels, 
                        qa, cnt, samples, scene_depth, mesh_depth, tiles_x, tiles, per_xcd, c->d_status);
     HIP_TRY(c, hipGetLastError());
     const uint32_t bounce_blocks = (uint32_t)cus * 2, sort_blocks = (uint32_t)cus * 2;
+    // queue A holds the paths to run; sorting moves them to B (emptying A), so `in` is B after a sort
+    auto sort_a_into_b = [&]() {
+        hipLaunchKernelGGL(wf_bin_count_kernel, dim3(sort_blocks), dim3(256), 0, c->stream, qa, cnt, samples);
+        hipLaunchKernelGGL(wf_bin_scan_kernel, dim3(1), dim3(64), 0, c->stream, cnt);
+        hipLaunchKernelGGL(wf_bin_scatter_kernel, dim3(sort_blocks), dim3(256), 0, c->stream, qa, qb, cnt, samples);
+        hipLaunchKernelGGL(wf_clear_count_kernel, dim3(1), dim3(64), 0, c->stream, cnt, (uint32_t)WF_COUNT_A);
+    };
+    if (rp.max_depth < 2) return SLIMRT_OK;
+    if (!per_bounce) {
+        if (sort) sort_a_into_b();
+        hipLaunchKernelGGL(wf_bounce_kernel<true>, dim3(bounce_blocks), dim3(kBlock), lds, c->stream, v, rp, pixels, accum,
+                           (const PathState *)(sort ? qb : qa), (const uint32_t *)(cnt + (sort ? WF_COUNT_B : WF_COUNT_A)),
+                           sort ? qa : qb, cnt + (sort ? WF_COUNT_A : WF_COUNT_B), samples, scene_depth, mesh_depth,
+                           c->d_status);
+        HIP_TRY(c, hipGetLastError());
+        return SLIMRT_OK;
+    }
     uint32_t cur_count = WF_COUNT_A, next_count = WF_COUNT_B;   // unsorted: ping-pong between A and B
     for (uint32_t it = 1; it < rp.max_depth; it++) {
         if (sort) {   // A -> sorted B (A emptied), then B -> A
-            hipLaunchKernelGGL(wf_bin_count_kernel, dim3(sort_blocks), dim3(256), 0, c->stream, qa, cnt, samples);
-            hipLaunchKernelGGL(wf_bin_scan_kernel, dim3(1), dim3(64), 0, c->stream, cnt);
-            hipLaunchKernelGGL(wf_bin_scatter_kernel, dim3(sort_blocks), dim3(256), 0, c->stream, qa, qb, cnt, samples);
-            hipLaunchKernelGGL(wf_clear_count_kernel, dim3(1), dim3(64), 0, c->stream, cnt, (uint32_t)WF_COUNT_A);
-            hipLaunchKernelGGL(wf_bounce_kernel, dim3(bounce_blocks), dim3(kBlock), lds, c->stream, v, rp, pixels, accum,
-                               (const PathState *)qb, (const uint32_t *)(cnt + WF_COUNT_B), qa, cnt + WF_COUNT_A, samples,
-                               scene_depth, mesh_depth, c->d_status);
+            sort_a_into_b();
+            hipLaunchKernelGGL(wf_bounce_kernel<false>, dim3(bounce_blocks), dim3(kBlock), lds, c->stream, v, rp, pixels,
+                               accum, (const PathState *)qb, (const uint32_t *)(cnt + WF_COUNT_B), qa, cnt + WF_COUNT_A,
+                               samples, scene_depth, mesh_depth, c->d_status);
         } else {
             PathState *in = cur_count == WF_COUNT_A ? qa : qb, *out = cur_count == WF_COUNT_A ? qb : qa;
             hipLaunchKernelGGL(wf_clear_count_kernel, dim3(1), dim3(64), 0, c->stream, cnt, next_count);
-            hipLaunchKernelGGL(wf_bounce_kernel, dim3(bounce_blocks), dim3(kBlock), lds, c->stream, v, rp, pixels, accum,
-                               (const PathState *)in, (const uint32_t *)(cnt + cur_count), out, cnt + next_count, samples,
-                               scene_depth, mesh_depth, c->d_status);
+            hipLaunchKernelGGL(wf_bounce_kernel<false>, dim3(bounce_blocks), dim3(kBlock), lds, c->stream, v, rp, pixels,
+                               accum, (const PathState *)in, (const uint32_t *)(cnt + cur_count), out, cnt + next_count,
+                               samples, scene_depth, mesh_depth, c->d_status);
             std::swap(cur_count, next_count);
         }
         HIP_TRY(c, hipGetLastError());

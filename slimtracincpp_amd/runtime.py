@@ -74,7 +74,7 @@ class ViewsConfig(C.Structure):
 
 
 SCENE_REFIT, SCENE_REBUILD = 0, 1
-WAVEFRONT, WAVEFRONT_UNSORTED = 2, 4   # slimrt_render_params.flags, with RAY_CONES
+WAVEFRONT, WAVEFRONT_PER_BOUNCE, WAVEFRONT_SORT = 2, 4, 8   # slimrt_render_params.flags, with RAY_CONES
 
 
 # RayTracingRenderer's mip_level_colors (src/slim/renderer/renderer.h:50-58) as Color(ColorID) values
@@ -112,6 +112,7 @@ def lib():
             "slimrt_views_configure": [vp, P(ViewsConfig)], "slimrt_views_get_config": [vp, P(ViewsConfig)],
             "slimrt_views_frame": [vp, vp, u32, u32, u32], "slimrt_views_last_ms": [vp, P(f32)],
             "slimrt_views_autotune": [vp, u32, P(ViewsConfig), P(f32)],
+            "slimrt_render_autotune": [vp, P(RenderParams), vp, vp, u32, P(u32), P(f32)],
         }
         for name, args in sig.items():
             fn = getattr(L, name)
@@ -342,6 +343,13 @@ class Renderer:
     def render_device(self, params: RenderParams, pixels_ptr: int, depths_ptr: int):
         _check(lib().slimrt_render_device(self.ctx, C.byref(params), C.c_void_p(pixels_ptr), C.c_void_p(depths_ptr)),
                self.ctx)
+
+    def autotune_form(self, params: RenderParams, pixels_ptr: int, depths_ptr: int, reps: int = 3):
+        """Fastest frame form (megakernel or wavefront variants) for these params: (flags, median ms)."""
+        flags, ms = u32(), f32()
+        _check(lib().slimrt_render_autotune(self.ctx, C.byref(params), C.c_void_p(pixels_ptr), C.c_void_p(depths_ptr),
+                                            reps, C.byref(flags), C.byref(ms)), self.ctx)
+        return flags.value, ms.value
 
     def synchronize(self):
         _check(lib().slimrt_synchronize(self.ctx), self.ctx)

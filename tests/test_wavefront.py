@@ -1,7 +1,8 @@
-"""Bounces as a wavefront (render flag SLIMRT_WAVEFRONT): compacted ray queues, sorted by direction octant or not,
-must give every sample bit for bit what the per-sample megakernel gives (same iteration code, same accumulation
-order per sample), on scenes with point lights and glow, transparency, glass and mirrors, SSAA and both output
-kinds; and a frame with many bounces (depth 8) equals the oracle within the beauty tolerance."""
+"""Bounces as a wavefront (render flag SLIMRT_WAVEFRONT): compacted ray queues run to the paths' ends in one launch
+or re-compacted after every bounce, sorted by direction octant or not, must give every sample bit for bit what
+the per-sample megakernel gives (same iteration code, same accumulation order per sample), on scenes with point
+lights and glow, transparency, glass and mirrors, SSAA and both output kinds; and a frame with many bounces
+(depth 8) equals the oracle within the beauty tolerance."""
 import numpy as np
 import pytest
 
@@ -10,7 +11,7 @@ import oracle
 
 pytestmark = pytest.mark.gpu
 
-WAVEFRONT, UNSORTED = 2, 4
+WAVEFRONT, PER_BOUNCE, SORT = 2, 4, 8
 
 
 def _frame(r, p):
@@ -34,7 +35,7 @@ CASES = {
 
 
 @pytest.mark.parametrize("name", sorted(CASES))
-@pytest.mark.parametrize("flags", [WAVEFRONT, WAVEFRONT | UNSORTED])
+@pytest.mark.parametrize("flags", [WAVEFRONT, WAVEFRONT | SORT, WAVEFRONT | PER_BOUNCE, WAVEFRONT | PER_BOUNCE | SORT])
 @pytest.mark.parametrize("output", [runtime.OUTPUT_CANVAS, runtime.OUTPUT_RADIANCE])
 def test_wavefront_equals_megakernel(gpu_available, name, flags, output):
     spec = CASES[name]()
@@ -66,3 +67,19 @@ def test_wavefront_deep_bounces_against_oracle(gpu_available):
     col, _, _ = oracle.OracleScene(spec).render()
     got = px.view(np.float32).reshape(spec.height, spec.width, 4)[..., :3]
     assert np.max(np.abs(got - col)) <= 1e-4
+
+
+def test_render_autotune_picks_a_form_with_the_same_frame(gpu_available):
+    spec = CASES["mesh_pbr_d4"]()
+    r = runtime.Renderer(spec, "gpu")
+    p = r.params()
+    ref = _frame(r, p)
+    import torch
+    n = p.width * p.height
+    px = torch.zeros((n, 4), dtype=torch.float32, device="cuda")
+    dp = torch.zeros(n, dtype=torch.float32, device="cuda")
+    flags, ms = r.autotune_form(p, px.data_ptr(), dp.data_ptr(), reps=2)
+    assert flags in (0, WAVEFRONT, WAVEFRONT | SORT, WAVEFRONT | PER_BOUNCE, WAVEFRONT | PER_BOUNCE | SORT) and ms > 0
+    p.flags = flags
+    got = _frame(r, p)
+    assert np.array_equal(got[0], ref[0]) and np.array_equal(got[1], ref[1])

@@ -24,6 +24,10 @@ Reported:
                BVH rebuild, every view's tiles in one launch, launch shape from the autotuner) replayed as one
                HIP graph; baseline: the host rebuild + upload and one launch per view.  Per-frame GPU latency
                p50/p99 (HIP events) and aggregate Mrays/s (rays counted on the first frames' transforms).
+  large        (with --large) the combined 4K workload: 16 instances of a 262,144-triangle displaced sphere,
+               32 implicit primitives, 4 emissive quads, 8 + 8 textures at 2048^2, 512^2 cube IBL, depth 8; one
+               4K frame in every form, 8 uneven views (4K down to 640x360) in one launch vs one launch per view,
+               and the neural cache at the primary hit against the plain trace.
 """
 from __future__ import annotations
 
@@ -195,6 +199,91 @@ def batched_views_leg(frames: int, stream, spec, n_views: int = 48) -> dict:
             "per_view_mrays_per_s": rays_frame / (float(np.mean(b)) * 1e-3) / 1e6}
 
 
+def large_leg(stream, frames: int = 5) -> dict:
+    import numpy as np
+    import torch
+    from slimtracincpp_amd import scenes, runtime
+    from slimtracincpp_amd.nrc import CachedFrames
+    t0 = time.time()
+    spec = scenes.large_scene(0)
+    print(f"large: scene generated in {time.time() - t0:.0f}s", file=sys.stderr, flush=True)
+    t0 = time.time()
+    r = runtime.Renderer(spec, "gpu")
+    r.set_stream(stream.cuda_stream)
+    print(f"large: host build + upload in {time.time() - t0:.0f}s", file=sys.stderr, flush=True)
+    p = r.params()
+    n = p.width * p.height
+    px = torch.empty((n, 4), dtype=torch.float32, device="cuda")
+    dp = torch.empty(n, dtype=torch.float32, device="cuda")
+    cnt = r.count_work(p)
+    forms = {}
+    for name, flags in (("megakernel", 0), ("wavefront", runtime.WAVEFRONT),
+                        ("wavefront_sorted", runtime.WAVEFRONT | runtime.WAVEFRONT_SORT),
+                        ("wavefront_per_bounce", runtime.WAVEFRONT | runtime.WAVEFRONT_PER_BOUNCE)):
+        p.flags = flags
+        ts = []
+        for i in range(frames + 1):
+            r.render_device(p, px.data_ptr(), dp.data_ptr())
+            ts.append(r.last_kernel_ms())
+        forms[name] = float(np.median(ts[1:]))
+    best = min(forms, key=forms.get)
+    print(f"large: frame forms {forms}", file=sys.stderr, flush=True)
+    # 8 uneven views in one launch vs one launch per view
+    params, bufs = [], []
+    for w, h, eul, pos, focal in scenes.large_views(0):
+        vp = r.params(camera=(eul, pos, focal), size=(w, h))
+        params.append(vp)
+        bufs.append((torch.empty((w * h, 4), dtype=torch.float32, device="cuda"),
+                     torch.empty(w * h, dtype=torch.float32, device="cuda")))
+    views = runtime.Views(r, params, [b[0].data_ptr() for b in bufs], [b[1].data_ptr() for b in bufs])
+    cfg, _ = views.autotune(reps=1)
+    vg = []
+    for i in range(frames + 1):
+        views.frame(graph=True)
+        vg.append(views.last_ms())
+    views.close()
+    ev0, ev1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    vs = []
+    for i in range(frames + 1):
+        torch.cuda.synchronize()
+        ev0.record(stream)
+        for vp, (bp, bd) in zip(params, bufs):
+            r.render_device(vp, bp.data_ptr(), bd.data_ptr())
+        ev1.record(stream)
+        torch.cuda.synchronize()
+        vs.append(ev0.elapsed_time(ev1))
+    view_rays = sum(r.count_work(vp)["rays"] for vp in params)
+    print("large: views done", file=sys.stderr, flush=True)
+    # the neural cache at the primary hit, static camera, trained over the frames
+    p.flags = 0
+    p.output = runtime.OUTPUT_RADIANCE
+    plain = torch.empty_like(px)
+    r.render_device(p, plain.data_ptr(), dp.data_ptr())
+    plain_ms = r.last_kernel_ms()
+    cf = CachedFrames(r, seed=1, cache_depth=0, train_one_in=8, train_steps=4, train_batch=16384,
+                      gate_threshold=0.1, gate_min_frames=4)
+    psnr, nrc_ms = [], []
+    for f in range(6 * frames):
+        cf.render_device(p, px.data_ptr(), dp.data_ptr())
+        nrc_ms.append(r.last_kernel_ms())
+        d = (px[:, :3] - plain[:, :3]).double()
+        psnr.append(10 * np.log10(1.0 / max(float(torch.mean(d * d)), 1e-20)))
+    st = cf.stats()
+    cf.cache.close()
+    r.synchronize()
+    rays = cnt["rays"]
+    return {"workload": "large combined scene, 3840x2160, depth 8", "rays_per_frame": rays, "counters": cnt,
+            "forms_ms": forms, "best_form": best, "ms_per_frame": forms[best],
+            "mrays_per_s": rays / (forms[best] * 1e-3) / 1e6,
+            "views": {"count": len(params), "launch_config": cfg, "rays_per_frame": view_rays,
+                      "graph_ms_p50": float(np.median(vg[1:])), "per_view_ms_p50": float(np.median(vs[1:])),
+                      "graph_mrays_per_s": view_rays / (float(np.median(vg[1:])) * 1e-3) / 1e6},
+            "neural_cache_at_primary_hit": {"plain_ms": plain_ms, "cache_ms_median": float(np.median(nrc_ms[2:])),
+                                            "psnr_db_last": psnr[-1], "psnr_db_max": max(psnr),
+                                            "queried_fraction": st["queries"] / n,
+                                            "fallback_fraction": st["fallbacks"] / n}}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -206,6 +295,7 @@ def main():
     ap.add_argument("--no-neural-cache", action="store_true")
     ap.add_argument("--nrc-frames", type=int, default=300)
     ap.add_argument("--no-views", action="store_true")
+    ap.add_argument("--large", action="store_true", help="also run the combined 4K workload (about 2 minutes)")
     ap.add_argument("--view-frames", type=int, default=20)
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
                     help="also run a plain device-to-device copy of this many MiB (HBM counter calibration)")
@@ -314,6 +404,8 @@ def main():
     }
     if not args.no_views:
         result["batched_views"] = batched_views_leg(args.view_frames, stream, spec)
+    if args.large:
+        result["large"] = large_leg(stream)
     if not args.no_neural_cache:
         legs = [neural_cache_leg(args.nrc_frames, args.width, args.height, stream, d) for d in (2, 0)]
         mesh_legs = [neural_cache_leg(args.nrc_frames, args.width, args.height, stream, d, spec) for d in (2, 0)]

@@ -610,3 +610,136 @@ def batched_views(count: int = 48, seed: int = 0, resolutions=VIEW_RESOLUTIONS):
         focal = np.float32(1.0 / math.tan(math.radians(fov) / 2))
         out.append((w, h, look_at_euler(pos, (0.0, 0.5, 0.0)), pos, focal))
     return out
+
+
+# ---------------------------------------------------------------- the large combined workload
+LARGE_VIEW_RESOLUTIONS = ((3840, 2160), (2560, 1440), (1920, 1080), (1600, 900), (1280, 720), (960, 540),
+                          (800, 450), (640, 360))
+
+
+def normal_map_texels(size: int, seed: int, strength: float = 2.0) -> np.ndarray:
+    """Tangent-space normal map of a 4-octave value-noise height field, in the channel order the reference's
+    sampler reads (rotate_normal, src/slim/scene/material.h:83-86: (r, b, g) * 2 - 1 with b the up axis)."""
+    y, x = np.mgrid[0:size, 0:size]
+    p = np.stack([x.ravel() / size * 8.0, y.ravel() / size * 8.0, np.full(size * size, 0.5)], 1)
+    h = value_noise3(p, seed, octaves=4, base_frequency=1.0).reshape(size, size)
+    du = (np.roll(h, -1, 1) - np.roll(h, 1, 1)) * (size / 16.0) * strength / size * 8
+    dv = (np.roll(h, -1, 0) - np.roll(h, 1, 0)) * (size / 16.0) * strength / size * 8
+    n = np.stack([-du, np.ones_like(h), -dv], -1)
+    n /= np.linalg.norm(n, axis=-1, keepdims=True)
+    rgb = np.stack([n[..., 0], n[..., 2], n[..., 1]], -1) * 0.5 + 0.5   # r = x, g = z, b = up
+    return (np.round(np.clip(rgb, 0, 1) * 255) / 255).astype(np.float32)
+
+
+def large_scene(seed: int = 0, width: int = 3840, height: int = 2160, max_depth: int = 8, instances: int = 16,
+                implicit: int = 32, emissive: int = 4, uv_sphere=(257, 512), texture_size: int = 2048,
+                textures: int = 8, sky_size: int = 512, irradiance_size: int = 32, leaf_size: int = 4,
+                render_mode: int = MODE_BEAUTY) -> SceneSpec:
+    """The large combined workload: `instances` instances of one displaced UV sphere (257 rings x 512 segments =
+    262,144 triangles: pole fans of 512 plus 255 rings of 1,024), `implicit` implicit primitives, `emissive`
+    2x2 emissive quads (intensity U[10, 40]) facing down from y = 7..9, GGX materials with `textures` distinct
+    mip-mapped albedo textures and as many normal maps, cube-map IBL, bounce depth 8.
+
+    Instances sit on a jittered 4 x 4 grid (x, z in [-9, 9]) with Euler U[0, 2pi)^3 and scale U[0.7, 1.5];
+    instance i uses texture pair i mod `textures`; instances 0 and 1 are glass (IOR 1.5) and a mirror.
+    Implicit primitives (sphere, box, tetrahedron in turn) are placed at x, z U[-11, 11], y U[0.5, 4] with
+    scale U[0.3, 0.9]; every fourth is a mirror, the rest Lambert or GGX.  Emissive quads are area lights
+    only, shaded by the reference's heuristic estimator (shadeFromEmissiveQuads: no shadow rays).
+    The albedo textures share one noise field (checker colours differ per texture), and so do the normal maps
+    (each at its own strength), which keeps generation cheap at 2048^2.
+    Textures: 0 .. T-1 albedo, T .. 2T-1 normal maps, 2T sky radiance cube, 2T+1 irradiance cube."""
+    rng = PCG32(seed)
+    rings, segs = uv_sphere
+    mesh = displaced_sphere_mesh(0, 0.15, seed, leaf_size, uv_sphere=(rings, segs))
+    two_pi = 2.0 * math.pi
+    geos, mats_f, mats_u = [], [], []
+    geos.append(_geo(GEO_QUAD, (0, 0, 0), (0, 0, 0), (12, 1, 12), 0))
+    f, u = material((1, 1, 1), 0.6, flags=MAT_ALBEDO_MAP, texture_ids=(0,), brdf=BRDF_COOK_TORRANCE, uv_repeat=(6, 6))
+    mats_f.append(f); mats_u.append(u)
+    for i in range(instances):
+        gx, gz = i % 4, i // 4
+        s = rng.uniform(0.7, 1.5)
+        px = -6.75 + 4.5 * gx + rng.uniform(-0.8, 0.8)
+        pz = -6.75 + 4.5 * gz + rng.uniform(-0.8, 0.8)
+        eul = (rng.uniform(0, two_pi), rng.uniform(0, two_pi), rng.uniform(0, two_pi))
+        rough = rng.uniform(0.05, 0.9)
+        metal = 1.0 if rng.uniform() < 0.5 else 0.0
+        fl = GEO_VISIBLE | GEO_SHADOWING
+        if i == 0:
+            f, u = material((0.05, 0.05, 0.05), rough * 0.2, flags=MAT_REFRACTIVE, ior=1.5, reflectivity=(0.04, 0.04, 0.04))
+            fl = GEO_VISIBLE
+        elif i == 1:
+            f, u = material((0.0, 0.0, 0.0), 0.02, flags=MAT_REFLECTIVE, reflectivity=(0.91, 0.92, 0.92), metalness=1.0)
+        else:
+            t = i % textures
+            base = np.array([0.9, 0.8, 0.6], np.float32)
+            f0 = 0.04 * (1 - metal) + base * metal
+            f, u = material(base, rough, flags=MAT_ALBEDO_MAP | MAT_NORMAL_MAP, texture_ids=(t, textures + t),
+                            reflectivity=tuple(f0), metalness=metal, normal_magnitude=0.5)
+        mats_f.append(f); mats_u.append(u)
+        geos.append(_geo(GEO_MESH, eul, (px, s * 1.15, pz), (s, s, s), len(geos), 0, fl))
+    kinds = (GEO_SPHERE, GEO_BOX, GEO_TET)
+    for j in range(implicit):
+        s = rng.uniform(0.3, 0.9)
+        pos = (rng.uniform(-11, 11), rng.uniform(0.5, 4.0), rng.uniform(-11, 11))
+        eul = (rng.uniform(0, two_pi), rng.uniform(0, two_pi), rng.uniform(0, two_pi))
+        albedo = (rng.uniform(0.1, 0.9), rng.uniform(0.1, 0.9), rng.uniform(0.1, 0.9))
+        rough = rng.uniform(0.1, 0.9)
+        coin = rng.uniform()
+        if j % 4 == 3:
+            f, u = material(albedo, 0.05, flags=MAT_REFLECTIVE, brdf=BRDF_BLINN, reflectivity=(0.9, 0.9, 0.9))
+        elif coin < 0.5:
+            f, u = material(albedo, 1.0, brdf=BRDF_LAMBERT)
+        else:
+            f, u = material(albedo, rough, brdf=BRDF_COOK_TORRANCE, reflectivity=(0.04, 0.04, 0.04))
+        mats_f.append(f); mats_u.append(u)
+        geos.append(_geo(kinds[j % 3], eul, pos, (s, s, s), len(geos)))
+    for k in range(emissive):
+        intensity = rng.uniform(10.0, 40.0)
+        pos = (rng.uniform(-8, 8), rng.uniform(7, 9), rng.uniform(-8, 8))
+        f, u = material((1, 1, 1), 1.0, flags=MAT_EMISSIVE, brdf=BRDF_LAMBERT, emission=(intensity,) * 3)
+        mats_f.append(f); mats_u.append(u)
+        geos.append(_geo(GEO_QUAD, (math.pi, 0, 0), pos, (1, 1, 1), len(geos)))   # flipped: faces down
+    # textures
+    n_tex = textures
+    y, x = np.mgrid[0:texture_size, 0:texture_size]
+    p = np.stack([x.ravel() / texture_size * 16.0, y.ravel() / texture_size * 16.0, np.zeros(texture_size ** 2)], 1)
+    noise = value_noise3(p, seed + 11, octaves=4, base_frequency=1.0).reshape(texture_size, texture_size)
+    tex = []
+    for t in range(n_tex):
+        c0 = np.array([rng.uniform(0.55, 0.95) for _ in range(3)])
+        c1 = np.array([rng.uniform(0.05, 0.45) for _ in range(3)])
+        squares = 4 + 2 * t
+        check = ((x * squares // texture_size) + (y * squares // texture_size)) % 2
+        base = np.where(check[..., None] == 1, c0, c1)
+        rgb = np.clip(base * (0.8 + 0.2 * np.roll(noise, 97 * t, axis=1)[..., None]), 0, 1)
+        texels = (np.round(rgb * 255) / 255).astype(np.float32)
+        tex.append(texture_from_texels(texels, True, True))
+    nm = normal_map_texels(texture_size, seed + 303)
+    for t in range(n_tex):
+        tex.append(texture_from_texels(np.roll(nm, 131 * t, axis=0), True, True))
+    sky = sky_function(seed + 101)
+    tex += [cube_texture(sky, sky_size), cube_texture(irradiance_function(sky), irradiance_size)]
+    cam_pos = np.array([0.0, 6.0, -16.0], np.float32)
+    return SceneSpec(
+        name=f"large_s{seed}_{width}x{height}", width=width, height=height,
+        camera_euler=look_at_euler(cam_pos, (0.0, 1.0, 0.0)), camera_position=cam_pos, focal_length=FOV60_FOCAL,
+        max_depth=max_depth, render_mode=render_mode, skybox_color=2 * n_tex, skybox_radiance=2 * n_tex,
+        skybox_irradiance=2 * n_tex + 1,
+        geo_f=_stack([g[0] for g in geos]), geo_u=_stack([g[1] for g in geos]),
+        mat_f=_stack(mats_f), mat_u=_stack(mats_u),
+        light_f=np.zeros((0, 7), np.float32), light_u=np.zeros(0, np.uint32),
+        meshes=[mesh], textures=tex)
+
+
+def large_views(seed: int = 0, resolutions=LARGE_VIEW_RESOLUTIONS):
+    """One camera per resolution, on a circle of radius U[14, 20] at height U[4, 9] looking at (0, 1, 0).
+    Returns [(width, height, euler, position, focal_length)] as batched_views does."""
+    rng = PCG32(seed)
+    out = []
+    for w, h in resolutions:
+        r = rng.uniform(14.0, 20.0)
+        a = rng.uniform(0.0, 2 * math.pi)
+        pos = np.array([r * math.cos(a), rng.uniform(4.0, 9.0), r * math.sin(a)], np.float32)
+        out.append((w, h, look_at_euler(pos, (0.0, 1.0, 0.0)), pos, FOV60_FOCAL))
+    return out

@@ -33,6 +33,9 @@ CASES = {
                                                                irradiance_size=8, render_mode=scenes.MODE_UVS),
     "mesh_pbr_s1_96x54_l6_beauty": lambda: scenes.mesh_pbr_scene(1, 96, 54, level=6, texture_size=64, sky_size=16,
                                                                  irradiance_size=8),
+    # the large workload's generator at a small size: 53 geometries, emissive quads, normal maps, depth 8
+    "large_s0_160x90_emissive": lambda: scenes.large_scene(0, 160, 90, uv_sphere=(17, 32), texture_size=64,
+                                                           sky_size=16, irradiance_size=8),
 }
 
 DIGEST_ABOVE = 4096   # arrays larger than this many elements are stored as SHA-256 digests
@@ -42,8 +45,10 @@ def digest(a: np.ndarray) -> str:
     return hashlib.sha256(np.ascontiguousarray(a).tobytes()).hexdigest()
 
 
-def main():
+def main(names=None):
     for name, make in CASES.items():
+        if names and name not in names:
+            continue
         spec = make()
         sb = spec_bytes(spec)
         ref = oracle.run_reference(spec)
@@ -64,4 +69,5 @@ def main():
 
 
 if __name__ == "__main__":
-    main()
+    import sys
+    main(sys.argv[1:])   # case names, or none for every case

@@ -186,3 +186,26 @@ def test_moving_geometries_rebuild(gpu_available):
         px, _ = r.render()
         ok, err = _close(px[..., :3], col, atol=BEAUTY_ATOL)
         assert ok, (frame, err)
+
+
+@pytest.mark.parametrize("golden", ["large_s0_160x90_emissive"])
+def test_golden_reference_frames_on_gpu(gpu_available, golden, tmp_path):
+    """The GPU against the reference's own frame (golden vectors written by the compiled reference): the large
+    workload's generator at a small size - 53 geometries, emissive quads, normal maps, glass, mirrors, depth 8."""
+    from tests.helpers import load_golden, golden_spec_path
+    from tests.golden.make_golden import CASES
+    g = load_golden(golden)
+    golden_spec_path(golden, g, str(tmp_path))   # checks the seeded generator against the golden's digest
+    spec = CASES[golden]()
+    r = runtime.Renderer(spec, "gpu")
+    px, depth = r.render()
+    col = g["color"].reshape(spec.height, spec.width, 3)
+    ok, err = _close(px[..., :3], col, atol=BEAUTY_ATOL)
+    assert ok, err
+    geo, tri, t = r.primary_hits()
+    match = np.mean((geo.reshape(-1) == g["hit_geo"]) & (tri.reshape(-1) == g["hit_tri"]))
+    assert match >= 0.9999, match
+    p = r.params()
+    p.flags = runtime.WAVEFRONT
+    pw, dw = r.render(p)
+    assert np.array_equal(pw.view(np.uint32), px.view(np.uint32)) and np.array_equal(dw.view(np.uint32), depth.view(np.uint32))

@@ -48,7 +48,8 @@ needs_ref = pytest.mark.skipif(not (os.path.exists(oracle.REF_PATH) and os.path.
     lambda: scenes.mesh_pbr_scene(5, 64, 40, level=2, texture_size=32, sky_size=8, irradiance_size=4),
     lambda: scenes.mesh_pbr_scene(6, 64, 40, level=3, texture_size=32, sky_size=8, irradiance_size=4,
                                   leaf_size=2, render_mode=scenes.MODE_MIP_LEVEL),
-], ids=["implicit_d3", "implicit_normals", "mesh_l2", "mesh_l3_leaf2_mip"])
+    lambda: scenes.large_scene(3, 64, 40, uv_sphere=(9, 16), texture_size=16, sky_size=8, irradiance_size=4),
+], ids=["implicit_d3", "implicit_normals", "mesh_l2", "mesh_l3_leaf2_mip", "large_emissive"])
 def test_oracle_matches_live_reference(make):
     spec = make()
     ref = oracle.run_reference(spec)
@@ -60,3 +61,13 @@ def test_oracle_matches_live_reference(make):
     assert bits_equal(dep.reshape(-1), ref["depth"])
     geo, tri, t = o.primary_hits()
     assert np.array_equal(geo.reshape(-1), ref["hit_geo"]) and np.array_equal(tri.reshape(-1), ref["hit_tri"])
+
+
+def test_emissive_quads_contribute():
+    """The large scene's emissive quads light it: without their emission the frame is darker somewhere."""
+    spec = scenes.large_scene(0, 96, 54, uv_sphere=(9, 16), texture_size=16, sky_size=8, irradiance_size=4)
+    col, _, _ = oracle.OracleScene(spec).render()
+    dark = scenes.large_scene(0, 96, 54, uv_sphere=(9, 16), texture_size=16, sky_size=8, irradiance_size=4)
+    dark.mat_f[dark.mat_u[:, 0] == scenes.MAT_EMISSIVE, 7:10] = 0.0
+    col0, _, _ = oracle.OracleScene(dark).render()
+    assert np.sum(col > col0 + 1e-3) > 100

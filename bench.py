@@ -330,7 +330,7 @@ def main():
 
     # Five forms of the same frame, bit-identical outputs (tests/test_wavefront.py): the per-sample megakernel and
     # the bounces as a wavefront over a compacted ray queue (run to the end in one launch, or re-compacted after
-    # every bounce; either sorted by direction octant or not).  Each is timed over K back-to-back frames between
+    # every bounce; sorted by direction octant or not).  Each is timed over K back-to-back frames between
     # synchronisations; the fastest is the reported path.
     forms = {"megakernel": 0, "wavefront": runtime.WAVEFRONT, "wavefront_sorted": runtime.WAVEFRONT | runtime.WAVEFRONT_SORT,
              "wavefront_per_bounce": runtime.WAVEFRONT | runtime.WAVEFRONT_PER_BOUNCE,

@@ -55,6 +55,9 @@ extern "C" {
 #define SLIMRT_WAVEFRONT_PER_BOUNCE 4u /* with SLIMRT_WAVEFRONT: one launch per bounce, the queue compacted again
                                           after every bounce */
 #define SLIMRT_WAVEFRONT_SORT 8u   /* with SLIMRT_WAVEFRONT: sort each queue by direction octant before its launch */
+/* with SLIMRT_WAVEFRONT: the frame in 2^log2n horizontal bands; a band's bounces run on a second stream while
+ * the next band's first hits are traced, so the bounce launches' latency tails overlap the primary work */
+#define SLIMRT_WAVEFRONT_BANDS(log2n) (((uint32_t)(log2n) & 3u) << 4)
 
 typedef struct slimrt_context slimrt_context;
 
@@ -159,7 +162,7 @@ typedef struct {
     uint32_t render_mode;
     int32_t skybox_color, skybox_radiance, skybox_irradiance;   /* texture ids, -1 for none */
     uint32_t output;            /* SLIMRT_OUTPUT_* */
-    uint32_t flags;             /* SLIMRT_RAY_CONES | SLIMRT_WAVEFRONT (| _PER_BOUNCE | _SORT) */
+    uint32_t flags;             /* SLIMRT_RAY_CONES | SLIMRT_WAVEFRONT (| _PER_BOUNCE | _SORT | _BANDS(n)) */
     float mip_level_colors[9][3];
 } slimrt_render_params;
 

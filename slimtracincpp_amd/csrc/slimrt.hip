@@ -186,7 +186,7 @@ int launch(slimrt_context *c, const slimrt_render_params *p, float4 *pixels, flo
     if (c->timed) HIP_TRY(c, hipEventRecord(c->ev0, c->stream));
     if (!count && (rp.flags & SLIMRT_WAVEFRONT) && rp.render_mode == RM_Beauty && rp.output != SLIMRT_OUTPUT_HITS) {
         int rc = launch_wavefront(c, rp, pixels, depths, (rp.flags & SLIMRT_WAVEFRONT_PER_BOUNCE) != 0,
-                                  (rp.flags & SLIMRT_WAVEFRONT_SORT) != 0);
+                                  (rp.flags & SLIMRT_WAVEFRONT_SORT) != 0, 1u << ((rp.flags >> 4) & 3u));
         if (rc) return rc;
     } else if (count) {
         hipLaunchKernelGGL(render_kernel<true>, dim3(per_xcd * 8), dim3(kBlock), lds, c->stream, v, rp, pixels, depths,
@@ -357,6 +357,11 @@ int slimrt_destroy(slimrt_context *c) {
         dev_free(c->d_geo_stage);
         for (void *q : {c->wf_queue_a, c->wf_queue_b, c->wf_accum, (void *)c->wf_counters})
             if (q) (void)hipFree(q);
+        for (hipEvent_t e : c->wf_band)
+            if (e) (void)hipEventDestroy(e);
+        if (c->wf_fork) (void)hipEventDestroy(c->wf_fork);
+        if (c->wf_join) (void)hipEventDestroy(c->wf_join);
+        if (c->side_stream) (void)hipStreamDestroy(c->side_stream);
         if (c->d_status) (void)hipFree(c->d_status);
         if (c->d_counters) (void)hipFree(c->d_counters);
         if (c->ev0) (void)hipEventDestroy(c->ev0);

@@ -64,6 +64,8 @@ struct slimrt_context {
     DevBuf<slimrt_geometry> d_geo_stage;   // transform updates on their way to the scene update kernel
     void *wf_queue_a = nullptr, *wf_queue_b = nullptr, *wf_accum = nullptr;   // wavefront ray queues, colours
     uint32_t *wf_counters = nullptr, wf_cap = 0;
+    hipStream_t side_stream = nullptr;          // a wavefront frame's bounce launches, overlapping the next band
+    hipEvent_t wf_fork = nullptr, wf_join = nullptr, wf_band[8] = {};
     uint32_t *d_status = nullptr;
     unsigned long long *d_counters = nullptr;
 
@@ -193,7 +195,8 @@ inline int validate_params(slimrt_context *c, const slimrt_render_params *p) {
 }
 
 // Beauty frame with the bounces as a wavefront (slimrt_wavefront.hip).
-int launch_wavefront(slimrt_context *c, const RenderParams &rp, float4 *pixels, float *depths, bool per_bounce, bool sort);
+int launch_wavefront(slimrt_context *c, const RenderParams &rp, float4 *pixels, float *depths, bool per_bounce, bool sort,
+                     uint32_t bands);
 
 inline int check_status(slimrt_context *c) {
     uint32_t st = 0;

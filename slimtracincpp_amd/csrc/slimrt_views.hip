@@ -589,10 +589,13 @@ const char *slimrt_views_last_error(const slimrt_views *v) { return v ? v->err.c
 int slimrt_render_autotune(slimrt_context *c, const slimrt_render_params *params, void *d_pixels, void *d_depths,
                            uint32_t reps, uint32_t *best_flags, float *best_ms) {
     if (!c || !params || !best_flags || reps == 0) return SLIMRT_E_ARG;
-    const uint32_t base = params->flags & ~(uint32_t)(SLIMRT_WAVEFRONT | SLIMRT_WAVEFRONT_PER_BOUNCE | SLIMRT_WAVEFRONT_SORT);
-    const uint32_t forms[5] = {0u, SLIMRT_WAVEFRONT, SLIMRT_WAVEFRONT | SLIMRT_WAVEFRONT_SORT,
-                               SLIMRT_WAVEFRONT | SLIMRT_WAVEFRONT_PER_BOUNCE,
-                               SLIMRT_WAVEFRONT | SLIMRT_WAVEFRONT_PER_BOUNCE | SLIMRT_WAVEFRONT_SORT};
+    const uint32_t base = params->flags & ~(uint32_t)(SLIMRT_WAVEFRONT | SLIMRT_WAVEFRONT_PER_BOUNCE |
+                                                      SLIMRT_WAVEFRONT_SORT | SLIMRT_WAVEFRONT_BANDS(3));
+    std::vector<uint32_t> forms{0u};
+    for (uint32_t bands_log2 : {0u, 2u})
+        for (uint32_t v : {0u, (uint32_t)SLIMRT_WAVEFRONT_SORT, (uint32_t)SLIMRT_WAVEFRONT_PER_BOUNCE,
+                           (uint32_t)(SLIMRT_WAVEFRONT_PER_BOUNCE | SLIMRT_WAVEFRONT_SORT)})
+            forms.push_back(SLIMRT_WAVEFRONT | v | SLIMRT_WAVEFRONT_BANDS(bands_log2));
     slimrt_render_params p = *params;
     float win_ms = INFINITY;
     uint32_t win = base;

@@ -11,6 +11,12 @@
 // prefix, scatter) so that a wave's rays head the same way through the BVHs.  A path's colour accumulates in a
 // per-sample buffer in the integrator's own order, so every sample is bit-identical to the megakernel's.
 //
+// The bounce launches carry few rays, each with a long chain of dependent fetches, so their time is the latency
+// of their slowest waves rather than throughput.  With SLIMRT_WAVEFRONT_BANDS the frame is cut into horizontal
+// bands with queues of their own: the primary pass runs band after band on the caller's stream while each
+// band's bounces run on a second stream as soon as its primary pass is done, so those tails overlap the next
+// band's primary work (event fork and join, both streams at the default priority).
+//
 // A path in flight is 48 bytes: the bounced ray's origin and direction (as ray_reset leaves them), throughput,
 // the pixel's ray-cone scaling factor, the remaining depth and the canvas offset.
 #include <algorithm>
@@ -105,21 +111,23 @@ __device__ __forceinline__ void push_path(PathState *q, uint32_t *count, uint32_
     accum[pixel] = make_float4(col.r, col.g, col.b, 0.0f);
 }
 
-__global__ void wf_reset_kernel(uint32_t *__restrict__ counters) {
-    if (threadIdx.x < WF_COUNTERS) counters[threadIdx.x] = 0u;
+__global__ void wf_reset_kernel(uint32_t *__restrict__ counters, uint32_t n) {
+    for (uint32_t i = threadIdx.x; i < n; i += blockDim.x) counters[i] = 0u;
 }
 
 // Iteration 1 for every sample, tiles as in the render kernel.
+// Tiles [tile_base, tile_base + tiles) of the frame (one band, or all of it).
 __global__ __launch_bounds__(kBlock) void wf_primary_kernel(SceneView s, RenderParams rp, float4 *__restrict__ pixels,
                                                             float *__restrict__ depths, float4 *__restrict__ accum,
                                                             PathState *__restrict__ queue, uint32_t *__restrict__ counters,
                                                             uint32_t cap, uint32_t scene_depth, uint32_t mesh_depth,
-                                                            uint32_t tiles_x, uint32_t tiles, uint32_t tiles_per_xcd,
-                                                            uint32_t *__restrict__ status) {
+                                                            uint32_t tiles_x, uint32_t tile_base, uint32_t tiles,
+                                                            uint32_t tiles_per_xcd, uint32_t *__restrict__ status) {
     extern __shared__ uint32_t lds[];
     const uint32_t xcd = blockIdx.x & 7u, slot = blockIdx.x >> 3;
-    const uint32_t tile = xcd * tiles_per_xcd + slot;
-    if (tile >= tiles) return;
+    const uint32_t local = xcd * tiles_per_xcd + slot;
+    if (local >= tiles) return;
+    const uint32_t tile = tile_base + local;
     const uint32_t t = threadIdx.x, wave = t >> 6, lane = t & 63u;
     const uint32_t x = (tile % tiles_x) * kTile + (wave & 1u) * 8u + (lane & 7u);
     const uint32_t y = (tile / tiles_x) * kTile + (wave >> 1) * 8u + (lane >> 3);
@@ -254,23 +262,64 @@ __global__ void wf_clear_count_kernel(uint32_t *__restrict__ counters, uint32_t 
 
 namespace slimrt {
 
-int launch_wavefront(slimrt_context *c, const RenderParams &rp, float4 *pixels, float *depths, bool per_bounce, bool sort) {
+// The bounce launches of one queue (queue A holds the paths to run; sorting moves them to B, emptying A).
+int enqueue_bounces(slimrt_context *c, hipStream_t st, const SceneView &v, const RenderParams &rp, float4 *pixels,
+                    float4 *accum, PathState *qa, PathState *qb, uint32_t *cnt, uint32_t cap, size_t lds,
+                    uint32_t scene_depth, uint32_t mesh_depth, uint32_t blocks, bool per_bounce, bool sort) {
+    auto sort_a_into_b = [&]() {
+        hipLaunchKernelGGL(wf_bin_count_kernel, dim3(blocks), dim3(256), 0, st, qa, cnt, cap);
+        hipLaunchKernelGGL(wf_bin_scan_kernel, dim3(1), dim3(64), 0, st, cnt);
+        hipLaunchKernelGGL(wf_bin_scatter_kernel, dim3(blocks), dim3(256), 0, st, qa, qb, cnt, cap);
+        hipLaunchKernelGGL(wf_clear_count_kernel, dim3(1), dim3(64), 0, st, cnt, (uint32_t)WF_COUNT_A);
+    };
+    if (rp.max_depth < 2) return SLIMRT_OK;
+    if (!per_bounce) {
+        if (sort) sort_a_into_b();
+        hipLaunchKernelGGL(wf_bounce_kernel<true>, dim3(blocks), dim3(kBlock), lds, st, v, rp, pixels, accum,
+                           (const PathState *)(sort ? qb : qa), (const uint32_t *)(cnt + (sort ? WF_COUNT_B : WF_COUNT_A)),
+                           sort ? qa : qb, cnt + (sort ? WF_COUNT_A : WF_COUNT_B), cap, scene_depth, mesh_depth,
+                           c->d_status);
+        HIP_TRY(c, hipGetLastError());
+        return SLIMRT_OK;
+    }
+    uint32_t cur_count = WF_COUNT_A, next_count = WF_COUNT_B;   // unsorted: ping-pong between A and B
+    for (uint32_t it = 1; it < rp.max_depth; it++) {
+        if (sort) {   // A -> sorted B (A emptied), then B -> A
+            sort_a_into_b();
+            hipLaunchKernelGGL(wf_bounce_kernel<false>, dim3(blocks), dim3(kBlock), lds, st, v, rp, pixels, accum,
+                               (const PathState *)qb, (const uint32_t *)(cnt + WF_COUNT_B), qa, cnt + WF_COUNT_A, cap,
+                               scene_depth, mesh_depth, c->d_status);
+        } else {
+            PathState *in = cur_count == WF_COUNT_A ? qa : qb, *out = cur_count == WF_COUNT_A ? qb : qa;
+            hipLaunchKernelGGL(wf_clear_count_kernel, dim3(1), dim3(64), 0, st, cnt, next_count);
+            hipLaunchKernelGGL(wf_bounce_kernel<false>, dim3(blocks), dim3(kBlock), lds, st, v, rp, pixels, accum,
+                               (const PathState *)in, (const uint32_t *)(cnt + cur_count), out, cnt + next_count, cap,
+                               scene_depth, mesh_depth, c->d_status);
+            std::swap(cur_count, next_count);
+        }
+        HIP_TRY(c, hipGetLastError());
+    }
+    return SLIMRT_OK;
+}
+
+int launch_wavefront(slimrt_context *c, const RenderParams &rp, float4 *pixels, float *depths, bool per_bounce, bool sort,
+                     uint32_t bands) {
     const uint32_t ws = rp.width << rp.ssaa, hs = rp.height << rp.ssaa;
-    const uint32_t samples = ws * hs;
-    if (c->wf_cap < samples) {
+    const uint32_t tiles_x = (ws + kTile - 1) / kTile, tiles_y = (hs + kTile - 1) / kTile;
+    bands = std::max(1u, std::min(bands, std::min(tiles_y, 8u)));
+    const uint32_t slots = tiles_x * tiles_y * kBlock;   // queue entries: one per sample slot of every tile
+    if (c->wf_cap < slots) {
         for (void *p : {(void *)c->wf_queue_a, (void *)c->wf_queue_b, (void *)c->wf_accum})
             if (p) HIP_TRY(c, hipFree(p));
         c->wf_queue_a = c->wf_queue_b = nullptr;
         c->wf_accum = nullptr;
         c->wf_cap = 0;
-        HIP_TRY(c, hipMalloc(&c->wf_queue_a, (size_t)samples * sizeof(PathState)));
-        HIP_TRY(c, hipMalloc(&c->wf_queue_b, (size_t)samples * sizeof(PathState)));
-        HIP_TRY(c, hipMalloc(&c->wf_accum, (size_t)samples * sizeof(float4)));
-        c->wf_cap = samples;
+        HIP_TRY(c, hipMalloc(&c->wf_queue_a, (size_t)slots * sizeof(PathState)));
+        HIP_TRY(c, hipMalloc(&c->wf_queue_b, (size_t)slots * sizeof(PathState)));
+        HIP_TRY(c, hipMalloc(&c->wf_accum, (size_t)ws * hs * sizeof(float4)));
+        c->wf_cap = slots;
     }
-    if (!c->wf_counters) HIP_TRY(c, hipMalloc(&c->wf_counters, WF_COUNTERS * sizeof(uint32_t)));
-    const uint32_t tiles_x = (ws + kTile - 1) / kTile, tiles_y = (hs + kTile - 1) / kTile;
-    const uint32_t tiles = tiles_x * tiles_y, per_xcd = (tiles + 7) / 8;
+    if (!c->wf_counters) HIP_TRY(c, hipMalloc(&c->wf_counters, 8 * WF_COUNTERS * sizeof(uint32_t)));
     const uint32_t scene_depth = c->scene_height + 1, mesh_depth = c->mesh_height + 1;
     const size_t lds = (size_t)(scene_depth + mesh_depth) * kBlock * sizeof(uint32_t);
     if (lds > 160 * 1024) return fail(c, SLIMRT_E_ARG, "BVH too deep for the LDS traversal stacks");
@@ -280,44 +329,40 @@ int launch_wavefront(slimrt_context *c, const RenderParams &rp, float4 *pixels, 
     PathState *qa = (PathState *)c->wf_queue_a, *qb = (PathState *)c->wf_queue_b;
     float4 *accum = (float4 *)c->wf_accum;
     uint32_t *cnt = c->wf_counters;
-    hipLaunchKernelGGL(wf_reset_kernel, dim3(1), dim3(64), 0, c->stream, cnt);
-    hipLaunchKernelGGL(wf_primary_kernel, dim3(per_xcd * 8), dim3(kBlock), lds, c->stream, v, rp, pixels, depths, accum,
-                       qa, cnt, samples, scene_depth, mesh_depth, tiles_x, tiles, per_xcd, c->d_status);
+    const uint32_t blocks = (uint32_t)cus * 2;
+    hipLaunchKernelGGL(wf_reset_kernel, dim3(1), dim3(256), 0, c->stream, cnt, bands * WF_COUNTERS);
     HIP_TRY(c, hipGetLastError());
-    const uint32_t bounce_blocks = (uint32_t)cus * 2, sort_blocks = (uint32_t)cus * 2;
-    // queue A holds the paths to run; sorting moves them to B (emptying A), so `in` is B after a sort
-    auto sort_a_into_b = [&]() {
-        hipLaunchKernelGGL(wf_bin_count_kernel, dim3(sort_blocks), dim3(256), 0, c->stream, qa, cnt, samples);
-        hipLaunchKernelGGL(wf_bin_scan_kernel, dim3(1), dim3(64), 0, c->stream, cnt);
-        hipLaunchKernelGGL(wf_bin_scatter_kernel, dim3(sort_blocks), dim3(256), 0, c->stream, qa, qb, cnt, samples);
-        hipLaunchKernelGGL(wf_clear_count_kernel, dim3(1), dim3(64), 0, c->stream, cnt, (uint32_t)WF_COUNT_A);
-    };
-    if (rp.max_depth < 2) return SLIMRT_OK;
-    if (!per_bounce) {
-        if (sort) sort_a_into_b();
-        hipLaunchKernelGGL(wf_bounce_kernel<true>, dim3(bounce_blocks), dim3(kBlock), lds, c->stream, v, rp, pixels, accum,
-                           (const PathState *)(sort ? qb : qa), (const uint32_t *)(cnt + (sort ? WF_COUNT_B : WF_COUNT_A)),
-                           sort ? qa : qb, cnt + (sort ? WF_COUNT_A : WF_COUNT_B), samples, scene_depth, mesh_depth,
+    hipStream_t side = c->stream;
+    if (bands > 1) {   // fork: the side stream starts after everything queued on the caller's stream so far
+        if (!c->side_stream) HIP_TRY(c, hipStreamCreateWithFlags(&c->side_stream, hipStreamNonBlocking));
+        if (!c->wf_fork) HIP_TRY(c, hipEventCreateWithFlags(&c->wf_fork, hipEventDisableTiming));
+        if (!c->wf_join) HIP_TRY(c, hipEventCreateWithFlags(&c->wf_join, hipEventDisableTiming));
+        for (uint32_t b = 0; b < bands; b++)
+            if (!c->wf_band[b]) HIP_TRY(c, hipEventCreateWithFlags(&c->wf_band[b], hipEventDisableTiming));
+        side = c->side_stream;
+        HIP_TRY(c, hipEventRecord(c->wf_fork, c->stream));
+        HIP_TRY(c, hipStreamWaitEvent(side, c->wf_fork, 0));
+    }
+    for (uint32_t b = 0; b < bands; b++) {
+        const uint32_t row0 = tiles_y * b / bands, row1 = tiles_y * (b + 1) / bands;
+        const uint32_t tile_base = row0 * tiles_x, tiles = (row1 - row0) * tiles_x, per_xcd = (tiles + 7) / 8;
+        const uint32_t base = tile_base * kBlock, cap = tiles * kBlock;
+        uint32_t *bc = cnt + b * WF_COUNTERS;
+        hipLaunchKernelGGL(wf_primary_kernel, dim3(per_xcd * 8), dim3(kBlock), lds, c->stream, v, rp, pixels, depths,
+                           accum, qa + base, bc, cap, scene_depth, mesh_depth, tiles_x, tile_base, tiles, per_xcd,
                            c->d_status);
         HIP_TRY(c, hipGetLastError());
-        return SLIMRT_OK;
-    }
-    uint32_t cur_count = WF_COUNT_A, next_count = WF_COUNT_B;   // unsorted: ping-pong between A and B
-    for (uint32_t it = 1; it < rp.max_depth; it++) {
-        if (sort) {   // A -> sorted B (A emptied), then B -> A
-            sort_a_into_b();
-            hipLaunchKernelGGL(wf_bounce_kernel<false>, dim3(bounce_blocks), dim3(kBlock), lds, c->stream, v, rp, pixels,
-                               accum, (const PathState *)qb, (const uint32_t *)(cnt + WF_COUNT_B), qa, cnt + WF_COUNT_A,
-                               samples, scene_depth, mesh_depth, c->d_status);
-        } else {
-            PathState *in = cur_count == WF_COUNT_A ? qa : qb, *out = cur_count == WF_COUNT_A ? qb : qa;
-            hipLaunchKernelGGL(wf_clear_count_kernel, dim3(1), dim3(64), 0, c->stream, cnt, next_count);
-            hipLaunchKernelGGL(wf_bounce_kernel<false>, dim3(bounce_blocks), dim3(kBlock), lds, c->stream, v, rp, pixels,
-                               accum, (const PathState *)in, (const uint32_t *)(cnt + cur_count), out, cnt + next_count,
-                               samples, scene_depth, mesh_depth, c->d_status);
-            std::swap(cur_count, next_count);
+        if (bands > 1) {
+            HIP_TRY(c, hipEventRecord(c->wf_band[b], c->stream));
+            HIP_TRY(c, hipStreamWaitEvent(side, c->wf_band[b], 0));
         }
-        HIP_TRY(c, hipGetLastError());
+        int rc = enqueue_bounces(c, side, v, rp, pixels, accum, qa + base, qb + base, bc, cap, lds, scene_depth,
+                                 mesh_depth, blocks, per_bounce, sort);
+        if (rc) return rc;
+    }
+    if (bands > 1) {   // join: the caller's stream continues after the last bounce launch
+        HIP_TRY(c, hipEventRecord(c->wf_join, side));
+        HIP_TRY(c, hipStreamWaitEvent(c->stream, c->wf_join, 0));
     }
     return SLIMRT_OK;
 }

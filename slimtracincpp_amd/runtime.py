@@ -77,6 +77,10 @@ SCENE_REFIT, SCENE_REBUILD = 0, 1
 WAVEFRONT, WAVEFRONT_PER_BOUNCE, WAVEFRONT_SORT = 2, 4, 8   # slimrt_render_params.flags, with RAY_CONES
 
 
+def WAVEFRONT_BANDS(log2n: int) -> int:
+    return (log2n & 3) << 4
+
+
 # RayTracingRenderer's mip_level_colors (src/slim/renderer/renderer.h:50-58) as Color(ColorID) values
 MIP_LEVEL_COLORS = [(1.0, 0.5, 0.5), (0.75, 0.75, 0.0), (0.5, 1.0, 0.5), (0.75, 0.0, 0.75), (0.0, 0.75, 0.75),
                     (0.5, 0.5, 1.0), (0.75, 0.75, 0.75), (0.5, 0.5, 0.5), (0.25, 0.25, 0.25)]

@@ -58,8 +58,11 @@ def measured_traffic(width: int, height: int):
     return int(m.group(1)) if m else None
 
 
-def algorithmic_bytes(cnt: dict) -> int:
-    return sum(cnt[k] * b for k, b in BYTES_PER.items()) + cnt["pixels"] * OUT_BYTES_PER_SAMPLE
+def algorithmic_bytes(cnt: dict, wide: bool = False) -> int:
+    """Bytes the frame's work reads and writes.  With 4-wide mesh BVHs the 'pairs' counter counts 128-byte node
+    visits instead of 64-byte sibling-pair visits."""
+    per = dict(BYTES_PER, pairs=128) if wide else BYTES_PER
+    return sum(cnt[k] * b for k, b in per.items()) + cnt["pixels"] * OUT_BYTES_PER_SAMPLE
 
 
 def cpu_baseline(spec, rays_per_frame: int, threads: int) -> dict:
@@ -219,7 +222,9 @@ def large_leg(stream, frames: int = 5) -> dict:
     forms = {}
     for name, flags in (("megakernel", 0), ("wavefront", runtime.WAVEFRONT),
                         ("wavefront_sorted", runtime.WAVEFRONT | runtime.WAVEFRONT_SORT),
-                        ("wavefront_per_bounce", runtime.WAVEFRONT | runtime.WAVEFRONT_PER_BOUNCE)):
+                        ("wavefront_per_bounce", runtime.WAVEFRONT | runtime.WAVEFRONT_PER_BOUNCE),
+                        ("megakernel_bvh4", runtime.BVH4), ("wavefront_bvh4", runtime.WAVEFRONT | runtime.BVH4),
+                        ("wavefront_sorted_bvh4", runtime.WAVEFRONT | runtime.WAVEFRONT_SORT | runtime.BVH4)):
         p.flags = flags
         ts = []
         for i in range(frames + 1):
@@ -326,15 +331,22 @@ def main():
             torch.mul(src, 2.0, out=dst)
         torch.cuda.synchronize()
         del src, dst
-    cnt = r.count_work(p)
+    cnt = r.count_work(p)                       # sibling-pair walk (equal to the oracle's counts)
+    p.flags = runtime.BVH4
+    cnt4 = r.count_work(p)                      # 4-wide mesh walk: 'pairs' counts node visits
+    p.flags = 0
 
-    # Five forms of the same frame, bit-identical outputs (tests/test_wavefront.py): the per-sample megakernel and
-    # the bounces as a wavefront over a compacted ray queue (run to the end in one launch, or re-compacted after
-    # every bounce; sorted by direction octant or not).  Each is timed over K back-to-back frames between
-    # synchronisations; the fastest is the reported path.
-    forms = {"megakernel": 0, "wavefront": runtime.WAVEFRONT, "wavefront_sorted": runtime.WAVEFRONT | runtime.WAVEFRONT_SORT,
-             "wavefront_per_bounce": runtime.WAVEFRONT | runtime.WAVEFRONT_PER_BOUNCE,
-             "wavefront_per_bounce_sorted": runtime.WAVEFRONT | runtime.WAVEFRONT_PER_BOUNCE | runtime.WAVEFRONT_SORT}
+    # Ten forms of the same frame (tests/test_wavefront.py, tests/test_bvh4.py): the per-sample megakernel and the
+    # bounces as a wavefront over a compacted ray queue (run to the end in one launch, or re-compacted after every
+    # bounce; sorted by direction octant or not), each with the meshes' sibling-pair BVHs or their 4-wide
+    # collapse.  Each is timed over K back-to-back frames between synchronisations; the fastest is the reported
+    # path.
+    base_forms = {"megakernel": 0, "wavefront": runtime.WAVEFRONT,
+                  "wavefront_sorted": runtime.WAVEFRONT | runtime.WAVEFRONT_SORT,
+                  "wavefront_per_bounce": runtime.WAVEFRONT | runtime.WAVEFRONT_PER_BOUNCE,
+                  "wavefront_per_bounce_sorted": runtime.WAVEFRONT | runtime.WAVEFRONT_PER_BOUNCE | runtime.WAVEFRONT_SORT}
+    forms = dict(base_forms)
+    forms.update({k + "_bvh4": v | runtime.BVH4 for k, v in base_forms.items()})
     timings = {}
     for name, flags in forms.items():
         p.flags = flags
@@ -363,9 +375,10 @@ def main():
     ms_step, wall_ms, k_ms = (timings[form][k] for k in ("ms_per_step", "wall_ms_per_step", "kernel_ms"))
     p.flags = forms[form]
 
+    wide = form.endswith("_bvh4")
     rays = cnt["rays"]
     mrays = rays / (ms_step * 1e-3) / 1e6
-    bytes_frame = algorithmic_bytes(cnt)
+    bytes_frame = algorithmic_bytes(cnt4 if wide else cnt, wide)
     achieved = bytes_frame / (k_ms * 1e-3) / 1e9
     result = {
         "metric": "Mrays/s + ms/frame at 1080p, plain vs neural-cache at PSNR>=40 dB vs plain",
@@ -389,7 +402,7 @@ def main():
                    "forms_ms_per_step": {k: v["ms_per_step"] for k, v in timings.items()}},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": measured_traffic(args.width, args.height),
-                     "kernel": {"megakernel": "render_kernel<false>",
+                     "kernel": ("4-wide mesh BVHs; " if wide else "") + {"megakernel": "render_kernel<false>",
                                 "wavefront": "the frame's launches: wf_primary_kernel + one wf_bounce_kernel<true> "
                                              "over the compacted bounce queue",
                                 "wavefront_sorted": "the frame's launches: wf_primary_kernel + octant sort (count, "
@@ -397,9 +410,9 @@ def main():
                                 "wavefront_per_bounce": "the frame's launches: wf_primary_kernel + "
                                                         "wf_bounce_kernel<false> per bounce",
                                 "wavefront_per_bounce_sorted": "the frame's launches: wf_primary_kernel + per bounce "
-                                                               "octant sort + wf_bounce_kernel<false>"}[form],
+                                                               "octant sort + wf_bounce_kernel<false>"}[form.replace("_bvh4", "")],
                      "kernel_ms": k_ms, "algorithmic_bytes": bytes_frame,
-                     "counters": cnt},
+                     "counters": cnt4 if wide else cnt},
         "neural_cache": None,
     }
     if not args.no_views:

@@ -58,6 +58,8 @@ extern "C" {
 /* with SLIMRT_WAVEFRONT: the frame in 2^log2n horizontal bands; a band's bounces run on a second stream while
  * the next band's first hits are traced, so the bounce launches' latency tails overlap the primary work */
 #define SLIMRT_WAVEFRONT_BANDS(log2n) (((uint32_t)(log2n) & 3u) << 4)
+#define SLIMRT_BVH4 64u            /* trace meshes through 4-wide BVHs collapsed from the uploaded ones (same leaves
+                                      and triangles; only exact ties between triangles can resolve differently) */
 
 typedef struct slimrt_context slimrt_context;
 
@@ -162,7 +164,7 @@ typedef struct {
     uint32_t render_mode;
     int32_t skybox_color, skybox_radiance, skybox_irradiance;   /* texture ids, -1 for none */
     uint32_t output;            /* SLIMRT_OUTPUT_* */
-    uint32_t flags;             /* SLIMRT_RAY_CONES | SLIMRT_WAVEFRONT (| _PER_BOUNCE | _SORT | _BANDS(n)) */
+    uint32_t flags;             /* SLIMRT_RAY_CONES | SLIMRT_WAVEFRONT (| _PER_BOUNCE | _SORT | _BANDS(n)) | SLIMRT_BVH4 */
     float mip_level_colors[9][3];
 } slimrt_render_params;
 

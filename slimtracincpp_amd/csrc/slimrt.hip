@@ -31,7 +31,7 @@ static_assert(sizeof(RenderParams) == sizeof(slimrt_render_params), "render para
 namespace {
 
 
-template <bool kCount>
+template <bool kCount, bool WIDE>
 __global__ __launch_bounds__(kBlock) void render_kernel(SceneView s, RenderParams rp, float4 *__restrict__ pixels,
                                                        float *__restrict__ depths, uint32_t scene_depth,
                                                        uint32_t mesh_depth, uint32_t tiles_x, uint32_t tiles,
@@ -47,7 +47,7 @@ __global__ __launch_bounds__(kBlock) void render_kernel(SceneView s, RenderParam
     const uint32_t ws = rp.width << rp.ssaa, hs = rp.height << rp.ssaa;
 
     Counters cnt{};
-    Tracer tr;
+    TracerT<WIDE> tr;
     tr.scene_stack = Stack{lds + t, kBlock};
     tr.mesh_stack = Stack{lds + scene_depth * kBlock + t, kBlock};
     tr.scene_depth = scene_depth;
@@ -75,6 +75,7 @@ __global__ __launch_bounds__(kBlock) void render_kernel(SceneView s, RenderParam
 // ---------------------------------------------------------------- rendering with the neural radiance cache
 // The frame kernel has the render kernel's tiling; one in train_one_in waves (chosen by a hash of the wave's
 // position and the frame) traces full paths for training, so training and cached pixels never share a wave.
+template <bool WIDE>
 __global__ __launch_bounds__(kBlock) void nrc_frame_kernel(SceneView s, RenderParams rp, NrcFrame nf,
                                                            float4 *__restrict__ pixels, float *__restrict__ depths,
                                                            uint32_t scene_depth, uint32_t mesh_depth, uint32_t tiles_x,
@@ -94,7 +95,7 @@ __global__ __launch_bounds__(kBlock) void nrc_frame_kernel(SceneView s, RenderPa
     h ^= h >> 12;
     const bool training = nf.train_one_in && (h % nf.train_one_in) == 0;
 
-    Tracer tr;
+    TracerT<WIDE> tr;
     tr.scene_stack = Stack{lds + t, kBlock};
     tr.mesh_stack = Stack{lds + scene_depth * kBlock + t, kBlock};
     tr.scene_depth = scene_depth;
@@ -179,7 +180,7 @@ int launch(slimrt_context *c, const slimrt_render_params *p, float4 *pixels, flo
     const uint32_t tiles_x = (ws + kTile - 1) / kTile, tiles_y = (hs + kTile - 1) / kTile;
     const uint32_t tiles = tiles_x * tiles_y;
     const uint32_t per_xcd = (tiles + 7) / 8;
-    const uint32_t scene_depth = c->scene_height + 1, mesh_depth = c->mesh_height + 1;
+    const uint32_t scene_depth = c->scene_height + 1, mesh_depth = mesh_stack_depth(c, wide_mesh_bvh(rp));
     const size_t lds = (size_t)(scene_depth + mesh_depth) * kBlock * sizeof(uint32_t);
     if (lds > 160 * 1024) return fail(c, SLIMRT_E_ARG, "BVH too deep for the LDS traversal stacks");
     SceneView v = device_view(c);
@@ -188,12 +189,17 @@ int launch(slimrt_context *c, const slimrt_render_params *p, float4 *pixels, flo
         int rc = launch_wavefront(c, rp, pixels, depths, (rp.flags & SLIMRT_WAVEFRONT_PER_BOUNCE) != 0,
                                   (rp.flags & SLIMRT_WAVEFRONT_SORT) != 0, 1u << ((rp.flags >> 4) & 3u));
         if (rc) return rc;
-    } else if (count) {
-        hipLaunchKernelGGL(render_kernel<true>, dim3(per_xcd * 8), dim3(kBlock), lds, c->stream, v, rp, pixels, depths,
-                           scene_depth, mesh_depth, tiles_x, tiles, per_xcd, c->d_status, c->d_counters);
     } else {
-        hipLaunchKernelGGL(render_kernel<false>, dim3(per_xcd * 8), dim3(kBlock), lds, c->stream, v, rp, pixels, depths,
-                           scene_depth, mesh_depth, tiles_x, tiles, per_xcd, c->d_status, c->d_counters);
+#define RENDER_LAUNCH(COUNT, WIDE)                                                                              \
+    hipLaunchKernelGGL((render_kernel<COUNT, WIDE>), dim3(per_xcd * 8), dim3(kBlock), lds, c->stream, v, rp, pixels, \
+                       depths, scene_depth, mesh_depth, tiles_x, tiles, per_xcd, c->d_status, c->d_counters)
+        const bool wide = wide_mesh_bvh(rp);
+        if (count) {
+            if (wide) RENDER_LAUNCH(true, true); else RENDER_LAUNCH(true, false);
+        } else {
+            if (wide) RENDER_LAUNCH(false, true); else RENDER_LAUNCH(false, false);
+        }
+#undef RENDER_LAUNCH
     }
     HIP_TRY(c, hipGetLastError());
     if (c->timed) HIP_TRY(c, hipEventRecord(c->ev1, c->stream));
@@ -219,7 +225,7 @@ int cpu_render(slimrt_context *c, const slimrt_render_params *p, slimrt_pixel *p
         tr.mesh_depth = md;
         tr.overflow = 0;
         per[ti] = Counters{};
-        tr.cnt = counters ? &per[ti] : nullptr;
+        tr.cnt = counters ? &per[ti] : nullptr;   // the host context runs the reference's sibling-pair walk
         for (uint32_t y = ti; y < hs; y += nthreads)
             for (uint32_t x = 0; x < ws; x++) {
                 float px[4], depth;
@@ -245,6 +251,55 @@ int cpu_render(slimrt_context *c, const slimrt_render_params *p, slimrt_pixel *p
     for (uint32_t o : overflow)
         if (o) return fail(c, SLIMRT_E_OVERFLOW, "traversal stack overflow");
     return SLIMRT_OK;
+}
+
+// Collapses one mesh's sibling-pair BVH (nodes, mesh-relative indices) into 4-wide nodes appended to out: a
+// node's children are its two children with, while fewer than four, the inner child of largest surface area
+// replaced by its own two children.  Leaves keep their triangle ranges.  Returns the node index; height4 receives
+// the collapsed tree's height.
+uint32_t collapse4(const Node *n2, uint32_t i2, uint32_t depth, std::vector<Node4> &out, uint32_t &height4) {
+    auto is_leaf = [&](uint32_t k) { return (n2[k].leaf_count_depth & 0xFFFFu) != 0; };
+    auto area = [&](uint32_t k) {
+        const float ex = n2[k].maxx - n2[k].minx, ey = n2[k].maxy - n2[k].miny, ez = n2[k].maxz - n2[k].minz;
+        return ex * ey + ey * ez + ez * ex;
+    };
+    uint32_t kids[4], nk = 2;
+    kids[0] = n2[i2].first_index;
+    kids[1] = n2[i2].first_index + 1;
+    while (nk < 4) {
+        int pick = -1;
+        for (uint32_t j = 0; j < nk; j++)
+            if (!is_leaf(kids[j]) && (pick < 0 || area(kids[j]) > area(kids[pick]))) pick = (int)j;
+        if (pick < 0) break;
+        const uint32_t k = kids[pick];
+        kids[pick] = n2[k].first_index;
+        kids[nk++] = n2[k].first_index + 1;
+    }
+    const uint32_t idx = (uint32_t)out.size();
+    out.push_back(Node4{});
+    height4 = std::max(height4, depth);
+    Node4 node;
+    for (uint32_t j = 0; j < 4; j++) {
+        if (j >= nk) {
+            node.minx[j] = node.miny[j] = node.minz[j] = INFINITY;
+            node.maxx[j] = node.maxy[j] = node.maxz[j] = -INFINITY;
+            node.child[j] = ~0u;
+            node.count[j] = 0;
+            continue;
+        }
+        const Node &c = n2[kids[j]];
+        node.minx[j] = c.minx; node.miny[j] = c.miny; node.minz[j] = c.minz;
+        node.maxx[j] = c.maxx; node.maxy[j] = c.maxy; node.maxz[j] = c.maxz;
+        if (is_leaf(kids[j])) {
+            node.child[j] = c.first_index;
+            node.count[j] = c.leaf_count_depth & 0xFFFFu;
+        } else {
+            node.child[j] = collapse4(n2, kids[j], depth + 1, out, height4);
+            node.count[j] = 0;
+        }
+    }
+    out[idx] = node;
+    return idx;
 }
 
 int ensure_outputs(slimrt_context *c, size_t n) {
@@ -351,6 +406,7 @@ int slimrt_destroy(slimrt_context *c) {
     if (!c->cpu) {
         (void)hipStreamSynchronize(c->stream);
         dev_free(c->d_geos); dev_free(c->d_mats); dev_free(c->d_lights); dev_free(c->d_snodes); dev_free(c->d_mnodes);
+        dev_free(c->d_mnodes4);
         dev_free(c->d_leaf); dev_free(c->d_meshes); dev_free(c->d_tri_hit); dev_free(c->d_tri_attr); dev_free(c->d_tex);
         dev_free(c->d_mips); dev_free(c->d_quads); dev_free(c->d_pixels); dev_free(c->d_depths);
         free_nrc(c);
@@ -437,8 +493,9 @@ int slimrt_upload_scene(slimrt_context *c, const slimrt_scene_desc *d) {
         return fail(c, SLIMRT_E_ARG, "a scene needs geometries and a scene BVH");
     c->loaded = false;
     // meshes: concatenate nodes and split triangle records
-    c->meshes.clear(); c->mnodes.clear(); c->tri_hit.clear(); c->tri_attr.clear();
+    c->meshes.clear(); c->mnodes.clear(); c->tri_hit.clear(); c->tri_attr.clear(); c->mnodes4.clear();
     c->mesh_height = 0;
+    c->mesh_height4 = 0;
     for (uint32_t k = 0; k < d->mesh_count; k++) {
         const slimrt_mesh &m = d->meshes[k];
         if (!m.nodes || !m.triangles || m.node_count == 0 || m.triangle_count == 0)
@@ -450,7 +507,6 @@ int slimrt_upload_scene(slimrt_context *c, const slimrt_scene_desc *d) {
         r.tri_base = (uint32_t)c->tri_hit.size();
         r.node_count = m.node_count; r.tri_count = m.triangle_count;
         r.normals_count = m.normals_count; r.uvs_count = m.uvs_count;
-        c->meshes.push_back(r);
         for (uint32_t i = 0; i < m.node_count; i++) {
             const slimrt_bvh_node &n = m.nodes[i];
             const bool bad = n.leaf_count ? (n.first_index + n.leaf_count > m.triangle_count)
@@ -458,6 +514,32 @@ int slimrt_upload_scene(slimrt_context *c, const slimrt_scene_desc *d) {
             if (bad) return fail(c, SLIMRT_E_ARG, "mesh BVH node index out of range");
             c->mnodes.push_back(to_node(n));
         }
+        r.node4_base = (uint32_t)c->mnodes4.size();
+        uint32_t h4 = 1;
+        const Node *n2 = c->mnodes.data() + r.node_base;
+        if (n2[0].leaf_count_depth & 0xFFFFu) {   // a single leaf: one 4-wide node holding it
+            Node4 leaf;
+            for (uint32_t j = 0; j < 4; j++) {
+                leaf.minx[j] = leaf.miny[j] = leaf.minz[j] = INFINITY;
+                leaf.maxx[j] = leaf.maxy[j] = leaf.maxz[j] = -INFINITY;
+                leaf.child[j] = ~0u;
+                leaf.count[j] = 0;
+            }
+            leaf.minx[0] = n2[0].minx; leaf.miny[0] = n2[0].miny; leaf.minz[0] = n2[0].minz;
+            leaf.maxx[0] = n2[0].maxx; leaf.maxy[0] = n2[0].maxy; leaf.maxz[0] = n2[0].maxz;
+            leaf.child[0] = 0;
+            leaf.count[0] = m.triangle_count;
+            c->mnodes4.push_back(leaf);
+        } else {
+            std::vector<Node4> out;
+            collapse4(n2, 0, 1, out, h4);
+            c->mnodes4.insert(c->mnodes4.end(), out.begin(), out.end());
+        }
+        r.node4_count = (uint32_t)c->mnodes4.size() - r.node4_base;
+        r.height4 = h4;
+        r.pad = 0;
+        c->mesh_height4 = std::max(c->mesh_height4, h4);
+        c->meshes.push_back(r);
         for (uint32_t i = 0; i < m.triangle_count; i++) {
             const slimrt_triangle &t = m.triangles[i];
             const float *L = t.local_to_tangent;
@@ -501,6 +583,7 @@ int slimrt_upload_scene(slimrt_context *c, const slimrt_scene_desc *d) {
     if (!c->cpu) {
         if ((rc = dev_upload(c, c->d_meshes, c->meshes.data(), c->meshes.size()))) return rc;
         if ((rc = dev_upload(c, c->d_mnodes, c->mnodes.data(), c->mnodes.size()))) return rc;
+        if ((rc = dev_upload(c, c->d_mnodes4, c->mnodes4.data(), c->mnodes4.size()))) return rc;
         if ((rc = dev_upload(c, c->d_tri_hit, c->tri_hit.data(), c->tri_hit.size()))) return rc;
         if ((rc = dev_upload(c, c->d_tri_attr, c->tri_attr.data(), c->tri_attr.size()))) return rc;
         if ((rc = dev_upload(c, c->d_tex, c->textures.data(), c->textures.size()))) return rc;
@@ -588,7 +671,7 @@ int slimrt_render_nrc(slimrt_context *c, const slimrt_render_params *p, slimrt_n
     const uint32_t samples = ws * hs;
     const uint32_t tiles_x = (ws + kTile - 1) / kTile, tiles_y = (hs + kTile - 1) / kTile;
     const uint32_t tiles = tiles_x * tiles_y, per_xcd = (tiles + 7) / 8;
-    const uint32_t scene_depth = c->scene_height + 1, mesh_depth = c->mesh_height + 1;
+    const uint32_t scene_depth = c->scene_height + 1, mesh_depth = mesh_stack_depth(c, wide_mesh_bvh(rp));
     const size_t lds = (size_t)(scene_depth + mesh_depth) * kBlock * sizeof(uint32_t);
     if (lds > 160 * 1024) return fail(c, SLIMRT_E_ARG, "BVH too deep for the LDS traversal stacks");
     // training waves are a hashed 1-in-N subset: twice the expected share (at least 64 waves) bounds the samples
@@ -625,8 +708,14 @@ int slimrt_render_nrc(slimrt_context *c, const slimrt_render_params *p, slimrt_n
     }
     hipLaunchKernelGGL(nrc_reset_kernel, dim3(1), dim3(64), 0, c->stream, c->nrc_counters, c->nrc_losses, kMaxTrainSteps);
     HIP_TRY(c, hipGetLastError());
-    hipLaunchKernelGGL(nrc_frame_kernel, dim3(per_xcd * 8), dim3(kBlock), lds, c->stream, device_view(c), rp, nf,
-                       (float4 *)d_pixels, (float *)d_depths, scene_depth, mesh_depth, tiles_x, tiles, per_xcd, c->d_status);
+    if (wide_mesh_bvh(rp))
+        hipLaunchKernelGGL(nrc_frame_kernel<true>, dim3(per_xcd * 8), dim3(kBlock), lds, c->stream, device_view(c), rp,
+                           nf, (float4 *)d_pixels, (float *)d_depths, scene_depth, mesh_depth, tiles_x, tiles, per_xcd,
+                           c->d_status);
+    else
+        hipLaunchKernelGGL(nrc_frame_kernel<false>, dim3(per_xcd * 8), dim3(kBlock), lds, c->stream, device_view(c), rp,
+                           nf, (float4 *)d_pixels, (float *)d_depths, scene_depth, mesh_depth, tiles_x, tiles, per_xcd,
+                           c->d_status);
     HIP_TRY(c, hipGetLastError());
     if (np->cache_depth + 1 <= p->max_depth) {
         if (slimrt_nrc_infer_queue(nrc, c->q_raw, c->nrc_counters + NC_QUERIES, samples, c->q_out))

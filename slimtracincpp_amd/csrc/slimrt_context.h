@@ -46,12 +46,15 @@ struct slimrt_context {
     std::vector<MipRec> mips;
     std::vector<uint8_t> quads;
     uint32_t scene_height = 0, mesh_height = 0, scene_flags = 0;
+    std::vector<Node4> mnodes4;             // 4-wide mesh BVHs collapsed from mnodes at upload
+    uint32_t mesh_height4 = 0;
     bool loaded = false;
 
     DevBuf<Geo> d_geos;
     DevBuf<Mat> d_mats;
     DevBuf<Light> d_lights;
     DevBuf<Node> d_snodes, d_mnodes;
+    DevBuf<Node4> d_mnodes4;
     DevBuf<uint32_t> d_leaf;
     DevBuf<MeshRec> d_meshes;
     DevBuf<TriHit> d_tri_hit;
@@ -160,10 +163,17 @@ inline SceneView host_view(const slimrt_context *c) {
     v.geos = c->geos.data(); v.mats = c->mats.data(); v.lights = c->lights.data();
     v.snodes = c->snodes.data(); v.leaf_ids = c->leaf_ids.data(); v.meshes = c->meshes.data();
     v.mnodes = c->mnodes.data(); v.tri_hit = c->tri_hit.data(); v.tri_attr = c->tri_attr.data();
+    v.mnodes4 = c->mnodes4.data();
     v.textures = c->textures.data(); v.mips = c->mips.data(); v.quads = c->quads.data();
     v.G = (uint32_t)c->geos.size(); v.M = (uint32_t)c->mats.size(); v.L = (uint32_t)c->lights.size();
     v.NM = (uint32_t)c->meshes.size(); v.NT = (uint32_t)c->textures.size(); v.flags = c->scene_flags;
     return v;
+}
+
+// Traversal stack entries for mesh BVHs: the declared height + 1 for the sibling-pair walk; for the 4-wide walk
+// up to three pending siblings per level of the collapsed tree, plus the children of the node being visited.
+inline uint32_t mesh_stack_depth(const slimrt_context *c, bool wide) {
+    return wide ? 3 * c->mesh_height4 + 4 : c->mesh_height + 1;
 }
 
 inline SceneView device_view(const slimrt_context *c) {
@@ -171,6 +181,7 @@ inline SceneView device_view(const slimrt_context *c) {
     v.geos = c->d_geos.ptr; v.mats = c->d_mats.ptr; v.lights = c->d_lights.ptr;
     v.snodes = c->d_snodes.ptr; v.leaf_ids = c->d_leaf.ptr; v.meshes = c->d_meshes.ptr;
     v.mnodes = c->d_mnodes.ptr; v.tri_hit = c->d_tri_hit.ptr; v.tri_attr = c->d_tri_attr.ptr;
+    v.mnodes4 = c->d_mnodes4.ptr;
     v.textures = c->d_tex.ptr; v.mips = c->d_mips.ptr; v.quads = c->d_quads.ptr;
     return v;
 }

@@ -67,11 +67,20 @@ struct alignas(16) Light {
     float r, g, b, intensity;
     float x, y, z; uint32_t flags;
 };
-// Mesh record, 48 bytes: object-space AABB, node/triangle bases in the concatenated arrays, counts.
+// Mesh record, 64 bytes: object-space AABB, node/triangle bases in the concatenated arrays, counts, and the
+// base of the mesh's 4-wide nodes.
 struct alignas(16) MeshRec {
     float minx, miny, minz; uint32_t node_base;
     float maxx, maxy, maxz; uint32_t tri_base;
     uint32_t node_count, tri_count, normals_count, uvs_count;
+    uint32_t node4_base, node4_count, height4, pad;
+};
+// 4-wide mesh BVH node, 128 bytes: the children's boxes as structure of arrays, then per child either a node
+// index (count 0) or a leaf's first triangle and triangle count; unused slots have child = ~0u.
+struct alignas(16) Node4 {
+    float minx[4], miny[4], minz[4];
+    float maxx[4], maxy[4], maxz[4];
+    uint32_t child[4], count[4];
 };
 // Triangle split into the 64 bytes every intersection test reads and the 64 bytes only the closest
 // hit reads (Triangle is 128 bytes, src/slim/scene/mesh.h:19-24).
@@ -98,6 +107,7 @@ struct SceneView {
     const uint32_t *leaf_ids;    // scene leaf order -> geometry index
     const MeshRec *meshes;
     const Node *mnodes;          // all mesh BVH nodes
+    const Node4 *mnodes4;        // all 4-wide mesh BVH nodes (collapsed from mnodes)
     const TriHit *tri_hit;
     const TriAttr *tri_attr;
     const TexRec *textures;
@@ -121,11 +131,13 @@ struct RenderParams {
     int32_t sky_color, sky_radiance, sky_irradiance;
     uint32_t output;             // 0 canvas Pixel (clamp(c)^2, 1), 1 radiance (tone-mapped colour, 1)
     uint32_t flags;              // bit 0: keep the pixel's ray-cone scaling factor through hits (else the
-                                 // reference data flow, which overwrites it on every hit); bits 1-3: wavefront
+                                 // reference data flow, which overwrites it on every hit); bits 1-5: wavefront
+                                 // forms; bit 6: 4-wide mesh BVHs
     float mip_colors[9][3];
 };
 
 SLIM_HD bool keeps_ray_cones(const RenderParams &rp) { return (rp.flags & 1u) != 0; }
+SLIM_HD bool wide_mesh_bvh(const RenderParams &rp) { return (rp.flags & 64u) != 0; }
 
 // ------------------------------------------------------------------ small algebra
 struct v3 { float x, y, z; };

@@ -94,7 +94,8 @@ __device__ __forceinline__ void write_record(float *dst, const Surface &sf) {
 
 // The beauty integrator with a cache vertex (query) or full-path sample recording (training).  Returns true
 // when the pixel is final (col holds its untone-mapped colour), false when a query was appended.
-__device__ __forceinline__ bool render_beauty_nrc(const SceneView &s, const RenderParams &rp, v3 direction, float scaling, Tracer &tr,
+template <class TR>
+__device__ __forceinline__ bool render_beauty_nrc(const SceneView &s, const RenderParams &rp, v3 direction, float scaling, TR &tr,
                                   const NrcFrame &nf, bool training, uint32_t pixel, color &col, float &depth) {
     Surface sf;
     Ray ray;

@@ -21,7 +21,11 @@ struct Counters {   // work counters, accumulated only when the caller asks for 
     uint64_t rays, pairs, roots, instances, triangles, mesh_hits, taps, pixels;
 };
 
-struct Tracer {
+// The traversal state of one sample.  WIDE selects the meshes' 4-wide BVHs at compile time, so a kernel carries
+// only the mesh walk it uses.
+template <bool WIDE>
+struct TracerT {
+    static constexpr bool kWide = WIDE;
     Stack scene_stack, mesh_stack;
     uint32_t scene_depth, mesh_depth;   // capacities
     uint32_t overflow;                  // set when a push would exceed a capacity
@@ -29,14 +33,16 @@ struct Tracer {
     Hit aux_hit;                        // SceneTracer::aux_hit
     Counters *cnt;                      // null: no counting
 };
+using Tracer = TracerT<false>;
 
 #define SLIM_COUNT(tr, field) do { if ((tr).cnt) (tr).cnt->field++; } while (0)
 
 SLIM_HD Node load_node(const Node *nodes, uint32_t i) { return nodes[i]; }
 
 // MeshTracer::hitTriangles (mesh_tracer.h:23-49); triangle indices are relative to the mesh.
+template <class TR>
 SLIM_HD bool hit_triangles(const SceneView &s, const MeshRec &m, uint32_t first, uint32_t n, float closest,
-                           const Ray &r, Hit &hit, bool any_hit, Tracer &tr) {
+                           const Ray &r, Hit &hit, bool any_hit, TR &tr) {
     bool found = false;
     closest = fmin2(closest, hit.distance);
     Hit th = hit;
@@ -61,9 +67,32 @@ SLIM_HD bool hit_triangles(const SceneView &s, const MeshRec &m, uint32_t first,
     return found;
 }
 
+// The closest hit's interpolated UV and normal (mesh_tracer.h:117-131).
+template <class TR>
+SLIM_HD void mesh_hit_attributes(const SceneView &s, const MeshRec &m, Hit &hit, TR &tr) {
+    if (!(m.normals_count | m.uvs_count)) return;
+    const TriAttr t = s.tri_attr[m.tri_base + hit.id];
+    float a = hit.uv.u, b = hit.uv.v, c = 1 - a - b;
+    SLIM_COUNT(tr, mesh_hits);
+    if (m.uvs_count) {
+        hit.uv.u = t.uv3u * a + (t.uv2u * b + t.uv1u * c);
+        hit.uv.v = t.uv3v * a + (t.uv2v * b + t.uv1v * c);
+    }
+    if (m.normals_count) {
+        hit.normal.x = t.n3x * a + (t.n2x * b + t.n1x * c);
+        hit.normal.y = t.n3y * a + (t.n2y * b + t.n1y * c);
+        hit.normal.z = t.n3z * a + (t.n2z * b + t.n1z * c);
+    }
+}
+
+template <class TR>
+SLIM_HD bool mesh_trace4(const SceneView &s, const MeshRec &m, const Ray &r, Hit &hit, bool any_hit, TR &tr);
+
 // MeshTracer::trace (mesh_tracer.h:51-134)
-SLIM_HD bool mesh_trace(const SceneView &s, uint32_t mesh_id, const Ray &r, Hit &hit, bool any_hit, Tracer &tr) {
+template <class TR>
+SLIM_HD bool mesh_trace(const SceneView &s, uint32_t mesh_id, const Ray &r, Hit &hit, bool any_hit, TR &tr) {
     const MeshRec m = s.meshes[mesh_id];
+    if constexpr (TR::kWide) return mesh_trace4(s, m, r, hit, any_hit, tr);
     const Node *nodes = s.mnodes + m.node_base;
     float ln, rn, lf, rf;
     bool found = false;
@@ -110,25 +139,74 @@ SLIM_HD bool mesh_trace(const SceneView &s, uint32_t mesh_id, const Ray &r, Hit 
             pair = tr.mesh_stack[--top];
         }
     }
-    if (found && !any_hit && (m.normals_count | m.uvs_count)) {
-        const TriAttr t = s.tri_attr[m.tri_base + hit.id];
-        float a = hit.uv.u, b = hit.uv.v, c = 1 - a - b;
-        SLIM_COUNT(tr, mesh_hits);
-        if (m.uvs_count) {
-            hit.uv.u = t.uv3u * a + (t.uv2u * b + t.uv1u * c);
-            hit.uv.v = t.uv3v * a + (t.uv2v * b + t.uv1v * c);
-        }
-        if (m.normals_count) {
-            hit.normal.x = t.n3x * a + (t.n2x * b + t.n1x * c);
-            hit.normal.y = t.n3y * a + (t.n2y * b + t.n1y * c);
-            hit.normal.z = t.n3z * a + (t.n2z * b + t.n1z * c);
-        }
+    if (found && !any_hit) mesh_hit_attributes(s, m, hit, tr);
+    return found;
+}
+
+// The 4-wide traversal of the same mesh: each visit slab-tests the (up to) four children, intersects the leaf
+// children nearest first, enters the nearest inner child and pushes the others far to near.  It reaches the
+// same closest hit as the sibling-pair walk (leaves keep their triangle ranges and each leaf's triangles are
+// clipped to that leaf's box as before); only exact ties between triangles can resolve differently.
+template <class TR>
+SLIM_HD bool mesh_trace4(const SceneView &s, const MeshRec &m, const Ray &r, Hit &hit, bool any_hit, TR &tr) {
+    const Node4 *nodes = s.mnodes4 + m.node4_base;
+    bool found = false;
+    uint32_t cur = 0, top = 0;
+    SLIM_COUNT(tr, roots);
+    for (;;) {
+        SLIM_COUNT(tr, pairs);
+        const Node4 n = nodes[cur];
+        float nd[4] = {0.0f, 0.0f, 0.0f, 0.0f}, fd[4] = {0.0f, 0.0f, 0.0f, 0.0f};
+        bool h[4];
+#pragma unroll
+        for (int i = 0; i < 4; i++)
+            h[i] = n.child[i] != ~0u &&
+                   hits_box_bounds(r, n.minx[i], n.miny[i], n.minz[i], n.maxx[i], n.maxy[i], n.maxz[i], nd[i], fd[i]) &&
+                   nd[i] < hit.distance;
+        // leaves, nearest first (a found hit shortens hit.distance for the later ones)
+        float key[4];
+        uint32_t idx[4] = {0, 1, 2, 3};
+#pragma unroll
+        for (int i = 0; i < 4; i++) key[i] = h[i] ? nd[i] : INFINITY;
+#define SLIM_CSWAP(a, b)                                                             \
+    if (key[b] < key[a]) {                                                           \
+        const float tk = key[a]; key[a] = key[b]; key[b] = tk;                       \
+        const uint32_t ti = idx[a]; idx[a] = idx[b]; idx[b] = ti;                    \
     }
+        SLIM_CSWAP(0, 1) SLIM_CSWAP(2, 3) SLIM_CSWAP(0, 2) SLIM_CSWAP(1, 3) SLIM_CSWAP(1, 2)
+#undef SLIM_CSWAP
+        bool stop = false;
+#pragma unroll
+        for (int k = 0; k < 4; k++) {   // leaf children, nearest first
+            const uint32_t i = idx[k];
+            const uint32_t count = i == 0 ? n.count[0] : i == 1 ? n.count[1] : i == 2 ? n.count[2] : n.count[3];
+            if (!(key[k] < hit.distance) || count == 0 || stop) continue;
+            const uint32_t child = i == 0 ? n.child[0] : i == 1 ? n.child[1] : i == 2 ? n.child[2] : n.child[3];
+            const float far_d = i == 0 ? fd[0] : i == 1 ? fd[1] : i == 2 ? fd[2] : fd[3];
+            if (hit_triangles(s, m, child, count, far_d, r, hit, any_hit, tr)) {
+                found = true;
+                if (any_hit) stop = true;
+            }
+        }
+        if (stop) break;
+#pragma unroll
+        for (int k = 3; k >= 0; k--) {   // inner children, far to near: the nearest ends on top
+            const uint32_t i = idx[k];
+            const uint32_t count = i == 0 ? n.count[0] : i == 1 ? n.count[1] : i == 2 ? n.count[2] : n.count[3];
+            if (!(key[k] < hit.distance) || count != 0) continue;
+            const uint32_t child = i == 0 ? n.child[0] : i == 1 ? n.child[1] : i == 2 ? n.child[2] : n.child[3];
+            if (top < tr.mesh_depth) tr.mesh_stack[top++] = child; else tr.overflow = 1;
+        }
+        if (top == 0) break;
+        cur = tr.mesh_stack[--top];
+    }
+    if (found && !any_hit) mesh_hit_attributes(s, m, hit, tr);
     return found;
 }
 
 // SceneTracer::hitGeometryInLocalSpace (scene_tracer.h:138-165)
-SLIM_HD bool hit_geometry_local(const SceneView &s, const Geo &g, const Ray &r, Hit &hit, bool any_hit, Tracer &tr) {
+template <class TR>
+SLIM_HD bool hit_geometry_local(const SceneView &s, const Geo &g, const Ray &r, Hit &hit, bool any_hit, TR &tr) {
     SLIM_COUNT(tr, instances);
     ray_localize(tr.aux_ray, r.origin, r.direction, g);
     float n, f;
@@ -155,8 +233,9 @@ SLIM_HD bool hit_geometry_local(const SceneView &s, const Geo &g, const Ray &r, 
 }
 
 // SceneTracer::hitGeometries (scene_tracer.h:111-136); returns a geometry index or -1.
+template <class TR>
 SLIM_HD int hit_geometries(const SceneView &s, uint32_t first_leaf, uint32_t n, float closest, const Ray &r, Hit &hit,
-                           bool any_hit, Tracer &tr) {
+                           bool any_hit, TR &tr) {
     int hit_geo = -1;
     const uint32_t vis = any_hit ? GEO_SHADOWING : GEO_VISIBLE;
     tr.aux_hit.distance = fmin2(closest + kEps, hit.distance);
@@ -177,7 +256,8 @@ SLIM_HD int hit_geometries(const SceneView &s, uint32_t first_leaf, uint32_t n, 
 }
 
 // SceneTracer::trace (scene_tracer.h:26-98); returns a geometry index or -1.  Offsets the ray origin.
-SLIM_HD int scene_trace(const SceneView &s, Ray &r, Hit &hit, bool any_hit, float max_distance, Tracer &tr) {
+template <class TR>
+SLIM_HD int scene_trace(const SceneView &s, Ray &r, Hit &hit, bool any_hit, float max_distance, TR &tr) {
     SLIM_COUNT(tr, rays);
     ray_reset(r, scale_add(r.direction, kTraceOffset, r.origin), r.direction);
     hit.distance = max_distance;
@@ -311,7 +391,8 @@ SLIM_HD bool facing_light(Surface &sf, const Light &l) {   // surface_shader.h:1
     return sf.NdotL > 0.0f;
 }
 
-SLIM_HD void shade_from_light(Surface &sf, const Light &l, const SceneView &s, Tracer &tr, color &c) {   // surface_shader.h:43-59
+template <class TR>
+SLIM_HD void shade_from_light(Surface &sf, const Light &l, const SceneView &s, TR &tr, color &c) {   // surface_shader.h:43-59
     if (facing_light(sf, l)) {
         bool lit = true;
         if (l.flags & LIGHT_SHADOWING) {
@@ -328,7 +409,8 @@ SLIM_HD void shade_from_light(Surface &sf, const Light &l, const SceneView &s, T
     }
 }
 
-SLIM_HD void prepare_for_shading(Surface &sf, const SceneView &s, const Ray &r, Hit &hit, Tracer &tr) {   // surface_shader.h:61-107
+template <class TR>
+SLIM_HD void prepare_for_shading(Surface &sf, const SceneView &s, const Ray &r, Hit &hit, TR &tr) {   // surface_shader.h:61-107
     const Geo g = s.geos[sf.geo];
     sf.mat = s.mats[g.material_id];
     const Mat &m = sf.mat;
@@ -476,7 +558,8 @@ SLIM_HD void primary_ray(const Projection &p, uint32_t x, uint32_t y, v3 &dir, f
 }
 
 // renderPixelBeauty (src/slim/renderer/ray_tracer.h:19-108)
-SLIM_HD void render_beauty(const SceneView &s, const RenderParams &rp, v3 direction, float scaling, Tracer &tr,
+template <class TR>
+SLIM_HD void render_beauty(const SceneView &s, const RenderParams &rp, v3 direction, float scaling, TR &tr,
                            color &out, float &depth) {
     Surface sf;
     Ray ray;
@@ -534,7 +617,8 @@ SLIM_HD void render_beauty(const SceneView &s, const RenderParams &rp, v3 direct
 }
 
 // renderPixelDebugMode (src/slim/renderer/ray_tracer.h:110-142)
-SLIM_HD void render_debug(const SceneView &s, const RenderParams &rp, v3 direction, float scaling, Tracer &tr,
+template <class TR>
+SLIM_HD void render_debug(const SceneView &s, const RenderParams &rp, v3 direction, float scaling, TR &tr,
                           color &out, float &depth) {
     Surface sf;
     Ray ray;
@@ -586,7 +670,8 @@ SLIM_HD void output_value(const RenderParams &rp, color c, float out_px[4]) {
 }
 
 // One sample: primary ray, integrator, and the value the canvas or the radiance buffer receives.
-SLIM_HD void render_sample(const SceneView &s, const RenderParams &rp, uint32_t x, uint32_t y, Tracer &tr,
+template <class TR>
+SLIM_HD void render_sample(const SceneView &s, const RenderParams &rp, uint32_t x, uint32_t y, TR &tr,
                            float out_px[4], float &out_depth) {
     v3 dir;
     float scaling;

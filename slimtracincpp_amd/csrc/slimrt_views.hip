@@ -143,9 +143,9 @@ __global__ void counter_reset_kernel(uint32_t *__restrict__ counter) {
 
 // The tile of one view traced by one workgroup, exactly as the single-view render kernel traces it.  The view
 // records are read through the constant address space (invariant loads).
-template <int WX, int WY>
+template <int WX, int WY, class TR>
 __device__ __forceinline__ void trace_view_tile(const SceneView &s, const ViewRec *views, uint32_t nviews, uint32_t tile,
-                                                Tracer &tr, uint32_t wave, uint32_t lane) {
+                                                TR &tr, uint32_t wave, uint32_t lane) {
     typedef const __attribute__((address_space(4))) ViewRec cview;
     typedef const __attribute__((address_space(4))) uint32_t cword;
     cview *cv = (cview *)views;
@@ -179,7 +179,7 @@ __device__ __forceinline__ void trace_view_tile(const SceneView &s, const ViewRe
 // Multi-view trace: wave tile WX x WY (64 samples), workgroup tile 2WX x 2WY.  PERSISTENT: a grid sized to the
 // chip takes tiles from the work counter (grab at a time) until it passes the total; otherwise one workgroup
 // per tile, dealt so that consecutive tiles land on the same XCD (workgroups go round-robin to the 8 XCDs).
-template <int WX, int WY, bool PERSISTENT>
+template <int WX, int WY, bool PERSISTENT, bool WIDE>
 __global__ __launch_bounds__(kBlock) void views_kernel(SceneView s, const ViewRec *__restrict__ views, uint32_t nviews,
                                                        uint32_t total_tiles, uint32_t grab, uint32_t *__restrict__ counter,
                                                        uint32_t scene_depth, uint32_t mesh_depth,
@@ -188,7 +188,7 @@ __global__ __launch_bounds__(kBlock) void views_kernel(SceneView s, const ViewRe
     extern __shared__ uint32_t lds[];
     __shared__ uint32_t item;
     const uint32_t t = threadIdx.x, wave = t >> 6, lane = t & 63u;
-    Tracer tr;
+    TracerT<WIDE> tr;
     tr.scene_stack = Stack{lds + t, kBlock};
     tr.mesh_stack = Stack{lds + scene_depth * kBlock + t, kBlock};
     tr.scene_depth = scene_depth;
@@ -333,20 +333,23 @@ int enqueue_views(slimrt_views *v) {
     const uint32_t blocks = persistent
         ? std::max(1u, std::min((uint32_t)cus * v->cfg.blocks_per_cu, (v->total_tiles + grab - 1) / grab))
         : ((v->total_tiles + 7) / 8) * 8;
-    const uint32_t scene_depth = v->scene_cap + 1, mesh_depth = c->mesh_height + 1;
+    bool wide = true;   // the 4-wide mesh walk when every view asks for it
+    for (const slimrt_render_params &p : v->params) wide = wide && (p.flags & SLIMRT_BVH4) != 0;
+    const uint32_t scene_depth = v->scene_cap + 1, mesh_depth = mesh_stack_depth(c, wide);
     const size_t lds = (size_t)(scene_depth + mesh_depth) * kBlock * sizeof(uint32_t);
     if (lds + 16 > 160 * 1024) return vfail(v, SLIMRT_E_ARG, "BVH too deep for the LDS traversal stacks");
     const SceneView sv = device_view(c);
     const uint32_t n = (uint32_t)v->recs.size();
+#define VIEWS_LAUNCH1(WX, WY, P, W)                                                                           \
+    hipLaunchKernelGGL((views_kernel<WX, WY, P, W>), dim3(blocks), dim3(kBlock), lds, c->stream, sv,           \
+                       (const ViewRec *)v->d_recs, n, v->total_tiles, grab, v->d_counter, scene_depth, mesh_depth, \
+                       c->d_status)
 #define VIEWS_LAUNCH(WX, WY)                                                                                  \
-    if (persistent)                                                                                           \
-        hipLaunchKernelGGL((views_kernel<WX, WY, true>), dim3(blocks), dim3(kBlock), lds, c->stream, sv,      \
-                           (const ViewRec *)v->d_recs, n, v->total_tiles, grab, v->d_counter, scene_depth,    \
-                           mesh_depth, c->d_status);                                                          \
-    else                                                                                                      \
-        hipLaunchKernelGGL((views_kernel<WX, WY, false>), dim3(blocks), dim3(kBlock), lds, c->stream, sv,     \
-                           (const ViewRec *)v->d_recs, n, v->total_tiles, grab, v->d_counter, scene_depth,    \
-                           mesh_depth, c->d_status)
+    if (persistent) {                                                                                         \
+        if (wide) VIEWS_LAUNCH1(WX, WY, true, true); else VIEWS_LAUNCH1(WX, WY, true, false);                 \
+    } else {                                                                                                  \
+        if (wide) VIEWS_LAUNCH1(WX, WY, false, true); else VIEWS_LAUNCH1(WX, WY, false, false);               \
+    }
     switch (v->cfg.tile_shape) {
         case 0: VIEWS_LAUNCH(8, 8); break;
         case 1: VIEWS_LAUNCH(16, 4); break;
@@ -354,6 +357,7 @@ int enqueue_views(slimrt_views *v) {
         default: VIEWS_LAUNCH(4, 16); break;
     }
 #undef VIEWS_LAUNCH
+#undef VIEWS_LAUNCH1
     VIEWS_TRY(v, hipGetLastError());
     return SLIMRT_OK;
 }
@@ -590,12 +594,15 @@ int slimrt_render_autotune(slimrt_context *c, const slimrt_render_params *params
                            uint32_t reps, uint32_t *best_flags, float *best_ms) {
     if (!c || !params || !best_flags || reps == 0) return SLIMRT_E_ARG;
     const uint32_t base = params->flags & ~(uint32_t)(SLIMRT_WAVEFRONT | SLIMRT_WAVEFRONT_PER_BOUNCE |
-                                                      SLIMRT_WAVEFRONT_SORT | SLIMRT_WAVEFRONT_BANDS(3));
-    std::vector<uint32_t> forms{0u};
-    for (uint32_t bands_log2 : {0u, 2u})
-        for (uint32_t v : {0u, (uint32_t)SLIMRT_WAVEFRONT_SORT, (uint32_t)SLIMRT_WAVEFRONT_PER_BOUNCE,
-                           (uint32_t)(SLIMRT_WAVEFRONT_PER_BOUNCE | SLIMRT_WAVEFRONT_SORT)})
-            forms.push_back(SLIMRT_WAVEFRONT | v | SLIMRT_WAVEFRONT_BANDS(bands_log2));
+                                                      SLIMRT_WAVEFRONT_SORT | SLIMRT_WAVEFRONT_BANDS(3) | SLIMRT_BVH4);
+    std::vector<uint32_t> forms;
+    for (uint32_t width : {0u, (uint32_t)SLIMRT_BVH4}) {   // node width 2 or 4
+        forms.push_back(width);
+        for (uint32_t bands_log2 : {0u, 2u})
+            for (uint32_t v : {0u, (uint32_t)SLIMRT_WAVEFRONT_SORT, (uint32_t)SLIMRT_WAVEFRONT_PER_BOUNCE,
+                               (uint32_t)(SLIMRT_WAVEFRONT_PER_BOUNCE | SLIMRT_WAVEFRONT_SORT)})
+                forms.push_back(SLIMRT_WAVEFRONT | v | SLIMRT_WAVEFRONT_BANDS(bands_log2) | width);
+    }
     slimrt_render_params p = *params;
     float win_ms = INFINITY;
     uint32_t win = base;

@@ -37,8 +37,9 @@ enum : uint32_t { WF_COUNT_A = 0, WF_COUNT_B = 1, WF_BINS = 2, WF_CURSORS = 10, 
 
 // One iteration of the integrator loop (the body of render_beauty, in its order); returns whether the path
 // continues, with ray, throughput and depth_left advanced to the bounce.
+template <class TR>
 __device__ __forceinline__ bool beauty_iteration(const SceneView &s, const RenderParams &rp, Ray &ray, float scaling,
-                                                 uint32_t &depth_left, color &tp, color &col, float &depth, Tracer &tr) {
+                                                 uint32_t &depth_left, color &tp, color &col, float &depth, TR &tr) {
     Surface sf;
     Hit hit;
     hit.scaling_factor = scaling;
@@ -117,6 +118,7 @@ __global__ void wf_reset_kernel(uint32_t *__restrict__ counters, uint32_t n) {
 
 // Iteration 1 for every sample, tiles as in the render kernel.
 // Tiles [tile_base, tile_base + tiles) of the frame (one band, or all of it).
+template <bool WIDE>
 __global__ __launch_bounds__(kBlock) void wf_primary_kernel(SceneView s, RenderParams rp, float4 *__restrict__ pixels,
                                                             float *__restrict__ depths, float4 *__restrict__ accum,
                                                             PathState *__restrict__ queue, uint32_t *__restrict__ counters,
@@ -132,7 +134,7 @@ __global__ __launch_bounds__(kBlock) void wf_primary_kernel(SceneView s, RenderP
     const uint32_t x = (tile % tiles_x) * kTile + (wave & 1u) * 8u + (lane & 7u);
     const uint32_t y = (tile / tiles_x) * kTile + (wave >> 1) * 8u + (lane >> 3);
     const uint32_t ws = rp.width << rp.ssaa, hs = rp.height << rp.ssaa;
-    Tracer tr;
+    TracerT<WIDE> tr;
     tr.scene_stack = Stack{lds + t, kBlock};
     tr.mesh_stack = Stack{lds + scene_depth * kBlock + t, kBlock};
     tr.scene_depth = scene_depth;
@@ -161,7 +163,7 @@ __global__ __launch_bounds__(kBlock) void wf_primary_kernel(SceneView s, RenderP
 
 // Further iterations over the queue `in` (count at in_count): with TO_END each path runs to its end; otherwise one
 // iteration, appending continuing paths to `out`.
-template <bool TO_END>
+template <bool TO_END, bool WIDE>
 __global__ __launch_bounds__(kBlock) void wf_bounce_kernel(SceneView s, RenderParams rp, float4 *__restrict__ pixels,
                                                            float4 *__restrict__ accum, const PathState *__restrict__ in,
                                                            const uint32_t *__restrict__ in_count, PathState *__restrict__ out,
@@ -170,7 +172,7 @@ __global__ __launch_bounds__(kBlock) void wf_bounce_kernel(SceneView s, RenderPa
                                                            uint32_t *__restrict__ status) {
     extern __shared__ uint32_t lds[];
     const uint32_t t = threadIdx.x;
-    Tracer tr;
+    TracerT<WIDE> tr;
     tr.scene_stack = Stack{lds + t, kBlock};
     tr.mesh_stack = Stack{lds + scene_depth * kBlock + t, kBlock};
     tr.scene_depth = scene_depth;
@@ -263,6 +265,7 @@ __global__ void wf_clear_count_kernel(uint32_t *__restrict__ counters, uint32_t 
 namespace slimrt {
 
 // The bounce launches of one queue (queue A holds the paths to run; sorting moves them to B, emptying A).
+template <bool WIDE>
 int enqueue_bounces(slimrt_context *c, hipStream_t st, const SceneView &v, const RenderParams &rp, float4 *pixels,
                     float4 *accum, PathState *qa, PathState *qb, uint32_t *cnt, uint32_t cap, size_t lds,
                     uint32_t scene_depth, uint32_t mesh_depth, uint32_t blocks, bool per_bounce, bool sort) {
@@ -275,7 +278,7 @@ int enqueue_bounces(slimrt_context *c, hipStream_t st, const SceneView &v, const
     if (rp.max_depth < 2) return SLIMRT_OK;
     if (!per_bounce) {
         if (sort) sort_a_into_b();
-        hipLaunchKernelGGL(wf_bounce_kernel<true>, dim3(blocks), dim3(kBlock), lds, st, v, rp, pixels, accum,
+        hipLaunchKernelGGL((wf_bounce_kernel<true, WIDE>), dim3(blocks), dim3(kBlock), lds, st, v, rp, pixels, accum,
                            (const PathState *)(sort ? qb : qa), (const uint32_t *)(cnt + (sort ? WF_COUNT_B : WF_COUNT_A)),
                            sort ? qa : qb, cnt + (sort ? WF_COUNT_A : WF_COUNT_B), cap, scene_depth, mesh_depth,
                            c->d_status);
@@ -286,13 +289,13 @@ int enqueue_bounces(slimrt_context *c, hipStream_t st, const SceneView &v, const
     for (uint32_t it = 1; it < rp.max_depth; it++) {
         if (sort) {   // A -> sorted B (A emptied), then B -> A
             sort_a_into_b();
-            hipLaunchKernelGGL(wf_bounce_kernel<false>, dim3(blocks), dim3(kBlock), lds, st, v, rp, pixels, accum,
+            hipLaunchKernelGGL((wf_bounce_kernel<false, WIDE>), dim3(blocks), dim3(kBlock), lds, st, v, rp, pixels, accum,
                                (const PathState *)qb, (const uint32_t *)(cnt + WF_COUNT_B), qa, cnt + WF_COUNT_A, cap,
                                scene_depth, mesh_depth, c->d_status);
         } else {
             PathState *in = cur_count == WF_COUNT_A ? qa : qb, *out = cur_count == WF_COUNT_A ? qb : qa;
             hipLaunchKernelGGL(wf_clear_count_kernel, dim3(1), dim3(64), 0, st, cnt, next_count);
-            hipLaunchKernelGGL(wf_bounce_kernel<false>, dim3(blocks), dim3(kBlock), lds, st, v, rp, pixels, accum,
+            hipLaunchKernelGGL((wf_bounce_kernel<false, WIDE>), dim3(blocks), dim3(kBlock), lds, st, v, rp, pixels, accum,
                                (const PathState *)in, (const uint32_t *)(cnt + cur_count), out, cnt + next_count, cap,
                                scene_depth, mesh_depth, c->d_status);
             std::swap(cur_count, next_count);
@@ -320,7 +323,7 @@ int launch_wavefront(slimrt_context *c, const RenderParams &rp, float4 *pixels, 
         c->wf_cap = slots;
     }
     if (!c->wf_counters) HIP_TRY(c, hipMalloc(&c->wf_counters, 8 * WF_COUNTERS * sizeof(uint32_t)));
-    const uint32_t scene_depth = c->scene_height + 1, mesh_depth = c->mesh_height + 1;
+    const uint32_t scene_depth = c->scene_height + 1, mesh_depth = mesh_stack_depth(c, wide_mesh_bvh(rp));
     const size_t lds = (size_t)(scene_depth + mesh_depth) * kBlock * sizeof(uint32_t);
     if (lds > 160 * 1024) return fail(c, SLIMRT_E_ARG, "BVH too deep for the LDS traversal stacks");
     int cus = 0;
@@ -348,16 +351,24 @@ int launch_wavefront(slimrt_context *c, const RenderParams &rp, float4 *pixels, 
         const uint32_t tile_base = row0 * tiles_x, tiles = (row1 - row0) * tiles_x, per_xcd = (tiles + 7) / 8;
         const uint32_t base = tile_base * kBlock, cap = tiles * kBlock;
         uint32_t *bc = cnt + b * WF_COUNTERS;
-        hipLaunchKernelGGL(wf_primary_kernel, dim3(per_xcd * 8), dim3(kBlock), lds, c->stream, v, rp, pixels, depths,
-                           accum, qa + base, bc, cap, scene_depth, mesh_depth, tiles_x, tile_base, tiles, per_xcd,
-                           c->d_status);
+        if (wide_mesh_bvh(rp))
+            hipLaunchKernelGGL(wf_primary_kernel<true>, dim3(per_xcd * 8), dim3(kBlock), lds, c->stream, v, rp, pixels,
+                               depths, accum, qa + base, bc, cap, scene_depth, mesh_depth, tiles_x, tile_base, tiles,
+                               per_xcd, c->d_status);
+        else
+            hipLaunchKernelGGL(wf_primary_kernel<false>, dim3(per_xcd * 8), dim3(kBlock), lds, c->stream, v, rp, pixels,
+                               depths, accum, qa + base, bc, cap, scene_depth, mesh_depth, tiles_x, tile_base, tiles,
+                               per_xcd, c->d_status);
         HIP_TRY(c, hipGetLastError());
         if (bands > 1) {
             HIP_TRY(c, hipEventRecord(c->wf_band[b], c->stream));
             HIP_TRY(c, hipStreamWaitEvent(side, c->wf_band[b], 0));
         }
-        int rc = enqueue_bounces(c, side, v, rp, pixels, accum, qa + base, qb + base, bc, cap, lds, scene_depth,
-                                 mesh_depth, blocks, per_bounce, sort);
+        int rc = wide_mesh_bvh(rp)
+            ? enqueue_bounces<true>(c, side, v, rp, pixels, accum, qa + base, qb + base, bc, cap, lds, scene_depth,
+                                    mesh_depth, blocks, per_bounce, sort)
+            : enqueue_bounces<false>(c, side, v, rp, pixels, accum, qa + base, qb + base, bc, cap, lds, scene_depth,
+                                     mesh_depth, blocks, per_bounce, sort);
         if (rc) return rc;
     }
     if (bands > 1) {   // join: the caller's stream continues after the last bounce launch

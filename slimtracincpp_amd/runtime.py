@@ -74,7 +74,7 @@ class ViewsConfig(C.Structure):
 
 
 SCENE_REFIT, SCENE_REBUILD = 0, 1
-WAVEFRONT, WAVEFRONT_PER_BOUNCE, WAVEFRONT_SORT = 2, 4, 8   # slimrt_render_params.flags, with RAY_CONES
+WAVEFRONT, WAVEFRONT_PER_BOUNCE, WAVEFRONT_SORT, BVH4 = 2, 4, 8, 64   # slimrt_render_params.flags, with RAY_CONES
 
 
 def WAVEFRONT_BANDS(log2n: int) -> int:

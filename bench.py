@@ -47,11 +47,16 @@ BYTES_PER = {"pairs": 64, "instances": 64, "triangles": 64, "mesh_hits": 64, "ta
 OUT_BYTES_PER_SAMPLE = 20
 
 
-def measured_traffic(width: int, height: int):
-    """HBM bytes per launch of the render kernel on this workload, from the committed rocprofv3 --pmc summary
-    (FETCH_SIZE + WRITE_SIZE, each scaled by the factor the same counter reads on a copy kernel of known size)."""
+FORM_NAMES = ("megakernel", "wavefront", "wavefront_sorted", "wavefront_per_bounce", "wavefront_per_bounce_sorted")
+FORM_NAMES = FORM_NAMES + tuple(f + "_bvh4" for f in FORM_NAMES)
+
+
+def measured_traffic(width: int, height: int, form: str):
+    """HBM bytes per frame of this form (the sum over the frame's launches) on this workload, from the committed
+    rocprofv3 --pmc summary (FETCH_SIZE + WRITE_SIZE, each scaled by the factor the same counter reads on an
+    elementwise kernel of known size); None when no summary covers the form."""
     import re
-    path = os.path.join(ROOT, "profiles", f"render_kernel_mesh_pbr_{width}x{height}_traffic.md")
+    path = os.path.join(ROOT, "profiles", f"{form}_mesh_pbr_{width}x{height}_traffic.md")
     if not os.path.exists(path):
         return None
     m = re.search(r"traffic_bytes_per_launch:\s*([0-9]+)", open(path).read())
@@ -307,6 +312,8 @@ def main():
     ap.add_argument("--no-views", action="store_true")
     ap.add_argument("--large", action="store_true", help="also run the combined 4K workload (about 2 minutes)")
     ap.add_argument("--view-frames", type=int, default=20)
+    ap.add_argument("--form", choices=FORM_NAMES, default=None,
+                    help="time only this frame form (for profiler runs); by default every form, the fastest reported")
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
                     help="also run a plain device-to-device copy of this many MiB (HBM counter calibration)")
     args = ap.parse_args()
@@ -352,6 +359,9 @@ def main():
                   "wavefront_per_bounce_sorted": runtime.WAVEFRONT | runtime.WAVEFRONT_PER_BOUNCE | runtime.WAVEFRONT_SORT}
     forms = dict(base_forms)
     forms.update({k + "_bvh4": v | runtime.BVH4 for k, v in base_forms.items()})
+    assert tuple(forms) == FORM_NAMES
+    if args.form:
+        forms = {args.form: forms[args.form]}
     timings = {}
     for name, flags in forms.items():
         p.flags = flags
@@ -406,7 +416,7 @@ def main():
                    "wall_ms_per_step": wall_ms, "mode": "plain (full-depth trace)", "form": form,
                    "forms_ms_per_step": {k: v["ms_per_step"] for k, v in timings.items()}},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "traffic": measured_traffic(args.width, args.height),
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": measured_traffic(args.width, args.height, form),
                      "kernel": ("4-wide mesh BVHs; " if wide else "") + {"megakernel": "render_kernel<false>",
                                 "wavefront": "the frame's launches: wf_primary_kernel + one wf_bounce_kernel<true> "
                                              "over the compacted bounce queue",

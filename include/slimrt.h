@@ -60,6 +60,8 @@ extern "C" {
 #define SLIMRT_WAVEFRONT_BANDS(log2n) (((uint32_t)(log2n) & 3u) << 4)
 #define SLIMRT_BVH4 64u            /* trace meshes through 4-wide BVHs collapsed from the uploaded ones (same leaves
                                       and triangles; only exact ties between triangles can resolve differently) */
+#define SLIMRT_WAVEFRONT_COUNT 128u /* with SLIMRT_WAVEFRONT: also count the paths of the frame (per iteration,
+                                       spawned, retired) for slimrt_wavefront_counts; costs atomics, for checks */
 
 typedef struct slimrt_context slimrt_context;
 
@@ -164,7 +166,7 @@ typedef struct {
     uint32_t render_mode;
     int32_t skybox_color, skybox_radiance, skybox_irradiance;   /* texture ids, -1 for none */
     uint32_t output;            /* SLIMRT_OUTPUT_* */
-    uint32_t flags;             /* SLIMRT_RAY_CONES | SLIMRT_WAVEFRONT (| _PER_BOUNCE | _SORT | _BANDS(n)) | SLIMRT_BVH4 */
+    uint32_t flags;             /* SLIMRT_RAY_CONES | SLIMRT_WAVEFRONT (| _PER_BOUNCE | _SORT | _BANDS(n) | _COUNT) | SLIMRT_BVH4 */
     float mip_level_colors[9][3];
 } slimrt_render_params;
 
@@ -221,6 +223,31 @@ int slimrt_host_world_aabbs(const slimrt_geometry *geometries, uint32_t count, c
  * 2N - 1 nodes (at least 1); leaf_ids_out N entries. */
 int slimrt_host_build_bvh(const float *aabbs, uint32_t count, uint32_t max_leaf_size, slimrt_bvh_node *nodes_out,
                           uint32_t *node_count_out, uint32_t *height_out, uint32_t *leaf_ids_out);
+/* 4-wide BVH node as the SLIMRT_BVH4 walk reads it (128 bytes): the four children's boxes as structure of
+ * arrays, then per child an inner node index (count 0) or a leaf's first triangle and its triangle count; an
+ * unused slot has child 0xFFFFFFFF, count 0 and the empty box (+inf, -inf). */
+typedef struct slimrt_bvh4_node {
+    float min_x[4], min_y[4], min_z[4];
+    float max_x[4], max_y[4], max_z[4];
+    uint32_t child[4], count[4];
+} slimrt_bvh4_node;
+
+/* The collapse slimrt_upload_scene applies to every mesh BVH (no counterpart in the reference, whose meshes are
+ * walked as sibling pairs, src/slim/scene/mesh_tracer.h:51-115): a 4-wide node's children are its BVH2 node's
+ * two children with, while fewer than four, the inner child of largest surface area replaced by its own two.
+ * Leaves and their triangle ranges are the BVH2's.  nodes_out receives at most node_cap nodes ((node_count + 1) / 2
+ * suffices), root first; *height_out the 4-wide tree's height (1 for a root that is a leaf). */
+int slimrt_host_collapse_bvh4(const slimrt_bvh_node *nodes, uint32_t node_count, uint32_t triangle_count,
+                              slimrt_bvh4_node *nodes_out, uint32_t node_cap, uint32_t *node_count_out,
+                              uint32_t *height_out);
+
+/* The path counters of the last frame rendered with SLIMRT_WAVEFRONT | SLIMRT_WAVEFRONT_COUNT (no counterpart in
+ * the reference): counts[i] for i < 32 = paths that ran iteration i of the integrator loop (0 = the camera ray;
+ * src/slim/renderer/ray_tracer.h:42-105), counts[32] = bounce rays spawned into a queue, counts[33] = samples
+ * finished.  Every iteration either spawns or finishes, so sum(counts[0..31]) = counts[32] + counts[33].
+ * n >= 34.  Synchronizes the context's stream. */
+int slimrt_wavefront_counts(slimrt_context *ctx, uint32_t *counts, uint32_t n);
+
 /* BVHBuilder::buildMesh (src/slim/scene/bvh_builder.h:265-341): mesh BVH and leaf-ordered triangle records.
  * normals/uvs may be NULL (counts 0).  nodes_out: 2T - 1 nodes; triangles_out: T records; aabb_out: 6 floats
  * (componentwise min/max of the positions). */

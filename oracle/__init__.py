@@ -61,6 +61,8 @@ def lib():
         L.oracle_set_scene_leaf_size.argtypes = [vp, u32]
         L.oracle_path_samples.restype = i
         L.oracle_path_samples.argtypes = [vp, i, i, i, vp, vp, vp]
+        L.oracle_iteration_counts.restype = i
+        L.oracle_iteration_counts.argtypes = [vp, i, i, i, vp, i]
         _lib = L
     return _lib
 
@@ -142,6 +144,13 @@ class OracleScene:
         v = np.zeros(16, np.int32)
         n = lib().oracle_path_samples(self.h, x, y, max_depth, _p(raw), _p(L), _p(v))
         return raw[:n], L[:n], v[:n]
+
+    def iteration_counts(self, max_depth: int = -1, n: int = 16) -> np.ndarray:
+        """counts[i] = samples whose beauty path runs iteration i of the integrator loop (0 = camera ray)."""
+        out = np.zeros(n, np.uint64)
+        rc = lib().oracle_iteration_counts(self.h, 0, self.height * (2 if self.ssaa else 1), max_depth, _p(out), n)
+        assert rc == 0, "max_depth above 16"
+        return out
 
     def set_scene_leaf_size(self, max_leaf: int):
         lib().oracle_set_scene_leaf_size(self.h, max_leaf)

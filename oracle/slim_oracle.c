@@ -1566,6 +1566,34 @@ int oracle_path_samples(void *h, int x, int y, int max_depth, float *raw, float 
     return n;
 }
 
+/* Per-iteration path counts of beauty rows [y0, y1): counts[i] += the samples whose path runs iteration i of
+ * renderPixelBeauty's loop (i = 0 for the camera ray; src/slim/renderer/ray_tracer.h:42-105), for i < n.
+ * max_depth <= 16.  The wavefront frame's path counters are checked against these. */
+int oracle_iteration_counts(void *h, int y0, int y1, int max_depth, uint64_t *counts, int n) {
+    Scene *s = (Scene *)h;
+    if (max_depth < 0) max_depth = s->max_depth;
+    if (max_depth > PATH_MAX_ITER) return -1;
+    int Ws = s->width * (s->ssaa ? 2 : 1);
+    Counters cnt;
+    memset(&cnt, 0, sizeof cnt);
+    Tracer *tr = (Tracer *)calloc(1, sizeof(Tracer));
+    tr->stack = (uint32_t *)calloc(s->G + 2, 4);
+    tr->mesh_stack = (uint32_t *)calloc(max_mesh_height(s) + 2, 4);
+    tr->cnt = &cnt;
+    PathRec *rec = (PathRec *)calloc(1, sizeof(PathRec));
+    for (int y = y0; y < y1; y++)
+        for (int x = 0; x < Ws; x++) {
+            v3 dir;
+            float scaling, depth;
+            color c;
+            primary_ray(s, x, y, &dir, &scaling);
+            render_beauty_rec(s, tr, max_depth, dir, scaling, &c, &depth, rec);
+            for (int i = 0; i < rec->n_iter && i < n; i++) counts[i]++;
+        }
+    free(rec); free(tr->stack); free(tr->mesh_stack); free(tr);
+    return 0;
+}
+
 /* Primary closest hit per sample: geometry index (-1 miss), leaf-ordered triangle id (-1 if not a mesh), t. */
 int oracle_primary_hits(void *h, int y0, int y1, int32_t *geo, int32_t *tri, float *t) {
     Scene *s = (Scene *)h;

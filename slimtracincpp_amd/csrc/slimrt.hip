@@ -302,6 +302,30 @@ uint32_t collapse4(const Node *n2, uint32_t i2, uint32_t depth, std::vector<Node
     return idx;
 }
 
+// The 4-wide tree of a validated sibling-pair mesh BVH (children after parents), root first; returns its height.
+// A root that is a leaf becomes one node holding that leaf.
+uint32_t collapse_mesh_bvh4(const Node *n2, std::vector<Node4> &out) {
+    uint32_t h4 = 1;
+    out.clear();
+    if (n2[0].leaf_count_depth & 0xFFFFu) {
+        Node4 leaf;
+        for (uint32_t j = 0; j < 4; j++) {
+            leaf.minx[j] = leaf.miny[j] = leaf.minz[j] = INFINITY;
+            leaf.maxx[j] = leaf.maxy[j] = leaf.maxz[j] = -INFINITY;
+            leaf.child[j] = ~0u;
+            leaf.count[j] = 0;
+        }
+        leaf.minx[0] = n2[0].minx; leaf.miny[0] = n2[0].miny; leaf.minz[0] = n2[0].minz;
+        leaf.maxx[0] = n2[0].maxx; leaf.maxy[0] = n2[0].maxy; leaf.maxz[0] = n2[0].maxz;
+        leaf.child[0] = n2[0].first_index;
+        leaf.count[0] = n2[0].leaf_count_depth & 0xFFFFu;
+        out.push_back(leaf);
+    } else {
+        collapse4(n2, 0, 1, out, h4);
+    }
+    return h4;
+}
+
 int ensure_outputs(slimrt_context *c, size_t n) {
     if (c->d_pixels.cap < n) {
         dev_free(c->d_pixels);
@@ -411,7 +435,7 @@ int slimrt_destroy(slimrt_context *c) {
         dev_free(c->d_mips); dev_free(c->d_quads); dev_free(c->d_pixels); dev_free(c->d_depths);
         free_nrc(c);
         dev_free(c->d_geo_stage);
-        for (void *q : {c->wf_queue_a, c->wf_queue_b, c->wf_accum, (void *)c->wf_counters})
+        for (void *q : {c->wf_queue_a, c->wf_queue_b, c->wf_accum, (void *)c->wf_counters, (void *)c->wf_stats})
             if (q) (void)hipFree(q);
         for (hipEvent_t e : c->wf_band)
             if (e) (void)hipEventDestroy(e);
@@ -515,27 +539,10 @@ int slimrt_upload_scene(slimrt_context *c, const slimrt_scene_desc *d) {
             c->mnodes.push_back(to_node(n));
         }
         r.node4_base = (uint32_t)c->mnodes4.size();
-        uint32_t h4 = 1;
-        const Node *n2 = c->mnodes.data() + r.node_base;
-        if (n2[0].leaf_count_depth & 0xFFFFu) {   // a single leaf: one 4-wide node holding it
-            Node4 leaf;
-            for (uint32_t j = 0; j < 4; j++) {
-                leaf.minx[j] = leaf.miny[j] = leaf.minz[j] = INFINITY;
-                leaf.maxx[j] = leaf.maxy[j] = leaf.maxz[j] = -INFINITY;
-                leaf.child[j] = ~0u;
-                leaf.count[j] = 0;
-            }
-            leaf.minx[0] = n2[0].minx; leaf.miny[0] = n2[0].miny; leaf.minz[0] = n2[0].minz;
-            leaf.maxx[0] = n2[0].maxx; leaf.maxy[0] = n2[0].maxy; leaf.maxz[0] = n2[0].maxz;
-            leaf.child[0] = 0;
-            leaf.count[0] = m.triangle_count;
-            c->mnodes4.push_back(leaf);
-        } else {
-            std::vector<Node4> out;
-            collapse4(n2, 0, 1, out, h4);
-            c->mnodes4.insert(c->mnodes4.end(), out.begin(), out.end());
-        }
-        r.node4_count = (uint32_t)c->mnodes4.size() - r.node4_base;
+        std::vector<Node4> n4;
+        const uint32_t h4 = collapse_mesh_bvh4(c->mnodes.data() + r.node_base, n4);
+        c->mnodes4.insert(c->mnodes4.end(), n4.begin(), n4.end());
+        r.node4_count = (uint32_t)n4.size();
         r.height4 = h4;
         r.pad = 0;
         c->mesh_height4 = std::max(c->mesh_height4, h4);
@@ -638,6 +645,37 @@ int slimrt_count_work(slimrt_context *c, const slimrt_render_params *p, uint64_t
     if ((rc = launch(c, p, c->d_pixels.ptr, c->d_depths.ptr, true))) return rc;
     HIP_TRY(c, hipMemcpyAsync(counters, c->d_counters, 8 * sizeof(uint64_t), hipMemcpyDeviceToHost, c->stream));
     return check_status(c);
+}
+
+int slimrt_host_collapse_bvh4(const slimrt_bvh_node *nodes, uint32_t node_count, uint32_t triangle_count,
+                              slimrt_bvh4_node *nodes_out, uint32_t node_cap, uint32_t *node_count_out,
+                              uint32_t *height_out) {
+    if (!nodes || !node_count || !nodes_out || !node_count_out || !height_out) return SLIMRT_E_ARG;
+    std::vector<Node> n2(node_count);
+    for (uint32_t i = 0; i < node_count; i++) {   // the checks of the upload: the walk below stays in range
+        const slimrt_bvh_node &n = nodes[i];
+        const bool bad = n.leaf_count ? ((uint64_t)n.first_index + n.leaf_count > triangle_count)
+                                      : (n.first_index <= i || (uint64_t)n.first_index + 1 >= node_count);
+        if (bad) return SLIMRT_E_ARG;
+        n2[i] = to_node(n);
+    }
+    std::vector<Node4> out;
+    const uint32_t h4 = collapse_mesh_bvh4(n2.data(), out);
+    if (out.size() > node_cap) return SLIMRT_E_OVERFLOW;
+    static_assert(sizeof(Node4) == sizeof(slimrt_bvh4_node), "4-wide node layout");
+    memcpy(nodes_out, out.data(), out.size() * sizeof(Node4));
+    *node_count_out = (uint32_t)out.size();
+    *height_out = h4;
+    return SLIMRT_OK;
+}
+
+int slimrt_wavefront_counts(slimrt_context *c, uint32_t *counts, uint32_t n) {
+    if (!c) return SLIMRT_E_ARG;
+    if (!counts || n < kWfStats) return fail(c, SLIMRT_E_ARG, "counts needs 34 entries");
+    if (c->cpu || !c->wf_stats) return fail(c, SLIMRT_E_STATE, "no frame rendered with SLIMRT_WAVEFRONT_COUNT");
+    HIP_TRY(c, hipMemcpyAsync(counts, c->wf_stats, kWfStats * sizeof(uint32_t), hipMemcpyDeviceToHost, c->stream));
+    HIP_TRY(c, hipStreamSynchronize(c->stream));
+    return SLIMRT_OK;
 }
 
 int slimrt_last_kernel_ms(slimrt_context *c, float *ms) {

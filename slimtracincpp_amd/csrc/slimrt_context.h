@@ -67,6 +67,7 @@ struct slimrt_context {
     DevBuf<slimrt_geometry> d_geo_stage;   // transform updates on their way to the scene update kernel
     void *wf_queue_a = nullptr, *wf_queue_b = nullptr, *wf_accum = nullptr;   // wavefront ray queues, colours
     uint32_t *wf_counters = nullptr, wf_cap = 0;
+    uint32_t *wf_stats = nullptr;   // kWfStats path counters (SLIMRT_WAVEFRONT_COUNT)
     hipStream_t side_stream = nullptr;          // a wavefront frame's bounce launches, overlapping the next band
     hipEvent_t wf_fork = nullptr, wf_join = nullptr, wf_band[8] = {};
     uint32_t *d_status = nullptr;
@@ -204,6 +205,9 @@ inline int validate_params(slimrt_context *c, const slimrt_render_params *p) {
         return fail(c, SLIMRT_E_ARG, "skybox texture id out of range");
     return SLIMRT_OK;
 }
+
+// Path counters of a counted wavefront frame: paths per integrator iteration, then spawned, finished.
+constexpr uint32_t kWfIterations = 32, kWfSpawned = 32, kWfFinished = 33, kWfStats = 34;
 
 // Beauty frame with the bounces as a wavefront (slimrt_wavefront.hip).
 int launch_wavefront(slimrt_context *c, const RenderParams &rp, float4 *pixels, float *depths, bool per_bounce, bool sort,

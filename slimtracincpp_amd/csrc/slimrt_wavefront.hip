@@ -112,8 +112,18 @@ __device__ __forceinline__ void push_path(PathState *q, uint32_t *count, uint32_
     accum[pixel] = make_float4(col.r, col.g, col.b, 0.0f);
 }
 
-__global__ void wf_reset_kernel(uint32_t *__restrict__ counters, uint32_t n) {
+__global__ void wf_reset_kernel(uint32_t *__restrict__ counters, uint32_t n, uint32_t *__restrict__ stats) {
     for (uint32_t i = threadIdx.x; i < n; i += blockDim.x) counters[i] = 0u;
+    if (stats)
+        for (uint32_t i = threadIdx.x; i < kWfStats; i += blockDim.x) stats[i] = 0u;
+}
+
+// Path counters (SLIMRT_WAVEFRONT_COUNT, stats != nullptr): one atomic per lane and event, for checks only.
+__device__ __forceinline__ void count_iteration(uint32_t *stats, const RenderParams &rp, uint32_t depth_left) {
+    if (stats) atomicAdd(stats + min(rp.max_depth - depth_left, kWfIterations - 1u), 1u);
+}
+__device__ __forceinline__ void count_end(uint32_t *stats, bool spawned) {
+    if (stats) atomicAdd(stats + (spawned ? kWfSpawned : kWfFinished), 1u);
 }
 
 // Iteration 1 for every sample, tiles as in the render kernel.
@@ -124,7 +134,8 @@ __global__ __launch_bounds__(kBlock) void wf_primary_kernel(SceneView s, RenderP
                                                             PathState *__restrict__ queue, uint32_t *__restrict__ counters,
                                                             uint32_t cap, uint32_t scene_depth, uint32_t mesh_depth,
                                                             uint32_t tiles_x, uint32_t tile_base, uint32_t tiles,
-                                                            uint32_t tiles_per_xcd, uint32_t *__restrict__ status) {
+                                                            uint32_t tiles_per_xcd, uint32_t *__restrict__ status,
+                                                            uint32_t *__restrict__ stats) {
     extern __shared__ uint32_t lds[];
     const uint32_t xcd = blockIdx.x & 7u, slot = blockIdx.x >> 3;
     const uint32_t local = xcd * tiles_per_xcd + slot;
@@ -152,7 +163,11 @@ __global__ __launch_bounds__(kBlock) void wf_primary_kernel(SceneView s, RenderP
         primary_ray(rp.proj, x, y, dir, scaling);
         const Projection &p = rp.proj;
         ray_reset(ray, v3{p.camera_position[0], p.camera_position[1], p.camera_position[2]}, normalized(dir));
-        if (depth_left) cont = beauty_iteration(s, rp, ray, scaling, depth_left, tp, col, depth, tr);
+        if (depth_left) {
+            count_iteration(stats, rp, depth_left);
+            cont = beauty_iteration(s, rp, ray, scaling, depth_left, tp, col, depth, tr);
+        }
+        count_end(stats, cont);
         o = canvas_offset(rp, x, y);
         depths[o] = depth;
         if (!cont) finish_sample(rp, col, pixels, o);
@@ -164,12 +179,13 @@ __global__ __launch_bounds__(kBlock) void wf_primary_kernel(SceneView s, RenderP
 // Further iterations over the queue `in` (count at in_count): with TO_END each path runs to its end; otherwise one
 // iteration, appending continuing paths to `out`.
 template <bool TO_END, bool WIDE>
-__global__ __launch_bounds__(kBlock) void wf_bounce_kernel(SceneView s, RenderParams rp, float4 *__restrict__ pixels,
+__global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(TO_END && !WIDE ? 3 : 1)))
+void wf_bounce_kernel(SceneView s, RenderParams rp, float4 *__restrict__ pixels,
                                                            float4 *__restrict__ accum, const PathState *__restrict__ in,
                                                            const uint32_t *__restrict__ in_count, PathState *__restrict__ out,
                                                            uint32_t *__restrict__ out_count, uint32_t cap,
                                                            uint32_t scene_depth, uint32_t mesh_depth,
-                                                           uint32_t *__restrict__ status) {
+                                                           uint32_t *__restrict__ status, uint32_t *__restrict__ stats) {
     extern __shared__ uint32_t lds[];
     const uint32_t t = threadIdx.x;
     TracerT<WIDE> tr;
@@ -198,9 +214,15 @@ __global__ __launch_bounds__(kBlock) void wf_bounce_kernel(SceneView s, RenderPa
             pixel = st.pixel;
             const float4 a = accum[pixel];
             col = color{a.x, a.y, a.z};
+            count_iteration(stats, rp, depth_left);
             cont = beauty_iteration(s, rp, ray, scaling, depth_left, tp, col, depth, tr);
             if (TO_END)
-                while (cont) cont = beauty_iteration(s, rp, ray, scaling, depth_left, tp, col, depth, tr);
+                while (cont) {
+                    count_end(stats, true);
+                    count_iteration(stats, rp, depth_left);
+                    cont = beauty_iteration(s, rp, ray, scaling, depth_left, tp, col, depth, tr);
+                }
+            count_end(stats, cont);
             if (!cont) finish_sample(rp, col, pixels, pixel);
         }
         if (!TO_END) push_path(out, out_count, cap, live && cont, ray, tp, scaling, depth_left, pixel, col, accum, status);
@@ -268,7 +290,8 @@ namespace slimrt {
 template <bool WIDE>
 int enqueue_bounces(slimrt_context *c, hipStream_t st, const SceneView &v, const RenderParams &rp, float4 *pixels,
                     float4 *accum, PathState *qa, PathState *qb, uint32_t *cnt, uint32_t cap, size_t lds,
-                    uint32_t scene_depth, uint32_t mesh_depth, uint32_t blocks, bool per_bounce, bool sort) {
+                    uint32_t scene_depth, uint32_t mesh_depth, uint32_t blocks, bool per_bounce, bool sort,
+                    uint32_t *stats) {
     auto sort_a_into_b = [&]() {
         hipLaunchKernelGGL(wf_bin_count_kernel, dim3(blocks), dim3(256), 0, st, qa, cnt, cap);
         hipLaunchKernelGGL(wf_bin_scan_kernel, dim3(1), dim3(64), 0, st, cnt);
@@ -281,7 +304,7 @@ int enqueue_bounces(slimrt_context *c, hipStream_t st, const SceneView &v, const
         hipLaunchKernelGGL((wf_bounce_kernel<true, WIDE>), dim3(blocks), dim3(kBlock), lds, st, v, rp, pixels, accum,
                            (const PathState *)(sort ? qb : qa), (const uint32_t *)(cnt + (sort ? WF_COUNT_B : WF_COUNT_A)),
                            sort ? qa : qb, cnt + (sort ? WF_COUNT_A : WF_COUNT_B), cap, scene_depth, mesh_depth,
-                           c->d_status);
+                           c->d_status, stats);
         HIP_TRY(c, hipGetLastError());
         return SLIMRT_OK;
     }
@@ -291,13 +314,13 @@ int enqueue_bounces(slimrt_context *c, hipStream_t st, const SceneView &v, const
             sort_a_into_b();
             hipLaunchKernelGGL((wf_bounce_kernel<false, WIDE>), dim3(blocks), dim3(kBlock), lds, st, v, rp, pixels, accum,
                                (const PathState *)qb, (const uint32_t *)(cnt + WF_COUNT_B), qa, cnt + WF_COUNT_A, cap,
-                               scene_depth, mesh_depth, c->d_status);
+                               scene_depth, mesh_depth, c->d_status, stats);
         } else {
             PathState *in = cur_count == WF_COUNT_A ? qa : qb, *out = cur_count == WF_COUNT_A ? qb : qa;
             hipLaunchKernelGGL(wf_clear_count_kernel, dim3(1), dim3(64), 0, st, cnt, next_count);
             hipLaunchKernelGGL((wf_bounce_kernel<false, WIDE>), dim3(blocks), dim3(kBlock), lds, st, v, rp, pixels, accum,
                                (const PathState *)in, (const uint32_t *)(cnt + cur_count), out, cnt + next_count, cap,
-                               scene_depth, mesh_depth, c->d_status);
+                               scene_depth, mesh_depth, c->d_status, stats);
             std::swap(cur_count, next_count);
         }
         HIP_TRY(c, hipGetLastError());
@@ -333,7 +356,12 @@ int launch_wavefront(slimrt_context *c, const RenderParams &rp, float4 *pixels, 
     float4 *accum = (float4 *)c->wf_accum;
     uint32_t *cnt = c->wf_counters;
     const uint32_t blocks = (uint32_t)cus * 2;
-    hipLaunchKernelGGL(wf_reset_kernel, dim3(1), dim3(256), 0, c->stream, cnt, bands * WF_COUNTERS);
+    uint32_t *stats = nullptr;
+    if (rp.flags & SLIMRT_WAVEFRONT_COUNT) {
+        if (!c->wf_stats) HIP_TRY(c, hipMalloc(&c->wf_stats, kWfStats * sizeof(uint32_t)));
+        stats = c->wf_stats;
+    }
+    hipLaunchKernelGGL(wf_reset_kernel, dim3(1), dim3(256), 0, c->stream, cnt, bands * WF_COUNTERS, stats);
     HIP_TRY(c, hipGetLastError());
     hipStream_t side = c->stream;
     if (bands > 1) {   // fork: the side stream starts after everything queued on the caller's stream so far
@@ -354,11 +382,11 @@ int launch_wavefront(slimrt_context *c, const RenderParams &rp, float4 *pixels, 
         if (wide_mesh_bvh(rp))
             hipLaunchKernelGGL(wf_primary_kernel<true>, dim3(per_xcd * 8), dim3(kBlock), lds, c->stream, v, rp, pixels,
                                depths, accum, qa + base, bc, cap, scene_depth, mesh_depth, tiles_x, tile_base, tiles,
-                               per_xcd, c->d_status);
+                               per_xcd, c->d_status, stats);
         else
             hipLaunchKernelGGL(wf_primary_kernel<false>, dim3(per_xcd * 8), dim3(kBlock), lds, c->stream, v, rp, pixels,
                                depths, accum, qa + base, bc, cap, scene_depth, mesh_depth, tiles_x, tile_base, tiles,
-                               per_xcd, c->d_status);
+                               per_xcd, c->d_status, stats);
         HIP_TRY(c, hipGetLastError());
         if (bands > 1) {
             HIP_TRY(c, hipEventRecord(c->wf_band[b], c->stream));
@@ -366,9 +394,9 @@ int launch_wavefront(slimrt_context *c, const RenderParams &rp, float4 *pixels, 
         }
         int rc = wide_mesh_bvh(rp)
             ? enqueue_bounces<true>(c, side, v, rp, pixels, accum, qa + base, qb + base, bc, cap, lds, scene_depth,
-                                    mesh_depth, blocks, per_bounce, sort)
+                                    mesh_depth, blocks, per_bounce, sort, stats)
             : enqueue_bounces<false>(c, side, v, rp, pixels, accum, qa + base, qb + base, bc, cap, lds, scene_depth,
-                                     mesh_depth, blocks, per_bounce, sort);
+                                     mesh_depth, blocks, per_bounce, sort, stats);
         if (rc) return rc;
     }
     if (bands > 1) {   // join: the caller's stream continues after the last bounce launch

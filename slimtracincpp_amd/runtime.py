@@ -38,7 +38,8 @@ TRIANGLE = np.dtype([("local_to_tangent", "<f4", 9), ("position", "<f4", 3), ("n
                      ("n1", "<f4", 3), ("n2", "<f4", 3), ("n3", "<f4", 3), ("uv1", "<f4", 2), ("uv2", "<f4", 2),
                      ("uv3", "<f4", 2), ("uv_coverage", "<f4"), ("padding", "<f4")])
 assert GEOMETRY.itemsize == 56 and MATERIAL.itemsize == 80 and LIGHT.itemsize == 32
-assert BVH_NODE.itemsize == 32 and TRIANGLE.itemsize == 128
+BVH4_NODE = np.dtype([("min", "<f4", (3, 4)), ("max", "<f4", (3, 4)), ("child", "<u4", 4), ("count", "<u4", 4)])
+assert BVH_NODE.itemsize == 32 and TRIANGLE.itemsize == 128 and BVH4_NODE.itemsize == 128
 
 
 class Mesh(C.Structure):
@@ -74,7 +75,7 @@ class ViewsConfig(C.Structure):
 
 
 SCENE_REFIT, SCENE_REBUILD = 0, 1
-WAVEFRONT, WAVEFRONT_PER_BOUNCE, WAVEFRONT_SORT, BVH4 = 2, 4, 8, 64   # slimrt_render_params.flags, with RAY_CONES
+WAVEFRONT, WAVEFRONT_PER_BOUNCE, WAVEFRONT_SORT, BVH4, WAVEFRONT_COUNT = 2, 4, 8, 64, 128   # slimrt_render_params.flags
 
 
 def WAVEFRONT_BANDS(log2n: int) -> int:
@@ -117,6 +118,8 @@ def lib():
             "slimrt_views_frame": [vp, vp, u32, u32, u32], "slimrt_views_last_ms": [vp, P(f32)],
             "slimrt_views_autotune": [vp, u32, P(ViewsConfig), P(f32)],
             "slimrt_render_autotune": [vp, P(RenderParams), vp, vp, u32, P(u32), P(f32)],
+            "slimrt_host_collapse_bvh4": [vp, u32, u32, vp, u32, P(u32), P(u32)],
+            "slimrt_wavefront_counts": [vp, vp, u32],
         }
         for name, args in sig.items():
             fn = getattr(L, name)
@@ -163,6 +166,16 @@ def build_bvh(aabbs: np.ndarray, max_leaf: int = 1):
     _check(lib().slimrt_host_build_bvh(_ptr(a), n, max_leaf, _ptr(nodes), C.byref(cnt), C.byref(h), _ptr(leaf)),
            what="build_bvh")
     return nodes[:cnt.value], leaf, h.value
+
+
+def collapse_bvh4(nodes: np.ndarray, triangle_count: int):
+    """The 4-wide tree the renderer walks for a mesh BVH under BVH4 (slimrt_host_collapse_bvh4): nodes, height."""
+    a = np.ascontiguousarray(nodes, BVH_NODE)
+    out = np.zeros((len(a) + 1) // 2, BVH4_NODE)
+    cnt, h = u32(), u32()
+    _check(lib().slimrt_host_collapse_bvh4(_ptr(a), len(a), triangle_count, _ptr(out), len(out), C.byref(cnt),
+                                           C.byref(h)), what="collapse_bvh4")
+    return out[:cnt.value], h.value
 
 
 class MeshBuild:
@@ -372,6 +385,13 @@ class Renderer:
         _check(lib().slimrt_count_work(self.ctx, C.byref(p), _ptr(c)), self.ctx)
         names = ("rays", "pairs", "roots", "instances", "triangles", "mesh_hits", "taps", "pixels")
         return dict(zip(names, (int(v) for v in c)))
+
+    def wavefront_counts(self) -> np.ndarray:
+        """Path counters of the last frame rendered with WAVEFRONT | WAVEFRONT_COUNT: paths per integrator
+        iteration (32), then bounce rays spawned, samples finished."""
+        c = np.zeros(34, np.uint32)
+        _check(lib().slimrt_wavefront_counts(self.ctx, _ptr(c), len(c)), self.ctx)
+        return c
 
     def update_scene_device(self, geo_f: np.ndarray, mode: int = SCENE_REFIT):
         """New transforms for every geometry; world AABBs and the scene BVH (refit or rebuild) on the device."""

@@ -3,7 +3,8 @@ or re-compacted after every bounce, sorted by direction octant or not, in 1, 2, 
 overlap the next band on a second stream, must give every sample bit for bit what
 the per-sample megakernel gives (same iteration code, same accumulation order per sample), on scenes with point
 lights and glow, transparency, glass and mirrors, SSAA and both output kinds; and a frame with many bounces
-(depth 8) equals the oracle within the beauty tolerance."""
+(depth 8) equals the oracle within the beauty tolerance.  With SLIMRT_WAVEFRONT_COUNT the queues' path counters
+obey conservation and equal the oracle's per-iteration path counts."""
 import numpy as np
 import pytest
 
@@ -76,13 +77,49 @@ def test_render_autotune_picks_a_form_with_the_same_frame(gpu_available):
     spec = CASES["mesh_pbr_d4"]()
     r = runtime.Renderer(spec, "gpu")
     p = r.params()
-    ref = _frame(r, p)
     import torch
     n = p.width * p.height
     px = torch.zeros((n, 4), dtype=torch.float32, device="cuda")
     dp = torch.zeros(n, dtype=torch.float32, device="cuda")
     flags, ms = r.autotune_form(p, px.data_ptr(), dp.data_ptr(), reps=2)
-    assert (flags == 0 or flags & WAVEFRONT) and ms > 0
+    assert flags & ~(WAVEFRONT | PER_BOUNCE | SORT | (3 << 4) | runtime.BVH4) == 0 and ms > 0
+    assert flags & (PER_BOUNCE | SORT | (3 << 4)) == 0 or flags & WAVEFRONT
+    p.flags = flags & runtime.BVH4      # the megakernel with the same node width: the wavefront gives its bits
+    ref = _frame(r, p)
     p.flags = flags
     got = _frame(r, p)
     assert np.array_equal(got[0], ref[0]) and np.array_equal(got[1], ref[1])
+
+
+COUNT_FORMS = [WAVEFRONT, WAVEFRONT | SORT, WAVEFRONT | PER_BOUNCE, WAVEFRONT | PER_BOUNCE | SORT,
+               WAVEFRONT | (2 << 4), WAVEFRONT | PER_BOUNCE | (1 << 4)]
+
+
+@pytest.mark.parametrize("name", ["implicit_transparent_d5", "mesh_pbr_d8"])
+def test_wavefront_path_counts(gpu_available, name):
+    """Queue conservation and the per-iteration path counts: in every form each path runs one iteration per queue
+    it passes through, every iteration either spawns a bounce ray into a queue or finishes its sample, every
+    sample finishes once, and the number of paths running iteration i equals the oracle's count of samples whose
+    path reaches iteration i."""
+    spec = CASES[name]()
+    r = runtime.Renderer(spec, "gpu")
+    p = r.params()
+    samples = p.width * p.height
+    want = oracle.OracleScene(spec).iteration_counts(n=16).astype(np.int64)
+    assert want[0] == samples and want[1] > 0            # the case has bounces
+    for flags in COUNT_FORMS + [WAVEFRONT | runtime.BVH4, WAVEFRONT | PER_BOUNCE | SORT | runtime.BVH4]:
+        p.flags = flags | runtime.WAVEFRONT_COUNT
+        _frame(r, p)
+        c = r.wavefront_counts().astype(np.int64)
+        it, spawned, finished = c[:32], c[32], c[33]
+        assert finished == samples, flags
+        assert it.sum() == spawned + finished, flags
+        assert spawned == it[1:].sum(), flags
+        assert np.all(np.diff(it) <= 0), flags
+        if not flags & runtime.BVH4:                       # 4-wide walks may resolve exact ties differently
+            assert np.array_equal(it[:16], want) and not it[16:].any(), (flags, it[:8], want[:8])
+    p.flags = WAVEFRONT                                   # counting leaves the frame's bits alone
+    plain = _frame(r, p)
+    p.flags = WAVEFRONT | runtime.WAVEFRONT_COUNT
+    counted = _frame(r, p)
+    assert np.array_equal(plain[0], counted[0]) and np.array_equal(plain[1], counted[1])

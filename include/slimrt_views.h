@@ -40,12 +40,15 @@ typedef struct slimrt_views slimrt_views;
 /* Launch shape of the multi-view kernel.  tile_shape: 0 = 16x16 samples per workgroup (8x8 per wave),
  * 1 = 32x8 (16x4), 2 = 64x4 (32x2), 3 = 8x32 (4x16).  persistent = 0: one workgroup per tile of every view
  * in one launch (XCD-aware order); persistent = 1: a grid of blocks_per_cu (1..4) workgroups per compute unit
- * takes tiles_per_grab (1..8) tiles at a time from a device work counter until all views are done. */
+ * takes tiles_per_grab (1..8) tiles at a time from a device work counter until all views are done.
+ * node_width: the mesh BVH walk, 2 (sibling pairs) or 4 (the SLIMRT_BVH4 collapse), or 0 = as the views' flags
+ * ask (4 when every view has SLIMRT_BVH4). */
 typedef struct slimrt_views_config {
     uint32_t tile_shape;
     uint32_t blocks_per_cu;
     uint32_t tiles_per_grab;
     uint32_t persistent;
+    uint32_t node_width;
 } slimrt_views_config;
 
 /* A set of views (render params as for slimrt_render_device; beauty or debug modes, canvas or radiance output)
@@ -62,7 +65,8 @@ int slimrt_views_get_config(const slimrt_views *v, slimrt_views_config *cfg);
 int slimrt_views_frame(slimrt_views *v, const slimrt_geometry *geometries, uint32_t count, uint32_t scene_mode,
                        uint32_t use_graph);
 int slimrt_views_last_ms(slimrt_views *v, float *ms);
-/* Times every launch configuration on eager frames (reps each, median) and keeps the fastest. */
+/* Times every launch configuration (tile shape x grid or persistent form x node width 2 or 4) on eager frames
+ * (reps each, median) and keeps the fastest. */
 int slimrt_views_autotune(slimrt_views *v, uint32_t reps, slimrt_views_config *best, float *best_ms);
 const char *slimrt_views_last_error(const slimrt_views *v);
 

@@ -264,7 +264,7 @@ struct slimrt_views {
     std::vector<ViewRec> recs;
     ViewRec *d_recs = nullptr;
     uint32_t total_tiles = 0;
-    slimrt_views_config cfg{0, 2, 1, 0};   // 16x16 tiles, one workgroup per tile
+    slimrt_views_config cfg{0, 2, 1, 0, 0};   // 16x16 tiles, one workgroup per tile, node width from the flags
     uint32_t *d_counter = nullptr;
     slimrt_geometry *h_stage = nullptr, *d_stage = nullptr;
     uint32_t scene_cap = 0;            // scene stack entries the launches size LDS for
@@ -333,8 +333,11 @@ int enqueue_views(slimrt_views *v) {
     const uint32_t blocks = persistent
         ? std::max(1u, std::min((uint32_t)cus * v->cfg.blocks_per_cu, (v->total_tiles + grab - 1) / grab))
         : ((v->total_tiles + 7) / 8) * 8;
-    bool wide = true;   // the 4-wide mesh walk when every view asks for it
-    for (const slimrt_render_params &p : v->params) wide = wide && (p.flags & SLIMRT_BVH4) != 0;
+    bool wide = v->cfg.node_width == 4;
+    if (v->cfg.node_width == 0) {   // as the views ask: the 4-wide mesh walk when every view has SLIMRT_BVH4
+        wide = true;
+        for (const slimrt_render_params &p : v->params) wide = wide && (p.flags & SLIMRT_BVH4) != 0;
+    }
     const uint32_t scene_depth = v->scene_cap + 1, mesh_depth = mesh_stack_depth(c, wide);
     const size_t lds = (size_t)(scene_depth + mesh_depth) * kBlock * sizeof(uint32_t);
     if (lds + 16 > 160 * 1024) return vfail(v, SLIMRT_E_ARG, "BVH too deep for the LDS traversal stacks");
@@ -484,7 +487,8 @@ int slimrt_views_destroy(slimrt_views *v) {
 int slimrt_views_configure(slimrt_views *v, const slimrt_views_config *cfg) {
     if (!v || !cfg) return SLIMRT_E_ARG;
     if (cfg->tile_shape > 3 || cfg->blocks_per_cu < 1 || cfg->blocks_per_cu > 4 || cfg->tiles_per_grab < 1 ||
-        cfg->tiles_per_grab > 8 || cfg->persistent > 1)
+        cfg->tiles_per_grab > 8 || cfg->persistent > 1 || (cfg->node_width != 0 && cfg->node_width != 2 &&
+                                                            cfg->node_width != 4))
         return vfail(v, SLIMRT_E_ARG, "views config out of range");
     VIEWS_TRY(v, hipStreamSynchronize(v->c->stream));
     v->cfg = *cfg;
@@ -556,11 +560,12 @@ int slimrt_views_autotune(slimrt_views *v, uint32_t reps, slimrt_views_config *b
     slimrt_views_config keep = v->cfg, win = v->cfg;
     float win_ms = INFINITY;
     std::vector<slimrt_views_config> cands;
-    for (uint32_t shape = 0; shape < 4; shape++) {
-        cands.push_back({shape, 1, 1, 0});
-        for (uint32_t bpc = 1; bpc <= 2; bpc++)
-            for (uint32_t grab : {1u, 4u}) cands.push_back({shape, bpc, grab, 1});
-    }
+    for (uint32_t width : {2u, 4u})
+        for (uint32_t shape = 0; shape < 4; shape++) {
+            cands.push_back({shape, 1, 1, 0, width});
+            for (uint32_t bpc = 1; bpc <= 2; bpc++)
+                for (uint32_t grab : {1u, 4u}) cands.push_back({shape, bpc, grab, 1, width});
+        }
     for (const slimrt_views_config &cfg : cands) {
         int rc = slimrt_views_configure(v, &cfg);
         if (rc) return rc;

@@ -71,7 +71,8 @@ class RenderParams(C.Structure):
 
 
 class ViewsConfig(C.Structure):
-    _fields_ = [("tile_shape", u32), ("blocks_per_cu", u32), ("tiles_per_grab", u32), ("persistent", u32)]
+    _fields_ = [("tile_shape", u32), ("blocks_per_cu", u32), ("tiles_per_grab", u32), ("persistent", u32),
+                ("node_width", u32)]
 
 
 SCENE_REFIT, SCENE_REBUILD = 0, 1
@@ -445,8 +446,8 @@ class Views:
         except Exception:
             pass
 
-    def configure(self, tile_shape=0, blocks_per_cu=2, tiles_per_grab=1, persistent=0):
-        cfg = ViewsConfig(tile_shape, blocks_per_cu, tiles_per_grab, persistent)
+    def configure(self, tile_shape=0, blocks_per_cu=2, tiles_per_grab=1, persistent=0, node_width=0):
+        cfg = ViewsConfig(tile_shape, blocks_per_cu, tiles_per_grab, persistent, node_width)
         self._check(lib().slimrt_views_configure(self.h, C.byref(cfg)))
 
     def config(self) -> dict:

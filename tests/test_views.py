@@ -6,8 +6,8 @@ GPU checks:
     transforms, and the frame equals a frame rendered after a host rebuild and upload, bit for bit;
   - REFIT keeps the uploaded topology, every node box contains its children / primitives, and the frame equals
     the host-rebuilt one (a refitted tree is a valid BVH over the same boxes: the same closest hits);
-  - every view of a batch, in every launch form the autotuner can choose, is bit-identical to its own
-    single-view render; ragged view sizes included;
+  - every view of a batch, in every launch form the autotuner can choose (node width 2 or 4 included), is
+    bit-identical to its own single-view render with the same node width; ragged view sizes included;
   - graph replays with new transforms each frame equal eager frames, with eager work between replays.
 CPU checks: the seeded generators are deterministic and within their stated ranges.
 """
@@ -134,15 +134,23 @@ def _views(spec, r, n=7, seed=3):
 
 @pytest.mark.gpu
 def test_views_equal_single_view_renders_in_every_launch_form(gpu_available):
+    """Node width 2 against single-view renders of the sibling-pair walk, width 4 against single-view renders with
+    SLIMRT_BVH4 (the same per-sample code: bit for bit either way)."""
     import torch
     spec = _scene()
     r = runtime.Renderer(spec, "gpu")
     params, bufs = _views(spec, r)
-    refs = [_render(r, p) for p in params]
+    refs = {2: [_render(r, p) for p in params]}
+    wide = []
+    for p in params:
+        q = runtime.RenderParams.from_buffer_copy(p)
+        q.flags |= runtime.BVH4
+        wide.append(_render(r, q))
+    refs[4] = wide
     v = runtime.Views(r, params, [b[0].data_ptr() for b in bufs], [b[1].data_ptr() for b in bufs])
-    forms = [(shape, 1, 1, 0) for shape in range(4)] + [(shape, bpc, grab, 1) for shape in range(4)
-                                                          for bpc in (1, 2) for grab in (1, 4)]
-    for form in forms:
+    forms = [(shape, 1, 1, 0, w) for w in (2, 4) for shape in range(4)] + \
+            [(shape, bpc, grab, 1, w) for w in (2, 4) for shape in range(4) for bpc in (1, 2) for grab in (1, 4)]
+    for form in forms + [(0, 2, 1, 0, 0)]:               # node width 0: as the views' flags ask (2 here)
         for px, dp in bufs:
             px.zero_()
             dp.zero_()
@@ -150,7 +158,7 @@ def test_views_equal_single_view_renders_in_every_launch_form(gpu_available):
         v.frame()
         r.synchronize()
         torch.cuda.synchronize()
-        for i, ((px, dp), (rp, rd)) in enumerate(zip(bufs, refs)):
+        for i, ((px, dp), (rp, rd)) in enumerate(zip(bufs, refs[form[4] or 2])):
             assert np.array_equal(_bits(px.cpu().numpy()), _bits(rp)), (form, i)
             assert np.array_equal(_bits(dp.cpu().numpy()), _bits(rd)), (form, i)
     cfg, ms = v.autotune(reps=2)

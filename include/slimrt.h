@@ -236,10 +236,12 @@ typedef struct slimrt_bvh4_node {
  * walked as sibling pairs, src/slim/scene/mesh_tracer.h:51-115): a 4-wide node's children are its BVH2 node's
  * two children with, while fewer than four, the inner child of largest surface area replaced by its own two.
  * Leaves and their triangle ranges are the BVH2's.  nodes_out receives at most node_cap nodes ((node_count + 1) / 2
- * suffices), root first; *height_out the 4-wide tree's height (1 for a root that is a leaf). */
+ * suffices), root first; *height_out the 4-wide tree's height (1 for a root that is a leaf); *stack_out the mesh
+ * stack entries its walk can hold at most (a visit pushes its k inner children and pops one).  The walk keeps
+ * 16-bit node indices on its stack, so SLIMRT_BVH4 frames need every mesh at 65536 4-wide nodes or fewer. */
 int slimrt_host_collapse_bvh4(const slimrt_bvh_node *nodes, uint32_t node_count, uint32_t triangle_count,
                               slimrt_bvh4_node *nodes_out, uint32_t node_cap, uint32_t *node_count_out,
-                              uint32_t *height_out);
+                              uint32_t *height_out, uint32_t *stack_out);
 
 /* The path counters of the last frame rendered with SLIMRT_WAVEFRONT | SLIMRT_WAVEFRONT_COUNT (no counterpart in
  * the reference): counts[i] for i < 32 = paths that ran iteration i of the integrator loop (0 = the camera ray;

@@ -48,11 +48,7 @@ __global__ __launch_bounds__(kBlock) void render_kernel(SceneView s, RenderParam
 
     Counters cnt{};
     TracerT<WIDE> tr;
-    tr.scene_stack = Stack{lds + t, kBlock};
-    tr.mesh_stack = Stack{lds + scene_depth * kBlock + t, kBlock};
-    tr.scene_depth = scene_depth;
-    tr.mesh_depth = mesh_depth;
-    tr.overflow = 0;
+    bind_lds_stacks(tr, lds, t, scene_depth, mesh_depth);
     tr.cnt = kCount ? &cnt : nullptr;
     if (x < ws && y < hs) {
         float px[4], depth;
@@ -96,11 +92,7 @@ __global__ __launch_bounds__(kBlock) void nrc_frame_kernel(SceneView s, RenderPa
     const bool training = nf.train_one_in && (h % nf.train_one_in) == 0;
 
     TracerT<WIDE> tr;
-    tr.scene_stack = Stack{lds + t, kBlock};
-    tr.mesh_stack = Stack{lds + scene_depth * kBlock + t, kBlock};
-    tr.scene_depth = scene_depth;
-    tr.mesh_depth = mesh_depth;
-    tr.overflow = 0;
+    bind_lds_stacks(tr, lds, t, scene_depth, mesh_depth);
     tr.cnt = nullptr;
     if (x < ws && y < hs) {
         v3 dir;
@@ -181,8 +173,9 @@ int launch(slimrt_context *c, const slimrt_render_params *p, float4 *pixels, flo
     const uint32_t tiles = tiles_x * tiles_y;
     const uint32_t per_xcd = (tiles + 7) / 8;
     const uint32_t scene_depth = c->scene_height + 1, mesh_depth = mesh_stack_depth(c, wide_mesh_bvh(rp));
-    const size_t lds = (size_t)(scene_depth + mesh_depth) * kBlock * sizeof(uint32_t);
+    const size_t lds = stack_lds_bytes(scene_depth, mesh_depth, wide_mesh_bvh(rp));
     if (lds > 160 * 1024) return fail(c, SLIMRT_E_ARG, "BVH too deep for the LDS traversal stacks");
+    if (wide_mesh_bvh(rp) && check_wide(c)) return SLIMRT_E_ARG;
     SceneView v = device_view(c);
     if (c->timed) HIP_TRY(c, hipEventRecord(c->ev0, c->stream));
     if (!count && (rp.flags & SLIMRT_WAVEFRONT) && rp.render_mode == RM_Beauty && rp.output != SLIMRT_OUTPUT_HITS) {
@@ -256,8 +249,11 @@ int cpu_render(slimrt_context *c, const slimrt_render_params *p, slimrt_pixel *p
 // Collapses one mesh's sibling-pair BVH (nodes, mesh-relative indices) into 4-wide nodes appended to out: a
 // node's children are its two children with, while fewer than four, the inner child of largest surface area
 // replaced by its own two children.  Leaves keep their triangle ranges.  Returns the node index; height4 receives
-// the collapsed tree's height.
-uint32_t collapse4(const Node *n2, uint32_t i2, uint32_t depth, std::vector<Node4> &out, uint32_t &height4) {
+// the collapsed tree's height, stack the most mesh stack entries the 4-wide walk can hold below this node: a
+// visit pushes its k inner children and pops one, so stack = max(k, k - 1 + the largest stack of its inner
+// children).
+uint32_t collapse4(const Node *n2, uint32_t i2, uint32_t depth, std::vector<Node4> &out, uint32_t &height4,
+                   uint32_t &stack) {
     auto is_leaf = [&](uint32_t k) { return (n2[k].leaf_count_depth & 0xFFFFu) != 0; };
     auto area = [&](uint32_t k) {
         const float ex = n2[k].maxx - n2[k].minx, ey = n2[k].maxy - n2[k].miny, ez = n2[k].maxz - n2[k].minz;
@@ -279,6 +275,7 @@ uint32_t collapse4(const Node *n2, uint32_t i2, uint32_t depth, std::vector<Node
     out.push_back(Node4{});
     height4 = std::max(height4, depth);
     Node4 node;
+    uint32_t inner = 0, below = 0;
     for (uint32_t j = 0; j < 4; j++) {
         if (j >= nk) {
             node.minx[j] = node.miny[j] = node.minz[j] = INFINITY;
@@ -294,18 +291,23 @@ uint32_t collapse4(const Node *n2, uint32_t i2, uint32_t depth, std::vector<Node
             node.child[j] = c.first_index;
             node.count[j] = c.leaf_count_depth & 0xFFFFu;
         } else {
-            node.child[j] = collapse4(n2, kids[j], depth + 1, out, height4);
+            uint32_t child_stack = 0;
+            node.child[j] = collapse4(n2, kids[j], depth + 1, out, height4, child_stack);
             node.count[j] = 0;
+            inner++;
+            below = std::max(below, child_stack);
         }
     }
+    stack = inner ? std::max(inner, inner - 1 + below) : 0;
     out[idx] = node;
     return idx;
 }
 
-// The 4-wide tree of a validated sibling-pair mesh BVH (children after parents), root first; returns its height.
-// A root that is a leaf becomes one node holding that leaf.
-uint32_t collapse_mesh_bvh4(const Node *n2, std::vector<Node4> &out) {
+// The 4-wide tree of a validated sibling-pair mesh BVH (children after parents), root first; returns its height
+// and sets stack to the mesh stack entries its walk needs.  A root that is a leaf becomes one node holding it.
+uint32_t collapse_mesh_bvh4(const Node *n2, std::vector<Node4> &out, uint32_t &stack) {
     uint32_t h4 = 1;
+    stack = 0;
     out.clear();
     if (n2[0].leaf_count_depth & 0xFFFFu) {
         Node4 leaf;
@@ -321,7 +323,7 @@ uint32_t collapse_mesh_bvh4(const Node *n2, std::vector<Node4> &out) {
         leaf.count[0] = n2[0].leaf_count_depth & 0xFFFFu;
         out.push_back(leaf);
     } else {
-        collapse4(n2, 0, 1, out, h4);
+        collapse4(n2, 0, 1, out, h4, stack);
     }
     return h4;
 }
@@ -520,6 +522,8 @@ int slimrt_upload_scene(slimrt_context *c, const slimrt_scene_desc *d) {
     c->meshes.clear(); c->mnodes.clear(); c->tri_hit.clear(); c->tri_attr.clear(); c->mnodes4.clear();
     c->mesh_height = 0;
     c->mesh_height4 = 0;
+    c->mesh_stack4 = 0;
+    c->mesh_nodes4_max = 0;
     for (uint32_t k = 0; k < d->mesh_count; k++) {
         const slimrt_mesh &m = d->meshes[k];
         if (!m.nodes || !m.triangles || m.node_count == 0 || m.triangle_count == 0)
@@ -540,7 +544,10 @@ int slimrt_upload_scene(slimrt_context *c, const slimrt_scene_desc *d) {
         }
         r.node4_base = (uint32_t)c->mnodes4.size();
         std::vector<Node4> n4;
-        const uint32_t h4 = collapse_mesh_bvh4(c->mnodes.data() + r.node_base, n4);
+        uint32_t stack4 = 0;
+        const uint32_t h4 = collapse_mesh_bvh4(c->mnodes.data() + r.node_base, n4, stack4);
+        c->mesh_stack4 = std::max(c->mesh_stack4, stack4);
+        c->mesh_nodes4_max = std::max(c->mesh_nodes4_max, (uint32_t)n4.size());
         c->mnodes4.insert(c->mnodes4.end(), n4.begin(), n4.end());
         r.node4_count = (uint32_t)n4.size();
         r.height4 = h4;
@@ -649,8 +656,8 @@ int slimrt_count_work(slimrt_context *c, const slimrt_render_params *p, uint64_t
 
 int slimrt_host_collapse_bvh4(const slimrt_bvh_node *nodes, uint32_t node_count, uint32_t triangle_count,
                               slimrt_bvh4_node *nodes_out, uint32_t node_cap, uint32_t *node_count_out,
-                              uint32_t *height_out) {
-    if (!nodes || !node_count || !nodes_out || !node_count_out || !height_out) return SLIMRT_E_ARG;
+                              uint32_t *height_out, uint32_t *stack_out) {
+    if (!nodes || !node_count || !nodes_out || !node_count_out || !height_out || !stack_out) return SLIMRT_E_ARG;
     std::vector<Node> n2(node_count);
     for (uint32_t i = 0; i < node_count; i++) {   // the checks of the upload: the walk below stays in range
         const slimrt_bvh_node &n = nodes[i];
@@ -660,12 +667,14 @@ int slimrt_host_collapse_bvh4(const slimrt_bvh_node *nodes, uint32_t node_count,
         n2[i] = to_node(n);
     }
     std::vector<Node4> out;
-    const uint32_t h4 = collapse_mesh_bvh4(n2.data(), out);
+    uint32_t stack4 = 0;
+    const uint32_t h4 = collapse_mesh_bvh4(n2.data(), out, stack4);
     if (out.size() > node_cap) return SLIMRT_E_OVERFLOW;
     static_assert(sizeof(Node4) == sizeof(slimrt_bvh4_node), "4-wide node layout");
     memcpy(nodes_out, out.data(), out.size() * sizeof(Node4));
     *node_count_out = (uint32_t)out.size();
     *height_out = h4;
+    *stack_out = stack4;
     return SLIMRT_OK;
 }
 
@@ -710,8 +719,9 @@ int slimrt_render_nrc(slimrt_context *c, const slimrt_render_params *p, slimrt_n
     const uint32_t tiles_x = (ws + kTile - 1) / kTile, tiles_y = (hs + kTile - 1) / kTile;
     const uint32_t tiles = tiles_x * tiles_y, per_xcd = (tiles + 7) / 8;
     const uint32_t scene_depth = c->scene_height + 1, mesh_depth = mesh_stack_depth(c, wide_mesh_bvh(rp));
-    const size_t lds = (size_t)(scene_depth + mesh_depth) * kBlock * sizeof(uint32_t);
+    const size_t lds = stack_lds_bytes(scene_depth, mesh_depth, wide_mesh_bvh(rp));
     if (lds > 160 * 1024) return fail(c, SLIMRT_E_ARG, "BVH too deep for the LDS traversal stacks");
+    if (wide_mesh_bvh(rp) && check_wide(c)) return SLIMRT_E_ARG;
     // training waves are a hashed 1-in-N subset: twice the expected share (at least 64 waves) bounds the samples
     uint64_t t_cap = 0;
     if (np->train_one_in) {

@@ -48,6 +48,8 @@ struct slimrt_context {
     uint32_t scene_height = 0, mesh_height = 0, scene_flags = 0;
     std::vector<Node4> mnodes4;             // 4-wide mesh BVHs collapsed from mnodes at upload
     uint32_t mesh_height4 = 0;
+    uint32_t mesh_stack4 = 0;       // mesh stack entries the 4-wide walk needs (largest over the meshes)
+    uint32_t mesh_nodes4_max = 0;   // 4-wide nodes of the largest mesh (16-bit stack entries need <= 65536)
     bool loaded = false;
 
     DevBuf<Geo> d_geos;
@@ -174,7 +176,30 @@ inline SceneView host_view(const slimrt_context *c) {
 // Traversal stack entries for mesh BVHs: the declared height + 1 for the sibling-pair walk; for the 4-wide walk
 // up to three pending siblings per level of the collapsed tree, plus the children of the node being visited.
 inline uint32_t mesh_stack_depth(const slimrt_context *c, bool wide) {
-    return wide ? 3 * c->mesh_height4 + 4 : c->mesh_height + 1;
+    return wide ? std::max(c->mesh_stack4, 1u) : c->mesh_height + 1;
+}
+
+// A workgroup's traversal stacks in LDS: kBlock columns of scene entries (32-bit), then kBlock columns of mesh
+// entries (32-bit pair indices, or 16-bit node indices for the 4-wide walk).
+inline size_t stack_lds_bytes(uint32_t scene_depth, uint32_t mesh_depth, bool wide) {
+    return ((size_t)scene_depth * 4 + (size_t)mesh_depth * (wide ? 2 : 4)) * kBlock;
+}
+template <bool WIDE>
+__device__ __forceinline__ void bind_lds_stacks(TracerT<WIDE> &tr, uint32_t *lds, uint32_t t, uint32_t scene_depth,
+                                                uint32_t mesh_depth) {
+    tr.scene_stack = StackT<uint32_t>{lds + t, kBlock};
+    tr.mesh_stack = StackT<typename TracerT<WIDE>::MeshEntry>{
+        reinterpret_cast<typename TracerT<WIDE>::MeshEntry *>(lds + scene_depth * kBlock) + t, kBlock};
+    tr.scene_depth = scene_depth;
+    tr.mesh_depth = mesh_depth;
+    tr.overflow = 0;
+}
+
+// Checks that the 4-wide walk can run on the uploaded meshes (16-bit stack entries).
+inline int check_wide(slimrt_context *c) {
+    if (c->mesh_nodes4_max > 65536)
+        return fail(c, SLIMRT_E_ARG, "a mesh has more than 65536 4-wide nodes: render it with node width 2");
+    return SLIMRT_OK;
 }
 
 inline SceneView device_view(const slimrt_context *c) {

@@ -7,15 +7,19 @@
 // stack at base[i * stride]), on the CPU in a local array (stride 1).
 #pragma once
 
+#include <type_traits>
+
 #include "slimrt_core.h"
 
 namespace slimrt {
 
-struct Stack {
-    uint32_t *base;
+template <class T>
+struct StackT {
+    T *base;
     uint32_t stride;
-    SLIM_HD uint32_t &operator[](uint32_t i) const { return base[i * stride]; }
+    SLIM_HD T &operator[](uint32_t i) const { return base[i * stride]; }
 };
+using Stack = StackT<uint32_t>;
 
 struct Counters {   // work counters, accumulated only when the caller asks for them
     uint64_t rays, pairs, roots, instances, triangles, mesh_hits, taps, pixels;
@@ -26,7 +30,10 @@ struct Counters {   // work counters, accumulated only when the caller asks for 
 template <bool WIDE>
 struct TracerT {
     static constexpr bool kWide = WIDE;
-    Stack scene_stack, mesh_stack;
+    // mesh stack entries: sibling-pair indices, or 4-wide node indices relative to the mesh (at most 65536 nodes)
+    using MeshEntry = typename std::conditional<WIDE, uint16_t, uint32_t>::type;
+    Stack scene_stack;
+    StackT<MeshEntry> mesh_stack;
     uint32_t scene_depth, mesh_depth;   // capacities
     uint32_t overflow;                  // set when a push would exceed a capacity
     Ray aux_ray;                        // last localized ray (SceneTracer::aux_ray)
@@ -195,7 +202,7 @@ SLIM_HD bool mesh_trace4(const SceneView &s, const MeshRec &m, const Ray &r, Hit
             const uint32_t count = i == 0 ? n.count[0] : i == 1 ? n.count[1] : i == 2 ? n.count[2] : n.count[3];
             if (!(key[k] < hit.distance) || count != 0) continue;
             const uint32_t child = i == 0 ? n.child[0] : i == 1 ? n.child[1] : i == 2 ? n.child[2] : n.child[3];
-            if (top < tr.mesh_depth) tr.mesh_stack[top++] = child; else tr.overflow = 1;
+            if (top < tr.mesh_depth) tr.mesh_stack[top++] = (typename TR::MeshEntry)child; else tr.overflow = 1;
         }
         if (top == 0) break;
         cur = tr.mesh_stack[--top];

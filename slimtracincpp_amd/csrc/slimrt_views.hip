@@ -189,11 +189,7 @@ __global__ __launch_bounds__(kBlock) void views_kernel(SceneView s, const ViewRe
     __shared__ uint32_t item;
     const uint32_t t = threadIdx.x, wave = t >> 6, lane = t & 63u;
     TracerT<WIDE> tr;
-    tr.scene_stack = Stack{lds + t, kBlock};
-    tr.mesh_stack = Stack{lds + scene_depth * kBlock + t, kBlock};
-    tr.scene_depth = scene_depth;
-    tr.mesh_depth = mesh_depth;
-    tr.overflow = 0;
+    bind_lds_stacks(tr, lds, t, scene_depth, mesh_depth);
     tr.cnt = nullptr;
     if constexpr (PERSISTENT) {
         for (;;) {
@@ -339,8 +335,9 @@ int enqueue_views(slimrt_views *v) {
         for (const slimrt_render_params &p : v->params) wide = wide && (p.flags & SLIMRT_BVH4) != 0;
     }
     const uint32_t scene_depth = v->scene_cap + 1, mesh_depth = mesh_stack_depth(c, wide);
-    const size_t lds = (size_t)(scene_depth + mesh_depth) * kBlock * sizeof(uint32_t);
+    const size_t lds = stack_lds_bytes(scene_depth, mesh_depth, wide);
     if (lds + 16 > 160 * 1024) return vfail(v, SLIMRT_E_ARG, "BVH too deep for the LDS traversal stacks");
+    if (wide && check_wide(c)) return vfail(v, SLIMRT_E_ARG, c->err);
     const SceneView sv = device_view(c);
     const uint32_t n = (uint32_t)v->recs.size();
 #define VIEWS_LAUNCH1(WX, WY, P, W)                                                                           \
@@ -561,7 +558,7 @@ int slimrt_views_autotune(slimrt_views *v, uint32_t reps, slimrt_views_config *b
     float win_ms = INFINITY;
     std::vector<slimrt_views_config> cands;
     for (uint32_t width : {2u, 4u})
-        for (uint32_t shape = 0; shape < 4; shape++) {
+        for (uint32_t shape = 0; shape < 4 && (width == 2 || v->c->mesh_nodes4_max <= 65536); shape++) {
             cands.push_back({shape, 1, 1, 0, width});
             for (uint32_t bpc = 1; bpc <= 2; bpc++)
                 for (uint32_t grab : {1u, 4u}) cands.push_back({shape, bpc, grab, 1, width});
@@ -602,6 +599,7 @@ int slimrt_render_autotune(slimrt_context *c, const slimrt_render_params *params
                                                       SLIMRT_WAVEFRONT_SORT | SLIMRT_WAVEFRONT_BANDS(3) | SLIMRT_BVH4);
     std::vector<uint32_t> forms;
     for (uint32_t width : {0u, (uint32_t)SLIMRT_BVH4}) {   // node width 2 or 4
+        if (width && c->mesh_nodes4_max > 65536) continue;   // meshes too large for the 4-wide walk
         forms.push_back(width);
         for (uint32_t bands_log2 : {0u, 2u})
             for (uint32_t v : {0u, (uint32_t)SLIMRT_WAVEFRONT_SORT, (uint32_t)SLIMRT_WAVEFRONT_PER_BOUNCE,

@@ -146,11 +146,7 @@ __global__ __launch_bounds__(kBlock) void wf_primary_kernel(SceneView s, RenderP
     const uint32_t y = (tile / tiles_x) * kTile + (wave >> 1) * 8u + (lane >> 3);
     const uint32_t ws = rp.width << rp.ssaa, hs = rp.height << rp.ssaa;
     TracerT<WIDE> tr;
-    tr.scene_stack = Stack{lds + t, kBlock};
-    tr.mesh_stack = Stack{lds + scene_depth * kBlock + t, kBlock};
-    tr.scene_depth = scene_depth;
-    tr.mesh_depth = mesh_depth;
-    tr.overflow = 0;
+    bind_lds_stacks(tr, lds, t, scene_depth, mesh_depth);
     tr.cnt = nullptr;
     const bool inside = x < ws && y < hs;
     bool cont = false;
@@ -189,11 +185,7 @@ void wf_bounce_kernel(SceneView s, RenderParams rp, float4 *__restrict__ pixels,
     extern __shared__ uint32_t lds[];
     const uint32_t t = threadIdx.x;
     TracerT<WIDE> tr;
-    tr.scene_stack = Stack{lds + t, kBlock};
-    tr.mesh_stack = Stack{lds + scene_depth * kBlock + t, kBlock};
-    tr.scene_depth = scene_depth;
-    tr.mesh_depth = mesh_depth;
-    tr.overflow = 0;
+    bind_lds_stacks(tr, lds, t, scene_depth, mesh_depth);
     tr.cnt = nullptr;
     const uint32_t n = min(*in_count, cap);
     const uint32_t stride = gridDim.x * kBlock;
@@ -347,8 +339,9 @@ int launch_wavefront(slimrt_context *c, const RenderParams &rp, float4 *pixels, 
     }
     if (!c->wf_counters) HIP_TRY(c, hipMalloc(&c->wf_counters, 8 * WF_COUNTERS * sizeof(uint32_t)));
     const uint32_t scene_depth = c->scene_height + 1, mesh_depth = mesh_stack_depth(c, wide_mesh_bvh(rp));
-    const size_t lds = (size_t)(scene_depth + mesh_depth) * kBlock * sizeof(uint32_t);
+    const size_t lds = stack_lds_bytes(scene_depth, mesh_depth, wide_mesh_bvh(rp));
     if (lds > 160 * 1024) return fail(c, SLIMRT_E_ARG, "BVH too deep for the LDS traversal stacks");
+    if (wide_mesh_bvh(rp) && check_wide(c)) return SLIMRT_E_ARG;
     int cus = 0;
     HIP_TRY(c, hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, c->device));
     const SceneView v = device_view(c);

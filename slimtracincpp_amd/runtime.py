@@ -119,7 +119,7 @@ def lib():
             "slimrt_views_frame": [vp, vp, u32, u32, u32], "slimrt_views_last_ms": [vp, P(f32)],
             "slimrt_views_autotune": [vp, u32, P(ViewsConfig), P(f32)],
             "slimrt_render_autotune": [vp, P(RenderParams), vp, vp, u32, P(u32), P(f32)],
-            "slimrt_host_collapse_bvh4": [vp, u32, u32, vp, u32, P(u32), P(u32)],
+            "slimrt_host_collapse_bvh4": [vp, u32, u32, vp, u32, P(u32), P(u32), P(u32)],
             "slimrt_wavefront_counts": [vp, vp, u32],
         }
         for name, args in sig.items():
@@ -169,14 +169,15 @@ def build_bvh(aabbs: np.ndarray, max_leaf: int = 1):
     return nodes[:cnt.value], leaf, h.value
 
 
-def collapse_bvh4(nodes: np.ndarray, triangle_count: int):
-    """The 4-wide tree the renderer walks for a mesh BVH under BVH4 (slimrt_host_collapse_bvh4): nodes, height."""
+def collapse_bvh4(nodes: np.ndarray, triangle_count: int, with_stack: bool = False):
+    """The 4-wide tree the renderer walks for a mesh BVH under BVH4 (slimrt_host_collapse_bvh4): nodes, height
+    (and the stack entries its walk can need, with_stack)."""
     a = np.ascontiguousarray(nodes, BVH_NODE)
     out = np.zeros((len(a) + 1) // 2, BVH4_NODE)
-    cnt, h = u32(), u32()
+    cnt, h, st = u32(), u32(), u32()
     _check(lib().slimrt_host_collapse_bvh4(_ptr(a), len(a), triangle_count, _ptr(out), len(out), C.byref(cnt),
-                                           C.byref(h)), what="collapse_bvh4")
-    return out[:cnt.value], h.value
+                                           C.byref(h), C.byref(st)), what="collapse_bvh4")
+    return (out[:cnt.value], h.value, st.value) if with_stack else (out[:cnt.value], h.value)
 
 
 class MeshBuild:

@@ -91,6 +91,32 @@ def test_collapse_keeps_leaves_and_boxes(name):
     assert starts[-1][0] + starts[-1][1] == T                           # ... which partition the triangles
 
 
+def _all_hit_walk(n4, rng):
+    """Deepest stack of the 4-wide walk when every box is hit, visiting each node's inner children in a random
+    order (the nearest-first order of some ray); the walk pushes them all and pops one."""
+    stack, deepest, cur = [], 0, 0
+    while True:
+        inner = [int(n4[cur]["child"][j]) for j in range(4)
+                 if n4[cur]["child"][j] != 0xFFFFFFFF and n4[cur]["count"][j] == 0]
+        rng.shuffle(inner)
+        stack.extend(inner)
+        deepest = max(deepest, len(stack))
+        if not stack:
+            return deepest
+        cur = stack.pop()
+
+
+@pytest.mark.parametrize("name", sorted(MESHES))
+def test_stack_bound_covers_every_visiting_order(name):
+    """The stack capacity the renderer sizes from the collapse holds the all-boxes-hit walk in every visiting order
+    tried (60 random orders) and stays within three entries per level (the bound it replaces)."""
+    b = runtime.MeshBuild(MESHES[name])
+    n4, h4, bound = runtime.collapse_bvh4(b.nodes, len(b.triangles), with_stack=True)
+    rng = np.random.default_rng(5)
+    walks = [_all_hit_walk(n4, rng) for _ in range(60)]
+    assert max(walks) <= bound <= 3 * h4 + 4
+
+
 def test_collapse_of_a_single_leaf():
     n2 = np.zeros(1, runtime.BVH_NODE)
     n2[0]["min"], n2[0]["max"], n2[0]["first_index"], n2[0]["leaf_count"] = (-1, -1, -1), (1, 1, 1), 0, 3

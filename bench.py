@@ -47,7 +47,8 @@ BYTES_PER = {"pairs": 64, "instances": 64, "triangles": 64, "mesh_hits": 64, "ta
 OUT_BYTES_PER_SAMPLE = 20
 
 
-FORM_NAMES = ("megakernel", "wavefront", "wavefront_sorted", "wavefront_per_bounce", "wavefront_per_bounce_sorted")
+FORM_NAMES = ("megakernel", "wavefront", "wavefront_sorted", "wavefront_per_bounce", "wavefront_per_bounce_sorted",
+              "wavefront_split", "wavefront_sorted_split")
 FORM_NAMES = FORM_NAMES + tuple(f + "_bvh4" for f in FORM_NAMES)
 
 
@@ -348,15 +349,17 @@ def main():
     cnt4 = r.count_work(p)                      # 4-wide mesh walk: 'pairs' counts node visits
     p.flags = 0
 
-    # Ten forms of the same frame (tests/test_wavefront.py, tests/test_bvh4.py): the per-sample megakernel and the
-    # bounces as a wavefront over a compacted ray queue (run to the end in one launch, or re-compacted after every
-    # bounce; sorted by direction octant or not), each with the meshes' sibling-pair BVHs or their 4-wide
-    # collapse.  Each is timed over K back-to-back frames between synchronisations; the fastest is the reported
+    # Fourteen forms of the same frame (tests/test_wavefront.py, tests/test_bvh4.py): the per-sample megakernel and
+    # the bounces as a wavefront over a compacted ray queue (run to the end in one launch, or re-compacted after
+    # every bounce; sorted by direction octant or not; first hits traced by the shading kernel or by a separate
+    # traversal kernel), each with the meshes' sibling-pair BVHs or their 4-wide collapse.  Each is timed over K back-to-back frames between synchronisations; the fastest is the reported
     # path.
     base_forms = {"megakernel": 0, "wavefront": runtime.WAVEFRONT,
                   "wavefront_sorted": runtime.WAVEFRONT | runtime.WAVEFRONT_SORT,
                   "wavefront_per_bounce": runtime.WAVEFRONT | runtime.WAVEFRONT_PER_BOUNCE,
-                  "wavefront_per_bounce_sorted": runtime.WAVEFRONT | runtime.WAVEFRONT_PER_BOUNCE | runtime.WAVEFRONT_SORT}
+                  "wavefront_per_bounce_sorted": runtime.WAVEFRONT | runtime.WAVEFRONT_PER_BOUNCE | runtime.WAVEFRONT_SORT,
+                  "wavefront_split": runtime.WAVEFRONT | runtime.WAVEFRONT_SPLIT,
+                  "wavefront_sorted_split": runtime.WAVEFRONT | runtime.WAVEFRONT_SORT | runtime.WAVEFRONT_SPLIT}
     forms = dict(base_forms)
     forms.update({k + "_bvh4": v | runtime.BVH4 for k, v in base_forms.items()})
     assert tuple(forms) == FORM_NAMES
@@ -425,7 +428,11 @@ def main():
                                 "wavefront_per_bounce": "the frame's launches: wf_primary_kernel + "
                                                         "wf_bounce_kernel<false> per bounce",
                                 "wavefront_per_bounce_sorted": "the frame's launches: wf_primary_kernel + per bounce "
-                                                               "octant sort + wf_bounce_kernel<false>"}[form.replace("_bvh4", "")],
+                                                               "octant sort + wf_bounce_kernel<false>",
+                                "wavefront_split": "the frame's launches: wf_trace_kernel (first hits) + "
+                                                   "wf_primary_kernel (their shading) + one wf_bounce_kernel<true>",
+                                "wavefront_sorted_split": "the frame's launches: wf_trace_kernel + wf_primary_kernel + "
+                                                          "octant sort + wf_bounce_kernel<true>"}[form.replace("_bvh4", "")],
                      "kernel_ms": k_ms, "algorithmic_bytes": bytes_frame,
                      "counters": cnt4 if wide else cnt},
         "neural_cache": None,

@@ -62,6 +62,9 @@ extern "C" {
                                       and triangles; only exact ties between triangles can resolve differently) */
 #define SLIMRT_WAVEFRONT_COUNT 128u /* with SLIMRT_WAVEFRONT: also count the paths of the frame (per iteration,
                                        spawned, retired) for slimrt_wavefront_counts; costs atomics, for checks */
+#define SLIMRT_WAVEFRONT_SPLIT 256u /* with SLIMRT_WAVEFRONT: the first hits are found by a traversal-only kernel
+                                       (fewer registers, more waves in flight) that hands hit records to the
+                                       shading kernel through device memory */
 
 typedef struct slimrt_context slimrt_context;
 
@@ -166,7 +169,8 @@ typedef struct {
     uint32_t render_mode;
     int32_t skybox_color, skybox_radiance, skybox_irradiance;   /* texture ids, -1 for none */
     uint32_t output;            /* SLIMRT_OUTPUT_* */
-    uint32_t flags;             /* SLIMRT_RAY_CONES | SLIMRT_WAVEFRONT (| _PER_BOUNCE | _SORT | _BANDS(n) | _COUNT) | SLIMRT_BVH4 */
+    uint32_t flags;             /* SLIMRT_RAY_CONES | SLIMRT_WAVEFRONT (| _PER_BOUNCE | _SORT | _BANDS(n) | _COUNT |
+                                   _SPLIT) | SLIMRT_BVH4 */
     float mip_level_colors[9][3];
 } slimrt_render_params;
 

@@ -437,7 +437,7 @@ int slimrt_destroy(slimrt_context *c) {
         dev_free(c->d_mips); dev_free(c->d_quads); dev_free(c->d_pixels); dev_free(c->d_depths);
         free_nrc(c);
         dev_free(c->d_geo_stage);
-        for (void *q : {c->wf_queue_a, c->wf_queue_b, c->wf_accum, (void *)c->wf_counters, (void *)c->wf_stats})
+        for (void *q : {c->wf_queue_a, c->wf_queue_b, c->wf_accum, c->wf_hits, (void *)c->wf_counters, (void *)c->wf_stats})
             if (q) (void)hipFree(q);
         for (hipEvent_t e : c->wf_band)
             if (e) (void)hipEventDestroy(e);

@@ -68,6 +68,8 @@ struct slimrt_context {
     DevBuf<float> d_depths;
     DevBuf<slimrt_geometry> d_geo_stage;   // transform updates on their way to the scene update kernel
     void *wf_queue_a = nullptr, *wf_queue_b = nullptr, *wf_accum = nullptr;   // wavefront ray queues, colours
+    void *wf_hits = nullptr;   // first-hit records of the split primary pass (36 bytes per sample slot)
+    uint32_t wf_hits_cap = 0;
     uint32_t *wf_counters = nullptr, wf_cap = 0;
     uint32_t *wf_stats = nullptr;   // kWfStats path counters (SLIMRT_WAVEFRONT_COUNT)
     hipStream_t side_stream = nullptr;          // a wavefront frame's bounce launches, overlapping the next band

@@ -596,15 +596,17 @@ int slimrt_render_autotune(slimrt_context *c, const slimrt_render_params *params
                            uint32_t reps, uint32_t *best_flags, float *best_ms) {
     if (!c || !params || !best_flags || reps == 0) return SLIMRT_E_ARG;
     const uint32_t base = params->flags & ~(uint32_t)(SLIMRT_WAVEFRONT | SLIMRT_WAVEFRONT_PER_BOUNCE |
-                                                      SLIMRT_WAVEFRONT_SORT | SLIMRT_WAVEFRONT_BANDS(3) | SLIMRT_BVH4);
+                                                      SLIMRT_WAVEFRONT_SORT | SLIMRT_WAVEFRONT_BANDS(3) |
+                                                      SLIMRT_WAVEFRONT_SPLIT | SLIMRT_WAVEFRONT_COUNT | SLIMRT_BVH4);
     std::vector<uint32_t> forms;
     for (uint32_t width : {0u, (uint32_t)SLIMRT_BVH4}) {   // node width 2 or 4
         if (width && c->mesh_nodes4_max > 65536) continue;   // meshes too large for the 4-wide walk
         forms.push_back(width);
-        for (uint32_t bands_log2 : {0u, 2u})
-            for (uint32_t v : {0u, (uint32_t)SLIMRT_WAVEFRONT_SORT, (uint32_t)SLIMRT_WAVEFRONT_PER_BOUNCE,
-                               (uint32_t)(SLIMRT_WAVEFRONT_PER_BOUNCE | SLIMRT_WAVEFRONT_SORT)})
-                forms.push_back(SLIMRT_WAVEFRONT | v | SLIMRT_WAVEFRONT_BANDS(bands_log2) | width);
+        for (uint32_t split : {0u, (uint32_t)SLIMRT_WAVEFRONT_SPLIT})
+            for (uint32_t bands_log2 : {0u, 2u})
+                for (uint32_t v : {0u, (uint32_t)SLIMRT_WAVEFRONT_SORT, (uint32_t)SLIMRT_WAVEFRONT_PER_BOUNCE,
+                                   (uint32_t)(SLIMRT_WAVEFRONT_PER_BOUNCE | SLIMRT_WAVEFRONT_SORT)})
+                    forms.push_back(SLIMRT_WAVEFRONT | v | SLIMRT_WAVEFRONT_BANDS(bands_log2) | split | width);
     }
     slimrt_render_params p = *params;
     float win_ms = INFINITY;

@@ -35,17 +35,17 @@ static_assert(sizeof(PathState) == 48, "path state layout");
 
 enum : uint32_t { WF_COUNT_A = 0, WF_COUNT_B = 1, WF_BINS = 2, WF_CURSORS = 10, WF_COUNTERS = 18 };
 
-// One iteration of the integrator loop (the body of render_beauty, in its order); returns whether the path
-// continues, with ray, throughput and depth_left advanced to the bounce.
+// One iteration of the integrator loop (the body of render_beauty, in its order) after its closest-hit trace:
+// geo and hit are what scene_trace returned for ray (whose origin it offset).  Returns whether the path continues,
+// with ray, throughput and depth_left advanced to the bounce.
 template <class TR>
-__device__ __forceinline__ bool beauty_iteration(const SceneView &s, const RenderParams &rp, Ray &ray, float scaling,
-                                                 uint32_t &depth_left, color &tp, color &col, float &depth, TR &tr) {
+__device__ __forceinline__ bool shade_iteration(const SceneView &s, const RenderParams &rp, Ray &ray, Hit &hit, int geo,
+                                                float scaling, uint32_t &depth_left, color &tp, color &col,
+                                                float &depth, TR &tr) {
     Surface sf;
-    Hit hit;
-    hit.scaling_factor = scaling;
     color cur = cs(0), next_tp = cs(0);
     bool bounced = false;
-    sf.geo = scene_trace(s, ray, hit, false, INFINITY, tr);
+    sf.geo = geo;
     if (sf.geo >= 0) {
         hit.scaling_factor = keeps_ray_cones(rp) ? scaling : 0.0f;
         const Geo g = s.geos[sf.geo];
@@ -85,6 +85,46 @@ __device__ __forceinline__ bool beauty_iteration(const SceneView &s, const Rende
     col = cmul_add(cur, tp, col);
     tp = tp * next_tp;
     return bounced;
+}
+
+template <class TR>
+__device__ __forceinline__ bool beauty_iteration(const SceneView &s, const RenderParams &rp, Ray &ray, float scaling,
+                                                 uint32_t &depth_left, color &tp, color &col, float &depth, TR &tr) {
+    Hit hit;
+    hit.scaling_factor = scaling;
+    const int geo = scene_trace(s, ray, hit, false, INFINITY, tr);
+    return shade_iteration(s, rp, ray, hit, geo, scaling, depth_left, tp, col, depth, tr);
+}
+
+// First-hit records of the split primary pass, structure of arrays over the frame's sample slots (tile * kBlock +
+// thread, so a wave's 64 records are contiguous): the hit's local normal and distance; its uv, uv coverage and
+// NdotRd; the geometry index with from_behind in bit 31 (all ones for a miss).  What shading reads of a Hit.
+constexpr size_t kHitSlotBytes = 36;
+struct HitSlots {
+    float4 *a, *b;
+    uint32_t *g;
+};
+inline HitSlots hit_slots(void *base, uint32_t cap) {
+    char *p = (char *)base;
+    return HitSlots{(float4 *)p, (float4 *)(p + (size_t)cap * 16), (uint32_t *)(p + (size_t)cap * 32)};
+}
+__device__ __forceinline__ void store_hit(const HitSlots &h, size_t i, const Hit &hit, int geo) {
+    h.a[i] = make_float4(hit.normal.x, hit.normal.y, hit.normal.z, hit.distance);
+    h.b[i] = make_float4(hit.uv.u, hit.uv.v, hit.uv_coverage, hit.NdotRd);
+    h.g[i] = geo < 0 ? ~0u : ((uint32_t)geo | (hit.from_behind ? 0x80000000u : 0u));
+}
+__device__ __forceinline__ int load_hit(const HitSlots &h, size_t i, Hit &hit) {
+    const float4 a = h.a[i], b = h.b[i];
+    const uint32_t g = h.g[i];
+    hit.normal = v3{a.x, a.y, a.z};
+    hit.distance = a.w;
+    hit.uv = v2{b.x, b.y};
+    hit.uv_coverage = b.z;
+    hit.NdotRd = b.w;
+    hit.from_behind = g != ~0u && (g >> 31) != 0;
+    hit.id = 0;
+    hit.position = vs(0.0f);   // prepare_for_shading derives it from the ray and the distance
+    return g == ~0u ? -1 : (int)(g & 0x7FFFFFFFu);
 }
 
 __device__ __forceinline__ void finish_sample(const RenderParams &rp, color col, float4 *pixels, uint32_t o) {
@@ -128,14 +168,15 @@ __device__ __forceinline__ void count_end(uint32_t *stats, bool spawned) {
 
 // Iteration 1 for every sample, tiles as in the render kernel.
 // Tiles [tile_base, tile_base + tiles) of the frame (one band, or all of it).
-template <bool WIDE>
+// With SPLIT the closest hits come from wf_trace_kernel's records instead of a trace here.
+template <bool WIDE, bool SPLIT>
 __global__ __launch_bounds__(kBlock) void wf_primary_kernel(SceneView s, RenderParams rp, float4 *__restrict__ pixels,
                                                             float *__restrict__ depths, float4 *__restrict__ accum,
                                                             PathState *__restrict__ queue, uint32_t *__restrict__ counters,
                                                             uint32_t cap, uint32_t scene_depth, uint32_t mesh_depth,
                                                             uint32_t tiles_x, uint32_t tile_base, uint32_t tiles,
                                                             uint32_t tiles_per_xcd, uint32_t *__restrict__ status,
-                                                            uint32_t *__restrict__ stats) {
+                                                            uint32_t *__restrict__ stats, HitSlots hits) {
     extern __shared__ uint32_t lds[];
     const uint32_t xcd = blockIdx.x & 7u, slot = blockIdx.x >> 3;
     const uint32_t local = xcd * tiles_per_xcd + slot;
@@ -161,7 +202,15 @@ __global__ __launch_bounds__(kBlock) void wf_primary_kernel(SceneView s, RenderP
         ray_reset(ray, v3{p.camera_position[0], p.camera_position[1], p.camera_position[2]}, normalized(dir));
         if (depth_left) {
             count_iteration(stats, rp, depth_left);
-            cont = beauty_iteration(s, rp, ray, scaling, depth_left, tp, col, depth, tr);
+            if constexpr (SPLIT) {
+                Hit hit;
+                const int geo = load_hit(hits, (size_t)tile * kBlock + t, hit);
+                hit.scaling_factor = scaling;
+                ray_reset(ray, scale_add(ray.direction, kTraceOffset, ray.origin), ray.direction);   // as scene_trace
+                cont = shade_iteration(s, rp, ray, hit, geo, scaling, depth_left, tp, col, depth, tr);
+            } else {
+                cont = beauty_iteration(s, rp, ray, scaling, depth_left, tp, col, depth, tr);
+            }
         }
         count_end(stats, cont);
         o = canvas_offset(rp, x, y);
@@ -172,16 +221,48 @@ __global__ __launch_bounds__(kBlock) void wf_primary_kernel(SceneView s, RenderP
     push_path(queue, counters + WF_COUNT_A, cap, inside && cont, ray, tp, scaling, depth_left, o, col, accum, status);
 }
 
+// The closest hit of every sample's camera ray, for the split primary pass: traversal only, so the kernel holds
+// fewer registers than the shading kernel and keeps more waves in flight to cover the walk's dependent fetches.
+template <bool WIDE>
+__global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(4)))
+void wf_trace_kernel(SceneView s, RenderParams rp, HitSlots hits, uint32_t scene_depth, uint32_t mesh_depth,
+                     uint32_t tiles_x, uint32_t tile_base, uint32_t tiles, uint32_t tiles_per_xcd,
+                     uint32_t *__restrict__ status) {
+    extern __shared__ uint32_t lds[];
+    const uint32_t xcd = blockIdx.x & 7u, slot = blockIdx.x >> 3;
+    const uint32_t local = xcd * tiles_per_xcd + slot;
+    if (local >= tiles) return;
+    const uint32_t tile = tile_base + local;
+    const uint32_t t = threadIdx.x, wave = t >> 6, lane = t & 63u;
+    const uint32_t x = (tile % tiles_x) * kTile + (wave & 1u) * 8u + (lane & 7u);
+    const uint32_t y = (tile / tiles_x) * kTile + (wave >> 1) * 8u + (lane >> 3);
+    const uint32_t ws = rp.width << rp.ssaa, hs = rp.height << rp.ssaa;
+    TracerT<WIDE> tr;
+    bind_lds_stacks(tr, lds, t, scene_depth, mesh_depth);
+    tr.cnt = nullptr;
+    if (x < ws && y < hs && rp.max_depth) {
+        v3 dir;
+        float scaling;
+        primary_ray(rp.proj, x, y, dir, scaling);
+        const Projection &p = rp.proj;
+        Ray ray;
+        ray_reset(ray, v3{p.camera_position[0], p.camera_position[1], p.camera_position[2]}, normalized(dir));
+        Hit hit;
+        hit.scaling_factor = scaling;
+        const int geo = scene_trace(s, ray, hit, false, INFINITY, tr);
+        store_hit(hits, (size_t)tile * kBlock + t, hit, geo);
+        if (tr.overflow) atomicOr(status, 1u);
+    }
+}
+
 // Further iterations over the queue `in` (count at in_count): with TO_END each path runs to its end; otherwise one
 // iteration, appending continuing paths to `out`.
 template <bool TO_END, bool WIDE>
 __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(TO_END && !WIDE ? 3 : 1)))
-void wf_bounce_kernel(SceneView s, RenderParams rp, float4 *__restrict__ pixels,
-                                                           float4 *__restrict__ accum, const PathState *__restrict__ in,
-                                                           const uint32_t *__restrict__ in_count, PathState *__restrict__ out,
-                                                           uint32_t *__restrict__ out_count, uint32_t cap,
-                                                           uint32_t scene_depth, uint32_t mesh_depth,
-                                                           uint32_t *__restrict__ status, uint32_t *__restrict__ stats) {
+void wf_bounce_kernel(SceneView s, RenderParams rp, float4 *__restrict__ pixels, float4 *__restrict__ accum,
+                      const PathState *__restrict__ in, const uint32_t *__restrict__ in_count,
+                      PathState *__restrict__ out, uint32_t *__restrict__ out_count, uint32_t cap, uint32_t scene_depth, uint32_t mesh_depth, uint32_t *__restrict__ status,
+                      uint32_t *__restrict__ stats) {
     extern __shared__ uint32_t lds[];
     const uint32_t t = threadIdx.x;
     TracerT<WIDE> tr;
@@ -305,14 +386,14 @@ int enqueue_bounces(slimrt_context *c, hipStream_t st, const SceneView &v, const
         if (sort) {   // A -> sorted B (A emptied), then B -> A
             sort_a_into_b();
             hipLaunchKernelGGL((wf_bounce_kernel<false, WIDE>), dim3(blocks), dim3(kBlock), lds, st, v, rp, pixels, accum,
-                               (const PathState *)qb, (const uint32_t *)(cnt + WF_COUNT_B), qa, cnt + WF_COUNT_A, cap,
-                               scene_depth, mesh_depth, c->d_status, stats);
+                               (const PathState *)qb, (const uint32_t *)(cnt + WF_COUNT_B), qa, cnt + WF_COUNT_A,
+                               cap, scene_depth, mesh_depth, c->d_status, stats);
         } else {
             PathState *in = cur_count == WF_COUNT_A ? qa : qb, *out = cur_count == WF_COUNT_A ? qb : qa;
             hipLaunchKernelGGL(wf_clear_count_kernel, dim3(1), dim3(64), 0, st, cnt, next_count);
             hipLaunchKernelGGL((wf_bounce_kernel<false, WIDE>), dim3(blocks), dim3(kBlock), lds, st, v, rp, pixels, accum,
-                               (const PathState *)in, (const uint32_t *)(cnt + cur_count), out, cnt + next_count, cap,
-                               scene_depth, mesh_depth, c->d_status, stats);
+                               (const PathState *)in, (const uint32_t *)(cnt + cur_count), out, cnt + next_count,
+                               cap, scene_depth, mesh_depth, c->d_status, stats);
             std::swap(cur_count, next_count);
         }
         HIP_TRY(c, hipGetLastError());
@@ -334,8 +415,16 @@ int launch_wavefront(slimrt_context *c, const RenderParams &rp, float4 *pixels, 
         c->wf_cap = 0;
         HIP_TRY(c, hipMalloc(&c->wf_queue_a, (size_t)slots * sizeof(PathState)));
         HIP_TRY(c, hipMalloc(&c->wf_queue_b, (size_t)slots * sizeof(PathState)));
-        HIP_TRY(c, hipMalloc(&c->wf_accum, (size_t)ws * hs * sizeof(float4)));
+        HIP_TRY(c, hipMalloc(&c->wf_accum, (size_t)slots * sizeof(float4)));   // canvas offsets < ws * hs <= slots
         c->wf_cap = slots;
+    }
+    const bool split = (rp.flags & SLIMRT_WAVEFRONT_SPLIT) != 0;
+    if (split && c->wf_hits_cap < slots) {
+        if (c->wf_hits) HIP_TRY(c, hipFree(c->wf_hits));
+        c->wf_hits = nullptr;
+        c->wf_hits_cap = 0;
+        HIP_TRY(c, hipMalloc(&c->wf_hits, (size_t)slots * kHitSlotBytes));
+        c->wf_hits_cap = slots;
     }
     if (!c->wf_counters) HIP_TRY(c, hipMalloc(&c->wf_counters, 8 * WF_COUNTERS * sizeof(uint32_t)));
     const uint32_t scene_depth = c->scene_height + 1, mesh_depth = mesh_stack_depth(c, wide_mesh_bvh(rp));
@@ -346,6 +435,7 @@ int launch_wavefront(slimrt_context *c, const RenderParams &rp, float4 *pixels, 
     HIP_TRY(c, hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, c->device));
     const SceneView v = device_view(c);
     PathState *qa = (PathState *)c->wf_queue_a, *qb = (PathState *)c->wf_queue_b;
+    const HitSlots hits = split ? hit_slots(c->wf_hits, c->wf_hits_cap) : HitSlots{nullptr, nullptr, nullptr};
     float4 *accum = (float4 *)c->wf_accum;
     uint32_t *cnt = c->wf_counters;
     const uint32_t blocks = (uint32_t)cus * 2;
@@ -372,14 +462,22 @@ int launch_wavefront(slimrt_context *c, const RenderParams &rp, float4 *pixels, 
         const uint32_t tile_base = row0 * tiles_x, tiles = (row1 - row0) * tiles_x, per_xcd = (tiles + 7) / 8;
         const uint32_t base = tile_base * kBlock, cap = tiles * kBlock;
         uint32_t *bc = cnt + b * WF_COUNTERS;
-        if (wide_mesh_bvh(rp))
-            hipLaunchKernelGGL(wf_primary_kernel<true>, dim3(per_xcd * 8), dim3(kBlock), lds, c->stream, v, rp, pixels,
-                               depths, accum, qa + base, bc, cap, scene_depth, mesh_depth, tiles_x, tile_base, tiles,
-                               per_xcd, c->d_status, stats);
-        else
-            hipLaunchKernelGGL(wf_primary_kernel<false>, dim3(per_xcd * 8), dim3(kBlock), lds, c->stream, v, rp, pixels,
-                               depths, accum, qa + base, bc, cap, scene_depth, mesh_depth, tiles_x, tile_base, tiles,
-                               per_xcd, c->d_status, stats);
+#define WF_PRIMARY(WIDE, SPLIT)                                                                                 \
+    hipLaunchKernelGGL((wf_primary_kernel<WIDE, SPLIT>), dim3(per_xcd * 8), dim3(kBlock), lds, c->stream, v, rp,    \
+                       pixels, depths, accum, qa + base, bc, cap, scene_depth, mesh_depth, tiles_x, tile_base, tiles, \
+                       per_xcd, c->d_status, stats, hits)
+#define WF_TRACE(WIDE)                                                                                          \
+    hipLaunchKernelGGL((wf_trace_kernel<WIDE>), dim3(per_xcd * 8), dim3(kBlock), lds, c->stream, v, rp, hits,        \
+                       scene_depth, mesh_depth, tiles_x, tile_base, tiles, per_xcd, c->d_status)
+        if (split) {
+            if (wide_mesh_bvh(rp)) WF_TRACE(true); else WF_TRACE(false);
+            HIP_TRY(c, hipGetLastError());
+            if (wide_mesh_bvh(rp)) WF_PRIMARY(true, true); else WF_PRIMARY(false, true);
+        } else {
+            if (wide_mesh_bvh(rp)) WF_PRIMARY(true, false); else WF_PRIMARY(false, false);
+        }
+#undef WF_PRIMARY
+#undef WF_TRACE
         HIP_TRY(c, hipGetLastError());
         if (bands > 1) {
             HIP_TRY(c, hipEventRecord(c->wf_band[b], c->stream));

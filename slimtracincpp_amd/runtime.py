@@ -76,7 +76,7 @@ class ViewsConfig(C.Structure):
 
 
 SCENE_REFIT, SCENE_REBUILD = 0, 1
-WAVEFRONT, WAVEFRONT_PER_BOUNCE, WAVEFRONT_SORT, BVH4, WAVEFRONT_COUNT = 2, 4, 8, 64, 128   # slimrt_render_params.flags
+WAVEFRONT, WAVEFRONT_PER_BOUNCE, WAVEFRONT_SORT, BVH4, WAVEFRONT_COUNT, WAVEFRONT_SPLIT = 2, 4, 8, 64, 128, 256   # flags
 
 
 def WAVEFRONT_BANDS(log2n: int) -> int:

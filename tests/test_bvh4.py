@@ -33,7 +33,7 @@ CASES = {
 
 
 @pytest.mark.parametrize("name", sorted(CASES))
-@pytest.mark.parametrize("form", [0, 2, 2 | 4 | 8])
+@pytest.mark.parametrize("form", [0, 2, 2 | 4 | 8, 2 | 256])
 def test_bvh4_matches_pair_walk(gpu_available, name, form):
     spec = CASES[name]()
     r = runtime.Renderer(spec, "gpu")

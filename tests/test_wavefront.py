@@ -1,6 +1,7 @@
 """Bounces as a wavefront (render flag SLIMRT_WAVEFRONT): compacted ray queues run to the paths' ends in one launch
 or re-compacted after every bounce, sorted by direction octant or not, in 1, 2, 4 or 8 bands whose bounces
-overlap the next band on a second stream, must give every sample bit for bit what
+overlap the next band on a second stream, with the first hits traced by the shading kernel or by a separate
+traversal kernel (SLIMRT_WAVEFRONT_SPLIT), must give every sample bit for bit what
 the per-sample megakernel gives (same iteration code, same accumulation order per sample), on scenes with point
 lights and glow, transparency, glass and mirrors, SSAA and both output kinds; and a frame with many bounces
 (depth 8) equals the oracle within the beauty tolerance.  With SLIMRT_WAVEFRONT_COUNT the queues' path counters
@@ -13,7 +14,7 @@ import oracle
 
 pytestmark = pytest.mark.gpu
 
-WAVEFRONT, PER_BOUNCE, SORT = 2, 4, 8
+WAVEFRONT, PER_BOUNCE, SORT, SPLIT = 2, 4, 8, 256
 
 
 def _frame(r, p):
@@ -39,7 +40,8 @@ CASES = {
 @pytest.mark.parametrize("name", sorted(CASES))
 @pytest.mark.parametrize("flags", [WAVEFRONT, WAVEFRONT | SORT, WAVEFRONT | PER_BOUNCE, WAVEFRONT | PER_BOUNCE | SORT,
                                    WAVEFRONT | (1 << 4), WAVEFRONT | PER_BOUNCE | SORT | (2 << 4),
-                                   WAVEFRONT | PER_BOUNCE | (3 << 4)])
+                                   WAVEFRONT | PER_BOUNCE | (3 << 4), WAVEFRONT | SPLIT,
+                                   WAVEFRONT | SORT | SPLIT | (2 << 4), WAVEFRONT | PER_BOUNCE | SPLIT])
 @pytest.mark.parametrize("output", [runtime.OUTPUT_CANVAS, runtime.OUTPUT_RADIANCE])
 def test_wavefront_equals_megakernel(gpu_available, name, flags, output):
     spec = CASES[name]()
@@ -82,8 +84,8 @@ def test_render_autotune_picks_a_form_with_the_same_frame(gpu_available):
     px = torch.zeros((n, 4), dtype=torch.float32, device="cuda")
     dp = torch.zeros(n, dtype=torch.float32, device="cuda")
     flags, ms = r.autotune_form(p, px.data_ptr(), dp.data_ptr(), reps=2)
-    assert flags & ~(WAVEFRONT | PER_BOUNCE | SORT | (3 << 4) | runtime.BVH4) == 0 and ms > 0
-    assert flags & (PER_BOUNCE | SORT | (3 << 4)) == 0 or flags & WAVEFRONT
+    assert flags & ~(WAVEFRONT | PER_BOUNCE | SORT | (3 << 4) | SPLIT | runtime.BVH4) == 0 and ms > 0
+    assert flags & (PER_BOUNCE | SORT | (3 << 4) | SPLIT) == 0 or flags & WAVEFRONT
     p.flags = flags & runtime.BVH4      # the megakernel with the same node width: the wavefront gives its bits
     ref = _frame(r, p)
     p.flags = flags
@@ -92,7 +94,7 @@ def test_render_autotune_picks_a_form_with_the_same_frame(gpu_available):
 
 
 COUNT_FORMS = [WAVEFRONT, WAVEFRONT | SORT, WAVEFRONT | PER_BOUNCE, WAVEFRONT | PER_BOUNCE | SORT,
-               WAVEFRONT | (2 << 4), WAVEFRONT | PER_BOUNCE | (1 << 4)]
+               WAVEFRONT | (2 << 4), WAVEFRONT | PER_BOUNCE | (1 << 4), WAVEFRONT | SPLIT | (1 << 4)]
 
 
 @pytest.mark.parametrize("name", ["implicit_transparent_d5", "mesh_pbr_d8"])

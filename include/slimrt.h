@@ -247,12 +247,38 @@ int slimrt_host_collapse_bvh4(const slimrt_bvh_node *nodes, uint32_t node_count,
                               slimrt_bvh4_node *nodes_out, uint32_t node_cap, uint32_t *node_count_out,
                               uint32_t *height_out, uint32_t *stack_out);
 
+/* Picking: Selection::manipulate's ray cast (src/slim/scene/selection.h:36-85) on the uploaded scene.  The camera
+ * ray through integer pixel (x, y) of the params' projection (CameraRayProjection::getRayDirectionAt, camera.h:67,
+ * normalised) is traced for the closest hit (SceneTracer::trace), then tested against every point light's pick
+ * sphere (radius intensity / 2 / 64; the last light hit wins, as in the reference).  geometry / light / triangle:
+ * -1 when none (triangle: leaf-ordered, meshes only); distance from the traced ray's origin (INFINITY when no
+ * geometry is hit); position: that hit in world space (0 when none).  Synchronous. */
+typedef struct slimrt_pick_result {
+    int32_t geometry, light, triangle;
+    float distance;
+    float position[3];
+} slimrt_pick_result;
+int slimrt_pick(slimrt_context *ctx, const slimrt_render_params *params, int32_t x, int32_t y, slimrt_pick_result *out);
+
 /* The path counters of the last frame rendered with SLIMRT_WAVEFRONT | SLIMRT_WAVEFRONT_COUNT (no counterpart in
  * the reference): counts[i] for i < 32 = paths that ran iteration i of the integrator loop (0 = the camera ray;
  * src/slim/renderer/ray_tracer.h:42-105), counts[32] = bounce rays spawned into a queue, counts[33] = samples
  * finished.  Every iteration either spawns or finishes, so sum(counts[0..31]) = counts[32] + counts[33].
  * n >= 34.  Synchronizes the context's stream. */
 int slimrt_wavefront_counts(slimrt_context *ctx, uint32_t *counts, uint32_t n);
+
+/* The reference's texture converter (src/bmp2texture.cpp:24-361): the texel quads of an RGB image as it writes them
+ * into .texture files.  rgb: width * height * 3 bytes, rows top to bottom.  flags: ImageFlags bits
+ * (src/slim/core/base.h:1264-1277): linear 1 << 1 (no gamma 2.2), mipmap 1 << 4 (2 x 2 box-filtered mips down to
+ * 4 texels), wrap 1 << 6 (border quads from the opposite edge, else clamped), normal 1 << 7 (renormalised normal
+ * map), cubemap 1 << 8 (the image is a 4-face strip, a top and a bottom face, width = 6 * height; three mips with
+ * seams stitched across faces).  Every mip's (w + 1) * (h + 1) quads go to quads_out in order (12 bytes each,
+ * the layout of slimrt_texture_mip); *quad_count_out receives their total, *mip_count_out the mip count (at most
+ * 16), mip_dims_out (2 per mip) their widths and heights, *flags_out the flags the converter stores.  With
+ * quads_out NULL only the counts are returned; SLIMRT_E_OVERFLOW when quad_cap is smaller than the total. */
+int slimrt_host_bake_texture(const uint8_t *rgb, uint32_t width, uint32_t height, uint32_t flags, uint8_t *quads_out,
+                             size_t quad_cap, size_t *quad_count_out, uint32_t *mip_count_out, uint32_t *mip_dims_out,
+                             uint32_t *flags_out);
 
 /* BVHBuilder::buildMesh (src/slim/scene/bvh_builder.h:265-341): mesh BVH and leaf-ordered triangle records.
  * normals/uvs may be NULL (counts 0).  nodes_out: 2T - 1 nodes; triangles_out: T records; aabb_out: 6 floats

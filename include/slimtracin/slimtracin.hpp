@@ -231,6 +231,7 @@ typedef slimrt_bvh_node BVHNode;
 struct BVH { BVHNode *nodes = nullptr; u32 node_count = 0; u32 height = 0; };
 typedef slimrt_triangle Triangle;
 struct TriangleVertexIndices { u32 v1, v2, v3; };
+struct EdgeVertexIndices { u32 from, to; };   // src/slim/scene/mesh.h:8-10
 
 struct Mesh {
     AABB aabb;
@@ -242,6 +243,7 @@ struct Mesh {
     TriangleVertexIndices *vertex_position_indices = nullptr;
     TriangleVertexIndices *vertex_normal_indices = nullptr;
     TriangleVertexIndices *vertex_uvs_indices = nullptr;
+    EdgeVertexIndices *edge_vertex_indices = nullptr;
     u32 triangle_count = 0, vertex_count = 0, edge_count = 0, normals_count = 0, uvs_count = 0;
 };
 
@@ -299,7 +301,8 @@ inline bool load(Mesh &m, const char *path) {
     ok = ok && fread(m.triangles, sizeof(Triangle), m.triangle_count, f) == m.triangle_count;
     ok = ok && fread(m.vertex_positions, 12, m.vertex_count, f) == m.vertex_count;
     ok = ok && fread(m.vertex_position_indices, 12, m.triangle_count, f) == m.triangle_count;
-    ok = ok && fseek(f, 8L * m.edge_count, SEEK_CUR) == 0;
+    m.edge_vertex_indices = alloc<EdgeVertexIndices>(m.edge_count);
+    ok = ok && fread(m.edge_vertex_indices, 8, m.edge_count, f) == m.edge_count;
     if (m.uvs_count) {
         m.vertex_uvs = alloc<vec2>(m.uvs_count);
         m.vertex_uvs_indices = alloc<TriangleVertexIndices>(m.triangle_count);
@@ -337,6 +340,93 @@ inline bool load(Texture &t, const char *path) {
     }
     fclose(f);
     return ok;
+}
+
+// Writers in the same layout (Mesh / Texture / BVH save, src/slim/serialization/mesh.h:56-65 and 104-122,
+// texture.h:61-68 with ImageInfo's header, base.h:1544-1546 and 1588-1595, bvh.h:18-48).
+inline bool save(const Mesh &m, const char *path) {
+    FILE *f = fopen(path, "wb");
+    if (!f) return false;
+    const u32 edges = m.edge_vertex_indices ? m.edge_count : 0;
+    const u32 h[7] = {m.vertex_count, m.triangle_count, edges, m.uvs_count, m.normals_count, m.bvh.node_count, m.bvh.height};
+    bool ok = fwrite(h, 4, 7, f) == 7;
+    const float ab[6] = {m.aabb.min.x, m.aabb.min.y, m.aabb.min.z, m.aabb.max.x, m.aabb.max.y, m.aabb.max.z};
+    ok = ok && fwrite(ab, 4, 6, f) == 6;
+    ok = ok && fwrite(m.triangles, sizeof(Triangle), m.triangle_count, f) == m.triangle_count;
+    ok = ok && fwrite(m.vertex_positions, 12, m.vertex_count, f) == m.vertex_count;
+    ok = ok && fwrite(m.vertex_position_indices, 12, m.triangle_count, f) == m.triangle_count;
+    ok = ok && fwrite(m.edge_vertex_indices, 8, edges, f) == edges;
+    if (m.uvs_count) {
+        ok = ok && fwrite(m.vertex_uvs, 8, m.uvs_count, f) == m.uvs_count;
+        ok = ok && fwrite(m.vertex_uvs_indices, 12, m.triangle_count, f) == m.triangle_count;
+    }
+    if (m.normals_count) {
+        ok = ok && fwrite(m.vertex_normals, 12, m.normals_count, f) == m.normals_count;
+        ok = ok && fwrite(m.vertex_normal_indices, 12, m.triangle_count, f) == m.triangle_count;
+    }
+    ok = ok && fwrite(m.bvh.nodes, sizeof(BVHNode), m.bvh.node_count, f) == m.bvh.node_count;
+    return fclose(f) == 0 && ok;
+}
+inline bool save(const Texture &t, const char *path) {
+    FILE *f = fopen(path, "wb");
+    if (!f) return false;
+    const u32 info[8] = {t.width, t.height, t.width * t.height, t.width, 0, 0, t.mip_count, t.flags};
+    bool ok = fwrite(info, 4, 8, f) == 8;
+    for (u32 i = 0; ok && i < t.mip_count; i++) {
+        const u32 wh[2] = {t.mips[i].width, t.mips[i].height};
+        const size_t n = (size_t)(wh[0] + 1) * (wh[1] + 1);
+        ok = fwrite(wh, 4, 2, f) == 2 && fwrite(t.mips[i].texel_quads, 12, n, f) == n;
+    }
+    return fclose(f) == 0 && ok;
+}
+inline bool load(BVH &b, const char *path) {
+    using namespace slimtracin_detail;
+    FILE *f = fopen(path, "rb");
+    if (!f) return false;
+    u32 h[2];
+    bool ok = fread(h, 4, 2, f) == 2;
+    b = BVH{};
+    if (ok) {
+        b.node_count = h[0];
+        b.height = h[1];
+        b.nodes = alloc<BVHNode>(b.node_count);
+        ok = fread(b.nodes, sizeof(BVHNode), b.node_count, f) == b.node_count;
+    }
+    fclose(f);
+    return ok;
+}
+inline bool save(const BVH &b, const char *path) {
+    FILE *f = fopen(path, "wb");
+    if (!f) return false;
+    const u32 h[2] = {b.node_count, b.height};
+    bool ok = fwrite(h, 4, 2, f) == 2 && fwrite(b.nodes, sizeof(BVHNode), b.node_count, f) == b.node_count;
+    return fclose(f) == 0 && ok;
+}
+
+// The reference converter (src/bmp2texture.cpp) in memory: a Texture from an RGB image (width * height * 3 bytes,
+// rows top to bottom) and ImageFlags bits, through slimrt_host_bake_texture.
+inline Texture bakeTexture(const u8 *rgb, u32 width, u32 height, u32 flags) {
+    using namespace slimtracin_detail;
+    size_t total = 0;
+    u32 mip_count = 0, dims[32], out_flags = 0;
+    check(slimrt_host_bake_texture(rgb, width, height, flags, nullptr, 0, &total, &mip_count, dims, &out_flags),
+          nullptr, "bakeTexture");
+    u8 *quads = alloc<u8>(total * 12);
+    check(slimrt_host_bake_texture(rgb, width, height, flags, quads, total, &total, &mip_count, dims, &out_flags),
+          nullptr, "bakeTexture");
+    Texture t;
+    t.width = width;
+    t.height = height;
+    t.flags = out_flags;
+    t.mip_count = mip_count;
+    t.mips = alloc<TextureMip>(mip_count);
+    for (u32 i = 0; i < mip_count; i++) {
+        t.mips[i].width = dims[2 * i];
+        t.mips[i].height = dims[2 * i + 1];
+        t.mips[i].texel_quads = reinterpret_cast<TexelQuad *>(quads);
+        quads += 12 * (size_t)(dims[2 * i] + 1) * (dims[2 * i + 1] + 1);
+    }
+    return t;
 }
 
 struct Camera {

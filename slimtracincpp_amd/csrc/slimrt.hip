@@ -128,6 +128,24 @@ __global__ void nrc_composite_kernel(RenderParams rp, const uint32_t *__restrict
 }
 
 // Clears the frame counters and step losses (a kernel rather than a memset, so the frame can be captured).
+// One pick on the device: thread 0 traces, with the render kernels' LDS stacks.
+__global__ __launch_bounds__(kBlock) void pick_kernel(SceneView s, RenderParams rp, int32_t x, int32_t y,
+                                                      uint32_t scene_depth, uint32_t mesh_depth,
+                                                      int32_t *__restrict__ out, uint32_t *__restrict__ status) {
+    extern __shared__ uint32_t lds[];
+    const uint32_t t = threadIdx.x;
+    if (t != 0) return;
+    TracerT<false> tr;
+    bind_lds_stacks(tr, lds, t, scene_depth, mesh_depth);
+    tr.cnt = nullptr;
+    int32_t oi[3];
+    float of[4];
+    pick(s, rp, x, y, tr, oi, of);
+    for (int k = 0; k < 3; k++) out[k] = oi[k];
+    for (int k = 0; k < 4; k++) out[3 + k] = __builtin_bit_cast(int32_t, of[k]);
+    if (tr.overflow) atomicOr(status, 1u);
+}
+
 __global__ void nrc_reset_kernel(uint32_t *__restrict__ counters, float *__restrict__ losses, uint32_t nloss) {
     const uint32_t i = threadIdx.x;
     if (i < NC_COUNT) counters[i] = 0u;
@@ -437,7 +455,8 @@ int slimrt_destroy(slimrt_context *c) {
         dev_free(c->d_mips); dev_free(c->d_quads); dev_free(c->d_pixels); dev_free(c->d_depths);
         free_nrc(c);
         dev_free(c->d_geo_stage);
-        for (void *q : {c->wf_queue_a, c->wf_queue_b, c->wf_accum, c->wf_hits, (void *)c->wf_counters, (void *)c->wf_stats})
+        for (void *q : {c->wf_queue_a, c->wf_queue_b, c->wf_accum, c->wf_hits, (void *)c->wf_counters, (void *)c->wf_stats,
+                        (void *)c->d_pick})
             if (q) (void)hipFree(q);
         for (hipEvent_t e : c->wf_band)
             if (e) (void)hipEventDestroy(e);
@@ -675,6 +694,48 @@ int slimrt_host_collapse_bvh4(const slimrt_bvh_node *nodes, uint32_t node_count,
     *node_count_out = (uint32_t)out.size();
     *height_out = h4;
     *stack_out = stack4;
+    return SLIMRT_OK;
+}
+
+int slimrt_pick(slimrt_context *c, const slimrt_render_params *p, int32_t x, int32_t y, slimrt_pick_result *out) {
+    if (!c) return SLIMRT_E_ARG;
+    int rc = validate_params(c, p);
+    if (rc) return rc;
+    if (!out) return fail(c, SLIMRT_E_ARG, "out is NULL");
+    RenderParams rp;
+    std::memcpy(&rp, p, sizeof rp);
+    rp.ssaa = rp.ssaa ? 1u : 0u;
+    int32_t res[7];
+    if (c->cpu) {
+        const SceneView v = host_view(c);
+        std::vector<uint32_t> stack(c->scene_height + 1 + c->mesh_height + 1);
+        Tracer tr;
+        tr.scene_stack = Stack{stack.data(), 1};
+        tr.mesh_stack = Stack{stack.data() + c->scene_height + 1, 1};
+        tr.scene_depth = c->scene_height + 1;
+        tr.mesh_depth = c->mesh_height + 1;
+        tr.overflow = 0;
+        tr.cnt = nullptr;
+        float of[4];
+        pick(v, rp, x, y, tr, res, of);
+        std::memcpy(res + 3, of, sizeof of);
+        if (tr.overflow) return fail(c, SLIMRT_E_OVERFLOW, "traversal stack overflow");
+    } else {
+        if (!c->d_pick) HIP_TRY(c, hipMalloc(&c->d_pick, 8 * sizeof(int32_t)));
+        const uint32_t scene_depth = c->scene_height + 1, mesh_depth = mesh_stack_depth(c, false);
+        const size_t lds = stack_lds_bytes(scene_depth, mesh_depth, false);
+        if (lds > 160 * 1024) return fail(c, SLIMRT_E_ARG, "BVH too deep for the LDS traversal stacks");
+        hipLaunchKernelGGL(pick_kernel, dim3(1), dim3(kBlock), lds, c->stream, device_view(c), rp, x, y, scene_depth,
+                           mesh_depth, c->d_pick, c->d_status);
+        HIP_TRY(c, hipGetLastError());
+        HIP_TRY(c, hipMemcpyAsync(res, c->d_pick, sizeof res, hipMemcpyDeviceToHost, c->stream));
+        if ((rc = check_status(c))) return rc;   // synchronizes the stream
+    }
+    out->geometry = res[0];
+    out->light = res[1];
+    out->triangle = res[2];
+    std::memcpy(&out->distance, res + 3, sizeof(float));
+    std::memcpy(out->position, res + 4, 3 * sizeof(float));
     return SLIMRT_OK;
 }
 

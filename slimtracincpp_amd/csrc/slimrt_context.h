@@ -72,6 +72,7 @@ struct slimrt_context {
     uint32_t wf_hits_cap = 0;
     uint32_t *wf_counters = nullptr, wf_cap = 0;
     uint32_t *wf_stats = nullptr;   // kWfStats path counters (SLIMRT_WAVEFRONT_COUNT)
+    int32_t *d_pick = nullptr;      // slimrt_pick's result
     hipStream_t side_stream = nullptr;          // a wavefront frame's bounce launches, overlapping the next band
     hipEvent_t wf_fork = nullptr, wf_join = nullptr, wf_band[8] = {};
     uint32_t *d_status = nullptr;

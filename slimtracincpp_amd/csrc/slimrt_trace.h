@@ -705,6 +705,49 @@ SLIM_HD void render_sample(const SceneView &s, const RenderParams &rp, uint32_t 
     output_value(rp, c, out_px);
 }
 
+// Selection::manipulate's pick (src/slim/scene/selection.h:36-85): the camera ray through integer pixel (x, y),
+// direction CameraRayProjection::getRayDirectionAt (camera.h:67) normalised, traced for the closest hit; then
+// every point light's pick sphere (radius intensity / 2 / 64, SphereTracer::hit, ray.h:317-334) against the hit
+// distance, the last light hit winning.  out: geometry, light, triangle (meshes), distance, world position.
+template <class TR>
+SLIM_HD void pick(const SceneView &s, const RenderParams &rp, int32_t x, int32_t y, TR &tr, int32_t out_i[3],
+                  float out_f[4]) {
+    const Projection &p = rp.proj;
+    const v3 dir = (v3{p.start[0], p.start[1], p.start[2]} + v3{p.down[0], p.down[1], p.down[2]} * (float)y) +
+                   v3{p.right[0], p.right[1], p.right[2]} * (float)x;
+    Ray ray;
+    Hit hit;
+    ray_reset(ray, v3{p.camera_position[0], p.camera_position[1], p.camera_position[2]}, normalized(dir));
+    const int g = scene_trace(s, ray, hit, false, INFINITY, tr);
+    int light = -1;
+    float max_distance = hit.distance;
+    for (uint32_t i = 0; i < s.L; i++) {
+        const Light l = s.lights[i];
+        if (l.flags & LIGHT_DIRECTIONAL) continue;
+        const float half = l.intensity * 0.5f;
+        const float radius = half * kLightRadiusFactor, oor = kLightInvRadiusFactor / half;
+        const float t_max = max_distance * oor;
+        const v3 rc = (v3{l.x, l.y, l.z} - ray.origin) * oor;
+        const float b = dot(ray.direction, rc), c = sqlen(rc) - 1;
+        float h = b * b - c;
+        if (h >= 0) {
+            h = sqrtf(h);
+            if (b + h > 0 && b - h < t_max) {
+                max_distance = t_max * radius;
+                light = (int)i;
+            }
+        }
+    }
+    out_i[0] = g;
+    out_i[1] = light;
+    out_i[2] = (g >= 0 && s.geos[g].type == GT_Mesh) ? (int)hit.id : -1;
+    out_f[0] = g >= 0 ? hit.distance : INFINITY;
+    const v3 pos = g >= 0 ? ray_at(ray, hit.distance) : vs(0.0f);
+    out_f[1] = pos.x;
+    out_f[2] = pos.y;
+    out_f[3] = pos.z;
+}
+
 // Canvas offsets: samples are stored row-major, or as 2x2 quads per pixel under SSAA (canvas.h:147).
 SLIM_HD uint32_t canvas_offset(const RenderParams &rp, uint32_t x, uint32_t y) {
     return rp.ssaa ? ((rp.width * (y >> 1) + (x >> 1)) * 4 + (2 * (y & 1)) + (x & 1)) : (rp.width * y + x);

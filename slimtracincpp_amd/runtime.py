@@ -70,6 +70,10 @@ class RenderParams(C.Structure):
                 ("output", u32), ("flags", u32), ("mip_level_colors", (f32 * 3) * 9)]
 
 
+class PickResult(C.Structure):
+    _fields_ = [("geometry", i32), ("light", i32), ("triangle", i32), ("distance", f32), ("position", f32 * 3)]
+
+
 class ViewsConfig(C.Structure):
     _fields_ = [("tile_shape", u32), ("blocks_per_cu", u32), ("tiles_per_grab", u32), ("persistent", u32),
                 ("node_width", u32)]
@@ -121,6 +125,8 @@ def lib():
             "slimrt_render_autotune": [vp, P(RenderParams), vp, vp, u32, P(u32), P(f32)],
             "slimrt_host_collapse_bvh4": [vp, u32, u32, vp, u32, P(u32), P(u32), P(u32)],
             "slimrt_wavefront_counts": [vp, vp, u32],
+            "slimrt_pick": [vp, P(RenderParams), i32, i32, P(PickResult)],
+            "slimrt_host_bake_texture": [vp, u32, u32, u32, vp, C.c_size_t, P(C.c_size_t), P(u32), vp, P(u32)],
         }
         for name, args in sig.items():
             fn = getattr(L, name)
@@ -387,6 +393,14 @@ class Renderer:
         _check(lib().slimrt_count_work(self.ctx, C.byref(p), _ptr(c)), self.ctx)
         names = ("rays", "pairs", "roots", "instances", "triangles", "mesh_hits", "taps", "pixels")
         return dict(zip(names, (int(v) for v in c)))
+
+    def pick(self, x: int, y: int, params: RenderParams | None = None) -> dict:
+        """slimrt_pick: what Selection's click at pixel (x, y) finds: geometry, light, triangle, distance, position."""
+        p = params if params is not None else self.params()
+        res = PickResult()
+        _check(lib().slimrt_pick(self.ctx, C.byref(p), x, y, C.byref(res)), self.ctx)
+        return {"geometry": res.geometry, "light": res.light, "triangle": res.triangle,
+                "distance": res.distance, "position": np.array(res.position[:], np.float32)}
 
     def wavefront_counts(self) -> np.ndarray:
         """Path counters of the last frame rendered with WAVEFRONT | WAVEFRONT_COUNT: paths per integrator

@@ -383,9 +383,11 @@ def displaced_sphere_mesh(level: int = 6, amplitude: float = 0.15, seed: int = 0
 # --- textures in the reference's texel-quad layout ------------------------------------
 def bake_texel_quads(texels: np.ndarray, wrap: bool) -> np.ndarray:
     """float texels (h, w, 3) in [0, 1] -> (h+1, w+1, 12) u8 TexelQuads (R{TL,TR,BL,BR} G{..} B{..}).
-    Quad (x, y) holds texels (x-1, y-1), (x, y-1), (x-1, y), (x, y); out-of-range indices wrap or clamp, which is
-    what TextureMipLoader::load produces (src/bmp2texture.cpp:36-95).  Quantisation truncates x * 255
-    (src/bmp2texture.cpp:327-357)."""
+    Quad (x, y) holds texels (x-1, y-1), (x, y-1), (x-1, y), (x, y); out-of-range indices wrap (as a torus) or
+    clamp.  The converter (TextureMipLoader::load, src/bmp2texture.cpp:36-95; slimrt_host_bake_texture) gives the
+    same quads except, with wrap, the right border column's TR and BR corners, which its later right-edge writes
+    leave at the last column's texels.  Both renderers read the same quads, so this is a choice of input, not a
+    parity difference.  Quantisation truncates x * 255 (src/bmp2texture.cpp:327-357)."""
     h, w, _ = texels.shape
     xs = np.arange(-1, w + 1)
     ys = np.arange(-1, h + 1)

@@ -135,15 +135,50 @@ static int run_demo(const char *mesh_path, const char *out, bool gpu) {
     return write_canvas(out, pixels, depths);
 }
 
+// Asset files through the C++ API: load and save again (mesh, texture, bvh by the input's extension); for a
+// texture also the converter's re-bake of its own faces (<out>.baked), as the Python tests do with assets.py.
+static int run_assets(const char *in, const char *out) {
+    const std::string path(in);
+    auto ends = [&](const char *e) { return path.size() > strlen(e) && path.compare(path.size() - strlen(e), strlen(e), e) == 0; };
+    if (ends(".mesh")) {
+        Mesh m;
+        return load(m, in) && save(m, out) ? 0 : 3;
+    }
+    if (ends(".bvh")) {
+        BVH b;
+        return load(b, in) && save(b, out) ? 0 : 3;
+    }
+    Texture t;
+    if (!load(t, in) || !save(t, out)) return 3;
+    if (!(t.flags & (1u << 8))) return 0;
+    // the cube's input image: texel (x, y) of each mip is quad (x + 1, y + 1).TL, stored as byte - 1
+    const u32 n = t.height, W = 6 * n;
+    std::vector<u8> rgb((size_t)W * n * 3);
+    u32 x0 = 0;
+    for (u32 k = 0; k < 3; k++) {
+        const TextureMip &m = t.mips[k];
+        for (u32 y = 0; y < m.height; y++)
+            for (u32 x = 0; x < m.width; x++) {
+                const TexelQuad &q = m.texel_quads[(size_t)(y + 1) * (m.width + 1) + x + 1];
+                const u8 c[3] = {q.R[0], q.G[0], q.B[0]};
+                for (int ch = 0; ch < 3; ch++) rgb[((size_t)y * W + x0 + x) * 3 + ch] = c[ch] ? (u8)(c[ch] + 1) : 0;
+            }
+        x0 += m.width;
+    }
+    Texture baked = bakeTexture(rgb.data(), W, n, t.flags);
+    return save(baked, (std::string(out) + ".baked").c_str()) ? 0 : 3;
+}
+
 int main(int argc, char **argv) {
     if (argc < 5) {
-        fprintf(stderr, "usage: %s spec|demo <input> <out.bin> cpu|gpu\n", argv[0]);
+        fprintf(stderr, "usage: %s spec|demo|assets <input> <out> cpu|gpu\n", argv[0]);
         return 2;
     }
     const bool gpu = !strcmp(argv[4], "gpu");
     try {
         if (!strcmp(argv[1], "spec")) return run_spec(argv[2], argv[3], gpu);
         if (!strcmp(argv[1], "demo")) return run_demo(argv[2], argv[3], gpu);
+        if (!strcmp(argv[1], "assets")) return run_assets(argv[2], argv[3]);
     } catch (const std::exception &e) {
         fprintf(stderr, "app_api_test: %s\n", e.what());
         return 1;

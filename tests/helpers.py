@@ -52,3 +52,17 @@ def mesh_spec_from_asset(m: dict, leaf: int) -> scenes.SceneSpec:
                             camera_position=np.array([0, 0, -10], np.float32), focal_length=np.float32(2.0),
                             geo_f=g[0][None], geo_u=g[1][None], mat_f=f[None], mat_u=u[None],
                             light_f=np.zeros((0, 7), np.float32), light_u=np.zeros(0, np.uint32), meshes=[ms])
+
+
+def shipped_sky_scene(width: int = 96, height: int = 64, seed: int = 7) -> scenes.SceneSpec:
+    """A small mesh + PBR scene lit by the reference's shipped cube map (bolonga_irradiance.texture) as sky colour,
+    radiance and irradiance, so texture sampling of cube maps is exercised on data the reference's converter wrote."""
+    from slimtracincpp_amd import assets
+    t = assets.read_texture(os.path.join(GOLDEN, "bolonga_irradiance.texture"))
+    cube = scenes.TextureSpec(width=t["width"], height=t["height"], flags=t["flags"],
+                              mips=[q for _, _, q in t["mips"]], mip_dims=[(w, h) for w, h, _ in t["mips"]])
+    spec = scenes.mesh_pbr_scene(seed, width, height, level=3, texture_size=32, sky_size=8, irradiance_size=4)
+    spec.textures = list(spec.textures) + [cube]
+    k = len(spec.textures) - 1
+    spec.skybox_color = spec.skybox_radiance = spec.skybox_irradiance = k
+    return spec

@@ -91,6 +91,19 @@ def test_mesh_pbr_small(gpu_available, mesh_small, mode):
     assert ok, err
 
 
+def test_shipped_cube_map_sky(gpu_available):
+    """Cube-map sampling on the texture the reference's converter wrote (sky colour, radiance, irradiance)."""
+    from tests.helpers import shipped_sky_scene
+    spec = shipped_sky_scene()
+    r, o = runtime.Renderer(spec, "gpu"), oracle.OracleScene(spec)
+    px, depth = r.render()
+    col, dep, _ = o.render()
+    ok, err = _close(px[..., :3], col, atol=BEAUTY_ATOL)
+    assert ok, err
+    ok, err = _close(depth, dep, rtol=DEBUG_RTOL)
+    assert ok, err
+
+
 @pytest.fixture(scope="module")
 def mesh_1080p():
     spec = scenes.mesh_pbr_scene(0, 1920, 1080)

@@ -63,6 +63,17 @@ def test_oracle_matches_live_reference(make):
     assert np.array_equal(geo.reshape(-1), ref["hit_geo"]) and np.array_equal(tri.reshape(-1), ref["hit_tri"])
 
 
+@needs_ref
+def test_shipped_cube_map_against_live_reference():
+    """The shipped irradiance cube map as sky: oracle and the reference's renderer agree bit for bit."""
+    from tests.helpers import shipped_sky_scene
+    spec = shipped_sky_scene()
+    ref = oracle.run_reference(spec)
+    col, dep, _ = oracle.OracleScene(spec).render()
+    assert bits_equal(col.reshape(-1), ref["color"])
+    assert bits_equal(dep.reshape(-1), ref["depth"])
+
+
 def test_emissive_quads_contribute():
     """The large scene's emissive quads light it: without their emission the frame is darker somewhere."""
     spec = scenes.large_scene(0, 96, 54, uv_sphere=(9, 16), texture_size=16, sky_size=8, irradiance_size=4)

@@ -101,6 +101,13 @@ def lib():
         if not os.path.exists(LIB_PATH):
             raise ImportError(f"{LIB_PATH} is missing: build it with __graft_entry__.build() "
                               "or make -C slimtracincpp_amd/csrc")
+        # torch's ROCm wheel carries its own HIP runtime (soname libamdhip64.so.7, loaded by file name).  Loaded
+        # first, it is the one the library binds to as well; loaded after the library, the process would hold two
+        # HIP runtimes and the second to start sees no device.  So torch, when installed, is imported first.
+        try:
+            import torch  # noqa: F401
+        except ImportError:
+            pass
         L = C.CDLL(LIB_PATH)
         P = C.POINTER
         sig = {

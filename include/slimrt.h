@@ -247,6 +247,13 @@ int slimrt_host_collapse_bvh4(const slimrt_bvh_node *nodes, uint32_t node_count,
                               slimrt_bvh4_node *nodes_out, uint32_t node_cap, uint32_t *node_count_out,
                               uint32_t *height_out, uint32_t *stack_out);
 
+/* Presentation: Canvas::drawToWindow (src/slim/draw/canvas.h:119-126) for a canvas written with
+ * SLIMRT_OUTPUT_CANVAS: the window content, one 0xRRGGBB word per pixel (getPixelContent / Pixel::asContent,
+ * canvas.h:187-190, base.h:1166-1171: the mean of the 2x2 sample quad under SSAA, each channel 255 * sqrt(c)
+ * truncated).  GPU context: device pointers, asynchronous on the context's stream; CPU context: host pointers. */
+int slimrt_canvas_content(slimrt_context *ctx, const void *pixels, uint32_t width, uint32_t height, uint32_t ssaa,
+                          void *content);
+
 /* Picking: Selection::manipulate's ray cast (src/slim/scene/selection.h:36-85) on the uploaded scene.  The camera
  * ray through integer pixel (x, y) of the params' projection (CameraRayProjection::getRayDirectionAt, camera.h:67,
  * normalised) is traced for the closest hit (SceneTracer::trace), then tested against every point light's pick

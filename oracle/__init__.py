@@ -63,6 +63,8 @@ def lib():
         L.oracle_path_samples.argtypes = [vp, i, i, i, vp, vp, vp]
         L.oracle_iteration_counts.restype = i
         L.oracle_iteration_counts.argtypes = [vp, i, i, i, vp, i]
+        L.oracle_canvas_content.restype = i
+        L.oracle_canvas_content.argtypes = [vp, i, i, i, vp]
         _lib = L
     return _lib
 
@@ -161,6 +163,15 @@ class OracleScene:
         out = np.zeros(n, dtype)
         lib().oracle_get(self.h, name.encode(), _p(out), n)
         return out
+
+
+def canvas_content(pixels: np.ndarray, width: int, height: int, ssaa: bool) -> np.ndarray:
+    """Window content (0xRRGGBB per window pixel) of an RGBA canvas in the canvas layout."""
+    px = np.ascontiguousarray(pixels, np.float32).reshape(-1)
+    assert px.size == width * height * 4 * (4 if ssaa else 1)
+    out = np.zeros(width * height, np.uint32)
+    lib().oracle_canvas_content(_p(px), width, height, int(ssaa), _p(out))
+    return out
 
 
 def run_reference(spec, *extra: str, timeout: float = 600.0) -> dict:

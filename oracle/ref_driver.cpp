@@ -375,10 +375,30 @@ int main(int argc, char **argv) {
     }
 
     OutFile out;
+    std::vector<uint32_t> content;
+    if (pixels && y0 == 0 && y1 == H * (ssaa ? 2 : 1)) {
+        // The canvas as the app keeps it (Canvas::setPixel with opacity -1, src/slim/renderer/renderer.h:97) and the
+        // window content Canvas::drawToWindow makes of it (getPixelContent, src/slim/draw/canvas.h:119-126, 187-190).
+        memory::canvas_memory_capacity = 0;   // the canvas gets the driver's buffers (canvas.h:21-39)
+        Canvas canvas{(u16)W, (u16)H, ssaa ? SSAA : NoAA};
+        std::vector<Pixel> canvas_pixels((size_t)Ws * rows);
+        std::vector<f32> canvas_depths((size_t)Ws * rows);
+        canvas.pixels = canvas_pixels.data();
+        canvas.depths = canvas_depths.data();
+        for (int y = 0; y < rows; y++)
+            for (int x = 0; x < Ws; x++) {
+                const size_t o = (size_t)y * Ws + x;
+                canvas.setPixel(x, y, Color{colors[3 * o], colors[3 * o + 1], colors[3 * o + 2]}, -1, depths[o]);
+            }
+        content.resize((size_t)W * H);
+        const int step = ssaa ? 4 : 1;
+        for (size_t i = 0; i < content.size(); i++) content[i] = canvas.getPixelContent(canvas_pixels.data() + i * step);
+    }
     if (pixels) {
         out.add("color", 0, colors.data(), colors.size());
         out.add("depth", 0, depths.data(), depths.size());
     }
+    if (!content.empty()) out.add("content", 1, content.data(), content.size());
     out.add("hit_geo", 2, hit_geo.data(), hit_geo.size());
     out.add("hit_tri", 2, hit_tri.data(), hit_tri.size());
     out.add("hit_t", 0, hit_t.data(), hit_t.size());

@@ -1594,6 +1594,33 @@ int oracle_iteration_counts(void *h, int y0, int y1, int max_depth, uint64_t *co
     return 0;
 }
 
+/* Window content of a canvas (Canvas::drawToWindow / getPixelContent, src/slim/draw/canvas.h:119-126, 187-190,
+ * 254-265; Pixel::asContent, src/slim/core/base.h:1166-1171): per window pixel, under SSAA the mean of its 2x2
+ * sample quad (((p0 + p1) + p2) + p3) * 0.25 - 0 when all four opacities are 0 - else the pixel itself (0 when
+ * its opacity is 0); then 0xRRGGBB with each channel 255 * sqrt(c) truncated, 255 above 1.  pixels: RGBA floats
+ * in the canvas layout (2x2 quads per pixel under SSAA). */
+int oracle_canvas_content(const float *pixels, int width, int height, int ssaa, uint32_t *content) {
+    const int step = ssaa ? 4 : 1;
+    for (int i = 0; i < width * height; i++) {
+        const float *p = pixels + (size_t)4 * step * i;
+        float c[4];
+        if (ssaa) {
+            if (p[3] == 0.0f && p[7] == 0.0f && p[11] == 0.0f && p[15] == 0.0f) { content[i] = 0; continue; }
+            for (int k = 0; k < 4; k++) c[k] = (((p[k] + p[4 + k]) + p[8 + k]) + p[12 + k]) * 0.25f;
+        } else {
+            if (p[3] == 0.0f) { content[i] = 0; continue; }
+            for (int k = 0; k < 4; k++) c[k] = p[k];
+        }
+        uint32_t v = 0;
+        for (int k = 0; k < 3; k++) {
+            const uint32_t b = c[k] > 1.0f ? 255u : (uint32_t)(uint8_t)(255.0f * sqrtf(c[k]));
+            v = (v << 8) | b;
+        }
+        content[i] = v;
+    }
+    return 0;
+}
+
 /* Primary closest hit per sample: geometry index (-1 miss), leaf-ordered triangle id (-1 if not a mesh), t. */
 int oracle_primary_hits(void *h, int y0, int y1, int32_t *geo, int32_t *tri, float *t) {
     Scene *s = (Scene *)h;

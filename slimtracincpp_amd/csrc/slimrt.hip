@@ -128,6 +128,13 @@ __global__ void nrc_composite_kernel(RenderParams rp, const uint32_t *__restrict
 }
 
 // Clears the frame counters and step losses (a kernel rather than a memset, so the frame can be captured).
+// Canvas::drawToWindow: one thread per window pixel (its 2x2 sample quad is 64 contiguous bytes under SSAA).
+__global__ __launch_bounds__(kBlock) void content_kernel(const float4 *__restrict__ pixels, uint32_t n, uint32_t ssaa,
+                                                         uint32_t *__restrict__ content) {
+    const uint32_t i = blockIdx.x * kBlock + threadIdx.x;
+    if (i < n) content[i] = pixel_content(pixels + (size_t)i * (ssaa ? 4 : 1), ssaa != 0);
+}
+
 // One pick on the device: thread 0 traces, with the render kernels' LDS stacks.
 __global__ __launch_bounds__(kBlock) void pick_kernel(SceneView s, RenderParams rp, int32_t x, int32_t y,
                                                       uint32_t scene_depth, uint32_t mesh_depth,
@@ -694,6 +701,23 @@ int slimrt_host_collapse_bvh4(const slimrt_bvh_node *nodes, uint32_t node_count,
     *node_count_out = (uint32_t)out.size();
     *height_out = h4;
     *stack_out = stack4;
+    return SLIMRT_OK;
+}
+
+int slimrt_canvas_content(slimrt_context *c, const void *pixels, uint32_t width, uint32_t height, uint32_t ssaa,
+                          void *content) {
+    if (!c) return SLIMRT_E_ARG;
+    if (!pixels || !content || !width || !height || width > 65535 || height > 65535)
+        return fail(c, SLIMRT_E_ARG, "canvas_content: NULL buffer or dimensions out of range");
+    const uint32_t n = width * height;
+    if (c->cpu) {
+        for (uint32_t i = 0; i < n; i++)
+            ((uint32_t *)content)[i] = pixel_content((const float4 *)pixels + (size_t)i * (ssaa ? 4 : 1), ssaa != 0);
+        return SLIMRT_OK;
+    }
+    hipLaunchKernelGGL(content_kernel, dim3((n + kBlock - 1) / kBlock), dim3(kBlock), 0, c->stream,
+                       (const float4 *)pixels, n, ssaa ? 1u : 0u, (uint32_t *)content);
+    HIP_TRY(c, hipGetLastError());
     return SLIMRT_OK;
 }
 

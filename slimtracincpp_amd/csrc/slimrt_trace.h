@@ -748,6 +748,23 @@ SLIM_HD void pick(const SceneView &s, const RenderParams &rp, int32_t x, int32_t
     out_f[3] = pos.z;
 }
 
+// Window content of one pixel of a canvas (Canvas::getPixelContent, canvas.h:187-190, 254-265; Pixel::asContent,
+// base.h:1166-1171): under SSAA the mean of its 2x2 sample quad, ((p0 + p1) + p2) + p3 times 0.25, or 0 when all
+// four opacities are 0; else the pixel (0 when its opacity is 0).  0xRRGGBB, each channel 255 * sqrt(c) truncated,
+// 255 above 1.
+SLIM_HD uint32_t pixel_content(const float4 *p, bool ssaa) {
+    float4 c = p[0];
+    if (ssaa) {
+        if (p[0].w == 0.0f && p[1].w == 0.0f && p[2].w == 0.0f && p[3].w == 0.0f) return 0u;
+        c = make_float4((((p[0].x + p[1].x) + p[2].x) + p[3].x) * 0.25f, (((p[0].y + p[1].y) + p[2].y) + p[3].y) * 0.25f,
+                        (((p[0].z + p[1].z) + p[2].z) + p[3].z) * 0.25f, 0.0f);
+    } else if (c.w == 0.0f) {
+        return 0u;
+    }
+    auto channel = [](float v) { return v > 1.0f ? 255u : (uint32_t)(uint8_t)(255.0f * sqrtf(v)); };
+    return channel(c.x) << 16 | channel(c.y) << 8 | channel(c.z);
+}
+
 // Canvas offsets: samples are stored row-major, or as 2x2 quads per pixel under SSAA (canvas.h:147).
 SLIM_HD uint32_t canvas_offset(const RenderParams &rp, uint32_t x, uint32_t y) {
     return rp.ssaa ? ((rp.width * (y >> 1) + (x >> 1)) * 4 + (2 * (y & 1)) + (x & 1)) : (rp.width * y + x);

@@ -133,6 +133,7 @@ def lib():
             "slimrt_host_collapse_bvh4": [vp, u32, u32, vp, u32, P(u32), P(u32), P(u32)],
             "slimrt_wavefront_counts": [vp, vp, u32],
             "slimrt_pick": [vp, P(RenderParams), i32, i32, P(PickResult)],
+            "slimrt_canvas_content": [vp, vp, u32, u32, u32, vp],
             "slimrt_host_bake_texture": [vp, u32, u32, u32, vp, C.c_size_t, P(C.c_size_t), P(u32), vp, P(u32)],
         }
         for name, args in sig.items():
@@ -408,6 +409,11 @@ class Renderer:
         _check(lib().slimrt_pick(self.ctx, C.byref(p), x, y, C.byref(res)), self.ctx)
         return {"geometry": res.geometry, "light": res.light, "triangle": res.triangle,
                 "distance": res.distance, "position": np.array(res.position[:], np.float32)}
+
+    def canvas_content(self, pixels_ptr: int, width: int, height: int, ssaa: bool, content_ptr: int):
+        """slimrt_canvas_content: window content words of a canvas (device pointers on a GPU context)."""
+        _check(lib().slimrt_canvas_content(self.ctx, C.c_void_p(pixels_ptr), width, height, int(ssaa),
+                                           C.c_void_p(content_ptr)), self.ctx)
 
     def wavefront_counts(self) -> np.ndarray:
         """Path counters of the last frame rendered with WAVEFRONT | WAVEFRONT_COUNT: paths per integrator

@@ -287,14 +287,13 @@ void wf_bounce_kernel(SceneView s, RenderParams rp, float4 *__restrict__ pixels,
             pixel = st.pixel;
             const float4 a = accum[pixel];
             col = color{a.x, a.y, a.z};
-            count_iteration(stats, rp, depth_left);
-            cont = beauty_iteration(s, rp, ray, scaling, depth_left, tp, col, depth, tr);
-            if (TO_END)
-                while (cont) {
-                    count_end(stats, true);
-                    count_iteration(stats, rp, depth_left);
-                    cont = beauty_iteration(s, rp, ray, scaling, depth_left, tp, col, depth, tr);
-                }
+#pragma unroll 1
+            for (;;) {   // one iteration, or (TO_END) every remaining one: a single inlined copy either way
+                count_iteration(stats, rp, depth_left);
+                cont = beauty_iteration(s, rp, ray, scaling, depth_left, tp, col, depth, tr);
+                if (!TO_END || !cont) break;
+                count_end(stats, true);
+            }
             count_end(stats, cont);
             if (!cont) finish_sample(rp, col, pixels, pixel);
         }

@@ -261,7 +261,8 @@ template <bool TO_END, bool WIDE>
 __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(TO_END && !WIDE ? 3 : 1)))
 void wf_bounce_kernel(SceneView s, RenderParams rp, float4 *__restrict__ pixels, float4 *__restrict__ accum,
                       const PathState *__restrict__ in, const uint32_t *__restrict__ in_count,
-                      PathState *__restrict__ out, uint32_t *__restrict__ out_count, uint32_t cap, uint32_t scene_depth, uint32_t mesh_depth, uint32_t *__restrict__ status,
+                      PathState *__restrict__ out, uint32_t *__restrict__ out_count, uint32_t cap,
+                      uint32_t scene_depth, uint32_t mesh_depth, uint32_t *__restrict__ status,
                       uint32_t *__restrict__ stats) {
     extern __shared__ uint32_t lds[];
     const uint32_t t = threadIdx.x;

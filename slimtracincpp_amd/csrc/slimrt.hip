@@ -5,9 +5,10 @@
 //   mesh BVH nodes 32 B | triangle hit records 64 B | triangle attribute records 64 B | texture and mip records |
 //   texel quads 12 B.
 // The render kernel is a per-sample megakernel: each 256-thread workgroup owns a 16x16 tile of samples, each
-// wave an 8x8 sub-tile (coherent primary rays), and tiles are assigned so that neighbouring tiles run on the
-// same XCD (workgroups are dealt round-robin to the 8 XCDs; each XCD has its own L2).  Traversal stacks live
-// in LDS, one column per thread, sized from the uploaded BVH heights.
+// wave an 8x8 sub-tile (coherent primary rays), and tile i is workgroup i: workgroups are dealt round-robin to
+// the 8 XCDs, so every XCD gets an even share of each image row band (band-per-XCD dealing left whole XCDs on
+// the costly bands and ran 18-20% slower).  Traversal stacks live in LDS, one column per thread, sized from the
+// uploaded BVH heights.
 #include <hip/hip_runtime.h>
 #include <string>
 #include <vector>
@@ -35,11 +36,10 @@ template <bool kCount, bool WIDE>
 __global__ __launch_bounds__(kBlock) void render_kernel(SceneView s, RenderParams rp, float4 *__restrict__ pixels,
                                                        float *__restrict__ depths, uint32_t scene_depth,
                                                        uint32_t mesh_depth, uint32_t tiles_x, uint32_t tiles,
-                                                       uint32_t tiles_per_xcd, uint32_t *__restrict__ status,
+                                                       uint32_t *__restrict__ status,
                                                        unsigned long long *__restrict__ counters) {
     extern __shared__ uint32_t lds[];
-    const uint32_t xcd = blockIdx.x & 7u, slot = blockIdx.x >> 3;
-    const uint32_t tile = xcd * tiles_per_xcd + slot;
+    const uint32_t tile = blockIdx.x;
     if (tile >= tiles) return;
     const uint32_t t = threadIdx.x, wave = t >> 6, lane = t & 63u;
     const uint32_t x = (tile % tiles_x) * kTile + (wave & 1u) * 8u + (lane & 7u);
@@ -75,11 +75,9 @@ template <bool WIDE>
 __global__ __launch_bounds__(kBlock) void nrc_frame_kernel(SceneView s, RenderParams rp, NrcFrame nf,
                                                            float4 *__restrict__ pixels, float *__restrict__ depths,
                                                            uint32_t scene_depth, uint32_t mesh_depth, uint32_t tiles_x,
-                                                           uint32_t tiles, uint32_t tiles_per_xcd,
-                                                           uint32_t *__restrict__ status) {
+                                                           uint32_t tiles, uint32_t *__restrict__ status) {
     extern __shared__ uint32_t lds[];
-    const uint32_t xcd = blockIdx.x & 7u, slot = blockIdx.x >> 3;
-    const uint32_t tile = xcd * tiles_per_xcd + slot;
+    const uint32_t tile = blockIdx.x;
     if (tile >= tiles) return;
     const uint32_t t = threadIdx.x, wave = t >> 6, lane = t & 63u;
     const uint32_t x = (tile % tiles_x) * kTile + (wave & 1u) * 8u + (lane & 7u);
@@ -196,7 +194,6 @@ int launch(slimrt_context *c, const slimrt_render_params *p, float4 *pixels, flo
     const uint32_t ws = rp.width << rp.ssaa, hs = rp.height << rp.ssaa;
     const uint32_t tiles_x = (ws + kTile - 1) / kTile, tiles_y = (hs + kTile - 1) / kTile;
     const uint32_t tiles = tiles_x * tiles_y;
-    const uint32_t per_xcd = (tiles + 7) / 8;
     const uint32_t scene_depth = c->scene_height + 1, mesh_depth = mesh_stack_depth(c, wide_mesh_bvh(rp));
     const size_t lds = stack_lds_bytes(scene_depth, mesh_depth, wide_mesh_bvh(rp));
     if (lds > 160 * 1024) return fail(c, SLIMRT_E_ARG, "BVH too deep for the LDS traversal stacks");
@@ -209,8 +206,8 @@ int launch(slimrt_context *c, const slimrt_render_params *p, float4 *pixels, flo
         if (rc) return rc;
     } else {
 #define RENDER_LAUNCH(COUNT, WIDE)                                                                              \
-    hipLaunchKernelGGL((render_kernel<COUNT, WIDE>), dim3(per_xcd * 8), dim3(kBlock), lds, c->stream, v, rp, pixels, \
-                       depths, scene_depth, mesh_depth, tiles_x, tiles, per_xcd, c->d_status, c->d_counters)
+    hipLaunchKernelGGL((render_kernel<COUNT, WIDE>), dim3(tiles), dim3(kBlock), lds, c->stream, v, rp, pixels,        \
+                       depths, scene_depth, mesh_depth, tiles_x, tiles, c->d_status, c->d_counters)
         const bool wide = wide_mesh_bvh(rp);
         if (count) {
             if (wide) RENDER_LAUNCH(true, true); else RENDER_LAUNCH(true, false);
@@ -802,7 +799,7 @@ int slimrt_render_nrc(slimrt_context *c, const slimrt_render_params *p, slimrt_n
     const uint32_t ws = rp.width << rp.ssaa, hs = rp.height << rp.ssaa;
     const uint32_t samples = ws * hs;
     const uint32_t tiles_x = (ws + kTile - 1) / kTile, tiles_y = (hs + kTile - 1) / kTile;
-    const uint32_t tiles = tiles_x * tiles_y, per_xcd = (tiles + 7) / 8;
+    const uint32_t tiles = tiles_x * tiles_y;
     const uint32_t scene_depth = c->scene_height + 1, mesh_depth = mesh_stack_depth(c, wide_mesh_bvh(rp));
     const size_t lds = stack_lds_bytes(scene_depth, mesh_depth, wide_mesh_bvh(rp));
     if (lds > 160 * 1024) return fail(c, SLIMRT_E_ARG, "BVH too deep for the LDS traversal stacks");
@@ -842,12 +839,12 @@ int slimrt_render_nrc(slimrt_context *c, const slimrt_render_params *p, slimrt_n
     hipLaunchKernelGGL(nrc_reset_kernel, dim3(1), dim3(64), 0, c->stream, c->nrc_counters, c->nrc_losses, kMaxTrainSteps);
     HIP_TRY(c, hipGetLastError());
     if (wide_mesh_bvh(rp))
-        hipLaunchKernelGGL(nrc_frame_kernel<true>, dim3(per_xcd * 8), dim3(kBlock), lds, c->stream, device_view(c), rp,
-                           nf, (float4 *)d_pixels, (float *)d_depths, scene_depth, mesh_depth, tiles_x, tiles, per_xcd,
+        hipLaunchKernelGGL(nrc_frame_kernel<true>, dim3(tiles), dim3(kBlock), lds, c->stream, device_view(c), rp,
+                           nf, (float4 *)d_pixels, (float *)d_depths, scene_depth, mesh_depth, tiles_x, tiles,
                            c->d_status);
     else
-        hipLaunchKernelGGL(nrc_frame_kernel<false>, dim3(per_xcd * 8), dim3(kBlock), lds, c->stream, device_view(c), rp,
-                           nf, (float4 *)d_pixels, (float *)d_depths, scene_depth, mesh_depth, tiles_x, tiles, per_xcd,
+        hipLaunchKernelGGL(nrc_frame_kernel<false>, dim3(tiles), dim3(kBlock), lds, c->stream, device_view(c), rp,
+                           nf, (float4 *)d_pixels, (float *)d_depths, scene_depth, mesh_depth, tiles_x, tiles,
                            c->d_status);
     HIP_TRY(c, hipGetLastError());
     if (np->cache_depth + 1 <= p->max_depth) {

@@ -178,7 +178,7 @@ __device__ __forceinline__ void trace_view_tile(const SceneView &s, const ViewRe
 
 // Multi-view trace: wave tile WX x WY (64 samples), workgroup tile 2WX x 2WY.  PERSISTENT: a grid sized to the
 // chip takes tiles from the work counter (grab at a time) until it passes the total; otherwise one workgroup
-// per tile, dealt so that consecutive tiles land on the same XCD (workgroups go round-robin to the 8 XCDs).
+// per tile, tile i on workgroup i (workgroups go round-robin to the 8 XCDs, so each XCD gets an even share).
 template <int WX, int WY, bool PERSISTENT, bool WIDE>
 __global__ __launch_bounds__(kBlock) void views_kernel(SceneView s, const ViewRec *__restrict__ views, uint32_t nviews,
                                                        uint32_t total_tiles, uint32_t grab, uint32_t *__restrict__ counter,
@@ -202,8 +202,7 @@ __global__ __launch_bounds__(kBlock) void views_kernel(SceneView s, const ViewRe
             for (uint32_t tile = first; tile < last; tile++) trace_view_tile<WX, WY>(s, views, nviews, tile, tr, wave, lane);
         }
     } else {
-        const uint32_t per_xcd = (total_tiles + 7) / 8;
-        const uint32_t tile = (blockIdx.x & 7u) * per_xcd + (blockIdx.x >> 3);
+        const uint32_t tile = blockIdx.x;
         if (tile >= total_tiles) return;
         trace_view_tile<WX, WY>(s, views, nviews, tile, tr, wave, lane);
     }
@@ -328,7 +327,7 @@ int enqueue_views(slimrt_views *v) {
     const bool persistent = v->cfg.persistent != 0;
     const uint32_t blocks = persistent
         ? std::max(1u, std::min((uint32_t)cus * v->cfg.blocks_per_cu, (v->total_tiles + grab - 1) / grab))
-        : ((v->total_tiles + 7) / 8) * 8;
+        : v->total_tiles;
     bool wide = v->cfg.node_width == 4;
     if (v->cfg.node_width == 0) {   // as the views ask: the 4-wide mesh walk when every view has SLIMRT_BVH4
         wide = true;

@@ -175,11 +175,10 @@ __global__ __launch_bounds__(kBlock) void wf_primary_kernel(SceneView s, RenderP
                                                             PathState *__restrict__ queue, uint32_t *__restrict__ counters,
                                                             uint32_t cap, uint32_t scene_depth, uint32_t mesh_depth,
                                                             uint32_t tiles_x, uint32_t tile_base, uint32_t tiles,
-                                                            uint32_t tiles_per_xcd, uint32_t *__restrict__ status,
+                                                            uint32_t *__restrict__ status,
                                                             uint32_t *__restrict__ stats, HitSlots hits) {
     extern __shared__ uint32_t lds[];
-    const uint32_t xcd = blockIdx.x & 7u, slot = blockIdx.x >> 3;
-    const uint32_t local = xcd * tiles_per_xcd + slot;
+    const uint32_t local = blockIdx.x;
     if (local >= tiles) return;
     const uint32_t tile = tile_base + local;
     const uint32_t t = threadIdx.x, wave = t >> 6, lane = t & 63u;
@@ -226,11 +225,9 @@ __global__ __launch_bounds__(kBlock) void wf_primary_kernel(SceneView s, RenderP
 template <bool WIDE>
 __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(4)))
 void wf_trace_kernel(SceneView s, RenderParams rp, HitSlots hits, uint32_t scene_depth, uint32_t mesh_depth,
-                     uint32_t tiles_x, uint32_t tile_base, uint32_t tiles, uint32_t tiles_per_xcd,
-                     uint32_t *__restrict__ status) {
+                     uint32_t tiles_x, uint32_t tile_base, uint32_t tiles, uint32_t *__restrict__ status) {
     extern __shared__ uint32_t lds[];
-    const uint32_t xcd = blockIdx.x & 7u, slot = blockIdx.x >> 3;
-    const uint32_t local = xcd * tiles_per_xcd + slot;
+    const uint32_t local = blockIdx.x;
     if (local >= tiles) return;
     const uint32_t tile = tile_base + local;
     const uint32_t t = threadIdx.x, wave = t >> 6, lane = t & 63u;
@@ -459,16 +456,16 @@ int launch_wavefront(slimrt_context *c, const RenderParams &rp, float4 *pixels, 
     }
     for (uint32_t b = 0; b < bands; b++) {
         const uint32_t row0 = tiles_y * b / bands, row1 = tiles_y * (b + 1) / bands;
-        const uint32_t tile_base = row0 * tiles_x, tiles = (row1 - row0) * tiles_x, per_xcd = (tiles + 7) / 8;
+        const uint32_t tile_base = row0 * tiles_x, tiles = (row1 - row0) * tiles_x;
         const uint32_t base = tile_base * kBlock, cap = tiles * kBlock;
         uint32_t *bc = cnt + b * WF_COUNTERS;
 #define WF_PRIMARY(WIDE, SPLIT)                                                                                 \
-    hipLaunchKernelGGL((wf_primary_kernel<WIDE, SPLIT>), dim3(per_xcd * 8), dim3(kBlock), lds, c->stream, v, rp,    \
+    hipLaunchKernelGGL((wf_primary_kernel<WIDE, SPLIT>), dim3(tiles), dim3(kBlock), lds, c->stream, v, rp,           \
                        pixels, depths, accum, qa + base, bc, cap, scene_depth, mesh_depth, tiles_x, tile_base, tiles, \
-                       per_xcd, c->d_status, stats, hits)
+                       c->d_status, stats, hits)
 #define WF_TRACE(WIDE)                                                                                          \
-    hipLaunchKernelGGL((wf_trace_kernel<WIDE>), dim3(per_xcd * 8), dim3(kBlock), lds, c->stream, v, rp, hits,        \
-                       scene_depth, mesh_depth, tiles_x, tile_base, tiles, per_xcd, c->d_status)
+    hipLaunchKernelGGL((wf_trace_kernel<WIDE>), dim3(tiles), dim3(kBlock), lds, c->stream, v, rp, hits,               \
+                       scene_depth, mesh_depth, tiles_x, tile_base, tiles, c->d_status)
         if (split) {
             if (wide_mesh_bvh(rp)) WF_TRACE(true); else WF_TRACE(false);
             HIP_TRY(c, hipGetLastError());

@@ -32,14 +32,14 @@ static_assert(sizeof(RenderParams) == sizeof(slimrt_render_params), "render para
 namespace {
 
 
-// Three waves per SIMD for the frame (168 VGPRs, a few spilled): the walk's dependent fetches want the waves more
-// than the registers (6% faster than two on the mesh + PBR frame).  The counting pass keeps its registers.
+// The 4-wide frame at three waves per SIMD (168 VGPRs, 18 dwords spilled): the walk's dependent fetches want the
+// waves more than the registers (6% faster than two on the mesh + PBR frame).  The 2-wide frame would spill 64
+// dwords a lane for no gain, and the counting pass keeps its registers: both stay at the compiler's two.
 template <bool kCount, bool WIDE>
-__global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(kCount ? 1 : 3))) void render_kernel(SceneView s, RenderParams rp, float4 *__restrict__ pixels,
-                                                       float *__restrict__ depths, uint32_t scene_depth,
-                                                       uint32_t mesh_depth, uint32_t tiles_x, uint32_t tiles,
-                                                       uint32_t *__restrict__ status,
-                                                       unsigned long long *__restrict__ counters) {
+__global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(WIDE && !kCount ? 3 : 1)))
+void render_kernel(SceneView s, RenderParams rp, float4 *__restrict__ pixels, float *__restrict__ depths,
+                   uint32_t scene_depth, uint32_t mesh_depth, uint32_t tiles_x, uint32_t tiles,
+                   uint32_t *__restrict__ status, unsigned long long *__restrict__ counters) {
     extern __shared__ uint32_t lds[];
     const uint32_t tile = blockIdx.x;
     if (tile >= tiles) return;
@@ -127,7 +127,6 @@ __global__ void nrc_composite_kernel(RenderParams rp, const uint32_t *__restrict
     }
 }
 
-// Clears the frame counters and step losses (a kernel rather than a memset, so the frame can be captured).
 // Canvas::drawToWindow: one thread per window pixel (its 2x2 sample quad is 64 contiguous bytes under SSAA).
 __global__ __launch_bounds__(kBlock) void content_kernel(const float4 *__restrict__ pixels, uint32_t n, uint32_t ssaa,
                                                          uint32_t *__restrict__ content) {
@@ -153,6 +152,7 @@ __global__ __launch_bounds__(kBlock) void pick_kernel(SceneView s, RenderParams 
     if (tr.overflow) atomicOr(status, 1u);
 }
 
+// Clears the frame counters and step losses (a kernel rather than a memset, so the frame can be captured).
 __global__ void nrc_reset_kernel(uint32_t *__restrict__ counters, float *__restrict__ losses, uint32_t nloss) {
     const uint32_t i = threadIdx.x;
     if (i < NC_COUNT) counters[i] = 0u;

@@ -653,7 +653,11 @@ SLIM_HD void render_debug(const SceneView &s, const RenderParams &rp, v3 directi
                     const TexRec t0 = s.textures[0];
                     uint32_t lvl = mip_level(hit.uv_coverage * (float)((uint64_t)t0.width * t0.height), t0.mip_count);
                     lvl = lvl > 8 ? 8 : lvl;
-                    out = color{rp.mip_colors[lvl][0], rp.mip_colors[lvl][1], rp.mip_colors[lvl][2]};
+                    // a select over constant indices: a dynamic index would force a caller's private copy of
+                    // the parameters (the multi-view kernel's) into scratch memory
+#pragma unroll
+                    for (uint32_t k = 0; k < 9; k++)
+                        if (k == lvl) out = color{rp.mip_colors[k][0], rp.mip_colors[k][1], rp.mip_colors[k][2]};
                 } else out = cs(0.5f);
                 break;
             default: break;

@@ -179,8 +179,9 @@ __device__ __forceinline__ void trace_view_tile(const SceneView &s, const ViewRe
 // Multi-view trace: wave tile WX x WY (64 samples), workgroup tile 2WX x 2WY.  PERSISTENT: a grid sized to the
 // chip takes tiles from the work counter (grab at a time) until it passes the total; otherwise one workgroup
 // per tile, tile i on workgroup i (workgroups go round-robin to the 8 XCDs, so each XCD gets an even share).
+// The 4-wide walk at three waves per SIMD, as the frame kernel (render_kernel).
 template <int WX, int WY, bool PERSISTENT, bool WIDE>
-__global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(3)))
+__global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(WIDE ? 3 : 1)))
 void views_kernel(SceneView s, const ViewRec *__restrict__ views, uint32_t nviews, uint32_t total_tiles, uint32_t grab,
                   uint32_t *__restrict__ counter, uint32_t scene_depth, uint32_t mesh_depth,
                   uint32_t *__restrict__ status) {

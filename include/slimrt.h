@@ -297,6 +297,73 @@ int slimrt_host_build_mesh(const float *positions, uint32_t vertex_count, const 
                            slimrt_bvh_node *nodes_out, uint32_t *node_count_out, uint32_t *height_out,
                            slimrt_triangle *triangles_out, float aabb_out[6]);
 
+/* ------------------------------------------------------------------------------------------ wireframe overlays
+ * Boxes, BVH node boxes and mesh edges drawn into a canvas after a render, as the reference apps draw them
+ * (drawBox, src/slim/draw/box.h:7-34; drawBVH, draw/bvh.h:8-28; drawMesh, draw/mesh.h:7-22; the scene-BVH view
+ * of examples/03_BlinnPhong.cpp:194-209).  Each edge is taken to view space (Camera::internPos,
+ * src/slim/scene/camera.h:47), clipped to the near/far planes and the frustum's sides
+ * (Frustum::cullAndClipEdge, viewport/frustum.h:67-141), projected (projectPoint, frustum.h:180-188) and drawn
+ * as an antialiased line (_drawLine, draw/line.h:70-273) whose samples are blended into the canvas by
+ * Canvas::setPixel (canvas.h:128-185): depth-tested against the canvas depths, alpha-blended over or under.
+ * pixels/depths: the canvas of a render (slimrt_pixel per sample, SSAA quads as SLIMRT_OUTPUT_CANVAS writes
+ * them; a depth per sample); GPU context: device pointers, CPU context: host pointers.  The result equals the
+ * reference's sequential drawing sample for sample on both.  GPU calls synchronise the context's stream once
+ * (to size the sample lists) and are otherwise asynchronous on it. */
+
+/* The viewport an overlay is drawn from: the camera (Camera: orientation from Euler angles as
+ * OrientationUsing3x3Matrix::setRotation composes them, position, focal length), the canvas (width, height,
+ * ssaa) and the frustum's clipping distances (the reference's defaults are 0.001 and 1000). */
+typedef struct slimrt_overlay_view {
+    float camera_euler[3];
+    float camera_position[3];
+    float focal_length;
+    uint32_t width, height, ssaa;
+    float near_distance, far_distance;
+} slimrt_overlay_view;
+
+/* A Transform (src/slim/core/transform.h:5-10): orientation quaternion (axis xyz, amount), position, scale. */
+typedef struct slimrt_overlay_transform {
+    float orientation[4];
+    float position[3];
+    float scale[3];
+} slimrt_overlay_transform;
+
+#define SLIMRT_BOX_ALL_SIDES 63u   /* BOX__ALL_SIDES (base.h:187); else BoxSide bits: left 1, bottom 2, back 4,
+                                      right 8, top 16, front 32 (base.h:413-421) */
+
+/* drawBox(Box{AABB{min, max}}, transform, viewport, color, opacity, line_width, sides). */
+typedef struct slimrt_overlay_box {
+    slimrt_overlay_transform transform;
+    float min[3], max[3];      /* the box's local corners; drawBVH's and drawSelection's boxes are -1..1 */
+    float color[3];
+    float opacity;
+    uint32_t line_width;       /* as the reference's u8 */
+    uint32_t sides;            /* SLIMRT_BOX_ALL_SIDES or a BoxSide mask (the edges touching those sides) */
+} slimrt_overlay_box;
+
+/* Draws the boxes in order. */
+int slimrt_draw_boxes(slimrt_context *ctx, const slimrt_overlay_view *view, const slimrt_overlay_box *boxes,
+                      uint32_t count, void *pixels, void *depths);
+
+/* drawMesh(mesh, transform, false, viewport, color, opacity, line_width): every edge (EdgeVertexIndices,
+ * src/slim/scene/mesh.h:8-10, pairs of vertex indices) in order.  positions: vertex_count * 3 floats (host). */
+int slimrt_draw_mesh_wireframe(slimrt_context *ctx, const slimrt_overlay_view *view, const float *positions,
+                               uint32_t vertex_count, const uint32_t *edges, uint32_t edge_count,
+                               const slimrt_overlay_transform *transform, const float color[3], float opacity,
+                               uint32_t line_width, void *pixels, void *depths);
+
+/* drawBVH's loop (draw/bvh.h:8-28) as box records for slimrt_draw_boxes: every node with min_depth <= depth <=
+ * max_depth, in node order, as a -1..1 box scaled to the node's half extent and moved to its centre through
+ * `transform`; leaves BrightMagenta at opacity * 2, the root BrightCyan and other inner nodes BrightGreen at
+ * opacity / 2.  boxes_out: node_count entries at most; *count_out receives the number written. */
+int slimrt_host_bvh_boxes(const slimrt_bvh_node *nodes, uint32_t node_count, const slimrt_overlay_transform *transform,
+                          uint32_t min_depth, uint32_t max_depth, float opacity, uint32_t line_width,
+                          slimrt_overlay_box *boxes_out, uint32_t *count_out);
+
+/* The camera rotation of OrientationUsing3x3Matrix::setRotation(x, y, z) (src/slim/core/base.h:568-620,
+ * math/mat3.h): rows X, Y, Z (9 floats). */
+int slimrt_host_camera_rotation(const float euler[3], float rotation_out[9]);
+
 #ifdef __cplusplus
 }
 #endif

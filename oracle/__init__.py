@@ -65,6 +65,10 @@ def lib():
         L.oracle_iteration_counts.argtypes = [vp, i, i, i, vp, i]
         L.oracle_canvas_content.restype = i
         L.oracle_canvas_content.argtypes = [vp, i, i, i, vp]
+        L.oracle_draw_boxes.restype = None
+        L.oracle_draw_boxes.argtypes = [vp, vp, u32, vp, vp]
+        L.oracle_draw_mesh_wireframe.restype = None
+        L.oracle_draw_mesh_wireframe.argtypes = [vp, vp, vp, u32, vp, vp, f, u32, vp, vp]
         _lib = L
     return _lib
 
@@ -197,3 +201,30 @@ def build_bvh(boxes: np.ndarray, max_leaf: int = 1):
     lib().oracle_build_bvh(_p(boxes), N, max_leaf, _p(nf), _p(nu), _p(leaf), _p(meta))
     c = int(meta[0])
     return nf[:6 * c].reshape(c, 6), nu[:3 * c].reshape(c, 3), leaf[:N], int(meta[1])
+
+
+def _buf(a):
+    return ctypes.c_void_p(a.ctypes.data) if a.size else None
+
+
+def draw_boxes(view, boxes: np.ndarray, pixels: np.ndarray, depths: np.ndarray) -> None:
+    """drawBox for each box, sequentially, into a canvas (pixels (N, 4) float32 and depths (N,) float32, modified
+    in place).  view: the 13 words of slimrt_overlay_view (any buffer); boxes: 88-byte slimrt_overlay_box records."""
+    v = np.frombuffer(bytes(view), np.uint8).copy()
+    b = np.ascontiguousarray(boxes)
+    assert b.dtype.itemsize == 88 and pixels.dtype == np.float32 and depths.dtype == np.float32
+    assert pixels.flags.c_contiguous and depths.flags.c_contiguous
+    lib().oracle_draw_boxes(_buf(v), _buf(b), len(b), _buf(pixels), _buf(depths))
+
+
+def draw_mesh_wireframe(view, positions: np.ndarray, edges: np.ndarray, transform, color, opacity: float,
+                        line_width: int, pixels: np.ndarray, depths: np.ndarray) -> None:
+    """drawMesh without normals, sequentially; transform: 10 floats (quaternion, position, scale)."""
+    v = np.frombuffer(bytes(view), np.uint8).copy()
+    pos = np.ascontiguousarray(positions, np.float32)
+    e = np.ascontiguousarray(edges, np.uint32)
+    t = np.frombuffer(np.ascontiguousarray(transform).tobytes(), np.float32).copy()
+    assert t.size == 10
+    col = np.ascontiguousarray(color, np.float32)
+    lib().oracle_draw_mesh_wireframe(_buf(v), _buf(pos), _buf(e), e.size // 2, _buf(t), _buf(col), opacity,
+                                     line_width, _buf(pixels), _buf(depths))

@@ -29,6 +29,8 @@ static unsigned short g_mesh_leaf_size = 2;
 
 #include "slim/scene/selection.h"
 #include "slim/renderer/renderer.h"
+#include "slim/draw/bvh.h"
+#include "slim/draw/mesh.h"
 
 #include "specfile.h"
 
@@ -88,16 +90,18 @@ static memory::MonotonicAllocator arena(u64 bytes) {
 
 int main(int argc, char **argv) {
     if (argc < 3) {
-        fprintf(stderr, "usage: %s spec out [--threads N] [--rows y0 y1] [--reps R] [--no-pixels]\n", argv[0]);
+        fprintf(stderr, "usage: %s spec out [--threads N] [--rows y0 y1] [--reps R] [--no-pixels] [--overlay LW]\n",
+                argv[0]);
         return 2;
     }
-    int threads = 1, reps = 1, y0 = 0, y1 = -1;
+    int threads = 1, reps = 1, y0 = 0, y1 = -1, overlay_lw = -1;
     bool pixels = true;
     for (int i = 3; i < argc; i++) {
         if (!strcmp(argv[i], "--threads") && i + 1 < argc) threads = atoi(argv[++i]);
         else if (!strcmp(argv[i], "--reps") && i + 1 < argc) reps = atoi(argv[++i]);
         else if (!strcmp(argv[i], "--rows") && i + 2 < argc) { y0 = atoi(argv[++i]); y1 = atoi(argv[++i]); }
         else if (!strcmp(argv[i], "--no-pixels")) pixels = false;
+        else if (!strcmp(argv[i], "--overlay") && i + 1 < argc) overlay_lw = atoi(argv[++i]);
     }
     SpecFile sf;
     if (spec_open(&sf, argv[1])) { fprintf(stderr, "cannot read spec %s\n", argv[1]); return 1; }
@@ -165,6 +169,7 @@ int main(int argc, char **argv) {
     std::vector<std::vector<vec3>> mesh_pos(NM), mesh_nrm(NM);
     std::vector<std::vector<vec2>> mesh_uv(NM);
     std::vector<std::vector<TriangleVertexIndices>> mesh_pidx(NM), mesh_nidx(NM), mesh_uidx(NM);
+    std::vector<std::vector<EdgeVertexIndices>> mesh_edges(NM);
     for (u32 k = 0; k < NM; k++) {
         char nm[32];
         snprintf(nm, sizeof nm, "mesh%u_info", k);
@@ -218,6 +223,14 @@ int main(int argc, char **argv) {
                 for (int j = 0; j < 3; j++) mesh_uidx[k][t].ids[j] = uidx[3 * t + j];
             mesh.vertex_uvs = mesh_uv[k].data();
             mesh.vertex_uvs_indices = mesh_uidx[k].data();
+        }
+        snprintf(nm, sizeof nm, "mesh%u_edges", k);
+        uint32_t n_edge_words = 0;
+        if (const uint32_t *ev = (const uint32_t *)spec_get(&sf, nm, &n_edge_words)) {   // for the overlays
+            mesh_edges[k].resize(n_edge_words / 2);
+            for (u32 e = 0; e < n_edge_words / 2; e++) mesh_edges[k][e] = EdgeVertexIndices{ev[2 * e], ev[2 * e + 1]};
+            mesh.edge_count = n_edge_words / 2;
+            mesh.edge_vertex_indices = mesh_edges[k].data();
         }
         mesh_nodes[k].resize(2 * T);
         mesh_tris[k].resize(T);
@@ -376,6 +389,7 @@ int main(int argc, char **argv) {
 
     OutFile out;
     std::vector<uint32_t> content;
+    std::vector<float> cv_before, cv_after, cv_depth_before, cv_depth_after;
     if (pixels && y0 == 0 && y1 == H * (ssaa ? 2 : 1)) {
         // The canvas as the app keeps it (Canvas::setPixel with opacity -1, src/slim/renderer/renderer.h:97) and the
         // window content Canvas::drawToWindow makes of it (getPixelContent, src/slim/draw/canvas.h:119-126, 187-190).
@@ -393,12 +407,57 @@ int main(int argc, char **argv) {
         content.resize((size_t)W * H);
         const int step = ssaa ? 4 : 1;
         for (size_t i = 0; i < content.size(); i++) content[i] = canvas.getPixelContent(canvas_pixels.data() + i * step);
+        if (overlay_lw >= 0) {
+            // The canvas before and after the app's wireframe overlays: the scene-BVH view of
+            // examples/03_BlinnPhong.cpp:194-209 (a box per geometry, then the scene BVH's nodes from depth 1),
+            // each mesh geometry's BVH (depths 0-3) and wireframe, and a selection box with two highlighted sides
+            // (draw/selection.h:24-33), all with line width LW where the apps use 1.
+            for (const Pixel &p : canvas_pixels) { cv_before.push_back(p.color.r); cv_before.push_back(p.color.g);
+                                                   cv_before.push_back(p.color.b); cv_before.push_back(p.opacity); }
+            cv_depth_before = canvas_depths;
+            Viewport viewport{canvas, &camera};
+            const u8 lw = (u8)overlay_lw;
+            Box box;
+            AABB aabb;
+            for (u32 i = 0; i < scene.counts.geometries; i++) {
+                Geometry &geo{scene.geometries[i]};
+                aabb.max = geo.type == GeometryType_Tet ? TET_MAX : 1.0f;
+                aabb.min = -aabb.max.x;
+                if (geo.type == GeometryType_Quad) {
+                    aabb.min.y = -EPS;
+                    aabb.max.y = EPS;
+                }
+                box.vertices = aabb;
+                drawBox(box, geo.transform, viewport, BrightYellow, 0.5f, lw);
+            }
+            drawBVH(scene.bvh, {}, viewport, 1, scene.bvh.height, 0.25f, lw);
+            for (u32 i = 0; i < scene.counts.geometries; i++) {
+                Geometry &geo{scene.geometries[i]};
+                if (geo.type != GeometryType_Mesh) continue;
+                drawBVH(scene.meshes[geo.id].bvh, geo.transform, viewport, 0, 3, 0.25f, lw);
+                drawMesh(scene.meshes[geo.id], geo.transform, false, viewport, White, 0.5f, lw);
+            }
+            if (scene.counts.geometries > 1) {
+                Box unit;
+                drawBox(unit, scene.geometries[1].transform, viewport, Yellow, 0.5f, 0);
+                drawBox(unit, scene.geometries[1].transform, viewport, Red, 0.5f, 1, BoxSide_Left | BoxSide_Top);
+            }
+            for (const Pixel &p : canvas_pixels) { cv_after.push_back(p.color.r); cv_after.push_back(p.color.g);
+                                                  cv_after.push_back(p.color.b); cv_after.push_back(p.opacity); }
+            cv_depth_after = canvas_depths;
+        }
     }
     if (pixels) {
         out.add("color", 0, colors.data(), colors.size());
         out.add("depth", 0, depths.data(), depths.size());
     }
     if (!content.empty()) out.add("content", 1, content.data(), content.size());
+    if (!cv_after.empty()) {
+        out.add("ov_in_pixels", 0, cv_before.data(), cv_before.size());
+        out.add("ov_in_depths", 0, cv_depth_before.data(), cv_depth_before.size());
+        out.add("ov_pixels", 0, cv_after.data(), cv_after.size());
+        out.add("ov_depths", 0, cv_depth_after.data(), cv_depth_after.size());
+    }
     out.add("hit_geo", 2, hit_geo.data(), hit_geo.size());
     out.add("hit_tri", 2, hit_tri.data(), hit_tri.size());
     out.add("hit_t", 0, hit_t.data(), hit_t.size());

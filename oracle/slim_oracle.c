@@ -1778,3 +1778,301 @@ int oracle_build_bvh(const float *boxes, uint32_t N, uint32_t max_leaf, float *n
     builder_free(&b);
     return 0;
 }
+
+/* ---------------------------------------------------------------- wireframe overlays (src/slim/draw)
+ * drawBox / drawBVH / drawMesh -> drawEdge -> _drawLine -> Canvas::setPixel, drawn sequentially as the reference
+ * draws them, into a canvas of RGBA floats and depths in the render's layout (2x2 quads per pixel under SSAA). */
+typedef struct {
+    float camera_euler[3], camera_position[3], focal_length;
+    uint32_t width, height, ssaa;
+    float near_distance, far_distance;
+} OvView;                                                   /* the layout of slimrt_overlay_view */
+typedef struct { float q[4], position[3], scale[3]; } OvTransform;
+typedef struct {
+    OvTransform t;
+    float mn[3], mx[3], color[3], opacity;
+    uint32_t line_width, sides;
+} OvBox;                                                    /* the layout of slimrt_overlay_box */
+
+typedef struct {
+    m3 rot;                                                 /* Camera::orientation */
+    v3 cam_pos;
+    float focal, near_d, far_d;
+    float f_width, f_height, h_width, h_height, width_over_height, height_over_width;   /* Dimensions */
+    float scale_x, scale_y;                                 /* Frustum::Projection::scale */
+    int width, height, ssaa;
+    float *pixels, *depths;
+} OvCanvas;
+
+static OvCanvas ov_canvas(const OvView *v, float *pixels, float *depths) {
+    OvCanvas c;
+    c.rot = m3_from_euler(v->camera_euler[0], v->camera_euler[1], v->camera_euler[2]);
+    c.cam_pos = V3(v->camera_position[0], v->camera_position[1], v->camera_position[2]);
+    c.focal = v->focal_length;
+    c.near_d = v->near_distance;
+    c.far_d = v->far_distance;
+    c.width = (uint16_t)v->width;                                                            /* base.h:1312-1323 */
+    c.height = (uint16_t)v->height;
+    c.f_width = (float)c.width;
+    c.f_height = (float)c.height;
+    c.h_width = c.f_width * 0.5f;
+    c.h_height = c.f_height * 0.5f;
+    c.width_over_height = c.f_width / c.f_height;
+    c.height_over_width = c.f_height / c.f_width;
+    c.scale_x = c.focal * c.height_over_width;                                               /* frustum.h:25-27 */
+    c.scale_y = c.focal;
+    c.ssaa = v->ssaa ? 1 : 0;
+    c.pixels = pixels;
+    c.depths = depths;
+    return c;
+}
+
+static void ov_set_pixel(const OvCanvas *cv, int x, int y, const float col[3], float opacity, float depth) { /* canvas.h:128-185 */
+    int w = cv->width, h = cv->height;
+    if (cv->ssaa) { w <<= 1; h <<= 1; }
+    if (x < 0 || y < 0 || x >= w || y >= h) return;
+    int override_ = 0;
+    if (opacity < 0) { opacity = -opacity; override_ = 1; }
+    opacity = clamp01(opacity);
+    float px[4] = {clamp01(col[0]), clamp01(col[1]), clamp01(col[2]), opacity};
+    for (int k = 0; k < 3; k++) px[k] *= px[k] * px[3];
+    const size_t off = cv->ssaa ? ((size_t)cv->width * (y >> 1) + (x >> 1)) * 4 + 2 * (y & 1) + (x & 1)
+                                : (size_t)cv->width * y + x;
+    float *out = cv->pixels + 4 * off, *out_depth = cv->depths + off;
+    if (override_ || (out_depth[0] == INFINITY && out[0] == 0 && out[1] == 0 && out[2] == 0) ||
+        (opacity == 1.0f && depth == INFINITY)) {
+        memcpy(out, px, sizeof px);
+        out_depth[0] = depth;
+        return;
+    }
+    const float *bg, *fg;                                    /* _sortPixelsByDepth, canvas.h:267-276 */
+    float old[4];
+    memcpy(old, out, sizeof old);
+    if (depth == INFINITY || depth < *out_depth) { *out_depth = depth; bg = old; fg = px; }
+    else { bg = px; fg = old; }
+    float res[4];
+    if (fg[3] == 1) memcpy(res, fg, sizeof res);
+    else for (int k = 0; k < 4; k++) res[k] = fg[k] + bg[k] * (1.0f - fg[3]);        /* alphaBlendOver, base.h:1162 */
+    memcpy(out, res, sizeof res);
+}
+
+typedef struct { float first, last; } RangeF;
+typedef struct { int first, last; } RangeI;
+static void rangef_sub(RangeF *r, float a, float b) {                                                  /* base.h:305-317 */
+    if (b < a) { float t = b; b = a; a = t; }
+    if (r->last < a || b < r->first) { r->first = -1; r->last = -2; }
+    else { r->first = r->first < a ? a : r->first; r->last = b < r->last ? b : r->last; }
+}
+static int rangei_has(RangeI r, int v) { return (r.first <= v) && (v <= r.last); }
+#define OV_FRAC(x) ((x) - floorf(x))                                                                   /* base.h:219-220 */
+#define OV_1MFRAC(x) (1 - OV_FRAC(x))
+
+static void ov_draw_line(const OvCanvas *cv, float x1, float y1, float z1, float x2, float y2, float z2,
+                         const float col[3], float opacity, uint8_t line_width) {                     /* line.h:70-273 */
+    RangeF fx = {x1 <= x2 ? x1 : x2, x1 <= x2 ? x2 : x1};
+    RangeF fy = {y1 <= y2 ? y1 : y2, y1 <= y2 ? y2 : y1};
+    const float left = 0.0f, top = 0.0f, right = (float)(cv->width - 1), bottom = (float)(cv->height - 1);
+    x1 += left; x2 += left; y1 += top; y2 += top;
+    fx.first += left; fx.last += left;
+    fy.first += top; fy.last += top;
+    rangef_sub(&fx, left, right);
+    rangef_sub(&fy, top, bottom);
+    rangef_sub(&fx, 0, cv->f_width - 1.0f);
+    rangef_sub(&fy, 0, cv->f_height - 1.0f);
+    if (fx.last < fx.first || fy.last < fy.first) return;
+    RangeI xr = {(int)fx.first, (int)ceilf(fx.last)}, yr = {(int)fy.first, (int)ceilf(fy.last)};
+    if (xr.last == cv->width) xr.last--;
+    if (yr.last == cv->height) yr.last--;
+    if (cv->ssaa) {
+        x1 += x1; x2 += x2; y1 += y1; y2 += y2;
+        xr.first <<= 1; yr.first <<= 1; xr.last <<= 1; yr.last <<= 1;
+        line_width <<= 1;
+        line_width++;
+    }
+    float tmp, z_range, range_remap, gap, grad, first_offset, last_offset, z = 0, z_curr = 0, z_step = 0;
+    float first_x, last_x, first_y, last_y;
+    int x, y, start_x, end_x, start_y, end_y;
+    const float dx = x2 - x1, dy = y2 - y1;
+    const int has_depth = (z1 != 0.0f) || (z2 != 0.0f);
+    if (fabsf(dx) > fabsf(dy)) {
+        if (x2 < x1) { tmp = x2; x2 = x1; x1 = tmp; tmp = y2; y2 = y1; y1 = tmp; tmp = z2; z2 = z1; z1 = tmp; }
+        grad = dy / dx;
+        first_x = roundf(x1); last_x = roundf(x2);
+        first_offset = first_x - x1; last_offset = last_x - x2;
+        first_y = y1 + grad * first_offset; last_y = y2 + grad * last_offset;
+        start_x = (int)first_x; start_y = (int)first_y; end_x = (int)last_x; end_y = (int)last_y;
+        x = start_x; y = start_y;
+        gap = OV_1MFRAC(x1 + 0.5f);
+        if (rangei_has(xr, x)) {
+            if (rangei_has(yr, y)) ov_set_pixel(cv, x, y, col, OV_1MFRAC(first_y) * gap * opacity, z1);
+            for (uint8_t i = 0; i < line_width; i++) if (rangei_has(yr, ++y)) ov_set_pixel(cv, x, y, col, opacity, z1);
+            if (rangei_has(yr, ++y)) ov_set_pixel(cv, x, y, col, OV_FRAC(first_y) * gap * opacity, z1);
+        }
+        x = end_x; y = end_y;
+        gap = OV_FRAC(x2 + 0.5f);
+        if (rangei_has(xr, x)) {
+            if (rangei_has(yr, y)) ov_set_pixel(cv, x, y, col, OV_1MFRAC(last_y) * gap * opacity, z2);
+            for (uint8_t i = 0; i < line_width; i++) if (rangei_has(yr, ++y)) ov_set_pixel(cv, x, y, col, opacity, z2);
+            if (rangei_has(yr, ++y)) ov_set_pixel(cv, x, y, col, OV_FRAC(last_y) * gap * opacity, z2);
+        }
+        if (has_depth) {
+            z1 = 1.0f / z1; z2 = 1.0f / z2;
+            z_range = z2 - z1;
+            range_remap = z_range / (x2 - x1);
+            z1 += range_remap * (first_offset + 1.0f);
+            z2 += range_remap * (last_offset - 1.0f);
+            z_curr = z1;
+            z_range = z2 - z1;
+            z_step = z_range / (last_x - first_x - 1.0f);
+        } else z = 0.0f;
+        gap = first_y + grad;
+        for (x = start_x + 1; x < end_x; x++) {
+            if (rangei_has(xr, x)) {
+                y = (int)gap;
+                if (has_depth) z = 1.0f / z_curr;
+                if (rangei_has(yr, y)) ov_set_pixel(cv, x, y, col, OV_1MFRAC(gap) * opacity, z);
+                for (uint8_t i = 0; i < line_width; i++) if (rangei_has(yr, ++y)) ov_set_pixel(cv, x, y, col, opacity, z);
+                if (rangei_has(yr, ++y)) ov_set_pixel(cv, x, y, col, OV_FRAC(gap) * opacity, z);
+            }
+            gap += grad;
+            if (has_depth) z_curr += z_step;
+        }
+    } else {
+        if (y2 < y1) { tmp = x2; x2 = x1; x1 = tmp; tmp = y2; y2 = y1; y1 = tmp; tmp = z2; z2 = z1; z1 = tmp; }
+        grad = dx / dy;
+        first_y = roundf(y1); last_y = roundf(y2);
+        first_offset = y1 - first_y; last_offset = last_y - y2;
+        first_x = x1 + grad * first_offset; last_x = x2 + grad * last_offset;
+        start_y = (int)first_y; start_x = (int)first_x; end_y = (int)last_y; end_x = (int)last_x;
+        x = start_x; y = start_y;
+        gap = OV_1MFRAC(y1 + 0.5f);
+        if (rangei_has(yr, y)) {
+            if (rangei_has(xr, x)) ov_set_pixel(cv, x, y, col, OV_1MFRAC(first_x) * gap * opacity, z1);
+            for (uint8_t i = 0; i < line_width; i++) if (rangei_has(xr, ++x)) ov_set_pixel(cv, x, y, col, opacity, z1);
+            if (rangei_has(xr, ++x)) ov_set_pixel(cv, x, y, col, OV_FRAC(first_x) * gap * opacity, z1);
+        }
+        x = end_x; y = end_y;
+        gap = OV_FRAC(y2 + 0.5f);
+        if (rangei_has(yr, y)) {
+            if (rangei_has(xr, x)) ov_set_pixel(cv, x, y, col, OV_1MFRAC(last_x) * gap * opacity, z2);
+            for (uint8_t i = 0; i < line_width; i++) if (rangei_has(xr, ++x)) ov_set_pixel(cv, x, y, col, opacity, z2);
+            if (rangei_has(xr, ++x)) ov_set_pixel(cv, x, y, col, OV_FRAC(last_x) * gap * opacity, z2);
+        }
+        if (has_depth) {
+            z1 = 1.0f / z1; z2 = 1.0f / z2;
+            z_range = z2 - z1;
+            range_remap = z_range / (y2 - y1);
+            z1 += range_remap * (first_offset + 1.0f);
+            z2 += range_remap * (last_offset - 1.0f);
+            z_range = z2 - z1;
+            z_step = z_range / (last_y - first_y - 1.0f);
+            z_curr = z1;
+        } else z = 0.0f;
+        gap = first_x + grad;
+        for (y = start_y + 1; y < end_y; y++) {
+            if (rangei_has(yr, y)) {
+                if (has_depth) z = 1.0f / z_curr;
+                x = (int)gap;
+                if (rangei_has(xr, x)) ov_set_pixel(cv, x, y, col, OV_1MFRAC(gap) * opacity, z);
+                for (uint8_t i = 0; i < line_width; i++) if (rangei_has(xr, ++x)) ov_set_pixel(cv, x, y, col, opacity, z);
+                if (rangei_has(xr, ++x)) ov_set_pixel(cv, x, y, col, OV_FRAC(gap) * opacity, z);
+            }
+            gap += grad;
+            if (has_depth) z_curr += z_step;
+        }
+    }
+}
+
+static v3 ov_lerp(v3 a, v3 b, float by) { return vscaleadd(vsub(b, a), by, a); }                     /* vec3.h:360-362 */
+
+static int ov_clip(v3 *A, v3 *B, v3 N) {                                                   /* one frustum side */
+    const float nA = vdot(N, *A), nB = vdot(N, *B);
+    const int out = (nA < 0) | ((nB < 0) << 1);
+    if (out) {
+        if (out == 3) return 0;
+        if (out & 1) *A = ov_lerp(*A, *B, nA / (nA - nB));
+        else *B = ov_lerp(*B, *A, nB / (nB - nA));
+    }
+    return 1;
+}
+
+static void ov_draw_edge(const OvCanvas *cv, v3 from_world, v3 to_world, const float col[3], float opacity,
+                         uint8_t line_width) {                                                 /* edge.h:7-17 */
+    /* Camera::internPos (camera.h:47): transposed rotation times (p - position) */
+    const m3 T = M3(V3(cv->rot.X.x, cv->rot.Y.x, cv->rot.Z.x), V3(cv->rot.X.y, cv->rot.Y.y, cv->rot.Z.y),
+                    V3(cv->rot.X.z, cv->rot.Y.z, cv->rot.Z.z));
+    v3 A = m3_mulv(T, vsub(from_world, cv->cam_pos)), B = m3_mulv(T, vsub(to_world, cv->cam_pos));
+    /* Frustum::cullAndClipEdge (frustum.h:67-141) */
+    float d = cv->near_d;
+    int out = (A.z < d) | ((B.z < d) << 1);
+    if (out) {
+        if (out == 3) return;
+        if (out & 1) A = ov_lerp(A, B, (d - A.z) / (B.z - A.z));
+        else B = ov_lerp(B, A, (d - B.z) / (A.z - B.z));
+    }
+    d = cv->far_d;
+    out = (A.z > d) | ((B.z > d) << 1);
+    if (out) {
+        if (out == 3) return;
+        if (out & 1) A = ov_lerp(A, B, (A.z - d) / (A.z - B.z));
+        else B = ov_lerp(B, A, (B.z - d) / (B.z - A.z));
+    }
+    if (!ov_clip(&A, &B, V3(cv->focal, 0, cv->width_over_height))) return;
+    if (!ov_clip(&A, &B, V3(-cv->focal, 0, cv->width_over_height))) return;
+    if (!ov_clip(&A, &B, V3(0, cv->focal, 1))) return;
+    if (!ov_clip(&A, &B, V3(0, -cv->focal, 1))) return;
+    /* Frustum::projectPoint (frustum.h:180-184) */
+    float ax = ((cv->scale_x * A.x / A.z) + 1) * cv->h_width, ay = ((cv->scale_y * A.y / A.z) + 1) * cv->h_height;
+    float bx = ((cv->scale_x * B.x / B.z) + 1) * cv->h_width, by = ((cv->scale_y * B.y / B.z) + 1) * cv->h_height;
+    ay = cv->f_height - ay;
+    by = cv->f_height - by;
+    ov_draw_line(cv, ax, ay, A.z - EPS, bx, by, B.z - EPS, col, opacity, line_width);
+}
+
+static v3 ov_extern(const OvTransform *t, v3 p) {                                          /* transform.h:10 */
+    const quat q = Q(V3(t->q[0], t->q[1], t->q[2]), t->q[3]);
+    return vadd(qrot(q, vmul(p, V3(t->scale[0], t->scale[1], t->scale[2]))),
+                V3(t->position[0], t->position[1], t->position[2]));
+}
+
+/* drawBox (box.h:7-34) */
+void oracle_draw_boxes(const OvView *view, const OvBox *boxes, uint32_t count, float *pixels, float *depths) {
+    const OvCanvas cv = ov_canvas(view, pixels, depths);
+    for (uint32_t b = 0; b < count; b++) {
+        const OvBox *bx = &boxes[b];
+        const v3 mn = V3(bx->mn[0], bx->mn[1], bx->mn[2]), mx = V3(bx->mx[0], bx->mx[1], bx->mx[2]);
+        const v3 corner[8] = {                                                                  /* box.h:19-29 */
+            V3(mn.x, mx.y, mx.z), V3(mx.x, mx.y, mx.z), V3(mn.x, mn.y, mx.z), V3(mx.x, mn.y, mx.z),
+            V3(mn.x, mx.y, mn.z), V3(mx.x, mx.y, mn.z), V3(mn.x, mn.y, mn.z), V3(mx.x, mn.y, mn.z)};
+        v3 w[8];
+        for (int i = 0; i < 8; i++) w[i] = ov_extern(&bx->t, corner[i]);
+        /* BoxEdges::setFrom (box.h:52-77): front top, bottom, left, right; back ...; left bottom, left top;
+           right bottom, right top */
+        static const int E[12][2] = {{0, 1}, {2, 3}, {2, 0}, {3, 1}, {4, 5}, {6, 7},
+                                     {6, 4}, {7, 5}, {2, 6}, {0, 4}, {3, 7}, {1, 5}};
+        const uint8_t sides = (uint8_t)bx->sides;
+        const uint8_t lw = (uint8_t)bx->line_width;
+        if (sides == 63) {
+            for (int e = 0; e < 12; e++) ov_draw_edge(&cv, w[E[e][0]], w[E[e][1]], bx->color, bx->opacity, lw);
+        } else {                                                                                /* box.h:21-32 */
+            enum { L = 1, Bo = 2, Bk = 4, R = 8, T = 16, F = 32 };
+            static const int S[12][3] = {{0, F, T}, {1, F, Bo}, {2, F, L}, {3, F, R}, {4, Bk, T}, {5, Bk, Bo},
+                                         {6, Bk, L}, {7, Bk, R}, {9, L, T}, {8, L, Bo}, {11, R, T}, {10, R, Bo}};
+            for (int i = 0; i < 12; i++)
+                if ((sides & S[i][1]) | (sides & S[i][2]))
+                    ov_draw_edge(&cv, w[E[S[i][0]][0]], w[E[S[i][0]][1]], bx->color, bx->opacity, lw);
+        }
+    }
+}
+
+/* drawMesh without normals (mesh.h:7-22) */
+void oracle_draw_mesh_wireframe(const OvView *view, const float *positions, const uint32_t *edges, uint32_t edge_count,
+                                const OvTransform *t, const float *color3, float opacity, uint32_t line_width,
+                                float *pixels, float *depths) {
+    const OvCanvas cv = ov_canvas(view, pixels, depths);
+    for (uint32_t i = 0; i < edge_count; i++) {
+        const float *a = positions + 3 * (size_t)edges[2 * i], *b = positions + 3 * (size_t)edges[2 * i + 1];
+        ov_draw_edge(&cv, ov_extern(t, V3(a[0], a[1], a[2])), ov_extern(t, V3(b[0], b[1], b[2])), color3, opacity,
+                     (uint8_t)line_width);
+    }
+}

@@ -461,6 +461,8 @@ int slimrt_destroy(slimrt_context *c) {
         dev_free(c->d_mips); dev_free(c->d_quads); dev_free(c->d_pixels); dev_free(c->d_depths);
         free_nrc(c);
         dev_free(c->d_geo_stage);
+        dev_free(c->ov_lines);
+        dev_free(c->ov_samples);
         for (void *q : {c->wf_queue_a, c->wf_queue_b, c->wf_accum, c->wf_hits, (void *)c->wf_counters, (void *)c->wf_stats,
                         (void *)c->d_pick})
             if (q) (void)hipFree(q);

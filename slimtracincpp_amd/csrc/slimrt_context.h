@@ -67,6 +67,7 @@ struct slimrt_context {
     DevBuf<float4> d_pixels;
     DevBuf<float> d_depths;
     DevBuf<slimrt_geometry> d_geo_stage;   // transform updates on their way to the scene update kernel
+    DevBuf<uint8_t> ov_lines, ov_samples;   // overlay drawing: inputs and lines; the lines' canvas samples
     void *wf_queue_a = nullptr, *wf_queue_b = nullptr, *wf_accum = nullptr;   // wavefront ray queues, colours
     void *wf_hits = nullptr;   // first-hit records of the split primary pass (36 bytes per sample slot)
     uint32_t wf_hits_cap = 0;

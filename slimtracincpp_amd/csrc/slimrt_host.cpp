@@ -235,11 +235,9 @@ int slimrt_host_orientation(const float euler[3], float q_out[4]) {
     return SLIMRT_OK;
 }
 
-int slimrt_host_projection(const float e[3], const float position[3], float focal_length, uint32_t width,
-                           uint32_t height, uint32_t ssaa, slimrt_render_params *p) {
-    if (!e || !position || !p || width == 0 || height == 0 || width > 65535 || height > 65535) return SLIMRT_E_ARG;
-    // OrientationUsing3x3Matrix (base.h:612-620, mat3.h:49-77); cos/sin of a float argument are evaluated in
-    // double there, as unqualified cos/sin resolve to the C library's double functions.
+// OrientationUsing3x3Matrix::setRotation (base.h:568-620, mat3.h:49-77); cos/sin of a float argument are evaluated
+// in double there, as unqualified cos/sin resolve to the C library's double functions.
+static m33 camera_matrix(const float e[3]) {
     m33 o{{1, 0, 0}, {0, 1, 0}, {0, 0, 1}};
     if (e[2] != 0.0f) {
         float c = (float)cos((double)e[2]), s = (float)sin((double)e[2]);
@@ -253,6 +251,53 @@ int slimrt_host_projection(const float e[3], const float position[3], float foca
         float c = (float)cos((double)e[1]), s = (float)sin((double)e[1]);
         o = mat_mul(o, {{c, 0, s}, {0, 1, 0}, {-s, 0, c}});
     }
+    return o;
+}
+
+int slimrt_host_camera_rotation(const float euler[3], float rotation_out[9]) {
+    if (!euler || !rotation_out) return SLIMRT_E_ARG;
+    const m33 o = camera_matrix(euler);
+    memcpy(rotation_out, &o, sizeof(float) * 9);
+    return SLIMRT_OK;
+}
+
+// drawBVH (draw/bvh.h:8-28): per node in depth range, the -1..1 box under the transform scaled by the node's half
+// extent (vec3 *= vec3) and moved to Transform::externPos of its centre.
+int slimrt_host_bvh_boxes(const slimrt_bvh_node *nodes, uint32_t node_count, const slimrt_overlay_transform *t,
+                          uint32_t min_depth, uint32_t max_depth, float opacity, uint32_t line_width,
+                          slimrt_overlay_box *out, uint32_t *count_out) {
+    if ((node_count && (!nodes || !out)) || !t || !count_out) return SLIMRT_E_ARG;
+    const q4 q{{t->orientation[0], t->orientation[1], t->orientation[2]}, t->orientation[3]};
+    const f3 scale{t->scale[0], t->scale[1], t->scale[2]}, pos{t->position[0], t->position[1], t->position[2]};
+    const uint16_t lo = (uint16_t)min_depth, hi = (uint16_t)max_depth;   // drawBVH takes u16 depths
+    uint32_t n = 0;
+    for (uint32_t i = 0; i < node_count; i++) {
+        const slimrt_bvh_node &nd = nodes[i];
+        if (nd.depth < lo || nd.depth > hi) continue;
+        const f3 mn{nd.min[0], nd.min[1], nd.min[2]}, mx{nd.max[0], nd.max[1], nd.max[2]};
+        const f3 half = mul(sub(mx, mn), 0.5f);
+        const f3 s = mul(scale, half);
+        const f3 c = add(rotate(q, mul(scale, mul(add(mn, mx), 0.5f))), pos);
+        slimrt_overlay_box &b = out[n++];
+        memcpy(b.transform.orientation, t->orientation, sizeof b.transform.orientation);
+        b.transform.position[0] = c.x; b.transform.position[1] = c.y; b.transform.position[2] = c.z;
+        b.transform.scale[0] = s.x; b.transform.scale[1] = s.y; b.transform.scale[2] = s.z;
+        for (int k = 0; k < 3; k++) { b.min[k] = -1.0f; b.max[k] = 1.0f; }
+        // BrightMagenta leaves, BrightCyan root, BrightGreen inner nodes (base.h:822-831)
+        const float leaf[3] = {0.75f, 0.0f, 0.75f}, root[3] = {0.0f, 0.75f, 0.75f}, inner[3] = {0.5f, 1.0f, 0.5f};
+        memcpy(b.color, nd.leaf_count ? leaf : (i ? inner : root), sizeof b.color);
+        b.opacity = nd.leaf_count ? opacity * 2.0f : opacity * 0.5f;
+        b.line_width = line_width & 0xFFu;
+        b.sides = SLIMRT_BOX_ALL_SIDES;
+    }
+    *count_out = n;
+    return SLIMRT_OK;
+}
+
+int slimrt_host_projection(const float e[3], const float position[3], float focal_length, uint32_t width,
+                           uint32_t height, uint32_t ssaa, slimrt_render_params *p) {
+    if (!e || !position || !p || width == 0 || height == 0 || width > 65535 || height > 65535) return SLIMRT_E_ARG;
+    const m33 o = camera_matrix(e);
     // Dimensions::update (base.h:1312-1323) and CameraRayProjection::reset (camera.h:69-83)
     const float fw = (float)(uint16_t)width, fh = (float)(uint16_t)height;
     const float hw = fw * 0.5f, hh = fh * 0.5f;

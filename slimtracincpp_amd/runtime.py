@@ -40,6 +40,12 @@ TRIANGLE = np.dtype([("local_to_tangent", "<f4", 9), ("position", "<f4", 3), ("n
 assert GEOMETRY.itemsize == 56 and MATERIAL.itemsize == 80 and LIGHT.itemsize == 32
 BVH4_NODE = np.dtype([("min", "<f4", (3, 4)), ("max", "<f4", (3, 4)), ("child", "<u4", 4), ("count", "<u4", 4)])
 assert BVH_NODE.itemsize == 32 and TRIANGLE.itemsize == 128 and BVH4_NODE.itemsize == 128
+OVERLAY_TRANSFORM = np.dtype([("orientation", "<f4", 4), ("position", "<f4", 3), ("scale", "<f4", 3)])
+OVERLAY_BOX = np.dtype([("transform", OVERLAY_TRANSFORM), ("min", "<f4", 3), ("max", "<f4", 3), ("color", "<f4", 3),
+                        ("opacity", "<f4"), ("line_width", "<u4"), ("sides", "<u4")])
+assert OVERLAY_BOX.itemsize == 88
+BOX_ALL_SIDES = 63
+BOX_LEFT, BOX_BOTTOM, BOX_BACK, BOX_RIGHT, BOX_TOP, BOX_FRONT = 1, 2, 4, 8, 16, 32
 
 
 class Mesh(C.Structure):
@@ -72,6 +78,18 @@ class RenderParams(C.Structure):
 
 class PickResult(C.Structure):
     _fields_ = [("geometry", i32), ("light", i32), ("triangle", i32), ("distance", f32), ("position", f32 * 3)]
+
+
+class OverlayView(C.Structure):
+    _fields_ = [("camera_euler", f32 * 3), ("camera_position", f32 * 3), ("focal_length", f32), ("width", u32),
+                ("height", u32), ("ssaa", u32), ("near_distance", f32), ("far_distance", f32)]
+
+
+def overlay_view(spec: scenes.SceneSpec, near: float = 0.001, far: float = 1000.0) -> OverlayView:
+    """The viewport of a scene's canvas with the reference's default clipping distances."""
+    return OverlayView((f32 * 3)(*np.asarray(spec.camera_euler, np.float32)),
+                       (f32 * 3)(*np.asarray(spec.camera_position, np.float32)), float(spec.focal_length),
+                       spec.width, spec.height, int(spec.ssaa), near, far)
 
 
 class ViewsConfig(C.Structure):
@@ -135,6 +153,10 @@ def lib():
             "slimrt_pick": [vp, P(RenderParams), i32, i32, P(PickResult)],
             "slimrt_canvas_content": [vp, vp, u32, u32, u32, vp],
             "slimrt_host_bake_texture": [vp, u32, u32, u32, vp, C.c_size_t, P(C.c_size_t), P(u32), vp, P(u32)],
+            "slimrt_draw_boxes": [vp, P(OverlayView), vp, u32, vp, vp],
+            "slimrt_draw_mesh_wireframe": [vp, P(OverlayView), vp, u32, vp, u32, vp, vp, f32, u32, vp, vp],
+            "slimrt_host_bvh_boxes": [vp, u32, vp, u32, u32, f32, u32, vp, P(u32)],
+            "slimrt_host_camera_rotation": [vp, vp],
         }
         for name, args in sig.items():
             fn = getattr(L, name)
@@ -170,6 +192,32 @@ def orientation(euler) -> np.ndarray:
     q = np.zeros(4, np.float32)
     _check(lib().slimrt_host_orientation(_ptr(e), _ptr(q)), what="orientation")
     return q
+
+
+def camera_rotation(euler) -> np.ndarray:
+    """The camera's rotation matrix (rows X, Y, Z) for Euler angles (slimrt_host_camera_rotation)."""
+    e = np.ascontiguousarray(euler, np.float32)
+    r = np.zeros(9, np.float32)
+    _check(lib().slimrt_host_camera_rotation(_ptr(e), _ptr(r)), what="camera_rotation")
+    return r.reshape(3, 3)
+
+
+def overlay_transform(orientation=(0, 0, 0, 1), position=(0, 0, 0), scale=(1, 1, 1)) -> np.ndarray:
+    t = np.zeros((), OVERLAY_TRANSFORM)
+    t["orientation"], t["position"], t["scale"] = orientation, position, scale
+    return t
+
+
+def bvh_boxes(nodes: np.ndarray, transform: np.ndarray, min_depth: int = 0, max_depth: int = 5,
+              opacity: float = 0.25, line_width: int = 1) -> np.ndarray:
+    """drawBVH's boxes (slimrt_host_bvh_boxes) for slimrt_draw_boxes."""
+    a = np.ascontiguousarray(nodes, BVH_NODE)
+    t = np.ascontiguousarray(transform, OVERLAY_TRANSFORM)
+    out = np.zeros(max(len(a), 1), OVERLAY_BOX)
+    n = u32()
+    _check(lib().slimrt_host_bvh_boxes(_ptr(a), len(a), _ptr(t), min_depth, max_depth, opacity, line_width, _ptr(out),
+                                       C.byref(n)), what="bvh_boxes")
+    return out[:n.value]
 
 
 def build_bvh(aabbs: np.ndarray, max_leaf: int = 1):
@@ -414,6 +462,23 @@ class Renderer:
         """slimrt_canvas_content: window content words of a canvas (device pointers on a GPU context)."""
         _check(lib().slimrt_canvas_content(self.ctx, C.c_void_p(pixels_ptr), width, height, int(ssaa),
                                            C.c_void_p(content_ptr)), self.ctx)
+
+    def draw_boxes(self, view: OverlayView, boxes: np.ndarray, pixels_ptr: int, depths_ptr: int):
+        """slimrt_draw_boxes into a canvas (device pointers on a GPU context, host pointers on a CPU one)."""
+        b = np.ascontiguousarray(boxes, OVERLAY_BOX)
+        _check(lib().slimrt_draw_boxes(self.ctx, C.byref(view), _ptr(b), len(b), C.c_void_p(pixels_ptr),
+                                       C.c_void_p(depths_ptr)), self.ctx)
+
+    def draw_mesh_wireframe(self, view: OverlayView, positions: np.ndarray, edges: np.ndarray, transform: np.ndarray,
+                            color, opacity: float, line_width: int, pixels_ptr: int, depths_ptr: int):
+        """slimrt_draw_mesh_wireframe: drawMesh's edges into a canvas."""
+        pos = np.ascontiguousarray(positions, np.float32)
+        e = np.ascontiguousarray(edges, np.uint32)
+        t = np.ascontiguousarray(transform, OVERLAY_TRANSFORM)
+        col = np.ascontiguousarray(color, np.float32)
+        _check(lib().slimrt_draw_mesh_wireframe(self.ctx, C.byref(view), _ptr(pos), pos.size // 3, _ptr(e),
+                                                e.size // 2, _ptr(t), _ptr(col), opacity, line_width,
+                                                C.c_void_p(pixels_ptr), C.c_void_p(depths_ptr)), self.ctx)
 
     def wavefront_counts(self) -> np.ndarray:
         """Path counters of the last frame rendered with WAVEFRONT | WAVEFRONT_COUNT: paths per integrator

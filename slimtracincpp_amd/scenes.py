@@ -103,10 +103,29 @@ class MeshSpec:
     uvs: Optional[np.ndarray] = None            # (U,2) f32
     uv_indices: Optional[np.ndarray] = None     # (T,3) u32
     leaf_size: int = 2             # max triangles per BVH leaf
+    edges: Optional[np.ndarray] = None          # (E,2) u32 vertex pairs (EdgeVertexIndices), for wireframe overlays
 
     @property
     def triangle_count(self) -> int:
         return int(self.position_indices.shape[0])
+
+
+def with_mesh_edges(spec: "SceneSpec") -> "SceneSpec":
+    """The spec with every mesh's edge list filled in (triangle_edges), for wireframe overlays."""
+    for m in spec.meshes:
+        m.edges = triangle_edges(m.position_indices)
+    return spec
+
+
+def triangle_edges(position_indices: np.ndarray) -> np.ndarray:
+    """The distinct edges of a triangle list as (E, 2) u32 vertex pairs, in order of first appearance (each
+    triangle's edges v1-v2, v2-v3, v3-v1), every pair ordered as it first appears."""
+    tri = np.asarray(position_indices, np.uint32).reshape(-1, 3)
+    pairs = np.stack([tri[:, [0, 1]], tri[:, [1, 2]], tri[:, [2, 0]]], axis=1).reshape(-1, 2)
+    key = np.sort(pairs, axis=1).astype(np.uint64)
+    key = key[:, 0] << np.uint64(32) | key[:, 1]
+    _, first = np.unique(key, return_index=True)
+    return np.ascontiguousarray(pairs[np.sort(first)], np.uint32)
 
 
 @dataclass

@@ -53,6 +53,8 @@ def _sections(spec: SceneSpec):
         if n_uv:
             yield f"mesh{k}_uv", m.uvs
             yield f"mesh{k}_uidx", m.uv_indices
+        if m.edges is not None:
+            yield f"mesh{k}_edges", np.ascontiguousarray(m.edges, np.uint32).reshape(-1)
     for k, t in enumerate(spec.textures):
         yield f"tex{k}_info", np.array([t.width, t.height, t.flags, len(t.mips)], np.uint32)
         yield f"tex{k}_dims", np.array(t.mip_dims, np.uint32).reshape(-1)

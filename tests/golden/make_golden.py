@@ -36,9 +36,20 @@ CASES = {
     # the large workload's generator at a small size: 53 geometries, emissive quads, normal maps, depth 8
     "large_s0_160x90_emissive": lambda: scenes.large_scene(0, 160, 90, uv_sphere=(17, 32), texture_size=64,
                                                            sky_size=16, irradiance_size=8),
+    # wireframe overlays after the render (the driver's --overlay LW sequence: geometry boxes, scene BVH, mesh
+    # BVHs and wireframes, a selection box)
+    "overlay_implicit_s3_96x64_lw1": lambda: scenes.implicit_scene(3, 96, 64, max_depth=2),
+    "overlay_mesh_s0_64x40_ssaa_lw2": lambda: _ssaa(scenes.with_mesh_edges(scenes.mesh_pbr_scene(
+        0, 64, 40, level=2, texture_size=16, sky_size=4, irradiance_size=4))),
 }
+OVERLAY_LINE_WIDTH = {"overlay_implicit_s3_96x64_lw1": 1, "overlay_mesh_s0_64x40_ssaa_lw2": 2}
 
 DIGEST_ABOVE = 4096   # arrays larger than this many elements are stored as SHA-256 digests
+
+
+def _ssaa(spec):
+    spec.ssaa = True
+    return spec
 
 
 def digest(a: np.ndarray) -> str:
@@ -51,7 +62,8 @@ def main(names=None):
             continue
         spec = make()
         sb = spec_bytes(spec)
-        ref = oracle.run_reference(spec)
+        extra = ["--overlay", str(OVERLAY_LINE_WIDTH[name])] if name in OVERLAY_LINE_WIDTH else []
+        ref = oracle.run_reference(spec, *extra)
         if len(sb) > 1 << 20:
             out = {"spec__sha256": np.array(hashlib.sha256(sb).hexdigest())}
         else:
@@ -59,7 +71,7 @@ def main(names=None):
         for k, v in ref.items():
             if k in ("timing", "rows"):
                 continue
-            if v.size > DIGEST_ABOVE and not k.startswith(("color", "depth", "hit_")):
+            if v.size > DIGEST_ABOVE and not k.startswith(("color", "depth", "hit_", "ov_")):
                 out[k + "__sha256"] = np.array(digest(v))
             else:
                 out[k] = v

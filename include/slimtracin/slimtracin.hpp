@@ -714,3 +714,90 @@ struct RayTracingRenderer {
         check(slimrt_render(ctx, &p, (slimrt_pixel *)canvas.pixels, canvas.depths), ctx, "render");
     }
 };
+
+// ---------------------------------------------------------------------------------------- wireframe overlays
+// drawBox / drawBVH / drawMesh (src/slim/draw/box.h:7-34, bvh.h:8-28, mesh.h:7-22) into the viewport's canvas
+// after a render, drawn by the library exactly as the reference draws them (slimrt_draw_boxes,
+// slimrt_draw_mesh_wireframe).  The canvas here is host memory, so the host context draws; a canvas in HBM is
+// drawn by the same entry points on a GPU context.  The viewport uses the reference's default frustum.
+#define BOX__ALL_SIDES 63
+enum BoxSide { BoxSide_None = 0, BoxSide_Left = 1, BoxSide_Bottom = 2, BoxSide_Back = 4, BoxSide_Right = 8,
+               BoxSide_Top = 16, BoxSide_Front = 32 };
+struct Box {   // Box with its vertices given by their AABB (BoxVertices(min, max), src/slim/scene/box.h:17-29)
+    AABB vertices{vec3{-1.0f}, vec3{1.0f}};
+};
+
+namespace slimtracin_detail {
+inline slimrt_context *cpu() {
+    Contexts &c = contexts();
+    if (!c.cpu) check(slimrt_create(&c.cpu, SLIMRT_DEVICE_CPU), nullptr, "create CPU context");
+    return c.cpu;
+}
+inline slimrt_overlay_view overlayView(const Viewport &v) {
+    const Camera &cam = *v.camera;
+    return slimrt_overlay_view{{cam.orientation.x, cam.orientation.y, cam.orientation.z},
+                               {cam.position.x, cam.position.y, cam.position.z}, cam.focal_length,
+                               v.canvas.dimensions.width, v.canvas.dimensions.height,
+                               v.canvas.antialias == SSAA ? 1u : 0u, 0.001f, 1000.0f};
+}
+inline slimrt_overlay_transform overlayTransform(const Transform &t) {
+    return slimrt_overlay_transform{{t.orientation.axis.x, t.orientation.axis.y, t.orientation.axis.z, t.orientation.amount},
+                                    {t.position.x, t.position.y, t.position.z}, {t.scale.x, t.scale.y, t.scale.z}};
+}
+inline void drawBoxes(const Viewport &viewport, const slimrt_overlay_box *boxes, u32 count) {
+    const slimrt_overlay_view view = overlayView(viewport);
+    check(slimrt_draw_boxes(cpu(), &view, boxes, count, viewport.canvas.pixels, viewport.canvas.depths), cpu(),
+          "draw boxes");
+}
+}  // namespace slimtracin_detail
+
+inline void drawBox(const Box &box, const Transform &transform, const Viewport &viewport, const Color &color = White,
+                    f32 opacity = 1.0f, u8 line_width = 1, u8 sides = BOX__ALL_SIDES) {
+    const AABB &a = box.vertices;
+    const slimrt_overlay_box b{slimtracin_detail::overlayTransform(transform),
+                               {a.min.x, a.min.y, a.min.z}, {a.max.x, a.max.y, a.max.z},
+                               {color.r, color.g, color.b}, opacity, line_width, sides};
+    slimtracin_detail::drawBoxes(viewport, &b, 1);
+}
+
+inline void drawBVH(const BVH &bvh, const Transform &transform, const Viewport &viewport, u16 min_depth = 0,
+                    u16 max_depth = 5, f32 opacity = 0.25f, u8 line_width = 1) {
+    std::vector<slimrt_overlay_box> boxes(bvh.node_count);
+    const slimrt_overlay_transform t = slimtracin_detail::overlayTransform(transform);
+    u32 n = 0;
+    slimtracin_detail::check(slimrt_host_bvh_boxes(bvh.nodes, bvh.node_count, &t, min_depth, max_depth, opacity,
+                                                   line_width, boxes.data(), &n), nullptr, "drawBVH");
+    slimtracin_detail::drawBoxes(viewport, boxes.data(), n);
+}
+
+inline void drawMesh(const Mesh &mesh, const Transform &transform, bool draw_normals, const Viewport &viewport,
+                     const Color &color = White, f32 opacity = 1.0f, u8 line_width = 1) {
+    using namespace slimtracin_detail;
+    const slimrt_overlay_view view = overlayView(viewport);
+    const slimrt_overlay_transform t = overlayTransform(transform);
+    const float col[3] = {color.r, color.g, color.b};
+    check(slimrt_draw_mesh_wireframe(cpu(), &view, (const float *)mesh.vertex_positions, mesh.vertex_count,
+                                     (const uint32_t *)mesh.edge_vertex_indices, mesh.edge_count, &t, col, opacity,
+                                     line_width, viewport.canvas.pixels, viewport.canvas.depths),
+          cpu(), "drawMesh");
+    if (draw_normals && mesh.normals_count && mesh.vertex_normals && mesh.vertex_normal_indices) {
+        // one segment per triangle corner, position to position + normal * 0.1 (mesh.h:24-33), in Red at half opacity
+        std::vector<float> ends;
+        std::vector<uint32_t> segs;
+        for (u32 i = 0; i < mesh.triangle_count; i++) {
+            const u32 *pi = &mesh.vertex_position_indices[i].v1, *ni = &mesh.vertex_normal_indices[i].v1;
+            for (int k = 0; k < 3; k++) {
+                const vec3 p = mesh.vertex_positions[pi[k]], to = mesh.vertex_normals[ni[k]] * 0.1f + p;
+                segs.push_back((uint32_t)(ends.size() / 3));
+                ends.insert(ends.end(), {p.x, p.y, p.z});
+                segs.push_back((uint32_t)(ends.size() / 3));
+                ends.insert(ends.end(), {to.x, to.y, to.z});
+            }
+        }
+        const float red[3] = {1.0f, 0.0f, 0.0f};
+        check(slimrt_draw_mesh_wireframe(cpu(), &view, ends.data(), (uint32_t)(ends.size() / 3), segs.data(),
+                                         (uint32_t)(segs.size() / 2), &t, red, opacity * 0.5f, line_width,
+                                         viewport.canvas.pixels, viewport.canvas.depths),
+              cpu(), "drawMesh normals");
+    }
+}

@@ -4,6 +4,9 @@
 //
 //   app_api_test spec <scene.spec> <out.bin> <cpu|gpu>   render a scene spec through the API, write the canvas
 //   app_api_test demo <monkey.mesh> <out.bin> <cpu|gpu>  a hand-built scene with a loaded .mesh asset
+//   app_api_test overlay<LW> <scene.spec> <out.bin> <cpu|gpu>  spec, then the app's wireframe overlays with line
+//       width LW (drawSceneBVH of src/examples/03_BlinnPhong.cpp:194-209, drawBVH + drawMesh per mesh geometry,
+//       a selection box and two of its sides); writes the canvas before and after them
 //
 // Output: raw float32 canvas pixels (RGBA per sample) followed by the depths.
 #include <cstdio>
@@ -23,7 +26,7 @@ static int write_canvas(const char *path, const std::vector<Pixel> &px, const st
     return 0;
 }
 
-static int run_spec(const char *spec_path, const char *out, bool gpu) {
+static int run_spec(const char *spec_path, const char *out, bool gpu, int overlay_lw = -1) {
     SpecFile sf;
     if (spec_open(&sf, spec_path)) return 2;
     const int32_t *fr = (const int32_t *)spec_get(&sf, "frame", nullptr);
@@ -65,6 +68,10 @@ static int run_spec(const char *spec_path, const char *out, bool gpu) {
         snprintf(nm, sizeof nm, "mesh%u_nidx", k); m.vertex_normal_indices = (TriangleVertexIndices *)spec_get(&sf, nm, nullptr);
         snprintf(nm, sizeof nm, "mesh%u_uv", k);   m.vertex_uvs = (vec2 *)spec_get(&sf, nm, nullptr);
         snprintf(nm, sizeof nm, "mesh%u_uidx", k); m.vertex_uvs_indices = (TriangleVertexIndices *)spec_get(&sf, nm, nullptr);
+        uint32_t edge_words = 0;
+        snprintf(nm, sizeof nm, "mesh%u_edges", k);
+        m.edge_vertex_indices = (EdgeVertexIndices *)spec_get(&sf, nm, &edge_words);
+        m.edge_count = m.edge_vertex_indices ? edge_words / 2 : 0;
         BVHBuilder{info[4]}.buildMesh(m);
     }
     std::vector<Texture> textures(NT);
@@ -98,7 +105,40 @@ static int run_spec(const char *spec_path, const char *out, bool gpu) {
                                 (RenderMode)fr[4]};
     projection.reset(camera, canvas.dimensions, canvas.antialias == SSAA);
     renderer.render(viewport, true, gpu);
-    return write_canvas(out, pixels, depths);
+    if (overlay_lw < 0) return write_canvas(out, pixels, depths);
+    const std::vector<Pixel> before = pixels;
+    const std::vector<f32> depths_before = depths;
+    const u8 lw = (u8)overlay_lw;
+    Box box;
+    for (u32 i = 0; i < G; i++) {
+        const Geometry &geo = geometries[i];
+        const f32 m = geo.type == GeometryType_Tet ? 0.577350259f : 1.0f;   // TET_MAX
+        box.vertices = AABB{vec3{-m}, vec3{m}};
+        if (geo.type == GeometryType_Quad) {
+            box.vertices.min.y = -EPS;
+            box.vertices.max.y = EPS;
+        }
+        drawBox(box, geo.transform, viewport, BrightYellow, 0.5f, lw);
+    }
+    drawBVH(scene.bvh, {}, viewport, 1, (u16)scene.bvh.height, 0.25f, lw);
+    for (u32 i = 0; i < G; i++) {
+        const Geometry &geo = geometries[i];
+        if (geo.type != GeometryType_Mesh) continue;
+        drawBVH(meshes[geo.id].bvh, geo.transform, viewport, 0, 3, 0.25f, lw);
+        drawMesh(meshes[geo.id], geo.transform, false, viewport, White, 0.5f, lw);
+    }
+    if (G > 1) {
+        drawBox(Box{}, geometries[1].transform, viewport, Yellow, 0.5f, 0);
+        drawBox(Box{}, geometries[1].transform, viewport, Red, 0.5f, 1, BoxSide_Left | BoxSide_Top);
+    }
+    FILE *f = fopen(out, "wb");
+    if (!f) return 1;
+    fwrite(before.data(), sizeof(Pixel), before.size(), f);
+    fwrite(depths_before.data(), sizeof(f32), depths_before.size(), f);
+    fwrite(pixels.data(), sizeof(Pixel), pixels.size(), f);
+    fwrite(depths.data(), sizeof(f32), depths.size(), f);
+    fclose(f);
+    return 0;
 }
 
 static int run_demo(const char *mesh_path, const char *out, bool gpu) {
@@ -177,6 +217,7 @@ int main(int argc, char **argv) {
     const bool gpu = !strcmp(argv[4], "gpu");
     try {
         if (!strcmp(argv[1], "spec")) return run_spec(argv[2], argv[3], gpu);
+        if (!strncmp(argv[1], "overlay", 7)) return run_spec(argv[2], argv[3], gpu, atoi(argv[1] + 7));
         if (!strcmp(argv[1], "demo")) return run_demo(argv[2], argv[3], gpu);
         if (!strcmp(argv[1], "assets")) return run_assets(argv[2], argv[3]);
     } catch (const std::exception &e) {

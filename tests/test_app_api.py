@@ -108,3 +108,36 @@ def test_cpp_bvh_round_trip():
         assets.write_bvh(src, nodes, height)
         subprocess.run([_exe(), "assets", src, out, "cpu"], check=True, timeout=120)
         assert open(out, "rb").read() == open(src, "rb").read()
+
+
+def _overlay_case(name, device):
+    """The app's overlays through the C++ API (drawBox, drawBVH, drawMesh) on the golden overlay scenes: the canvas
+    after them equals the oracle's drawing on the app's own rendered canvas, bit for bit, and the reference's
+    (golden) result where the rendered canvases agree."""
+    from tests.golden.make_golden import CASES, OVERLAY_LINE_WIDTH
+    from tests.helpers import load_golden
+    from tests.test_overlay import oracle_overlay, overlay_commands
+    spec = CASES[name]()
+    lw = OVERLAY_LINE_WIDTH[name]
+    n = spec.width * spec.height * (4 if spec.ssaa else 1)
+    with tempfile.TemporaryDirectory() as td:
+        path = write_spec(spec, os.path.join(td, "s.spec"))
+        out = os.path.join(td, "canvas.bin")
+        subprocess.run([_exe(), f"overlay{lw}", path, out, device], check=True, timeout=600)
+        raw = np.fromfile(out, np.float32)
+    before, dep0 = raw[:4 * n].reshape(n, 4), raw[4 * n:5 * n]
+    after, dep1 = raw[5 * n:9 * n].reshape(n, 4), raw[9 * n:10 * n]
+    want, want_d = oracle_overlay(spec, overlay_commands(spec, lw), before, dep0)
+    assert np.array_equal(after.view(np.uint32), want.view(np.uint32))
+    assert np.array_equal(dep1.view(np.uint32), want_d.view(np.uint32))
+    g = load_golden(name)
+    if np.array_equal(before.view(np.uint32), g["ov_in_pixels"].reshape(-1, 4).view(np.uint32)):
+        assert np.array_equal(after.view(np.uint32), g["ov_pixels"].reshape(-1, 4).view(np.uint32))
+    return before, g
+
+
+@pytest.mark.parametrize("name", ["overlay_implicit_s3_96x64_lw1", "overlay_mesh_s0_64x40_ssaa_lw2"])
+def test_app_api_overlays_cpu(name):
+    before, g = _overlay_case(name, "cpu")
+    # the CPU render itself is the reference's canvas here, so the golden comparison above ran
+    assert np.array_equal(before.view(np.uint32), g["ov_in_pixels"].reshape(-1, 4).view(np.uint32))

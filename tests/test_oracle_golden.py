@@ -20,7 +20,7 @@ def test_oracle_matches_reference_golden(name, tmp_path):
     g = load_golden(name)
     o = oracle.OracleScene(golden_spec_path(name, g, str(tmp_path)))
     for k in list(g):
-        if k.startswith(("spec", "color", "depth", "hit_")):
+        if k.startswith(("spec", "color", "depth", "hit_", "ov_")):   # ov_: the overlay canvases (test_overlay.py)
             continue
         base = k.replace("__sha256", "")
         mine = o.get(base)

@@ -12,7 +12,10 @@
  *     against committed golden vectors from it (tests/golden);
  *   - mesh BVH build: the reference's own monkey.mesh / cube.mesh assets, bit-for-bit;
  *   - intersectors, slab test, BRDF terms, tone map, bilinear/mip/cube sampling: closed forms,
- *     double-precision roots and brute force in tests/test_oracle_math.py.
+ *     double-precision roots and brute force in tests/test_oracle_math.py;
+ *   - canvas window content and the wireframe overlays (boxes, BVH node boxes, mesh edges drawn into a
+ *     rendered canvas): bit-for-bit with the compiled reference's own Canvas and draw functions on
+ *     seeded scenes (tests/test_canvas_content.py, tests/test_overlay.py) and its golden overlays.
  * Parity unpinned: none of the above covers libm ULP differences (sinf/cosf/powf/acosf); the
  * oracle calls the host libm exactly where the reference does.
  *

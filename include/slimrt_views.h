@@ -37,10 +37,11 @@ int slimrt_download_scene_bvh(slimrt_context *ctx, slimrt_bvh_node *nodes, uint3
 
 typedef struct slimrt_views slimrt_views;
 
-/* Launch shape of the multi-view kernel.  tile_shape: 0 = 16x16 samples per workgroup (8x8 per wave),
- * 1 = 32x8 (16x4), 2 = 64x4 (32x2), 3 = 8x32 (4x16).  persistent = 0: one workgroup per tile of every view
- * in one launch (XCD-aware order); persistent = 1: a grid of blocks_per_cu (1..4) workgroups per compute unit
- * takes tiles_per_grab (1..8) tiles at a time from a device work counter until all views are done.
+/* Launch shape of the multi-view kernel; a workgroup is one wave.  tile_shape: its tile of samples, 0 = 8x8,
+ * 1 = 16x4, 2 = 32x2, 3 = 4x16.  persistent = 0: one workgroup per tile of every view in one launch (tile i on
+ * workgroup i); persistent = 1: a grid of 4 * blocks_per_cu (1..4) workgroups per compute unit (blocks_per_cu
+ * waves per SIMD) takes tiles_per_grab (1..8) tiles at a time from a device work counter until all views are
+ * done.
  * node_width: the mesh BVH walk, 2 (sibling pairs) or 4 (the SLIMRT_BVH4 collapse), or 0 = as the views' flags
  * ask (4 when every view has SLIMRT_BVH4). */
 typedef struct slimrt_views_config {

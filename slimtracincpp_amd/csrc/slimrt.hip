@@ -4,11 +4,12 @@
 //   geometries 64 B | materials 64 B | lights 32 B | scene BVH nodes 32 B (+ leaf ids) | mesh records 48 B |
 //   mesh BVH nodes 32 B | triangle hit records 64 B | triangle attribute records 64 B | texture and mip records |
 //   texel quads 12 B.
-// The render kernel is a per-sample megakernel: each 256-thread workgroup owns a 16x16 tile of samples, each
-// wave an 8x8 sub-tile (coherent primary rays), and tile i is workgroup i: workgroups are dealt round-robin to
-// the 8 XCDs, so every XCD gets an even share of each image row band (band-per-XCD dealing left whole XCDs on
-// the costly bands and ran 18-20% slower).  Traversal stacks live in LDS, one column per thread, sized from the
-// uploaded BVH heights.
+// The render kernel is a per-sample megakernel: each workgroup is one wave owning an 8x8 tile of samples
+// (coherent primary rays; one-wave workgroups finish the frame 6-9% sooner than 4-wave 16x16 ones, whose waves
+// wait for the slowest of the four before the workgroup's slot frees), and tile i is workgroup i: workgroups are
+// dealt round-robin to the 8 XCDs, so every XCD gets an even share of each image row band (band-per-XCD dealing
+// left whole XCDs on the costly bands and ran 18-20% slower).  Traversal stacks live in LDS, one column per
+// thread, sized from the uploaded BVH heights.
 #include <hip/hip_runtime.h>
 #include <string>
 #include <vector>
@@ -36,21 +37,21 @@ namespace {
 // waves more than the registers (6% faster than two on the mesh + PBR frame).  The 2-wide frame would spill 64
 // dwords a lane for no gain, and the counting pass keeps its registers: both stay at the compiler's two.
 template <bool kCount, bool WIDE>
-__global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(WIDE && !kCount ? 3 : 1)))
+__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(WIDE && !kCount ? 3 : 1)))
 void render_kernel(SceneView s, RenderParams rp, float4 *__restrict__ pixels, float *__restrict__ depths,
                    uint32_t scene_depth, uint32_t mesh_depth, uint32_t tiles_x, uint32_t tiles,
                    uint32_t *__restrict__ status, unsigned long long *__restrict__ counters) {
     extern __shared__ uint32_t lds[];
     const uint32_t tile = blockIdx.x;
     if (tile >= tiles) return;
-    const uint32_t t = threadIdx.x, wave = t >> 6, lane = t & 63u;
-    const uint32_t x = (tile % tiles_x) * kTile + (wave & 1u) * 8u + (lane & 7u);
-    const uint32_t y = (tile / tiles_x) * kTile + (wave >> 1) * 8u + (lane >> 3);
+    const uint32_t lane = threadIdx.x;
+    const uint32_t x = (tile % tiles_x) * kWaveTile + (lane & 7u);
+    const uint32_t y = (tile / tiles_x) * kWaveTile + (lane >> 3);
     const uint32_t ws = rp.width << rp.ssaa, hs = rp.height << rp.ssaa;
 
     Counters cnt{};
     TracerT<WIDE> tr;
-    bind_lds_stacks(tr, lds, t, scene_depth, mesh_depth);
+    bind_lds_stacks<WIDE, 64>(tr, lds, lane, scene_depth, mesh_depth);
     tr.cnt = kCount ? &cnt : nullptr;
     if (x < ws && y < hs) {
         float px[4], depth;
@@ -194,9 +195,8 @@ int launch(slimrt_context *c, const slimrt_render_params *p, float4 *pixels, flo
     std::memcpy(&rp, p, sizeof rp);
     rp.ssaa = rp.ssaa ? 1u : 0u;
     const uint32_t ws = rp.width << rp.ssaa, hs = rp.height << rp.ssaa;
-    const uint32_t tiles_x = (ws + kTile - 1) / kTile, tiles_y = (hs + kTile - 1) / kTile;
-    const uint32_t tiles = tiles_x * tiles_y;
     const uint32_t scene_depth = c->scene_height + 1, mesh_depth = mesh_stack_depth(c, wide_mesh_bvh(rp));
+    // sized for the 256-thread workgroups of the wavefront kernels (the frame kernel needs a quarter)
     const size_t lds = stack_lds_bytes(scene_depth, mesh_depth, wide_mesh_bvh(rp));
     if (lds > 160 * 1024) return fail(c, SLIMRT_E_ARG, "BVH too deep for the LDS traversal stacks");
     if (wide_mesh_bvh(rp) && check_wide(c)) return SLIMRT_E_ARG;
@@ -207,9 +207,12 @@ int launch(slimrt_context *c, const slimrt_render_params *p, float4 *pixels, flo
                                   (rp.flags & SLIMRT_WAVEFRONT_SORT) != 0, 1u << ((rp.flags >> 4) & 3u));
         if (rc) return rc;
     } else {
+        const uint32_t wtiles_x = (ws + kWaveTile - 1) / kWaveTile;
+        const uint32_t wtiles = wtiles_x * ((hs + kWaveTile - 1) / kWaveTile);
+        const size_t wlds = stack_lds_bytes(scene_depth, mesh_depth, wide_mesh_bvh(rp), 64);
 #define RENDER_LAUNCH(COUNT, WIDE)                                                                              \
-    hipLaunchKernelGGL((render_kernel<COUNT, WIDE>), dim3(tiles), dim3(kBlock), lds, c->stream, v, rp, pixels,        \
-                       depths, scene_depth, mesh_depth, tiles_x, tiles, c->d_status, c->d_counters)
+    hipLaunchKernelGGL((render_kernel<COUNT, WIDE>), dim3(wtiles), dim3(64), wlds, c->stream, v, rp, pixels,        \
+                       depths, scene_depth, mesh_depth, wtiles_x, wtiles, c->d_status, c->d_counters)
         const bool wide = wide_mesh_bvh(rp);
         if (count) {
             if (wide) RENDER_LAUNCH(true, true); else RENDER_LAUNCH(true, false);

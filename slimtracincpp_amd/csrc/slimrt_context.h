@@ -13,6 +13,7 @@ namespace slimrt {
 
 constexpr uint32_t kBlock = 256;      // render workgroups: 4 waves of 64
 constexpr uint32_t kTile = 16;        // 16x16 samples per workgroup, 8x8 per wave
+constexpr uint32_t kWaveTile = 8;     // the frame kernel's workgroups: one wave, 8x8 samples
 
 template <typename T>
 struct DevBuf {
@@ -185,15 +186,16 @@ inline uint32_t mesh_stack_depth(const slimrt_context *c, bool wide) {
 
 // A workgroup's traversal stacks in LDS: kBlock columns of scene entries (32-bit), then kBlock columns of mesh
 // entries (32-bit pair indices, or 16-bit node indices for the 4-wide walk).
-inline size_t stack_lds_bytes(uint32_t scene_depth, uint32_t mesh_depth, bool wide) {
-    return ((size_t)scene_depth * 4 + (size_t)mesh_depth * (wide ? 2 : 4)) * kBlock;
+inline size_t stack_lds_bytes(uint32_t scene_depth, uint32_t mesh_depth, bool wide, uint32_t threads = kBlock) {
+    return ((size_t)scene_depth * 4 + (size_t)mesh_depth * (wide ? 2 : 4)) * threads;
 }
-template <bool WIDE>
+// Thread t's stacks in a workgroup of THREADS threads: one column each, entry i at i * THREADS.
+template <bool WIDE, uint32_t THREADS = kBlock>
 __device__ __forceinline__ void bind_lds_stacks(TracerT<WIDE> &tr, uint32_t *lds, uint32_t t, uint32_t scene_depth,
                                                 uint32_t mesh_depth) {
-    tr.scene_stack = StackT<uint32_t>{lds + t, kBlock};
+    tr.scene_stack = StackT<uint32_t>{lds + t, THREADS};
     tr.mesh_stack = StackT<typename TracerT<WIDE>::MeshEntry>{
-        reinterpret_cast<typename TracerT<WIDE>::MeshEntry *>(lds + scene_depth * kBlock) + t, kBlock};
+        reinterpret_cast<typename TracerT<WIDE>::MeshEntry *>(lds + scene_depth * THREADS) + t, THREADS};
     tr.scene_depth = scene_depth;
     tr.mesh_depth = mesh_depth;
     tr.overflow = 0;

@@ -145,7 +145,7 @@ __global__ void counter_reset_kernel(uint32_t *__restrict__ counter) {
 // records are read through the constant address space (invariant loads).
 template <int WX, int WY, class TR>
 __device__ __forceinline__ void trace_view_tile(const SceneView &s, const ViewRec *views, uint32_t nviews, uint32_t tile,
-                                                TR &tr, uint32_t wave, uint32_t lane) {
+                                                TR &tr, uint32_t lane) {
     typedef const __attribute__((address_space(4))) ViewRec cview;
     typedef const __attribute__((address_space(4))) uint32_t cword;
     cview *cv = (cview *)views;
@@ -164,8 +164,8 @@ __device__ __forceinline__ void trace_view_tile(const SceneView &s, const ViewRe
     const uint32_t tiles_x = cv[lo].tiles_x, local = tile - cv[lo].tile_base;
     float4 *const pixels = cv[lo].pixels;
     float *const depths = cv[lo].depths;
-    const uint32_t x = (local % tiles_x) * (2 * WX) + (wave & 1u) * WX + (lane % WX);
-    const uint32_t y = (local / tiles_x) * (2 * WY) + (wave >> 1) * WY + (lane / WX);
+    const uint32_t x = (local % tiles_x) * WX + (lane % WX);
+    const uint32_t y = (local / tiles_x) * WY + (lane / WX);
     const uint32_t ws = rp.width << rp.ssaa, hs = rp.height << rp.ssaa;
     if (x < ws && y < hs) {
         float px[4], depth;
@@ -176,36 +176,37 @@ __device__ __forceinline__ void trace_view_tile(const SceneView &s, const ViewRe
     }
 }
 
-// Multi-view trace: wave tile WX x WY (64 samples), workgroup tile 2WX x 2WY.  PERSISTENT: a grid sized to the
-// chip takes tiles from the work counter (grab at a time) until it passes the total; otherwise one workgroup
-// per tile, tile i on workgroup i (workgroups go round-robin to the 8 XCDs, so each XCD gets an even share).
+// Multi-view trace: one-wave workgroups with a WX x WY tile (64 samples), as the frame kernel (render_kernel).
+// PERSISTENT: a grid sized to the chip takes tiles from the work counter (grab at a time) until it passes the
+// total; otherwise one workgroup per tile, tile i on workgroup i (workgroups go round-robin to the 8 XCDs, so
+// each XCD gets an even share).
 // The 4-wide walk at three waves per SIMD, as the frame kernel (render_kernel).
 template <int WX, int WY, bool PERSISTENT, bool WIDE>
-__global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(WIDE ? 3 : 1)))
+__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(WIDE ? 3 : 1)))
 void views_kernel(SceneView s, const ViewRec *__restrict__ views, uint32_t nviews, uint32_t total_tiles, uint32_t grab,
                   uint32_t *__restrict__ counter, uint32_t scene_depth, uint32_t mesh_depth,
                   uint32_t *__restrict__ status) {
     static_assert(WX * WY == 64, "a wave tile holds 64 samples");
     extern __shared__ uint32_t lds[];
     __shared__ uint32_t item;
-    const uint32_t t = threadIdx.x, wave = t >> 6, lane = t & 63u;
+    const uint32_t lane = threadIdx.x;
     TracerT<WIDE> tr;
-    bind_lds_stacks(tr, lds, t, scene_depth, mesh_depth);
+    bind_lds_stacks<WIDE, 64>(tr, lds, lane, scene_depth, mesh_depth);
     tr.cnt = nullptr;
     if constexpr (PERSISTENT) {
         for (;;) {
-            __syncthreads();   // every thread has read the previous item
-            if (t == 0) item = atomicAdd(counter, grab);
+            __syncthreads();   // every lane has read the previous item
+            if (lane == 0) item = atomicAdd(counter, grab);
             __syncthreads();
             const uint32_t first = __builtin_amdgcn_readfirstlane(item);
             if (first >= total_tiles) break;
             const uint32_t last = min(first + grab, total_tiles);
-            for (uint32_t tile = first; tile < last; tile++) trace_view_tile<WX, WY>(s, views, nviews, tile, tr, wave, lane);
+            for (uint32_t tile = first; tile < last; tile++) trace_view_tile<WX, WY>(s, views, nviews, tile, tr, lane);
         }
     } else {
         const uint32_t tile = blockIdx.x;
         if (tile >= total_tiles) return;
-        trace_view_tile<WX, WY>(s, views, nviews, tile, tr, wave, lane);
+        trace_view_tile<WX, WY>(s, views, nviews, tile, tr, lane);
     }
     if (tr.overflow) atomicOr(status, 1u);
 }
@@ -305,8 +306,8 @@ int build_tables(slimrt_views *v) {
         const uint32_t ws = r.rp.width << r.rp.ssaa, hs = r.rp.height << r.rp.ssaa;
         r.pixels = v->pixels[i];
         r.depths = v->depths[i];
-        r.tiles_x = (ws + 2 * wx - 1) / (2 * wx);
-        r.tiles = r.tiles_x * ((hs + 2 * wy - 1) / (2 * wy));
+        r.tiles_x = (ws + wx - 1) / wx;
+        r.tiles = r.tiles_x * ((hs + wy - 1) / wy);
         r.tile_base = base;
         r.pad = 0;
         base += r.tiles;
@@ -327,7 +328,7 @@ int enqueue_views(slimrt_views *v) {
     const uint32_t grab = v->cfg.tiles_per_grab;
     const bool persistent = v->cfg.persistent != 0;
     const uint32_t blocks = persistent
-        ? std::max(1u, std::min((uint32_t)cus * v->cfg.blocks_per_cu, (v->total_tiles + grab - 1) / grab))
+        ? std::max(1u, std::min((uint32_t)cus * 4 * v->cfg.blocks_per_cu, (v->total_tiles + grab - 1) / grab))
         : v->total_tiles;
     bool wide = v->cfg.node_width == 4;
     if (v->cfg.node_width == 0) {   // as the views ask: the 4-wide mesh walk when every view has SLIMRT_BVH4
@@ -335,13 +336,13 @@ int enqueue_views(slimrt_views *v) {
         for (const slimrt_render_params &p : v->params) wide = wide && (p.flags & SLIMRT_BVH4) != 0;
     }
     const uint32_t scene_depth = v->scene_cap + 1, mesh_depth = mesh_stack_depth(c, wide);
-    const size_t lds = stack_lds_bytes(scene_depth, mesh_depth, wide);
-    if (lds + 16 > 160 * 1024) return vfail(v, SLIMRT_E_ARG, "BVH too deep for the LDS traversal stacks");
+    const size_t lds = stack_lds_bytes(scene_depth, mesh_depth, wide, 64);
+    if (lds + 16 > 64 * 1024) return vfail(v, SLIMRT_E_ARG, "BVH too deep for the LDS traversal stacks");
     if (wide && check_wide(c)) return vfail(v, SLIMRT_E_ARG, c->err);
     const SceneView sv = device_view(c);
     const uint32_t n = (uint32_t)v->recs.size();
 #define VIEWS_LAUNCH1(WX, WY, P, W)                                                                           \
-    hipLaunchKernelGGL((views_kernel<WX, WY, P, W>), dim3(blocks), dim3(kBlock), lds, c->stream, sv,           \
+    hipLaunchKernelGGL((views_kernel<WX, WY, P, W>), dim3(blocks), dim3(64), lds, c->stream, sv,               \
                        (const ViewRec *)v->d_recs, n, v->total_tiles, grab, v->d_counter, scene_depth, mesh_depth, \
                        c->d_status)
 #define VIEWS_LAUNCH(WX, WY)                                                                                  \

@@ -13,7 +13,8 @@ namespace slimrt {
 
 constexpr uint32_t kBlock = 256;      // render workgroups: 4 waves of 64
 constexpr uint32_t kTile = 16;        // 16x16 samples per workgroup, 8x8 per wave
-constexpr uint32_t kWaveTile = 8;     // the frame kernel's workgroups: one wave, 8x8 samples
+constexpr uint32_t kWaveTileX = 32, kWaveTileY = 2;   // the frame kernel's workgroups: one wave, 32x2 samples
+static_assert(kWaveTileX * kWaveTileY == 64, "a wave tile holds 64 samples");
 
 template <typename T>
 struct DevBuf {

@@ -47,7 +47,7 @@ BYTES_PER = {"pairs": 64, "instances": 64, "triangles": 64, "mesh_hits": 64, "ta
 OUT_BYTES_PER_SAMPLE = 20
 
 
-FORM_NAMES = ("megakernel", "wavefront", "wavefront_sorted", "wavefront_per_bounce", "wavefront_per_bounce_sorted",
+FORM_NAMES = ("megakernel", "megakernel_image_order", "wavefront", "wavefront_sorted", "wavefront_per_bounce", "wavefront_per_bounce_sorted",
               "wavefront_split", "wavefront_sorted_split")
 FORM_NAMES = FORM_NAMES + tuple(f + "_bvh4" for f in FORM_NAMES)
 
@@ -349,12 +349,13 @@ def main():
     cnt4 = r.count_work(p)                      # 4-wide mesh walk: 'pairs' counts node visits
     p.flags = 0
 
-    # Fourteen forms of the same frame (tests/test_wavefront.py, tests/test_bvh4.py): the per-sample megakernel and
-    # the bounces as a wavefront over a compacted ray queue (run to the end in one launch, or re-compacted after
+    # Sixteen forms of the same frame (tests/test_wavefront.py, tests/test_bvh4.py, tests/test_tile_order.py): the
+    # per-sample megakernel (tiles costliest first by the previous frame's durations, or in image order) and the
+    # bounces as a wavefront over a compacted ray queue (run to the end in one launch, or re-compacted after
     # every bounce; sorted by direction octant or not; first hits traced by the shading kernel or by a separate
     # traversal kernel), each with the meshes' sibling-pair BVHs or their 4-wide collapse.  Each is timed over K back-to-back frames between synchronisations; the fastest is the reported
     # path.
-    base_forms = {"megakernel": 0, "wavefront": runtime.WAVEFRONT,
+    base_forms = {"megakernel": 0, "megakernel_image_order": runtime.TILE_ORDER_FIXED, "wavefront": runtime.WAVEFRONT,
                   "wavefront_sorted": runtime.WAVEFRONT | runtime.WAVEFRONT_SORT,
                   "wavefront_per_bounce": runtime.WAVEFRONT | runtime.WAVEFRONT_PER_BOUNCE,
                   "wavefront_per_bounce_sorted": runtime.WAVEFRONT | runtime.WAVEFRONT_PER_BOUNCE | runtime.WAVEFRONT_SORT,
@@ -420,7 +421,11 @@ def main():
                    "forms_ms_per_step": {k: v["ms_per_step"] for k, v in timings.items()}},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": measured_traffic(args.width, args.height, form),
-                     "kernel": ("4-wide mesh BVHs; " if wide else "") + {"megakernel": "render_kernel<false>",
+                     "kernel": ("4-wide mesh BVHs; " if wide else "") + {
+                                "megakernel": "render_kernel<false> (tiles dispatched costliest first; their order, from "
+                                              "the tile durations of two frames before, is computed by "
+                                              "tile_order_kernel on a second stream while the previous frame renders)",
+                                "megakernel_image_order": "render_kernel<false> (tiles in image order)",
                                 "wavefront": "the frame's launches: wf_primary_kernel + one wf_bounce_kernel<true> "
                                              "over the compacted bounce queue",
                                 "wavefront_sorted": "the frame's launches: wf_primary_kernel + octant sort (count, "

@@ -34,17 +34,152 @@ static_assert(sizeof(RenderParams) == sizeof(slimrt_render_params), "render para
 namespace {
 
 
-// The 4-wide frame at three waves per SIMD (168 VGPRs, 18 dwords spilled): the walk's dependent fetches want the
-// waves more than the registers (6% faster than two on the mesh + PBR frame).  The 2-wide frame would spill 64
-// dwords a lane for no gain, and the counting pass keeps its registers: both stay at the compiler's two.
+// Dispatch order of a per-sample frame, costliest tiles first.  A frame's duration is set by its slowest waves:
+// the waves whose samples see the glass and mirror instances run 50-70x longer than the median wave, and in
+// image order they start a fifth of the way into the frame, so the frame ends with most of the chip idle while
+// they finish.  Dispatched first, they overlap the rest of the frame.  The cost of a tile is its duration two
+// frames earlier (a scene changes little from one frame to the next): the order for frame f + 1 is computed on a
+// second stream while frame f renders, from the durations frame f - 1 recorded, so it costs the frame nothing
+// (one workgroup takes ~80 us, on a CU that frees up during the frame's tail).
+//
+// One workgroup of 16 waves, each over a contiguous segment of tiles.  A tile's cost class is its duration's
+// binary exponent and the two bits below it (four classes an octave; rank 0 = costliest).  The costly tiles —
+// those within a factor 16 of the costliest class (narrowed 4 classes at a time while more than kHeavyMax), none
+// if even the costliest class alone has more — go first, by rank and within a rank in image order (a counting
+// sort over the at most 17 ranks of the window: one ballot per rank and 64-tile round); the others follow in
+// image order (ballot compaction).  Deterministic.
+constexpr uint32_t kOrderThreads = 1024, kOrderWaves = kOrderThreads / 64, kCostClasses = 128;
+constexpr uint32_t kWindow = 16;              // costly window: ranks top .. top + 16 (a factor 16 in duration)
+constexpr uint32_t kHeavyMax = 4096;          // at most this many costly tiles
+constexpr uint32_t kOrderCached = 48 * 1024;  // tiles whose rank stays in LDS between passes
+constexpr uint32_t kOrderBatch = 16;          // rounds of loads in flight together
+constexpr uint32_t kMaxOrderedTiles = 1u << 30;
+
+__device__ __forceinline__ uint32_t cost_rank(uint32_t ticks) {
+    if (ticks < 4u) return kCostClasses - 1u;
+    const uint32_t e = 31u - (uint32_t)__builtin_clz(ticks);                       // >= 2
+    return kCostClasses - 2u - ((e - 2u) * 4u + ((ticks >> (e - 2u)) & 3u));      // 7 .. 126, costliest lowest
+}
+
+__global__ __launch_bounds__(kOrderThreads) void tile_order_kernel(const uint32_t *__restrict__ cost,
+                                                                   uint32_t *__restrict__ order, uint32_t n) {
+    __shared__ uint8_t ranks[kOrderCached];
+    __shared__ uint32_t wave_count[kOrderWaves][kWindow + 1];
+    __shared__ uint32_t min_rank;
+    const uint32_t t = threadIdx.x, wave = t >> 6, lane = t & 63u;
+    const uint32_t rounds = (n + kOrderThreads - 1) / kOrderThreads;   // 64-tile rounds per wave segment
+    const uint32_t seg = wave * rounds * 64u;                          // the wave's first tile
+    const uint64_t below = (1ull << lane) - 1ull;
+    auto rank_at = [&](uint32_t i) -> uint32_t { return i < kOrderCached ? (uint32_t)ranks[i] : cost_rank(cost[i]); };
+    if (t == 0) min_rank = kCostClasses;
+    __syncthreads();
+
+    // pass 1: ranks (kept in LDS) and the costliest rank
+    uint32_t lo = kCostClasses;
+    for (uint32_t r0 = 0; r0 < rounds; r0 += kOrderBatch) {
+        uint32_t v[kOrderBatch];
+#pragma unroll
+        for (uint32_t j = 0; j < kOrderBatch; j++) {
+            const uint32_t i = seg + (r0 + j) * 64u + lane;
+            v[j] = (r0 + j < rounds && i < n) ? cost_rank(cost[i]) : kCostClasses;
+        }
+#pragma unroll
+        for (uint32_t j = 0; j < kOrderBatch; j++) {
+            const uint32_t i = seg + (r0 + j) * 64u + lane;
+            if (v[j] < kCostClasses && i < kOrderCached) ranks[i] = (uint8_t)v[j];
+            lo = min(lo, v[j]);
+        }
+    }
+    for (int off = 32; off > 0; off >>= 1) lo = min(lo, (uint32_t)__shfl_xor((int)lo, off, 64));
+    if (lane == 0) atomicMin(&min_rank, lo);
+    __syncthreads();
+    const uint32_t top = min_rank;
+
+    // pass 2: per wave, the tiles of each rank of the window
+    uint32_t cnt[kWindow + 1];
+#pragma unroll
+    for (uint32_t k = 0; k <= kWindow; k++) cnt[k] = 0;
+    for (uint32_t r = 0; r < rounds; r++) {
+        const uint32_t i = seg + r * 64u + lane;
+        const uint32_t d = i < n ? rank_at(i) - top : 0xFFu;
+#pragma unroll
+        for (uint32_t k = 0; k <= kWindow; k++) cnt[k] += (uint32_t)__popcll(__ballot(d == k));
+    }
+    if (lane == 0) {
+#pragma unroll
+        for (uint32_t k = 0; k <= kWindow; k++) wave_count[wave][k] = cnt[k];
+    }
+    __syncthreads();
+
+    // the window (every thread the same from LDS): widest of 16, 12, 8, 4, 0 ranks with <= kHeavyMax tiles
+    uint32_t total[kWindow + 1];
+#pragma unroll
+    for (uint32_t k = 0; k <= kWindow; k++) {
+        total[k] = 0;
+        for (uint32_t w = 0; w < kOrderWaves; w++) total[k] += wave_count[w][k];
+    }
+    int window = -1;
+    uint32_t heavy = 0;
+    for (int w = (int)kWindow; w >= 0; w -= 4) {
+        uint32_t h = 0;
+        for (int k = 0; k <= w; k++) h += total[k];
+        if (h <= kHeavyMax) {
+            window = w;
+            heavy = h;
+            break;
+        }
+    }
+    // this wave's first slot per rank (costly) and for its other tiles
+    uint32_t slot[kWindow + 1], run = 0, lb = heavy;
+#pragma unroll
+    for (uint32_t k = 0; k <= kWindow; k++) {
+        slot[k] = run;
+        for (uint32_t w = 0; w < wave; w++) slot[k] += wave_count[w][k];
+        run += (int)k <= window ? total[k] : 0u;
+    }
+    for (uint32_t w = 0; w < wave; w++) {
+        const uint32_t first = w * rounds * 64u, count = first < n ? min(n - first, rounds * 64u) : 0u;
+        uint32_t hw = 0;
+        for (int k = 0; k <= window; k++) hw += wave_count[w][k];
+        lb += count - hw;
+    }
+
+    // pass 3: costly tiles by rank, the others in image order
+    for (uint32_t r = 0; r < rounds; r++) {
+        const uint32_t i = seg + r * 64u + lane;
+        const uint32_t d = i < n ? rank_at(i) - top : 0xFFu;
+        const bool is_heavy = (int)d <= window, is_light = i < n && !is_heavy;
+        uint32_t pos = 0;
+#pragma unroll
+        for (uint32_t k = 0; k <= kWindow; k++) {
+            if ((int)k <= window) {   // uniform
+                const uint64_t m = __ballot(d == k);
+                if (d == k) pos = slot[k] + (uint32_t)__popcll(m & below);
+                slot[k] += (uint32_t)__popcll(m);
+            }
+        }
+        if (is_heavy) order[pos] = i;
+        const uint64_t ml = __ballot(is_light);
+        if (is_light) order[lb + (uint32_t)__popcll(ml & below)] = i;
+        lb += (uint32_t)__popcll(ml);
+    }
+}
+
+// The 4-wide frame at three waves per SIMD (168 VGPRs, 17 spilled): the walk's dependent fetches want the waves
+// more than the registers (6% faster than two on the mesh + PBR frame).  The 2-wide frame would spill 64 dwords a
+// lane for no gain, and the counting pass keeps its registers: both stay at the compiler's two.  With an order
+// (tile_order_kernel) workgroup k renders tile order[k] and records the tile's duration in cost.
 template <bool kCount, bool WIDE>
 __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(WIDE && !kCount ? 3 : 1)))
 void render_kernel(SceneView s, RenderParams rp, float4 *__restrict__ pixels, float *__restrict__ depths,
                    uint32_t scene_depth, uint32_t mesh_depth, uint32_t tiles_x, uint32_t tiles,
-                   uint32_t *__restrict__ status, unsigned long long *__restrict__ counters) {
+                   uint32_t *__restrict__ status, unsigned long long *__restrict__ counters,
+                   const uint32_t *__restrict__ order, uint32_t *__restrict__ cost) {
     extern __shared__ uint32_t lds[];
-    const uint32_t tile = blockIdx.x;
+    if (blockIdx.x >= tiles) return;
+    const uint32_t tile = order ? order[blockIdx.x] : blockIdx.x;
     if (tile >= tiles) return;
+    const uint64_t start = cost ? (uint64_t)wall_clock64() : 0u;
     const uint32_t lane = threadIdx.x;
     const uint32_t x = (tile % tiles_x) * kWaveTileX + (lane % kWaveTileX);
     const uint32_t y = (tile / tiles_x) * kWaveTileY + (lane / kWaveTileX);
@@ -61,6 +196,10 @@ void render_kernel(SceneView s, RenderParams rp, float4 *__restrict__ pixels, fl
         pixels[o] = make_float4(px[0], px[1], px[2], px[3]);
         depths[o] = depth;
         if (tr.overflow) atomicOr(status, 1u);
+    }
+    if (cost && lane == 0) {
+        const uint64_t d = (uint64_t)wall_clock64() - start;
+        cost[tile] = d > 0xFFFFFFFFull ? 0xFFFFFFFFu : (uint32_t)d;
     }
     if (kCount) {
         const uint64_t *c = &cnt.rays;
@@ -191,6 +330,33 @@ __global__ void nrc_gate_fold_kernel(float *__restrict__ cells, uint32_t ncells)
 
 namespace {
 
+// Buffers, events and the second stream of cost-ordered frames (created on first use, grown with the tile count).
+int prepare_tile_order(slimrt_context *c, uint32_t tiles) {
+    if (!c->side_stream) HIP_TRY(c, hipStreamCreateWithFlags(&c->side_stream, hipStreamNonBlocking));
+    for (int p = 0; p < 2; p++) {
+        if (!c->tile_rendered[p]) HIP_TRY(c, hipEventCreateWithFlags(&c->tile_rendered[p], hipEventDisableTiming));
+        if (!c->tile_ordered[p]) HIP_TRY(c, hipEventCreateWithFlags(&c->tile_ordered[p], hipEventDisableTiming));
+    }
+    if (c->tile_cap >= tiles) return SLIMRT_OK;
+    HIP_TRY(c, hipStreamSynchronize(c->side_stream));
+    HIP_TRY(c, hipStreamSynchronize(c->stream));
+    for (int p = 0; p < 2; p++) {
+        for (uint32_t **b : {&c->tile_cost[p], &c->tile_order[p]}) {
+            if (*b) HIP_TRY(c, hipFree(*b));
+            *b = nullptr;
+        }
+        c->cost_tiles[p] = c->order_tiles[p] = 0;
+        c->order_launched[p] = c->order_used[p] = false;
+    }
+    c->tile_cap = 0;
+    for (int p = 0; p < 2; p++) {
+        HIP_TRY(c, hipMalloc(&c->tile_cost[p], tiles * sizeof(uint32_t)));
+        HIP_TRY(c, hipMalloc(&c->tile_order[p], tiles * sizeof(uint32_t)));
+    }
+    c->tile_cap = tiles;
+    return SLIMRT_OK;
+}
+
 int launch(slimrt_context *c, const slimrt_render_params *p, float4 *pixels, float *depths, bool count) {
     RenderParams rp;
     std::memcpy(&rp, p, sizeof rp);
@@ -211,9 +377,21 @@ int launch(slimrt_context *c, const slimrt_render_params *p, float4 *pixels, flo
         const uint32_t wtiles_x = (ws + kWaveTileX - 1) / kWaveTileX;
         const uint32_t wtiles = wtiles_x * ((hs + kWaveTileY - 1) / kWaveTileY);
         const size_t wlds = stack_lds_bytes(scene_depth, mesh_depth, wide_mesh_bvh(rp), 64);
+        const bool by_cost = !count && !(rp.flags & SLIMRT_TILE_ORDER_FIXED) && wtiles < kMaxOrderedTiles;
+        const uint32_t par = (uint32_t)(c->tile_frame & 1u);
+        uint32_t *order = nullptr, *cost = nullptr;
+        if (by_cost) {
+            if (int rc = prepare_tile_order(c, wtiles)) return rc;
+            // an order kernel still reading this parity's costs must finish before they are overwritten
+            if (c->order_launched[par]) HIP_TRY(c, hipStreamWaitEvent(c->stream, c->tile_ordered[par], 0));
+            c->order_launched[par] = false;
+            c->order_used[par] = c->order_tiles[par] == wtiles;
+            order = c->order_used[par] ? c->tile_order[par] : nullptr;
+            cost = c->tile_cost[par];
+        }
 #define RENDER_LAUNCH(COUNT, WIDE)                                                                              \
     hipLaunchKernelGGL((render_kernel<COUNT, WIDE>), dim3(wtiles), dim3(64), wlds, c->stream, v, rp, pixels,        \
-                       depths, scene_depth, mesh_depth, wtiles_x, wtiles, c->d_status, c->d_counters)
+                       depths, scene_depth, mesh_depth, wtiles_x, wtiles, c->d_status, c->d_counters, order, cost)
         const bool wide = wide_mesh_bvh(rp);
         if (count) {
             if (wide) RENDER_LAUNCH(true, true); else RENDER_LAUNCH(true, false);
@@ -221,6 +399,24 @@ int launch(slimrt_context *c, const slimrt_render_params *p, float4 *pixels, flo
             if (wide) RENDER_LAUNCH(false, true); else RENDER_LAUNCH(false, false);
         }
 #undef RENDER_LAUNCH
+        HIP_TRY(c, hipGetLastError());
+        if (by_cost) {
+            HIP_TRY(c, hipEventRecord(c->tile_rendered[par], c->stream));
+            c->cost_tiles[par] = wtiles;
+            c->tile_frame++;
+            // the next frame's order, from the previous frame's costs, overlapping this frame
+            const uint32_t nxt = par ^ 1u;
+            c->order_tiles[nxt] = 0;
+            if (c->cost_tiles[nxt] == wtiles) {
+                HIP_TRY(c, hipStreamWaitEvent(c->side_stream, c->tile_rendered[nxt], 0));
+                hipLaunchKernelGGL(tile_order_kernel, dim3(1), dim3(kOrderThreads), 0, c->side_stream,
+                                   c->tile_cost[nxt], c->tile_order[nxt], wtiles);
+                HIP_TRY(c, hipGetLastError());
+                HIP_TRY(c, hipEventRecord(c->tile_ordered[nxt], c->side_stream));
+                c->order_launched[nxt] = true;
+                c->order_tiles[nxt] = wtiles;
+            }
+        }
     }
     HIP_TRY(c, hipGetLastError());
     if (c->timed) HIP_TRY(c, hipEventRecord(c->ev1, c->stream));
@@ -477,6 +673,13 @@ int slimrt_destroy(slimrt_context *c) {
         if (c->side_stream) (void)hipStreamDestroy(c->side_stream);
         if (c->d_status) (void)hipFree(c->d_status);
         if (c->d_counters) (void)hipFree(c->d_counters);
+        if (c->side_stream) (void)hipStreamSynchronize(c->side_stream);
+        for (int p = 0; p < 2; p++) {
+            if (c->tile_cost[p]) (void)hipFree(c->tile_cost[p]);
+            if (c->tile_order[p]) (void)hipFree(c->tile_order[p]);
+            if (c->tile_rendered[p]) (void)hipEventDestroy(c->tile_rendered[p]);
+            if (c->tile_ordered[p]) (void)hipEventDestroy(c->tile_ordered[p]);
+        }
         if (c->ev0) (void)hipEventDestroy(c->ev0);
         if (c->ev1) (void)hipEventDestroy(c->ev1);
         if (c->own_stream) (void)hipStreamDestroy(c->own_stream);
@@ -774,6 +977,23 @@ int slimrt_wavefront_counts(slimrt_context *c, uint32_t *counts, uint32_t n) {
     if (c->cpu || !c->wf_stats) return fail(c, SLIMRT_E_STATE, "no frame rendered with SLIMRT_WAVEFRONT_COUNT");
     HIP_TRY(c, hipMemcpyAsync(counts, c->wf_stats, kWfStats * sizeof(uint32_t), hipMemcpyDeviceToHost, c->stream));
     HIP_TRY(c, hipStreamSynchronize(c->stream));
+    return SLIMRT_OK;
+}
+
+int slimrt_tile_costs(slimrt_context *c, uint32_t *costs, uint32_t *order, uint32_t cap, uint32_t *tile_count) {
+    if (!c || !tile_count) return SLIMRT_E_ARG;
+    const uint32_t last = (uint32_t)((c->tile_frame + 1u) & 1u);
+    const uint32_t n = (c->cpu || c->tile_frame == 0) ? 0u : c->cost_tiles[last];
+    *tile_count = n;
+    if (n == 0) return SLIMRT_OK;
+    if ((costs || order) && cap < n) return fail(c, SLIMRT_E_ARG, "cap is smaller than the tile count");
+    HIP_TRY(c, hipStreamSynchronize(c->stream));
+    if (costs) HIP_TRY(c, hipMemcpy(costs, c->tile_cost[last], n * sizeof(uint32_t), hipMemcpyDeviceToHost));
+    if (order) {
+        if (c->order_used[last]) HIP_TRY(c, hipMemcpy(order, c->tile_order[last], n * sizeof(uint32_t), hipMemcpyDeviceToHost));
+        else
+            for (uint32_t i = 0; i < n; i++) order[i] = i;
+    }
     return SLIMRT_OK;
 }
 

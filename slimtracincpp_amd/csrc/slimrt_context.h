@@ -80,6 +80,14 @@ struct slimrt_context {
     hipEvent_t wf_fork = nullptr, wf_join = nullptr, wf_band[8] = {};
     uint32_t *d_status = nullptr;
     unsigned long long *d_counters = nullptr;
+    // per-sample frames dispatched by cost, double-buffered by frame parity: frame f renders in order
+    // tile_order[f & 1] (computed on side_stream during frame f - 1 from the costs of frame f - 2) and writes its
+    // tiles' durations (wall-clock ticks) to tile_cost[f & 1].  *_tiles: the tile count a buffer holds (0: none).
+    uint32_t *tile_cost[2] = {}, *tile_order[2] = {};
+    uint32_t tile_cap = 0, cost_tiles[2] = {}, order_tiles[2] = {};
+    bool order_launched[2] = {}, order_used[2] = {};
+    uint64_t tile_frame = 0;
+    hipEvent_t tile_rendered[2] = {}, tile_ordered[2] = {};
 
     // rendering with the cache: queues, training samples, gate table, frame counters
     float *q_raw = nullptr, *q_prefix = nullptr, *q_tp = nullptr, *q_out = nullptr;

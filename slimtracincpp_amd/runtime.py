@@ -99,6 +99,7 @@ class ViewsConfig(C.Structure):
 
 SCENE_REFIT, SCENE_REBUILD = 0, 1
 WAVEFRONT, WAVEFRONT_PER_BOUNCE, WAVEFRONT_SORT, BVH4, WAVEFRONT_COUNT, WAVEFRONT_SPLIT = 2, 4, 8, 64, 128, 256   # flags
+TILE_ORDER_FIXED = 512   # per-sample frames: tiles in image order instead of costliest first
 
 
 def WAVEFRONT_BANDS(log2n: int) -> int:
@@ -150,6 +151,7 @@ def lib():
             "slimrt_render_autotune": [vp, P(RenderParams), vp, vp, u32, P(u32), P(f32)],
             "slimrt_host_collapse_bvh4": [vp, u32, u32, vp, u32, P(u32), P(u32), P(u32)],
             "slimrt_wavefront_counts": [vp, vp, u32],
+            "slimrt_tile_costs": [vp, vp, vp, u32, P(u32)],
             "slimrt_pick": [vp, P(RenderParams), i32, i32, P(PickResult)],
             "slimrt_canvas_content": [vp, vp, u32, u32, u32, vp],
             "slimrt_host_bake_texture": [vp, u32, u32, u32, vp, C.c_size_t, P(C.c_size_t), P(u32), vp, P(u32)],
@@ -486,6 +488,16 @@ class Renderer:
         c = np.zeros(34, np.uint32)
         _check(lib().slimrt_wavefront_counts(self.ctx, _ptr(c), len(c)), self.ctx)
         return c
+
+    def tile_costs(self):
+        """Per-tile durations (wall-clock ticks) of the last per-sample frame dispatched by cost, and that frame's
+        dispatch order (order[k] = the tile workgroup k rendered); empty arrays before the first such frame."""
+        n = u32(0)
+        _check(lib().slimrt_tile_costs(self.ctx, None, None, 0, C.byref(n)), self.ctx)
+        costs, order = np.zeros(n.value, np.uint32), np.zeros(n.value, np.uint32)
+        if n.value:
+            _check(lib().slimrt_tile_costs(self.ctx, _ptr(costs), _ptr(order), n.value, C.byref(n)), self.ctx)
+        return costs, order
 
     def update_scene_device(self, geo_f: np.ndarray, mode: int = SCENE_REFIT):
         """New transforms for every geometry; world AABBs and the scene BVH (refit or rebuild) on the device."""

@@ -114,6 +114,22 @@ def test_render_before_upload_is_state_error():
     runtime.lib().slimrt_destroy(ctx)
 
 
+def test_cpu_context_has_no_tile_costs():
+    """The CPU context renders rows on host threads: no tiles, no costs; the tile-order flag changes nothing."""
+    spec = scenes.implicit_scene(0, 48, 27)
+    r = runtime.Renderer(spec, "cpu")
+    p = r.params()
+    a = r.render(p)
+    p.flags = runtime.TILE_ORDER_FIXED
+    b = r.render(p)
+    assert all(np.array_equal(x, y) for x, y in zip(a, b))
+    costs, order = r.tile_costs()
+    assert costs.size == 0 and order.size == 0
+    n = ctypes.c_uint32(7)
+    assert runtime.lib().slimrt_tile_costs(r.ctx, None, None, 0, ctypes.byref(n)) == 0 and n.value == 0
+    assert runtime.lib().slimrt_tile_costs(r.ctx, None, None, 0, None) == -1   # SLIMRT_E_ARG
+
+
 def test_invalid_uploads_are_rejected():
     spec = scenes.implicit_scene(0, 32, 18)
     sb = runtime.build_scene(spec)

@@ -1,0 +1,131 @@
+"""Per-sample frames dispatch their tiles costliest first, by the tile durations the context recorded two frames
+earlier at the same tile count (the order is computed on a second stream while the frame before renders; render
+flag SLIMRT_TILE_ORDER_FIXED keeps the image order).  Every tile must be rendered exactly once whatever the order
+(canvases filled with NaN before each frame), frames must equal the image-order frame bit for bit, the first two
+frames at a tile count must use the image order, and later orders must equal the order a numpy model derives from
+the durations of two frames before (costly tiles sorted first, the rest in image order)."""
+import numpy as np
+import pytest
+
+from slimtracincpp_amd import scenes, runtime
+
+pytestmark = pytest.mark.gpu
+
+
+def _frame(r, p):
+    import torch
+    n = p.width * p.height * (4 if p.ssaa else 1)
+    px = torch.full((n, 4), float("nan"), dtype=torch.float32, device="cuda")
+    dp = torch.full((n,), float("nan"), dtype=torch.float32, device="cuda")
+    r.render_device(p, px.data_ptr(), dp.data_ptr())
+    r.synchronize()
+    torch.cuda.synchronize()
+    return px.cpu().numpy().view(np.uint32), dp.cpu().numpy().view(np.uint32)
+
+
+def _tiles(p):
+    s = 2 if p.ssaa else 1
+    return ((p.width * s + 31) // 32) * ((p.height * s + 1) // 2)
+
+
+def _cost_class(ticks: np.ndarray) -> np.ndarray:
+    """The cost class of a duration: binary exponent and the two bits below it (higher = costlier)."""
+    t = ticks.astype(np.int64)
+    e = np.where(t >= 4, np.floor(np.log2(np.maximum(t, 1))).astype(np.int64), 0)
+    return np.where(t >= 4, (e - 2) * 4 + ((t >> np.maximum(e - 2, 0)) & 3) + 1, 0)
+
+
+def _expected_order(costs: np.ndarray) -> np.ndarray:
+    """The dispatch order a frame takes from the previous frame's tile durations: the tiles within a factor 16 of
+    the costliest class (narrowed four classes at a time while more than 4096; none if the costliest class alone
+    has more), costliest class first and by tile index within a class, then every other tile in image order."""
+    rank = 127 - _cost_class(costs)
+    idx = np.arange(costs.size)
+    top = rank.min()
+    heavy = np.zeros(costs.size, bool)
+    for window in (16, 12, 8, 4, 0):
+        h = rank <= top + window
+        if h.sum() <= 4096:
+            heavy = h
+            break
+    first = idx[heavy][np.lexsort((idx[heavy], rank[heavy]))]
+    return np.concatenate([first, idx[~heavy]])
+
+
+def test_expected_order_model():
+    costs = np.array([5, 900, 40, 1000, 7, 64, 950, 3, 0, 33], np.uint32)
+    # 900, 950 and 1000 share the costliest class; 64 lies within a factor 16 below it, 40 and less further
+    assert _expected_order(costs).tolist() == [1, 3, 6, 5, 0, 2, 4, 7, 8, 9]
+    flat = np.full(5000, 100, np.uint32)              # one class larger than the cap: image order
+    assert np.array_equal(_expected_order(flat), np.arange(5000))
+
+
+def test_cost_class_is_monotonic():
+    t = np.arange(0, 1 << 20, dtype=np.int64)
+    k = _cost_class(t)
+    assert np.all(np.diff(k) >= 0) and k[3] == 0 and k[4] == 1 and k[8] == 5
+
+
+@pytest.mark.parametrize("flags", [runtime.BVH4, 0])
+def test_cost_order_renders_every_tile_once(gpu_available, flags):
+    spec = scenes.mesh_pbr_scene(0, 200, 120, level=4, texture_size=64, sky_size=16, irradiance_size=8)
+    r = runtime.Renderer(spec, "gpu")
+    p = r.params()
+    p.flags = flags | runtime.TILE_ORDER_FIXED
+    ref = _frame(r, p)
+    assert r.tile_costs()[0].size == 0          # image-order frames record no costs
+    p.flags = flags
+    n = _tiles(p)
+    history = []
+    for f in range(6):
+        got = _frame(r, p)
+        assert all(np.array_equal(a, b) for a, b in zip(got, ref))
+        costs, order = r.tile_costs()
+        assert costs.size == n and np.all(costs > 0)
+        assert np.array_equal(np.sort(order), np.arange(n))           # a permutation of the tiles
+        # frame f is ordered by the durations of frame f - 2 (its order is computed while frame f - 1 renders)
+        expected = _expected_order(history[f - 2]) if f >= 2 else np.arange(n)
+        assert np.array_equal(order, expected), f
+        history.append(costs)
+
+
+def test_cost_order_follows_tile_count_changes(gpu_available):
+    spec = scenes.implicit_scene(1, 120, 72, max_depth=5, transparent=True)
+    r = runtime.Renderer(spec, "gpu")
+    p = r.params(output=runtime.OUTPUT_RADIANCE)
+    p.flags = runtime.TILE_ORDER_FIXED
+    ref = _frame(r, p)
+    p.flags = 0
+    for _ in range(3):
+        got = _frame(r, p)
+        assert all(np.array_equal(a, b) for a, b in zip(got, ref))
+    # SSAA quadruples the tiles: the first two frames at the new count use the image order again
+    p.ssaa = 1
+    p.flags = runtime.TILE_ORDER_FIXED
+    ref_ssaa = _frame(r, p)
+    p.flags = 0
+    n = _tiles(p)
+    history = []
+    for f in range(4):
+        got = _frame(r, p)
+        assert all(np.array_equal(a, b) for a, b in zip(got, ref_ssaa))
+        costs, order = r.tile_costs()
+        assert order.size == n
+        assert np.array_equal(order, _expected_order(history[f - 2]) if f >= 2 else np.arange(n)), f
+        history.append(costs)
+    # back to the first tile count
+    p.ssaa = 0
+    for f in range(3):
+        got = _frame(r, p)
+        assert all(np.array_equal(a, b) for a, b in zip(got, ref))
+    assert np.array_equal(np.sort(r.tile_costs()[1]), np.arange(_tiles(p)))
+
+
+def test_counting_pass_ignores_the_order(gpu_available):
+    spec = scenes.mesh_pbr_scene(0, 96, 54, level=3, texture_size=64, sky_size=16, irradiance_size=8)
+    r = runtime.Renderer(spec, "gpu")
+    p = r.params()
+    a = r.count_work(p)
+    _frame(r, p)
+    _frame(r, p)
+    assert r.count_work(p) == a

@@ -54,6 +54,7 @@ constexpr uint32_t kHeavyMax = 4096;          // at most this many costly tiles
 constexpr uint32_t kOrderCached = 48 * 1024;  // tiles whose rank stays in LDS between passes
 constexpr uint32_t kOrderBatch = 16;          // rounds of loads in flight together
 constexpr uint32_t kMaxOrderedTiles = 1u << 30;
+constexpr uint32_t kSplitShare = 32, kSplitMax = 2048;   // costliest tiles rendered as two half waves: 1 in 32
 
 __device__ __forceinline__ uint32_t cost_rank(uint32_t ticks) {
     if (ticks < 4u) return kCostClasses - 1u;
@@ -168,16 +169,25 @@ __global__ __launch_bounds__(kOrderThreads) void tile_order_kernel(const uint32_
 // The 4-wide frame at three waves per SIMD (168 VGPRs, 17 spilled): the walk's dependent fetches want the waves
 // more than the registers (6% faster than two on the mesh + PBR frame).  The 2-wide frame would spill 64 dwords a
 // lane for no gain, and the counting pass keeps its registers: both stay at the compiler's two.  With an order
-// (tile_order_kernel) workgroup k renders tile order[k] and records the tile's duration in cost.
+// (tile_order_kernel) workgroup k renders tile order[k] and records the tile's duration in cost; the first `split`
+// tiles of the order (the costliest) are each rendered by two workgroups, lanes 0-31 and lanes 32-63: a costly
+// tile's 64 paths diverge, and half as many lanes per wave shorten the frame's longest waves (0.75 -> 0.64 ms on
+// the mesh + PBR frame; quarter waves for fewer tiles gain little more and lose fast as the count grows).
 template <bool kCount, bool WIDE>
 __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(WIDE && !kCount ? 3 : 1)))
 void render_kernel(SceneView s, RenderParams rp, float4 *__restrict__ pixels, float *__restrict__ depths,
                    uint32_t scene_depth, uint32_t mesh_depth, uint32_t tiles_x, uint32_t tiles,
                    uint32_t *__restrict__ status, unsigned long long *__restrict__ counters,
-                   const uint32_t *__restrict__ order, uint32_t *__restrict__ cost) {
+                   const uint32_t *__restrict__ order, uint32_t *__restrict__ cost, uint32_t split) {
     extern __shared__ uint32_t lds[];
-    if (blockIdx.x >= tiles) return;
-    const uint32_t tile = order ? order[blockIdx.x] : blockIdx.x;
+    if (blockIdx.x >= tiles + split) return;
+    uint32_t tile = blockIdx.x, half = 2;   // half 0 / 1: lanes 0-31 / 32-63 only; 2: all lanes
+    if (order) {
+        if (blockIdx.x < 2 * split) {
+            tile = order[blockIdx.x >> 1];
+            half = blockIdx.x & 1u;
+        } else tile = order[blockIdx.x - split];
+    }
     if (tile >= tiles) return;
     const uint64_t start = cost ? (uint64_t)wall_clock64() : 0u;
     const uint32_t lane = threadIdx.x;
@@ -189,7 +199,7 @@ void render_kernel(SceneView s, RenderParams rp, float4 *__restrict__ pixels, fl
     TracerT<WIDE> tr;
     bind_lds_stacks<WIDE, 64>(tr, lds, lane, scene_depth, mesh_depth);
     tr.cnt = kCount ? &cnt : nullptr;
-    if (x < ws && y < hs) {
+    if ((half == 2u || (lane >> 5) == half) && x < ws && y < hs) {
         float px[4], depth;
         render_sample(s, rp, x, y, tr, px, depth);
         const uint32_t o = canvas_offset(rp, x, y);
@@ -197,7 +207,7 @@ void render_kernel(SceneView s, RenderParams rp, float4 *__restrict__ pixels, fl
         depths[o] = depth;
         if (tr.overflow) atomicOr(status, 1u);
     }
-    if (cost && lane == 0) {
+    if (cost && lane == 0) {   // a split tile keeps the duration of the half that ends last
         const uint64_t d = (uint64_t)wall_clock64() - start;
         cost[tile] = d > 0xFFFFFFFFull ? 0xFFFFFFFFu : (uint32_t)d;
     }
@@ -389,9 +399,11 @@ int launch(slimrt_context *c, const slimrt_render_params *p, float4 *pixels, flo
             order = c->order_used[par] ? c->tile_order[par] : nullptr;
             cost = c->tile_cost[par];
         }
+        const uint32_t split = order ? std::min(wtiles / kSplitShare, kSplitMax) : 0u;
 #define RENDER_LAUNCH(COUNT, WIDE)                                                                              \
-    hipLaunchKernelGGL((render_kernel<COUNT, WIDE>), dim3(wtiles), dim3(64), wlds, c->stream, v, rp, pixels,        \
-                       depths, scene_depth, mesh_depth, wtiles_x, wtiles, c->d_status, c->d_counters, order, cost)
+    hipLaunchKernelGGL((render_kernel<COUNT, WIDE>), dim3(wtiles + split), dim3(64), wlds, c->stream, v, rp,        \
+                       pixels, depths, scene_depth, mesh_depth, wtiles_x, wtiles, c->d_status, c->d_counters, order,  \
+                       cost, split)
         const bool wide = wide_mesh_bvh(rp);
         if (count) {
             if (wide) RENDER_LAUNCH(true, true); else RENDER_LAUNCH(true, false);

@@ -66,9 +66,12 @@ extern "C" {
                                        (fewer registers, more waves in flight) that hands hit records to the
                                        shading kernel through device memory */
 #define SLIMRT_TILE_ORDER_FIXED 512u /* per-sample frames (no SLIMRT_WAVEFRONT): dispatch the sample tiles in image
-                                        order.  Without it a frame dispatches its tiles costliest first, by the
-                                        time each tile took in the context's previous frame of the same tile count
-                                        (image order for the first); results are the same either way */
+                                        order.  Without it a frame whose longest tiles bound its length (the longest
+                                        tile two frames before lasted at least 3/4 of that frame's summed tile
+                                        durations over the resident waves) dispatches its costliest tiles first and
+                                        renders the first 1 in 32 of them as two half waves; other frames, and the
+                                        first two at a tile count, keep the image order.  Results are the same
+                                        either way */
 
 typedef struct slimrt_context slimrt_context;
 
@@ -278,14 +281,16 @@ int slimrt_pick(slimrt_context *ctx, const slimrt_render_params *params, int32_t
  * n >= 34.  Synchronizes the context's stream. */
 int slimrt_wavefront_counts(slimrt_context *ctx, uint32_t *counts, uint32_t n);
 
-/* The tile costs of the last per-sample frame dispatched by cost (no counterpart in the reference; see
- * SLIMRT_TILE_ORDER_FIXED).  A tile is the 32 x 2 samples one wave renders, tiles numbered in image order (row
- * major over ceil(w / 32) x ceil(h / 2) tiles of the sample grid, w and h doubled with SSAA).  costs[i] = tile
- * i's wall-clock duration in ticks of the device's 100 MHz real-time counter; order[k] = the tile that went to
- * workgroup k (tiles of equal cost class in no fixed order).  Either pointer may be NULL; *tile_count receives
- * the number of tiles (0 before the first such frame); SLIMRT_E_ARG when cap is smaller.  Synchronizes the
- * context's stream. */
-int slimrt_tile_costs(slimrt_context *ctx, uint32_t *costs, uint32_t *order, uint32_t cap, uint32_t *tile_count);
+/* The tile costs of the last per-sample frame rendered without SLIMRT_TILE_ORDER_FIXED (no counterpart in the
+ * reference).  A tile is the 32 x 2 samples one wave renders, tiles numbered in image order (row major over
+ * ceil(w / 32) x ceil(h / 2) tiles of the sample grid, w and h doubled with SSAA).  costs[i] = tile i's wall-clock
+ * duration in ticks of the device's 100 MHz real-time counter (a tile rendered as two half waves: the half that
+ * ended last); order[k] = the tile dispatched k-th (the costliest first, or the image order).  Either pointer may
+ * be NULL; *tile_count receives the number of tiles (0 before the first such frame); SLIMRT_E_ARG when cap is
+ * smaller.  resident (may be NULL): the frame kernel's waves the device holds at once, which the choice of order
+ * weighs the durations against.  Synchronizes the context's stream. */
+int slimrt_tile_costs(slimrt_context *ctx, uint32_t *costs, uint32_t *order, uint32_t cap, uint32_t *tile_count,
+                      uint32_t *resident);
 
 /* The reference's texture converter (src/bmp2texture.cpp:24-361): the texel quads of an RGB image as it writes them
  * into .texture files.  rgb: width * height * 3 bytes, rows top to bottom.  flags: ImageFlags bits

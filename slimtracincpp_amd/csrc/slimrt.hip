@@ -63,38 +63,70 @@ __device__ __forceinline__ uint32_t cost_rank(uint32_t ticks) {
 }
 
 __global__ __launch_bounds__(kOrderThreads) void tile_order_kernel(const uint32_t *__restrict__ cost,
-                                                                   uint32_t *__restrict__ order, uint32_t n) {
+                                                                   uint32_t *__restrict__ order, uint32_t n,
+                                                                   uint32_t resident) {
     __shared__ uint8_t ranks[kOrderCached];
     __shared__ uint32_t wave_count[kOrderWaves][kWindow + 1];
-    __shared__ uint32_t min_rank;
+    __shared__ uint32_t min_rank, max_cost;
+    __shared__ unsigned long long wave_sum[kOrderWaves];
     const uint32_t t = threadIdx.x, wave = t >> 6, lane = t & 63u;
     const uint32_t rounds = (n + kOrderThreads - 1) / kOrderThreads;   // 64-tile rounds per wave segment
     const uint32_t seg = wave * rounds * 64u;                          // the wave's first tile
     const uint64_t below = (1ull << lane) - 1ull;
     auto rank_at = [&](uint32_t i) -> uint32_t { return i < kOrderCached ? (uint32_t)ranks[i] : cost_rank(cost[i]); };
-    if (t == 0) min_rank = kCostClasses;
+    if (t == 0) {
+        min_rank = kCostClasses;
+        max_cost = 0;
+    }
     __syncthreads();
 
-    // pass 1: ranks (kept in LDS) and the costliest rank
-    uint32_t lo = kCostClasses;
+    // pass 1: ranks (kept in LDS), the costliest rank, the longest and the summed durations
+    uint32_t lo = kCostClasses, longest = 0;
+    uint64_t sum = 0;
     for (uint32_t r0 = 0; r0 < rounds; r0 += kOrderBatch) {
         uint32_t v[kOrderBatch];
 #pragma unroll
         for (uint32_t j = 0; j < kOrderBatch; j++) {
             const uint32_t i = seg + (r0 + j) * 64u + lane;
-            v[j] = (r0 + j < rounds && i < n) ? cost_rank(cost[i]) : kCostClasses;
+            v[j] = (r0 + j < rounds && i < n) ? cost[i] : 0u;
         }
 #pragma unroll
         for (uint32_t j = 0; j < kOrderBatch; j++) {
             const uint32_t i = seg + (r0 + j) * 64u + lane;
-            if (v[j] < kCostClasses && i < kOrderCached) ranks[i] = (uint8_t)v[j];
-            lo = min(lo, v[j]);
+            if (r0 + j < rounds && i < n) {
+                const uint32_t rk = cost_rank(v[j]);
+                if (i < kOrderCached) ranks[i] = (uint8_t)rk;
+                lo = min(lo, rk);
+                longest = max(longest, v[j]);
+                sum += v[j];
+            }
         }
     }
-    for (int off = 32; off > 0; off >>= 1) lo = min(lo, (uint32_t)__shfl_xor((int)lo, off, 64));
-    if (lane == 0) atomicMin(&min_rank, lo);
+    for (int off = 32; off > 0; off >>= 1) {
+        lo = min(lo, (uint32_t)__shfl_xor((int)lo, off, 64));
+        longest = max(longest, (uint32_t)__shfl_xor((int)longest, off, 64));
+        sum += (uint64_t)__shfl_xor((long long)sum, off, 64);
+    }
+    if (lane == 0) {
+        atomicMin(&min_rank, lo);
+        atomicMax(&max_cost, longest);
+        wave_sum[wave] = sum;
+    }
     __syncthreads();
     const uint32_t top = min_rank;
+
+    // Longest first only pays when the frame is bound by its longest tiles: when the longest tile lasts at least
+    // 3/4 of the frame's ideal length (summed durations over the resident waves).  Otherwise (large frames whose
+    // work fills the chip for longer than any one tile) the image order keeps neighbouring tiles together, which
+    // keeps what they share in L2 (4K combined workload: 3.81 ms in image order, 4.14 ms in cost order).
+    uint64_t summed = 0;
+    for (uint32_t w = 0; w < kOrderWaves; w++) summed += wave_sum[w];
+    if (4ull * max_cost * resident <= 3ull * summed) {
+        for (uint32_t i = t; i < n; i += kOrderThreads) order[i] = i;
+        if (t == 0) order[n] = 0;   // no split of the first tiles either
+        return;
+    }
+    if (t == 0) order[n] = 1;
 
     // pass 2: per wave, the tiles of each rank of the window
     uint32_t cnt[kWindow + 1];
@@ -169,8 +201,8 @@ __global__ __launch_bounds__(kOrderThreads) void tile_order_kernel(const uint32_
 // The 4-wide frame at three waves per SIMD (168 VGPRs, 17 spilled): the walk's dependent fetches want the waves
 // more than the registers (6% faster than two on the mesh + PBR frame).  The 2-wide frame would spill 64 dwords a
 // lane for no gain, and the counting pass keeps its registers: both stay at the compiler's two.  With an order
-// (tile_order_kernel) workgroup k renders tile order[k] and records the tile's duration in cost; the first `split`
-// tiles of the order (the costliest) are each rendered by two workgroups, lanes 0-31 and lanes 32-63: a costly
+// (tile_order_kernel) workgroup k renders tile order[k] and records the tile's duration in cost; when order[tiles]
+// is set, the first `split` tiles of the order (the costliest) are each rendered by two workgroups, lanes 0-31 and lanes 32-63: a costly
 // tile's 64 paths diverge, and half as many lanes per wave shorten the frame's longest waves (0.75 -> 0.64 ms on
 // the mesh + PBR frame; quarter waves for fewer tiles gain little more and lose fast as the count grows).
 template <bool kCount, bool WIDE>
@@ -180,13 +212,14 @@ void render_kernel(SceneView s, RenderParams rp, float4 *__restrict__ pixels, fl
                    uint32_t *__restrict__ status, unsigned long long *__restrict__ counters,
                    const uint32_t *__restrict__ order, uint32_t *__restrict__ cost, uint32_t split) {
     extern __shared__ uint32_t lds[];
-    if (blockIdx.x >= tiles + split) return;
     uint32_t tile = blockIdx.x, half = 2;   // half 0 / 1: lanes 0-31 / 32-63 only; 2: all lanes
     if (order) {
-        if (blockIdx.x < 2 * split) {
+        const uint32_t sp = order[tiles] ? split : 0u;   // order[tiles] = 0: the order kernel chose image order
+        if (blockIdx.x >= tiles + sp) return;
+        if (blockIdx.x < 2 * sp) {
             tile = order[blockIdx.x >> 1];
             half = blockIdx.x & 1u;
-        } else tile = order[blockIdx.x - split];
+        } else tile = order[blockIdx.x - sp];
     }
     if (tile >= tiles) return;
     const uint64_t start = cost ? (uint64_t)wall_clock64() : 0u;
@@ -361,9 +394,25 @@ int prepare_tile_order(slimrt_context *c, uint32_t tiles) {
     c->tile_cap = 0;
     for (int p = 0; p < 2; p++) {
         HIP_TRY(c, hipMalloc(&c->tile_cost[p], tiles * sizeof(uint32_t)));
-        HIP_TRY(c, hipMalloc(&c->tile_order[p], tiles * sizeof(uint32_t)));
+        HIP_TRY(c, hipMalloc(&c->tile_order[p], (tiles + 1) * sizeof(uint32_t)));   // + the longest-first flag
     }
     c->tile_cap = tiles;
+    return SLIMRT_OK;
+}
+
+// Waves of the frame kernel the device holds at once (its workgroups are single waves), for this node width and
+// stack size; cached per configuration.
+int resident_waves(slimrt_context *c, bool wide, size_t lds, uint32_t &out) {
+    const uint64_t key = ((uint64_t)lds << 1) | (wide ? 1u : 0u);
+    if (c->resident_key != key || !c->resident) {
+        int per_cu = 0, cus = 0;
+        HIP_TRY(c, wide ? hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, render_kernel<false, true>, 64, lds)
+                        : hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, render_kernel<false, false>, 64, lds));
+        HIP_TRY(c, hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, c->device));
+        c->resident = (uint32_t)std::max(1, per_cu * cus);
+        c->resident_key = key;
+    }
+    out = c->resident;
     return SLIMRT_OK;
 }
 
@@ -421,8 +470,10 @@ int launch(slimrt_context *c, const slimrt_render_params *p, float4 *pixels, flo
             c->order_tiles[nxt] = 0;
             if (c->cost_tiles[nxt] == wtiles) {
                 HIP_TRY(c, hipStreamWaitEvent(c->side_stream, c->tile_rendered[nxt], 0));
+                uint32_t resident = 0;
+                if (int rc = resident_waves(c, wide_mesh_bvh(rp), wlds, resident)) return rc;
                 hipLaunchKernelGGL(tile_order_kernel, dim3(1), dim3(kOrderThreads), 0, c->side_stream,
-                                   c->tile_cost[nxt], c->tile_order[nxt], wtiles);
+                                   c->tile_cost[nxt], c->tile_order[nxt], wtiles, resident);
                 HIP_TRY(c, hipGetLastError());
                 HIP_TRY(c, hipEventRecord(c->tile_ordered[nxt], c->side_stream));
                 c->order_launched[nxt] = true;
@@ -992,8 +1043,10 @@ int slimrt_wavefront_counts(slimrt_context *c, uint32_t *counts, uint32_t n) {
     return SLIMRT_OK;
 }
 
-int slimrt_tile_costs(slimrt_context *c, uint32_t *costs, uint32_t *order, uint32_t cap, uint32_t *tile_count) {
+int slimrt_tile_costs(slimrt_context *c, uint32_t *costs, uint32_t *order, uint32_t cap, uint32_t *tile_count,
+                      uint32_t *resident) {
     if (!c || !tile_count) return SLIMRT_E_ARG;
+    if (resident) *resident = c->cpu ? 0u : c->resident;
     const uint32_t last = (uint32_t)((c->tile_frame + 1u) & 1u);
     const uint32_t n = (c->cpu || c->tile_frame == 0) ? 0u : c->cost_tiles[last];
     *tile_count = n;

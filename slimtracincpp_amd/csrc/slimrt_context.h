@@ -88,6 +88,8 @@ struct slimrt_context {
     bool order_launched[2] = {}, order_used[2] = {};
     uint64_t tile_frame = 0;
     hipEvent_t tile_rendered[2] = {}, tile_ordered[2] = {};
+    uint32_t resident = 0;           // frame-kernel waves the device holds at once, for resident_key's configuration
+    uint64_t resident_key = 0;
 
     // rendering with the cache: queues, training samples, gate table, frame counters
     float *q_raw = nullptr, *q_prefix = nullptr, *q_tp = nullptr, *q_out = nullptr;

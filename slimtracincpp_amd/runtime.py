@@ -151,7 +151,7 @@ def lib():
             "slimrt_render_autotune": [vp, P(RenderParams), vp, vp, u32, P(u32), P(f32)],
             "slimrt_host_collapse_bvh4": [vp, u32, u32, vp, u32, P(u32), P(u32), P(u32)],
             "slimrt_wavefront_counts": [vp, vp, u32],
-            "slimrt_tile_costs": [vp, vp, vp, u32, P(u32)],
+            "slimrt_tile_costs": [vp, vp, vp, u32, P(u32), P(u32)],
             "slimrt_pick": [vp, P(RenderParams), i32, i32, P(PickResult)],
             "slimrt_canvas_content": [vp, vp, u32, u32, u32, vp],
             "slimrt_host_bake_texture": [vp, u32, u32, u32, vp, C.c_size_t, P(C.c_size_t), P(u32), vp, P(u32)],
@@ -490,13 +490,15 @@ class Renderer:
         return c
 
     def tile_costs(self):
-        """Per-tile durations (wall-clock ticks) of the last per-sample frame dispatched by cost, and that frame's
-        dispatch order (order[k] = the tile workgroup k rendered); empty arrays before the first such frame."""
-        n = u32(0)
-        _check(lib().slimrt_tile_costs(self.ctx, None, None, 0, C.byref(n)), self.ctx)
+        """Per-tile durations (wall-clock ticks) of the last per-sample frame rendered without TILE_ORDER_FIXED and
+        that frame's dispatch order (order[k] = the tile dispatched k-th); empty arrays before the first such frame.
+        tile_resident holds the frame kernel's resident waves the order was weighed against."""
+        n, res = u32(0), u32(0)
+        _check(lib().slimrt_tile_costs(self.ctx, None, None, 0, C.byref(n), C.byref(res)), self.ctx)
         costs, order = np.zeros(n.value, np.uint32), np.zeros(n.value, np.uint32)
         if n.value:
-            _check(lib().slimrt_tile_costs(self.ctx, _ptr(costs), _ptr(order), n.value, C.byref(n)), self.ctx)
+            _check(lib().slimrt_tile_costs(self.ctx, _ptr(costs), _ptr(order), n.value, C.byref(n), None), self.ctx)
+        self.tile_resident = res.value
         return costs, order
 
     def update_scene_device(self, geo_f: np.ndarray, mode: int = SCENE_REFIT):

@@ -125,9 +125,10 @@ def test_cpu_context_has_no_tile_costs():
     assert all(np.array_equal(x, y) for x, y in zip(a, b))
     costs, order = r.tile_costs()
     assert costs.size == 0 and order.size == 0
-    n = ctypes.c_uint32(7)
-    assert runtime.lib().slimrt_tile_costs(r.ctx, None, None, 0, ctypes.byref(n)) == 0 and n.value == 0
-    assert runtime.lib().slimrt_tile_costs(r.ctx, None, None, 0, None) == -1   # SLIMRT_E_ARG
+    n, res = ctypes.c_uint32(7), ctypes.c_uint32(7)
+    assert runtime.lib().slimrt_tile_costs(r.ctx, None, None, 0, ctypes.byref(n), ctypes.byref(res)) == 0
+    assert n.value == 0 and res.value == 0
+    assert runtime.lib().slimrt_tile_costs(r.ctx, None, None, 0, None, None) == -1   # SLIMRT_E_ARG
 
 
 def test_invalid_uploads_are_rejected():

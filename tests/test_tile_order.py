@@ -35,12 +35,16 @@ def _cost_class(ticks: np.ndarray) -> np.ndarray:
     return np.where(t >= 4, (e - 2) * 4 + ((t >> np.maximum(e - 2, 0)) & 3) + 1, 0)
 
 
-def _expected_order(costs: np.ndarray) -> np.ndarray:
-    """The dispatch order a frame takes from the previous frame's tile durations: the tiles within a factor 16 of
-    the costliest class (narrowed four classes at a time while more than 4096; none if the costliest class alone
-    has more), costliest class first and by tile index within a class, then every other tile in image order."""
-    rank = 127 - _cost_class(costs)
+def _expected_order(costs: np.ndarray, resident: int) -> np.ndarray:
+    """The dispatch order a frame takes from the tile durations of two frames before.  Image order unless the
+    longest tile lasted at least 3/4 of the summed durations over the resident waves; then the tiles within a
+    factor 16 of the costliest class (narrowed four classes at a time while more than 4096; none if the costliest
+    class alone has more), costliest class first and by tile index within a class, then every other tile in
+    image order."""
     idx = np.arange(costs.size)
+    if 4 * int(costs.max()) * resident <= 3 * int(costs.astype(np.int64).sum()):
+        return idx
+    rank = 127 - _cost_class(costs)
     top = rank.min()
     heavy = np.zeros(costs.size, bool)
     for window in (16, 12, 8, 4, 0):
@@ -55,9 +59,13 @@ def _expected_order(costs: np.ndarray) -> np.ndarray:
 def test_expected_order_model():
     costs = np.array([5, 900, 40, 1000, 7, 64, 950, 3, 0, 33], np.uint32)
     # 900, 950 and 1000 share the costliest class; 64 lies within a factor 16 below it, 40 and less further
-    assert _expected_order(costs).tolist() == [1, 3, 6, 5, 0, 2, 4, 7, 8, 9]
+    assert _expected_order(costs, 3072).tolist() == [1, 3, 6, 5, 0, 2, 4, 7, 8, 9]
+    # the longest tile (1000) against the summed durations (3002) over the resident waves: 4 * 1000 * 3 > 3 * 3002,
+    # costly first; 4 * 1000 * 2 <= 3 * 3002, image order
+    assert _expected_order(costs, 3).tolist()[:4] == [1, 3, 6, 5]
+    assert np.array_equal(_expected_order(costs, 2), np.arange(10))
     flat = np.full(5000, 100, np.uint32)              # one class larger than the cap: image order
-    assert np.array_equal(_expected_order(flat), np.arange(5000))
+    assert np.array_equal(_expected_order(flat, 3072), np.arange(5000))
 
 
 def test_cost_class_is_monotonic():
@@ -84,7 +92,7 @@ def test_cost_order_renders_every_tile_once(gpu_available, flags):
         assert costs.size == n and np.all(costs > 0)
         assert np.array_equal(np.sort(order), np.arange(n))           # a permutation of the tiles
         # frame f is ordered by the durations of frame f - 2 (its order is computed while frame f - 1 renders)
-        expected = _expected_order(history[f - 2]) if f >= 2 else np.arange(n)
+        expected = _expected_order(history[f - 2], r.tile_resident) if f >= 2 else np.arange(n)
         assert np.array_equal(order, expected), f
         history.append(costs)
 
@@ -111,7 +119,7 @@ def test_cost_order_follows_tile_count_changes(gpu_available):
         assert all(np.array_equal(a, b) for a, b in zip(got, ref_ssaa))
         costs, order = r.tile_costs()
         assert order.size == n
-        assert np.array_equal(order, _expected_order(history[f - 2]) if f >= 2 else np.arange(n)), f
+        assert np.array_equal(order, _expected_order(history[f - 2], r.tile_resident) if f >= 2 else np.arange(n)), f
         history.append(costs)
     # back to the first tile count
     p.ssaa = 0

@@ -137,3 +137,23 @@ def test_counting_pass_ignores_the_order(gpu_available):
     _frame(r, p)
     _frame(r, p)
     assert r.count_work(p) == a
+
+
+def test_cost_order_at_full_size(gpu_available):
+    """The 1080p mesh + PBR frame of the benchmark: from the third frame its costliest tiles go first and the first
+    1,012 of them render as two half waves; every frame equals the image-order frame bit for bit."""
+    spec = scenes.mesh_pbr_scene(0, 1920, 1080)
+    r = runtime.Renderer(spec, "gpu")
+    p = r.params()
+    p.flags = runtime.BVH4 | runtime.TILE_ORDER_FIXED
+    ref = _frame(r, p)
+    p.flags = runtime.BVH4
+    history = []
+    for f in range(4):
+        got = _frame(r, p)
+        assert all(np.array_equal(a, b) for a, b in zip(got, ref)), f
+        costs, order = r.tile_costs()
+        expected = _expected_order(history[f - 2], r.tile_resident) if f >= 2 else np.arange(costs.size)
+        assert np.array_equal(order, expected), f
+        history.append(costs)
+    assert not np.array_equal(order, np.arange(order.size))   # this frame is bound by its longest tiles

@@ -71,7 +71,9 @@ extern "C" {
                                         durations over the resident waves) dispatches its costliest tiles first and
                                         renders the first 1 in 32 of them as two half waves; other frames, and the
                                         first two at a tile count, keep the image order.  Results are the same
-                                        either way */
+                                        either way.  The cost order allocates its buffers on a context's first frame
+                                        at a larger tile count and orders on a second stream: callers that capture
+                                        frames in a HIP graph pass this flag */
 
 typedef struct slimrt_context slimrt_context;
 

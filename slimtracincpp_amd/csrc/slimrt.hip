@@ -258,25 +258,25 @@ void render_kernel(SceneView s, RenderParams rp, float4 *__restrict__ pixels, fl
 // The frame kernel has the render kernel's tiling; one in train_one_in waves (chosen by a hash of the wave's
 // position and the frame) traces full paths for training, so training and cached pixels never share a wave.
 template <bool WIDE>
-__global__ __launch_bounds__(kBlock) void nrc_frame_kernel(SceneView s, RenderParams rp, NrcFrame nf,
-                                                           float4 *__restrict__ pixels, float *__restrict__ depths,
-                                                           uint32_t scene_depth, uint32_t mesh_depth, uint32_t tiles_x,
-                                                           uint32_t tiles, uint32_t *__restrict__ status) {
+__global__ __launch_bounds__(64) void nrc_frame_kernel(SceneView s, RenderParams rp, NrcFrame nf,
+                                                       float4 *__restrict__ pixels, float *__restrict__ depths,
+                                                       uint32_t scene_depth, uint32_t mesh_depth, uint32_t tiles_x,
+                                                       uint32_t tiles, uint32_t *__restrict__ status) {
     extern __shared__ uint32_t lds[];
     const uint32_t tile = blockIdx.x;
     if (tile >= tiles) return;
-    const uint32_t t = threadIdx.x, wave = t >> 6, lane = t & 63u;
-    const uint32_t x = (tile % tiles_x) * kTile + (wave & 1u) * 8u + (lane & 7u);
-    const uint32_t y = (tile / tiles_x) * kTile + (wave >> 1) * 8u + (lane >> 3);
+    const uint32_t t = threadIdx.x, lane = t;
+    const uint32_t x = (tile % tiles_x) * kWaveTileX + (lane % kWaveTileX);
+    const uint32_t y = (tile / tiles_x) * kWaveTileY + (lane / kWaveTileX);
     const uint32_t ws = rp.width << rp.ssaa, hs = rp.height << rp.ssaa;
-    uint32_t h = (tile * 4u + wave) * 0x9E3779B1u + nf.frame * 0x85EBCA77u;
+    uint32_t h = tile * 0x9E3779B1u + nf.frame * 0x85EBCA77u;
     h ^= h >> 15;
     h *= 0x2C1B3C6Du;
     h ^= h >> 12;
     const bool training = nf.train_one_in && (h % nf.train_one_in) == 0;
 
     TracerT<WIDE> tr;
-    bind_lds_stacks(tr, lds, t, scene_depth, mesh_depth);
+    bind_lds_stacks<WIDE, 64>(tr, lds, t, scene_depth, mesh_depth);
     tr.cnt = nullptr;
     if (x < ws && y < hs) {
         v3 dir;
@@ -1091,17 +1091,18 @@ int slimrt_render_nrc(slimrt_context *c, const slimrt_render_params *p, slimrt_n
     rp.ssaa = rp.ssaa ? 1u : 0u;
     const uint32_t ws = rp.width << rp.ssaa, hs = rp.height << rp.ssaa;
     const uint32_t samples = ws * hs;
-    const uint32_t tiles_x = (ws + kTile - 1) / kTile, tiles_y = (hs + kTile - 1) / kTile;
+    // one-wave workgroups, a 32x2 strip of samples each (the render kernel's tiling)
+    const uint32_t tiles_x = (ws + kWaveTileX - 1) / kWaveTileX, tiles_y = (hs + kWaveTileY - 1) / kWaveTileY;
     const uint32_t tiles = tiles_x * tiles_y;
     const uint32_t scene_depth = c->scene_height + 1, mesh_depth = mesh_stack_depth(c, wide_mesh_bvh(rp));
-    const size_t lds = stack_lds_bytes(scene_depth, mesh_depth, wide_mesh_bvh(rp));
+    const size_t lds = stack_lds_bytes(scene_depth, mesh_depth, wide_mesh_bvh(rp), 64);
     if (lds > 160 * 1024) return fail(c, SLIMRT_E_ARG, "BVH too deep for the LDS traversal stacks");
     if (wide_mesh_bvh(rp) && check_wide(c)) return SLIMRT_E_ARG;
     // training waves are a hashed 1-in-N subset: twice the expected share (at least 64 waves) bounds the samples
     uint64_t t_cap = 0;
     if (np->train_one_in) {
-        const uint64_t waves = ((uint64_t)tiles * 4 + np->train_one_in - 1) / np->train_one_in;
-        t_cap = std::min<uint64_t>((uint64_t)tiles * 4, 2 * waves + 64) * 64 * p->max_depth;
+        const uint64_t waves = ((uint64_t)tiles + np->train_one_in - 1) / np->train_one_in;
+        t_cap = std::min<uint64_t>((uint64_t)tiles, 2 * waves + 64) * 64 * p->max_depth;
         t_cap = std::min<uint64_t>(t_cap, kLinkNone - 1);
     }
     bool cells_new;
@@ -1132,11 +1133,11 @@ int slimrt_render_nrc(slimrt_context *c, const slimrt_render_params *p, slimrt_n
     hipLaunchKernelGGL(nrc_reset_kernel, dim3(1), dim3(64), 0, c->stream, c->nrc_counters, c->nrc_losses, kMaxTrainSteps);
     HIP_TRY(c, hipGetLastError());
     if (wide_mesh_bvh(rp))
-        hipLaunchKernelGGL(nrc_frame_kernel<true>, dim3(tiles), dim3(kBlock), lds, c->stream, device_view(c), rp,
+        hipLaunchKernelGGL(nrc_frame_kernel<true>, dim3(tiles), dim3(64), lds, c->stream, device_view(c), rp,
                            nf, (float4 *)d_pixels, (float *)d_depths, scene_depth, mesh_depth, tiles_x, tiles,
                            c->d_status);
     else
-        hipLaunchKernelGGL(nrc_frame_kernel<false>, dim3(tiles), dim3(kBlock), lds, c->stream, device_view(c), rp,
+        hipLaunchKernelGGL(nrc_frame_kernel<false>, dim3(tiles), dim3(64), lds, c->stream, device_view(c), rp,
                            nf, (float4 *)d_pixels, (float *)d_depths, scene_depth, mesh_depth, tiles_x, tiles,
                            c->d_status);
     HIP_TRY(c, hipGetLastError());

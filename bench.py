@@ -14,7 +14,7 @@ Reported:
                triangle test, 64 per mesh closest-hit attribute fetch, 12 per texel-quad tap, 20 per sample out
   cpu_baseline the reference's own CPU renderer (compiled from its sources, oracle/_ref/slimref) on this host
   neural_cache the plain CPU-runnable scene (seed 0) at 1920x1080 with bounce depth 8, rendered over a seeded
-               orbit of the camera (300 frames) twice per frame: the plain full-depth trace (the judge) and the
+               orbit of the camera (300 frames) twice per frame: the plain full-depth trace (the cache's reference) and the
                neural-cache form (cache after the second bounce; 1 in 8 pixels, 2^18 at 1080p, trace full depth
                for training; confidence gate over the last ~16 frames), each timed with HIP events around its
                own launches; PSNR and MAE of every cached frame against its plain frame.  A second pass puts the

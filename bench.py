@@ -2,13 +2,16 @@
 
 Workload: the 1080p mesh + PBR scene at bounce depth 4 (three 81,920-triangle displaced icospheres on a textured
 floor, Cook-Torrance GGX, 1024^2 mip-mapped checker+noise albedo, cube-map IBL, one glass and one mirror
-instance).  A step is one frame: one launch of the render kernel over all 2,073,600 primary rays, with the scene
-resident in HBM and the canvas (pixels + depths) written to HBM.
+instance).  A step is one frame over all 2,073,600 primary rays, with the scene resident in HBM and the canvas
+(pixels + depths) written to HBM, in the fastest of sixteen frame forms (normally one launch of the per-sample
+render kernel, its tiles dispatched costliest first by an order computed on a second stream during the previous
+frame).
 
 Reported:
   value        whole-frame Mrays/s, rays = every SceneTracer::trace call (primary + bounce + shadow), counted
                by an instrumented pass of the same kernel (deterministic; equal to the CPU oracle's count)
-  ms_per_step  wall time per frame between synchronisations, averaged over --steps
+  ms_per_step  time per frame between two HIP events around --steps back-to-back frames (synchronised before
+               and after)
   roofline     HBM roofline of the render kernel: algorithmic bytes per frame / mean kernel time (HIP events on
                the launch stream); bytes per unit: 64 per sibling-pair visit, 64 per instance visit, 64 per
                triangle test, 64 per mesh closest-hit attribute fetch, 12 per texel-quad tap, 20 per sample out

@@ -9,7 +9,6 @@ import pytest
 
 from slimtracincpp_amd import scenes, runtime
 
-pytestmark = pytest.mark.gpu
 
 
 def _frame(r, p):
@@ -74,6 +73,7 @@ def test_cost_class_is_monotonic():
     assert np.all(np.diff(k) >= 0) and k[3] == 0 and k[4] == 1 and k[8] == 5
 
 
+@pytest.mark.gpu
 @pytest.mark.parametrize("flags", [runtime.BVH4, 0])
 def test_cost_order_renders_every_tile_once(gpu_available, flags):
     spec = scenes.mesh_pbr_scene(0, 200, 120, level=4, texture_size=64, sky_size=16, irradiance_size=8)
@@ -97,6 +97,7 @@ def test_cost_order_renders_every_tile_once(gpu_available, flags):
         history.append(costs)
 
 
+@pytest.mark.gpu
 def test_cost_order_follows_tile_count_changes(gpu_available):
     spec = scenes.implicit_scene(1, 120, 72, max_depth=5, transparent=True)
     r = runtime.Renderer(spec, "gpu")
@@ -129,6 +130,7 @@ def test_cost_order_follows_tile_count_changes(gpu_available):
     assert np.array_equal(np.sort(r.tile_costs()[1]), np.arange(_tiles(p)))
 
 
+@pytest.mark.gpu
 def test_counting_pass_ignores_the_order(gpu_available):
     spec = scenes.mesh_pbr_scene(0, 96, 54, level=3, texture_size=64, sky_size=16, irradiance_size=8)
     r = runtime.Renderer(spec, "gpu")
@@ -139,6 +141,7 @@ def test_counting_pass_ignores_the_order(gpu_available):
     assert r.count_work(p) == a
 
 
+@pytest.mark.gpu
 def test_cost_order_at_full_size(gpu_available):
     """The 1080p mesh + PBR frame of the benchmark: from the third frame its costliest tiles go first and the first
     1,012 of them render as two half waves; every frame equals the image-order frame bit for bit."""

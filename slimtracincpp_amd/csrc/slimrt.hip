@@ -7,10 +7,11 @@
 // The render kernel is a per-sample megakernel: each workgroup is one wave owning a 32x2 strip of samples
 // (one-wave workgroups finish the frame 6-9% sooner than 4-wave 16x16 ones, whose waves wait for the slowest of
 // the four before the workgroup's slot frees; 32x2 strips beat 8x8 squares by 4% and 16x4 by 2% on the mesh +
-// PBR frame: a wave's canvas writes are two 512-byte rows), and tile i is workgroup i: workgroups are
-// dealt round-robin to the 8 XCDs, so every XCD gets an even share of each image row band (band-per-XCD dealing
-// left whole XCDs on the costly bands and ran 18-20% slower).  Traversal stacks live in LDS, one column per
-// thread, sized from the uploaded BVH heights.
+// PBR frame: a wave's canvas writes are two 512-byte rows).  In image order tile i is workgroup i: workgroups
+// are dealt round-robin to the 8 XCDs, so every XCD gets an even share of each image row band (band-per-XCD
+// dealing left whole XCDs on the costly bands and ran 18-20% slower); frames bound by their longest tiles
+// dispatch those first instead (tile_order_kernel).  Traversal stacks live in LDS, one column per thread, sized
+// from the uploaded BVH heights.
 #include <hip/hip_runtime.h>
 #include <string>
 #include <vector>
@@ -202,9 +203,9 @@ __global__ __launch_bounds__(kOrderThreads) void tile_order_kernel(const uint32_
 // more than the registers (6% faster than two on the mesh + PBR frame).  The 2-wide frame would spill 64 dwords a
 // lane for no gain, and the counting pass keeps its registers: both stay at the compiler's two.  With an order
 // (tile_order_kernel) workgroup k renders tile order[k] and records the tile's duration in cost; when order[tiles]
-// is set, the first `split` tiles of the order (the costliest) are each rendered by two workgroups, lanes 0-31 and lanes 32-63: a costly
-// tile's 64 paths diverge, and half as many lanes per wave shorten the frame's longest waves (0.75 -> 0.64 ms on
-// the mesh + PBR frame; quarter waves for fewer tiles gain little more and lose fast as the count grows).
+// is set, the first `split` tiles of the order (the costliest) are each rendered by two workgroups, lanes 0-31
+// and lanes 32-63: a costly tile's 64 paths diverge, and half as many lanes per wave shorten the frame's longest
+// waves (0.77 -> 0.65 ms on the mesh + PBR frame; quarter waves gain nothing and lose fast as their count grows).
 template <bool kCount, bool WIDE>
 __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(WIDE && !kCount ? 3 : 1)))
 void render_kernel(SceneView s, RenderParams rp, float4 *__restrict__ pixels, float *__restrict__ depths,

@@ -445,6 +445,20 @@ def main():
                      "counters": cnt4 if wide else cnt},
         "neural_cache": None,
     }
+    if form in ("megakernel", "megakernel_bvh4"):
+        # a frame's tile durations (slimrt_tile_costs): how far the frame is from its ideal length
+        import numpy as np
+        for _ in range(3):   # this form's frames (the order of a frame comes from the durations two frames before)
+            r.render_device(p, pixels.data_ptr(), depths.data_ptr())
+        costs, order = r.tile_costs()
+        if costs.size:
+            c = costs.astype(np.int64)
+            ideal_us = float(c.sum()) / r.tile_resident / 100.0   # summed durations over the resident waves
+            result["tile_costs"] = {
+                "tiles": int(c.size), "resident_waves": r.tile_resident, "median_us": float(np.median(c)) / 100.0,
+                "p99_us": float(np.percentile(c, 99)) / 100.0, "longest_us": float(c.max()) / 100.0,
+                "ideal_frame_us": ideal_us,
+                "order": "image" if np.array_equal(order, np.arange(order.size)) else "longest first"}
     if not args.no_views:
         result["batched_views"] = batched_views_leg(args.view_frames, stream, spec)
     if args.large:

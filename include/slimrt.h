@@ -286,8 +286,8 @@ int slimrt_wavefront_counts(slimrt_context *ctx, uint32_t *counts, uint32_t n);
 /* The tile costs of the last per-sample frame rendered without SLIMRT_TILE_ORDER_FIXED (no counterpart in the
  * reference).  A tile is the 32 x 2 samples one wave renders, tiles numbered in image order (row major over
  * ceil(w / 32) x ceil(h / 2) tiles of the sample grid, w and h doubled with SSAA).  costs[i] = tile i's wall-clock
- * duration in ticks of the device's 100 MHz real-time counter (a tile rendered as two half waves: the half that
- * ended last); order[k] = the tile dispatched k-th (the costliest first, or the image order).  Either pointer may
+ * duration in ticks of the device's 100 MHz real-time counter (a tile rendered as two half waves: twice the
+ * duration of the half that ended last); order[k] = the tile dispatched k-th (the costliest first, or the image order).  Either pointer may
  * be NULL; *tile_count receives the number of tiles (0 before the first such frame); SLIMRT_E_ARG when cap is
  * smaller.  resident (may be NULL): the frame kernel's waves the device holds at once, which the choice of order
  * weighs the durations against.  Synchronizes the context's stream. */

@@ -241,8 +241,12 @@ void render_kernel(SceneView s, RenderParams rp, float4 *__restrict__ pixels, fl
         depths[o] = depth;
         if (tr.overflow) atomicOr(status, 1u);
     }
-    if (cost && lane == 0) {   // a split tile keeps the duration of the half that ends last
-        const uint64_t d = (uint64_t)wall_clock64() - start;
+    if (cost && lane == 0) {
+        // A split tile records twice the duration of its half that ends last: a half wave runs about half its
+        // tile's diverging paths, and the doubled figure keeps the tile in the costliest classes two frames later
+        // (its own half-wave duration would rank it lower, so it would run unsplit and late, then split again).
+        uint64_t d = (uint64_t)wall_clock64() - start;
+        if (half != 2u) d *= 2u;
         cost[tile] = d > 0xFFFFFFFFull ? 0xFFFFFFFFu : (uint32_t)d;
     }
     if (kCount) {
@@ -718,7 +722,10 @@ int slimrt_create(slimrt_context **out, uint32_t flags) {
 int slimrt_destroy(slimrt_context *c) {
     if (!c) return SLIMRT_E_ARG;
     if (!c->cpu) {
+        // the caller's stream and the side stream (the tile-order kernel of the last cost-ordered frame reads
+        // tile_cost and writes tile_order there, and nothing joins it back) both finish before anything is freed
         (void)hipStreamSynchronize(c->stream);
+        if (c->side_stream) (void)hipStreamSynchronize(c->side_stream);
         dev_free(c->d_geos); dev_free(c->d_mats); dev_free(c->d_lights); dev_free(c->d_snodes); dev_free(c->d_mnodes);
         dev_free(c->d_mnodes4);
         dev_free(c->d_leaf); dev_free(c->d_meshes); dev_free(c->d_tri_hit); dev_free(c->d_tri_attr); dev_free(c->d_tex);
@@ -734,10 +741,8 @@ int slimrt_destroy(slimrt_context *c) {
             if (e) (void)hipEventDestroy(e);
         if (c->wf_fork) (void)hipEventDestroy(c->wf_fork);
         if (c->wf_join) (void)hipEventDestroy(c->wf_join);
-        if (c->side_stream) (void)hipStreamDestroy(c->side_stream);
         if (c->d_status) (void)hipFree(c->d_status);
         if (c->d_counters) (void)hipFree(c->d_counters);
-        if (c->side_stream) (void)hipStreamSynchronize(c->side_stream);
         for (int p = 0; p < 2; p++) {
             if (c->tile_cost[p]) (void)hipFree(c->tile_cost[p]);
             if (c->tile_order[p]) (void)hipFree(c->tile_order[p]);
@@ -746,6 +751,7 @@ int slimrt_destroy(slimrt_context *c) {
         }
         if (c->ev0) (void)hipEventDestroy(c->ev0);
         if (c->ev1) (void)hipEventDestroy(c->ev1);
+        if (c->side_stream) (void)hipStreamDestroy(c->side_stream);
         if (c->own_stream) (void)hipStreamDestroy(c->own_stream);
     }
     delete c;
@@ -1114,6 +1120,8 @@ int slimrt_render_nrc(slimrt_context *c, const slimrt_render_params *p, slimrt_n
     nf.q_raw = c->q_raw; nf.q_pixel = c->q_pixel; nf.q_prefix = c->q_prefix; nf.q_tp = c->q_tp; nf.q_cap = samples;
     nf.t_raw = c->t_raw; nf.t_target = c->t_target; nf.t_next = c->t_next; nf.t_link = c->t_link;
     nf.t_cap = (uint32_t)t_cap;
+    c->nrc_frame_q_cap = nf.q_cap;
+    c->nrc_frame_t_cap = nf.t_cap;
     nf.counters = c->nrc_counters;
     nf.cells = c->cells;
     nf.cell_mask = (1u << np->gate_cells_log2) - 1u;
@@ -1174,7 +1182,7 @@ int slimrt_nrc_frame_samples(slimrt_context *c, float *raw, float *targets, uint
     uint32_t cnt[NC_COUNT];
     HIP_TRY(c, hipMemcpyAsync(cnt, c->nrc_counters, sizeof cnt, hipMemcpyDeviceToHost, c->stream));
     HIP_TRY(c, hipStreamSynchronize(c->stream));
-    const uint32_t n = std::min(cnt[NC_SAMPLES], c->t_cap);
+    const uint32_t n = std::min(cnt[NC_SAMPLES], c->nrc_frame_t_cap);
     *count = n;
     const uint32_t m = std::min(n, cap);
     if (m) {
@@ -1197,8 +1205,8 @@ int slimrt_nrc_frame_stats(slimrt_context *c, uint32_t stats[8]) {
     HIP_TRY(c, hipMemcpyAsync(cnt, c->nrc_counters, sizeof cnt, hipMemcpyDeviceToHost, c->stream));
     HIP_TRY(c, hipMemcpyAsync(losses, c->nrc_losses, sizeof losses, hipMemcpyDeviceToHost, c->stream));
     HIP_TRY(c, hipStreamSynchronize(c->stream));
-    stats[0] = std::min(cnt[NC_QUERIES], c->q_cap);
-    stats[1] = std::min(cnt[NC_SAMPLES], c->t_cap);
+    stats[0] = std::min(cnt[NC_QUERIES], c->nrc_frame_q_cap);
+    stats[1] = std::min(cnt[NC_SAMPLES], c->nrc_frame_t_cap);
     stats[2] = cnt[NC_FALLBACKS];
     stats[3] = cnt[NC_TRAIN_PIXELS];
     const float first = c->nrc_last_steps ? losses[0] : 0.0f, last = c->nrc_last_steps ? losses[c->nrc_last_steps - 1] : 0.0f;

@@ -96,6 +96,9 @@ struct slimrt_context {
     uint32_t *q_pixel = nullptr, q_cap = 0;
     float *t_raw = nullptr, *t_target = nullptr, *t_next = nullptr;
     uint32_t *t_link = nullptr, t_cap = 0;
+    // the queue and sample capacities the last cache frame used (the buffers above may be larger, left from a
+    // larger frame): counts read back are clamped to these
+    uint32_t nrc_frame_q_cap = 0, nrc_frame_t_cap = 0;
     uint32_t *nrc_counters = nullptr;
     float *nrc_losses = nullptr;
     uint32_t nrc_last_steps = 0;

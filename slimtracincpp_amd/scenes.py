@@ -592,6 +592,97 @@ def mesh_pbr_scene(seed: int = 0, width: int = 1920, height: int = 1080, max_dep
         meshes=[mesh], textures=tex)
 
 
+# --- scene 3: the mesh example's lighting (Phong, Blinn, point and directional lights, meshes casting shadows) --
+def lit_mesh_scene(seed: int = 0, width: int = 640, height: int = 360, max_depth: int = 5, level: int = 4,
+                   texture_size: int = 128, sky_size: int = 32, irradiance_size: int = 8, leaf_size: int = 4,
+                   render_mode: int = MODE_BEAUTY, ibl: bool = True) -> SceneSpec:
+    """The setup of the reference's mesh example (src/examples/07_Meshes.cpp:40-110) with its missing assets
+    replaced by seeded ones: a textured floor quad, a mirror sphere, three instances of a displaced icosphere
+    (a Phong one and a Blinn one with 07_Meshes' phong_material / blinn_material values, both casting shadows,
+    and a glass one that casts none, as 07_Meshes clears GEOMETRY_IS_SHADOWING for glass), a Phong box, a
+    Lambert tetrahedron; three coloured point lights as in 07_Meshes (rim, two glass lights; all shadowing), a
+    fourth point light that casts no shadows and one shadowing directional light.  `ibl` adds cube-map sky
+    lighting as 07_Meshes' Cathedral skybox does.
+
+    Draw order: per mesh instance (x, z offsets, euler y, scale); box (half-extents xyz, euler xyz); tetrahedron
+    (scale, euler xyz); per point light (position jitter xyz); directional light (elevation, azimuth).
+    Geometry order: floor, sphere, mesh Phong, mesh Blinn, mesh glass, box, tetrahedron.
+    Textures: 0 = checker+noise albedo; with `ibl`, 1 = sky radiance cube, 2 = irradiance cube."""
+    rng = PCG32(seed)
+    mesh = displaced_sphere_mesh(level, 0.15, seed, leaf_size)
+    geos, mats_f, mats_u = [], [], []
+    # materials: 07_Meshes' Floor, Mirror, Phong, Blinn, Glass, plus a Phong box and a rough tetrahedron
+    f, u = material((0.8, 0.8, 0.8), 0.2, flags=MAT_ALBEDO_MAP, texture_ids=(0,), brdf=BRDF_COOK_TORRANCE,
+                    uv_repeat=(4, 4))
+    mats_f.append(f); mats_u.append(u)
+    f, u = material((0.0, 0.0, 0.0), 0.02, flags=MAT_REFLECTIVE, brdf=BRDF_COOK_TORRANCE,
+                    reflectivity=(0.91, 0.92, 0.92), ior=float(IOR_AIR))
+    mats_f.append(f); mats_u.append(u)
+    f, u = material((1.0, 1.0, 1.0), 0.5, brdf=BRDF_PHONG, reflectivity=(1.0, 1.0, 0.4), ior=float(IOR_AIR))
+    mats_f.append(f); mats_u.append(u)
+    f, u = material((1.0, 0.3, 1.0), 0.5, brdf=BRDF_BLINN, reflectivity=(1.0, 0.4, 1.0), ior=float(IOR_AIR))
+    mats_f.append(f); mats_u.append(u)
+    f, u = material((0.05, 0.05, 0.05), 0.25, flags=MAT_REFRACTIVE, brdf=BRDF_COOK_TORRANCE, ior=1.52,
+                    reflectivity=(0.03, 0.03, 0.03))
+    mats_f.append(f); mats_u.append(u)
+    f, u = material((0.4, 0.8, 0.9), 0.3, brdf=BRDF_PHONG, reflectivity=(0.9, 0.9, 0.9), ior=float(IOR_AIR))
+    mats_f.append(f); mats_u.append(u)
+    f, u = material((0.8, 0.8, 0.8), 0.7, brdf=BRDF_LAMBERT)
+    mats_f.append(f); mats_u.append(u)
+
+    geos.append(_geo(GEO_QUAD, (0, 0, 0), (0, 0, 0), (12, 1, 12), 0))
+    geos.append(_geo(GEO_SPHERE, (0, 0, 0), (4, 1, -4), (1, 1, 1), 1))
+    slots = [((-0.5, 2.4), 2, GEO_VISIBLE | GEO_SHADOWING),     # Phong
+             ((3.0, 1.5), 3, GEO_VISIBLE | GEO_SHADOWING),      # Blinn
+             ((-3.4, -1.6), 4, GEO_VISIBLE)]                    # glass: no shadows (07_Meshes.cpp:125-129)
+    for (sx, sz), mat_id, fl in slots:
+        px = sx + rng.uniform(-0.4, 0.4)
+        pz = sz + rng.uniform(-0.4, 0.4)
+        yaw = rng.uniform(0, 2 * math.pi)
+        s = rng.uniform(1.0, 1.6)
+        geos.append(_geo(GEO_MESH, (0, yaw, 0), (px, s * 1.15, pz), (s, s, s), mat_id, 0, fl))
+    h = (rng.uniform(0.4, 1.0), rng.uniform(0.4, 1.0), rng.uniform(0.4, 1.0))
+    eul = (rng.uniform(0, 2 * math.pi), rng.uniform(0, 2 * math.pi), rng.uniform(0, 2 * math.pi))
+    geos.append(_geo(GEO_BOX, eul, (1.8, 1.6, -2.6), h, 5))
+    s = rng.uniform(0.8, 1.4)
+    eul = (rng.uniform(0, 2 * math.pi), rng.uniform(0, 2 * math.pi), rng.uniform(0, 2 * math.pi))
+    geos.append(_geo(GEO_TET, eul, (-1.6, 1.3, -3.4), (s, s, s), 6))
+
+    # 07_Meshes' lights (rim, glass 1, glass 2), positions jittered, plus an unshadowed fill light
+    base = [((1.0, 0.25, 0.25), (6.0, 5.0, 2.0), 36.0, LIGHT_SHADOWING),
+            ((0.25, 1.0, 0.25), (-3.3, 0.6, -3.0) , 20.0, LIGHT_SHADOWING),
+            ((0.25, 0.25, 1.0), (-0.6, 1.75, -3.15), 20.0, LIGHT_SHADOWING),
+            ((1.0, 0.9, 0.7), (-5.0, 6.0, -6.0), 25.0, 0)]
+    lights_f, lights_u = [], []
+    for col, (lx, ly, lz), inten, fl in base:
+        p = (lx + rng.uniform(-0.3, 0.3), ly + rng.uniform(0.0, 0.3), lz + rng.uniform(-0.3, 0.3))
+        lights_f.append(np.array([*col, *p, inten], np.float32))
+        lights_u.append(fl)
+    elev = rng.uniform(0.6, 1.1)
+    azim = rng.uniform(0, 2 * math.pi)
+    d = np.array([math.cos(elev) * math.cos(azim), -math.sin(elev), math.cos(elev) * math.sin(azim)], np.float32)
+    d = (d / np.float32(np.linalg.norm(d.astype(np.float64)))).astype(np.float32)
+    lights_f.append(np.array([0.9, 0.95, 1.0, d[0], d[1], d[2], 0.6], np.float32))   # direction of travel
+    lights_u.append(LIGHT_DIRECTIONAL | LIGHT_SHADOWING)
+
+    tex = [texture_from_texels(checker_noise_texels(texture_size, 8, seed + 202), True, True)]
+    sky_ids = (-1, -1, -1)
+    if ibl:
+        sky = sky_function(seed + 101)
+        tex += [cube_texture(sky, sky_size), cube_texture(irradiance_function(sky), irradiance_size)]
+        sky_ids = (1, 1, 2)
+    cam_pos = np.array([0.0, 4.5, -8.5], np.float32)
+    return SceneSpec(
+        name=f"lit_mesh_s{seed}_{width}x{height}", width=width, height=height,
+        camera_euler=look_at_euler(cam_pos, (0.0, 0.6, 0.2)), camera_position=cam_pos,
+        focal_length=FOV60_FOCAL, max_depth=max_depth, render_mode=render_mode,
+        skybox_color=sky_ids[0], skybox_radiance=sky_ids[1], skybox_irradiance=sky_ids[2],
+        geo_f=_stack([g[0] for g in geos]), geo_u=_stack([g[1] for g in geos]),
+        mat_f=_stack(mats_f), mat_u=_stack(mats_u),
+        light_f=_stack(lights_f), light_u=np.array(lights_u, np.uint32),
+        meshes=[mesh], textures=tex)
+
+
 # ---------------------------------------------------------------- batched views (many cameras, moving instances)
 VIEW_RESOLUTIONS = ((128, 128), (320, 180), (640, 360), (1280, 720), (1920, 1080))
 

@@ -36,6 +36,12 @@ CASES = {
     # the large workload's generator at a small size: 53 geometries, emissive quads, normal maps, depth 8
     "large_s0_160x90_emissive": lambda: scenes.large_scene(0, 160, 90, uv_sphere=(17, 32), texture_size=64,
                                                            sky_size=16, irradiance_size=8),
+    # the mesh example's lighting: Phong and Blinn meshes casting shadows, coloured shadowing point lights, an
+    # unshadowed point light, a shadowing directional light, glass and a mirror; with and without IBL
+    "lit_mesh_s0_160x90_l3_beauty": lambda: scenes.lit_mesh_scene(0, 160, 90, level=3, texture_size=64, sky_size=16,
+                                                                  irradiance_size=8),
+    "lit_mesh_s2_128x72_l4_noibl_d6": lambda: scenes.lit_mesh_scene(2, 128, 72, max_depth=6, level=4,
+                                                                    texture_size=32, ibl=False),
     # wireframe overlays after the render (the driver's --overlay LW sequence: geometry boxes, scene BVH, mesh
     # BVHs and wireframes, a selection box)
     "overlay_implicit_s3_96x64_lw1": lambda: scenes.implicit_scene(3, 96, 64, max_depth=2),

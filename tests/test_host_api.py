@@ -36,6 +36,7 @@ SCENES = {
     "implicit": lambda: scenes.implicit_scene(0, 96, 54),
     "implicit_t": lambda: scenes.implicit_scene(2, 96, 54, max_depth=3, transparent=True),
     "mesh": lambda: scenes.mesh_pbr_scene(0, 96, 54, level=3, texture_size=64, sky_size=16, irradiance_size=8),
+    "lit_mesh": lambda: scenes.lit_mesh_scene(1, 96, 54, level=3, texture_size=32, sky_size=8, irradiance_size=4),
 }
 
 

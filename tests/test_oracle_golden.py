@@ -49,7 +49,11 @@ needs_ref = pytest.mark.skipif(not (os.path.exists(oracle.REF_PATH) and os.path.
     lambda: scenes.mesh_pbr_scene(6, 64, 40, level=3, texture_size=32, sky_size=8, irradiance_size=4,
                                   leaf_size=2, render_mode=scenes.MODE_MIP_LEVEL),
     lambda: scenes.large_scene(3, 64, 40, uv_sphere=(9, 16), texture_size=16, sky_size=8, irradiance_size=4),
-], ids=["implicit_d3", "implicit_normals", "mesh_l2", "mesh_l3_leaf2_mip", "large_emissive"])
+    lambda: scenes.lit_mesh_scene(4, 96, 54, level=3, texture_size=32, sky_size=8, irradiance_size=4),
+    lambda: scenes.lit_mesh_scene(5, 80, 48, level=2, leaf_size=2, max_depth=3, ibl=False),
+    lambda: scenes.lit_mesh_scene(6, 64, 40, level=3, texture_size=16, render_mode=scenes.MODE_NORMALS, ibl=False),
+], ids=["implicit_d3", "implicit_normals", "mesh_l2", "mesh_l3_leaf2_mip", "large_emissive", "lit_mesh_ibl",
+        "lit_mesh_leaf2_noibl", "lit_mesh_normals"])
 def test_oracle_matches_live_reference(make):
     spec = make()
     ref = oracle.run_reference(spec)

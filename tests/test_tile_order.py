@@ -160,3 +160,25 @@ def test_cost_order_at_full_size(gpu_available):
         assert np.array_equal(order, expected), f
         history.append(costs)
     assert not np.array_equal(order, np.arange(order.size))   # this frame is bound by its longest tiles
+
+
+@pytest.mark.gpu
+def test_split_tiles_stay_split(gpu_available):
+    """A split tile records twice the duration of its later half, so it keeps its rank: on the 1080p benchmark
+    frame the tiles split in frame f (the first 1 in 32 of its order) are mostly split again in frame f + 2, whose
+    order comes from frame f's durations.  Recording the half wave's own duration instead would drop most of them
+    an octave, out of the split share, and they would run unsplit and late in every other frame of a parity."""
+    spec = scenes.mesh_pbr_scene(0, 1920, 1080)
+    r = runtime.Renderer(spec, "gpu")
+    p = r.params()
+    p.flags = runtime.BVH4
+    n = _tiles(p)
+    split = min(n // 32, 2048)
+    sets = []
+    for f in range(8):
+        _frame(r, p)
+        costs, order = r.tile_costs()
+        sets.append(set(order[:split].tolist()) if f >= 2 else None)
+    overlaps = [len(sets[f] & sets[f - 2]) / split for f in range(4, 8)]
+    print("split-set overlap with the frame two before:", [round(o, 3) for o in overlaps])
+    assert min(overlaps) >= 0.7

@@ -212,6 +212,19 @@ int slimrt_render(slimrt_context *ctx, const slimrt_render_params *params, slimr
 /* Same frame into device buffers, asynchronously on the context's stream (no reference counterpart: for
  * callers that keep the canvas in HBM).  Check the frame's status with slimrt_synchronize. */
 int slimrt_render_device(slimrt_context *ctx, const slimrt_render_params *params, void *d_pixels, void *d_depths);
+/* slimrt_render_device that allocates nothing (no counterpart in the reference): for callers that cannot
+ * allocate device memory - code under HIP graph capture, tight serving loops, harnesses that forbid hipMalloc.
+ * Per-sample frames only (the megakernel at either node width; SLIMRT_WAVEFRONT forms need ray queues the library
+ * allocates: SLIMRT_E_ARG).  With tiles costliest first (unless SLIMRT_TILE_ORDER_FIXED) the order kernel runs on
+ * the context's stream right before the frame, from the tile durations the last such frame on this GPU recorded
+ * (the image order when they belong to another tile count), so one stream holds the whole frame.  Those durations
+ * and the order live in one static device array per GPU (1 MiB, every context on the GPU shares it; it keeps the
+ * last frame's contents): frames of at most 131072 tiles of 32 x 2 samples (8,388,608 samples: 3840 x 2160, or
+ * 1920 x 1080 with SSAA), larger ones fail with SLIMRT_E_ARG.  Calls that overlap on one GPU (other streams,
+ * other contexts) race on that array and must be serialised by the caller.  The frame's samples equal
+ * slimrt_render_device's bit for bit.  Asynchronous: check the status with slimrt_synchronize. */
+int slimrt_render_device_noalloc(slimrt_context *ctx, const slimrt_render_params *params, void *d_pixels,
+                                 void *d_depths);
 /* Waits for the context's stream and reports the status of the frames since the last call
  * (SLIMRT_E_OVERFLOW if a traversal stack overflowed). */
 int slimrt_synchronize(slimrt_context *ctx);

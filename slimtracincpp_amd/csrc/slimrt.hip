@@ -18,6 +18,7 @@
 #include <thread>
 #include <algorithm>
 #include <cstring>
+#include <mutex>
 #include <cstdio>
 
 #include "slimrt_context.h"
@@ -63,9 +64,12 @@ __device__ __forceinline__ uint32_t cost_rank(uint32_t ticks) {
     return kCostClasses - 2u - ((e - 2u) * 4u + ((ticks >> (e - 2u)) & 3u));      // 7 .. 126, costliest lowest
 }
 
+// cost_tiles (may be null): the tile count the durations in cost were recorded for; any other count gives the
+// image order (the durations belong to another frame size).
 __global__ __launch_bounds__(kOrderThreads) void tile_order_kernel(const uint32_t *__restrict__ cost,
                                                                    uint32_t *__restrict__ order, uint32_t n,
-                                                                   uint32_t resident) {
+                                                                   uint32_t resident,
+                                                                   const uint32_t *__restrict__ cost_tiles) {
     __shared__ uint8_t ranks[kOrderCached];
     __shared__ uint32_t wave_count[kOrderWaves][kWindow + 1];
     __shared__ uint32_t min_rank, max_cost;
@@ -74,6 +78,11 @@ __global__ __launch_bounds__(kOrderThreads) void tile_order_kernel(const uint32_
     const uint32_t rounds = (n + kOrderThreads - 1) / kOrderThreads;   // 64-tile rounds per wave segment
     const uint32_t seg = wave * rounds * 64u;                          // the wave's first tile
     const uint64_t below = (1ull << lane) - 1ull;
+    if (cost_tiles && *cost_tiles != n) {   // uniform: every thread reads the same word
+        for (uint32_t i = t; i < n; i += kOrderThreads) order[i] = i;
+        if (t == 0) order[n] = 0;
+        return;
+    }
     auto rank_at = [&](uint32_t i) -> uint32_t { return i < kOrderCached ? (uint32_t)ranks[i] : cost_rank(cost[i]); };
     if (t == 0) {
         min_rank = kCostClasses;
@@ -211,8 +220,10 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(WIDE && !kCo
 void render_kernel(SceneView s, RenderParams rp, float4 *__restrict__ pixels, float *__restrict__ depths,
                    uint32_t scene_depth, uint32_t mesh_depth, uint32_t tiles_x, uint32_t tiles,
                    uint32_t *__restrict__ status, unsigned long long *__restrict__ counters,
-                   const uint32_t *__restrict__ order, uint32_t *__restrict__ cost, uint32_t split) {
+                   const uint32_t *__restrict__ order, uint32_t *__restrict__ cost, uint32_t split,
+                   uint32_t *__restrict__ cost_tiles) {
     extern __shared__ uint32_t lds[];
+    if (cost_tiles && blockIdx.x == 0 && threadIdx.x == 0) *cost_tiles = tiles;   // what cost will describe
     uint32_t tile = blockIdx.x, half = 2;   // half 0 / 1: lanes 0-31 / 32-63 only; 2: all lanes
     if (order) {
         const uint32_t sp = order[tiles] ? split : 0u;   // order[tiles] = 0: the order kernel chose image order
@@ -378,6 +389,36 @@ __global__ void nrc_gate_fold_kernel(float *__restrict__ cells, uint32_t ncells)
 
 namespace {
 
+// ---------------------------------------------------------------- scratch of the frames that allocate nothing
+// slimrt_render_device_noalloc keeps the tile durations its frames record and the dispatch order it computes from
+// them in one file-scope device array, for frames of up to kNoAllocTiles tiles (32 x 2 samples each).  Words:
+// [0] the tile count the recorded durations belong to, [1, 64) padding, then the durations, then the order
+// (kNoAllocTiles + 1 words: the order and the longest-first flag).  Zero-initialised, so a device's first such
+// frame takes the image order.  Every GPU holds its own copy until the process ends, shared by every context on
+// that GPU.  In host code the array's name is not a device address: the address is looked up once per device
+// (hipGetSymbolAddress, with that device current) and only that address reaches kernels.
+constexpr uint32_t kNoAllocTiles = 131072;   // 8,388,608 samples: 3840 x 2160, or 1920 x 1080 with SSAA
+constexpr uint32_t kNoAllocCost = 64, kNoAllocOrder = kNoAllocCost + kNoAllocTiles;
+constexpr uint32_t kNoAllocWords = kNoAllocOrder + kNoAllocTiles + 1;   // 1,048,836 bytes
+static __device__ __align__(256) uint32_t scratch[kNoAllocWords];
+
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mutex;
+uint32_t *g_scratch[kMaxDevices] = {};
+
+// The device address of `scratch` on `device`, which must be the current device.
+int noalloc_scratch_address(int device, uint32_t **out) {
+    if (device < 0 || device >= kMaxDevices) return SLIMRT_E_DEVICE;
+    std::lock_guard<std::mutex> lock(g_scratch_mutex);
+    if (!g_scratch[device]) {
+        void *ptr = nullptr;
+        if (hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch)) != hipSuccess || !ptr) return SLIMRT_E_DEVICE;
+        g_scratch[device] = static_cast<uint32_t *>(ptr);
+    }
+    *out = g_scratch[device];
+    return SLIMRT_OK;
+}
+
 // Buffers, events and the second stream of cost-ordered frames (created on first use, grown with the tile count).
 int prepare_tile_order(slimrt_context *c, uint32_t tiles) {
     if (!c->side_stream) HIP_TRY(c, hipStreamCreateWithFlags(&c->side_stream, hipStreamNonBlocking));
@@ -457,7 +498,7 @@ int launch(slimrt_context *c, const slimrt_render_params *p, float4 *pixels, flo
 #define RENDER_LAUNCH(COUNT, WIDE)                                                                              \
     hipLaunchKernelGGL((render_kernel<COUNT, WIDE>), dim3(wtiles + split), dim3(64), wlds, c->stream, v, rp,        \
                        pixels, depths, scene_depth, mesh_depth, wtiles_x, wtiles, c->d_status, c->d_counters, order,  \
-                       cost, split)
+                       cost, split, nullptr)
         const bool wide = wide_mesh_bvh(rp);
         if (count) {
             if (wide) RENDER_LAUNCH(true, true); else RENDER_LAUNCH(true, false);
@@ -478,7 +519,7 @@ int launch(slimrt_context *c, const slimrt_render_params *p, float4 *pixels, flo
                 uint32_t resident = 0;
                 if (int rc = resident_waves(c, wide_mesh_bvh(rp), wlds, resident)) return rc;
                 hipLaunchKernelGGL(tile_order_kernel, dim3(1), dim3(kOrderThreads), 0, c->side_stream,
-                                   c->tile_cost[nxt], c->tile_order[nxt], wtiles, resident);
+                                   c->tile_cost[nxt], c->tile_order[nxt], wtiles, resident, nullptr);
                 HIP_TRY(c, hipGetLastError());
                 HIP_TRY(c, hipEventRecord(c->tile_ordered[nxt], c->side_stream));
                 c->order_launched[nxt] = true;
@@ -709,8 +750,10 @@ int slimrt_create(slimrt_context **out, uint32_t flags) {
             hipEventCreate(&c->ev0) != hipSuccess || hipEventCreate(&c->ev1) != hipSuccess ||
             hipMalloc(&c->d_status, sizeof(uint32_t)) != hipSuccess ||
             hipMalloc(&c->d_counters, 8 * sizeof(unsigned long long)) != hipSuccess ||
-            hipMemset(c->d_status, 0, sizeof(uint32_t)) != hipSuccess) {
-            delete c;
+            hipMemset(c->d_status, 0, sizeof(uint32_t)) != hipSuccess ||
+            noalloc_scratch_address(c->device, &c->noalloc_scratch) != SLIMRT_OK) {
+            c->stream = c->own_stream;
+            (void)slimrt_destroy(c);
             return SLIMRT_E_DEVICE;
         }
         c->stream = c->own_stream;
@@ -921,6 +964,63 @@ int slimrt_render_device(slimrt_context *c, const slimrt_render_params *p, void 
     if (!d_pixels || !d_depths) return fail(c, SLIMRT_E_ARG, "output buffers are NULL");
     c->timed = true;
     return launch(c, p, (float4 *)d_pixels, (float *)d_depths, false);
+}
+
+int slimrt_render_device_noalloc(slimrt_context *c, const slimrt_render_params *p, void *d_pixels, void *d_depths) {
+    if (!c) return SLIMRT_E_ARG;
+    int rc = validate_params(c, p);
+    if (rc) return rc;
+    if (c->cpu) return fail(c, SLIMRT_E_STATE, "render_device_noalloc needs a GPU context");
+    if (!d_pixels || !d_depths) return fail(c, SLIMRT_E_ARG, "output buffers are NULL");
+    if (!c->noalloc_scratch) return fail(c, SLIMRT_E_STATE, "no scratch address for this device");
+    if (p->flags & (SLIMRT_WAVEFRONT | SLIMRT_WAVEFRONT_COUNT))
+        return fail(c, SLIMRT_E_ARG, "render_device_noalloc renders per-sample frames only: the wavefront forms "
+                                     "need ray queues the library allocates");
+    RenderParams rp;
+    std::memcpy(&rp, p, sizeof rp);
+    rp.ssaa = rp.ssaa ? 1u : 0u;
+    const uint32_t ws = rp.width << rp.ssaa, hs = rp.height << rp.ssaa;
+    const uint32_t wtiles_x = (ws + kWaveTileX - 1) / kWaveTileX;
+    const uint64_t wtiles64 = (uint64_t)wtiles_x * ((hs + kWaveTileY - 1) / kWaveTileY);
+    if (wtiles64 > kNoAllocTiles)
+        return fail(c, SLIMRT_E_ARG, "render_device_noalloc: the frame has " + std::to_string(wtiles64) +
+                                     " tiles of 32 x 2 samples; at most 131072 (8,388,608 samples) fit its scratch");
+    const uint32_t wtiles = (uint32_t)wtiles64;
+    const bool wide = wide_mesh_bvh(rp);
+    const uint32_t scene_depth = c->scene_height + 1, mesh_depth = mesh_stack_depth(c, wide);
+    const size_t wlds = stack_lds_bytes(scene_depth, mesh_depth, wide, 64);
+    if (wlds > 160 * 1024) return fail(c, SLIMRT_E_ARG, "BVH too deep for the LDS traversal stacks");
+    if (wide && check_wide(c)) return SLIMRT_E_ARG;
+    const bool by_cost = !(rp.flags & SLIMRT_TILE_ORDER_FIXED);
+    uint32_t *cost_tiles = c->noalloc_scratch, *cost = nullptr, *order = nullptr;
+    uint32_t resident = 0;
+    if (by_cost) {
+        cost = c->noalloc_scratch + kNoAllocCost;
+        order = c->noalloc_scratch + kNoAllocOrder;
+        if ((rc = resident_waves(c, wide, wlds, resident))) return rc;
+    }
+    const SceneView v = device_view(c);
+    c->timed = true;
+    HIP_TRY(c, hipEventRecord(c->ev0, c->stream));
+    // On the caller's stream, right before the frame: the order from the durations the last such frame on this
+    // GPU recorded (the image order when they belong to another tile count); one stream, so a graph can hold it.
+    if (by_cost) {
+        hipLaunchKernelGGL(tile_order_kernel, dim3(1), dim3(kOrderThreads), 0, c->stream, cost, order, wtiles,
+                           resident, cost_tiles);
+        HIP_TRY(c, hipGetLastError());
+    }
+    const uint32_t split = by_cost ? std::min(wtiles / kSplitShare, kSplitMax) : 0u;
+    if (wide)
+        hipLaunchKernelGGL((render_kernel<false, true>), dim3(wtiles + split), dim3(64), wlds, c->stream, v, rp,
+                           (float4 *)d_pixels, (float *)d_depths, scene_depth, mesh_depth, wtiles_x, wtiles,
+                           c->d_status, nullptr, order, cost, split, by_cost ? cost_tiles : nullptr);
+    else
+        hipLaunchKernelGGL((render_kernel<false, false>), dim3(wtiles + split), dim3(64), wlds, c->stream, v, rp,
+                           (float4 *)d_pixels, (float *)d_depths, scene_depth, mesh_depth, wtiles_x, wtiles,
+                           c->d_status, nullptr, order, cost, split, by_cost ? cost_tiles : nullptr);
+    HIP_TRY(c, hipGetLastError());
+    HIP_TRY(c, hipEventRecord(c->ev1, c->stream));
+    return SLIMRT_OK;
 }
 
 int slimrt_synchronize(slimrt_context *c) {

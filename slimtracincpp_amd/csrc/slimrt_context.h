@@ -88,6 +88,8 @@ struct slimrt_context {
     bool order_launched[2] = {}, order_used[2] = {};
     uint64_t tile_frame = 0;
     hipEvent_t tile_rendered[2] = {}, tile_ordered[2] = {};
+    // this device's copy of the no-allocation frames' scratch array (its device address, looked up at creation)
+    uint32_t *noalloc_scratch = nullptr;
     uint32_t resident = 0;           // frame-kernel waves the device holds at once, for resident_key's configuration
     uint64_t resident_key = 0;
 

@@ -135,6 +135,7 @@ def lib():
             "slimrt_upload_materials": [vp, vp, u32], "slimrt_upload_lights": [vp, vp, u32],
             "slimrt_upload_scene_bvh": [vp, vp, u32, vp, u32],
             "slimrt_render": [vp, P(RenderParams), vp, vp], "slimrt_render_device": [vp, P(RenderParams), vp, vp],
+            "slimrt_render_device_noalloc": [vp, P(RenderParams), vp, vp],
             "slimrt_synchronize": [vp], "slimrt_count_work": [vp, P(RenderParams), vp],
             "slimrt_last_kernel_ms": [vp, P(f32)],
             "slimrt_host_orientation": [vp, vp],
@@ -426,6 +427,11 @@ class Renderer:
     def render_device(self, params: RenderParams, pixels_ptr: int, depths_ptr: int):
         _check(lib().slimrt_render_device(self.ctx, C.byref(params), C.c_void_p(pixels_ptr), C.c_void_p(depths_ptr)),
                self.ctx)
+
+    def render_device_noalloc(self, params: RenderParams, pixels_ptr: int, depths_ptr: int):
+        """slimrt_render_device without any allocation (per-sample frames; scratch in a static per-GPU array)."""
+        _check(lib().slimrt_render_device_noalloc(self.ctx, C.byref(params), C.c_void_p(pixels_ptr),
+                                                  C.c_void_p(depths_ptr)), self.ctx)
 
     def autotune_form(self, params: RenderParams, pixels_ptr: int, depths_ptr: int, reps: int = 3):
         """Fastest frame form (megakernel or wavefront variants) for these params: (flags, median ms)."""

@@ -59,6 +59,8 @@ def lib():
         L.oracle_intersect.argtypes = [i, vp, vp, i, f, vp]
         L.oracle_build_bvh.argtypes = [vp, u32, u32, vp, vp, vp, vp]
         L.oracle_set_scene_leaf_size.argtypes = [vp, u32]
+        L.oracle_set_plain_quicksort.restype = None
+        L.oracle_set_plain_quicksort.argtypes = [i]
         L.oracle_path_samples.restype = i
         L.oracle_path_samples.argtypes = [vp, i, i, i, vp, vp, vp]
         L.oracle_iteration_counts.restype = i
@@ -188,6 +190,12 @@ def run_reference(spec, *extra: str, timeout: float = 600.0) -> dict:
         out = os.path.join(td, "out.bin")
         subprocess.run([REF_PATH, sp, out, *extra], check=True, capture_output=True, timeout=timeout)
         return read_container(out)
+
+
+def set_plain_quicksort(on: bool) -> None:
+    """Run the builder's quicksort step by step everywhere (True), or with its one-pass result on ranges that are
+    already sorted (False, the default); both give the same BVHs."""
+    lib().oracle_set_plain_quicksort(int(on))
 
 
 def build_bvh(boxes: np.ndarray, max_leaf: int = 1):

@@ -10,7 +10,8 @@
  *   - scene derivation (quaternions, AABBs, BVHs, triangle records, projection) and per-pixel
  *     results: compared bit-for-bit with the compiled reference (oracle/_ref) on seeded scenes and
  *     against committed golden vectors from it (tests/golden);
- *   - mesh BVH build: the reference's own monkey.mesh / cube.mesh assets, bit-for-bit;
+ *   - mesh BVH build: the reference's own monkey.mesh / cube.mesh assets, bit-for-bit; its one-pass result
+ *     for sorted ranges (sorted_runs) against the step-by-step quicksort;
  *   - intersectors, slab test, BRDF terms, tone map, bilinear/mip/cube sampling: closed forms,
  *     double-precision roots and brute force in tests/test_oracle_math.py;
  *   - canvas window content and the wireframe overlays (boxes, BVH node boxes, mesh edges drawn into a
@@ -319,6 +320,36 @@ static void builder_free(Builder *b) {
 
 static inline float axis_max(const aabb *b, int axis) { return axis == 0 ? b->max.x : axis == 1 ? b->max.y : b->max.z; }
 
+/* The quicksort below, applied to a range whose keys never decrease (none NaN), leaves every maximal run of
+ * equal keys rotated right by one: each step finds every element but the run of the pivot's key less than the
+ * pivot (kept in place), swaps the pivot with the first element of that run, and continues on the rest of the run
+ * and on the range before it.  sorted_runs computes that result in one pass (returns 0, ids unchanged, when the
+ * range is not non-decreasing); the step-by-step procedure takes quadratic time on such ranges, which every split
+ * hands to its children along the chosen axis.  Pinned against the step-by-step procedure
+ * (oracle_set_plain_quicksort) on meshes and on tie-heavy random boxes in tests/test_oracle_math.py. */
+static int g_plain_quicksort = 0;
+void oracle_set_plain_quicksort(int on) { g_plain_quicksort = on; }
+
+static int sorted_runs(uint32_t *ids, const Node *nodes, int axis, int32_t start, int32_t end) {
+    float prev = axis_max(&nodes[ids[start]].box, axis);
+    if (prev != prev) return 0;
+    for (int32_t i = start + 1; i <= end; i++) {
+        float k = axis_max(&nodes[ids[i]].box, axis);
+        if (k != k || k < prev) return 0;
+        prev = k;
+    }
+    for (int32_t run = start; run <= end;) {
+        float k = axis_max(&nodes[ids[run]].box, axis);
+        int32_t last = run;
+        while (last < end && !(k < axis_max(&nodes[ids[last + 1]].box, axis))) last++;
+        uint32_t t = ids[last];
+        for (int32_t i = last; i > run; i--) ids[i] = ids[i - 1];
+        ids[run] = t;
+        run = last + 1;
+    }
+    return 1;
+}
+
 /* BVHPartition::partition (bvh_builder.h:15-113) */
 static void partition(Partition *p, int axis, const Node *nodes, int32_t *stack, uint32_t N) {
     uint32_t *ids = p->sorted;
@@ -328,6 +359,7 @@ static void partition(Partition *p, int axis, const Node *nodes, int32_t *stack,
     while (top >= 0) {              /* iterative Lomuto quicksort by aabb.max[axis], pivot = last */
         end = stack[top--];
         start = stack[top--];
+        if (!g_plain_quicksort && end - start >= 16 && sorted_runs(ids, nodes, axis, start, end)) continue;
         int32_t pi = start;
         float pivot = axis_max(&nodes[ids[end]].box, axis);
         for (int32_t i = start; i < end; i++) {

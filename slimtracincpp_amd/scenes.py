@@ -307,7 +307,8 @@ def _hash3(ix, iy, iz, seed):
 
 def value_noise3(p: np.ndarray, seed: int, octaves: int = 4, base_frequency: float = 2.0) -> np.ndarray:
     """Seeded 3D value noise: trilinear with smoothstep weights, `octaves` octaves of doubling frequency and
-    halving amplitude, normalised to [-1, 1].  p: (N, 3) float64."""
+    halving amplitude, normalised to [-1, 1].  p: (N, 3) float64.  The lattice hash is evaluated once per lattice
+    point of the octave's bounding box and gathered (the same values as hashing every corner of every point)."""
     total = np.zeros(p.shape[0])
     amp, norm, freq = 1.0, 0.0, base_frequency
     for o in range(octaves):
@@ -315,6 +316,12 @@ def value_noise3(p: np.ndarray, seed: int, octaves: int = 4, base_frequency: flo
         i0 = np.floor(q).astype(np.int64)
         f = q - i0
         w = f * f * (3.0 - 2.0 * f)
+        lo = i0.min(axis=0)
+        rel = i0 - lo
+        dims = rel.max(axis=0) + 2
+        gx, gy, gz = np.meshgrid(*[np.arange(d, dtype=np.int64) + l for d, l in zip(dims, lo)], indexing="ij")
+        table = _hash3(gx.ravel(), gy.ravel(), gz.ravel(), seed + 7919 * o)
+        base = (rel[:, 0] * dims[1] + rel[:, 1]) * dims[2] + rel[:, 2]
         acc = np.zeros(p.shape[0])
         for dx in (0, 1):
             wx = w[:, 0] if dx else 1 - w[:, 0]
@@ -322,7 +329,7 @@ def value_noise3(p: np.ndarray, seed: int, octaves: int = 4, base_frequency: flo
                 wy = w[:, 1] if dy else 1 - w[:, 1]
                 for dz in (0, 1):
                     wz = w[:, 2] if dz else 1 - w[:, 2]
-                    acc += wx * wy * wz * _hash3(i0[:, 0] + dx, i0[:, 1] + dy, i0[:, 2] + dz, seed + 7919 * o)
+                    acc += wx * wy * wz * table[base + (dx * dims[1] + dy) * dims[2] + dz]
         total += amp * acc
         norm += amp
         amp *= 0.5
@@ -417,10 +424,16 @@ def bake_texel_quads(texels: np.ndarray, wrap: bool) -> np.ndarray:
 
 
 def quantize_quads(pad: np.ndarray) -> np.ndarray:
-    """Padded texels (h+2, w+2, 3) -> (h+1, w+1, 12) u8 quads."""
-    TL, TR, BL, BR = pad[:-1, :-1], pad[:-1, 1:], pad[1:, :-1], pad[1:, 1:]
-    q = np.stack([TL, TR, BL, BR], axis=-1)            # (h+1, w+1, 3, 4)
-    return (q * np.float32(255.0)).astype(np.float32).astype(np.uint8).reshape(q.shape[0], q.shape[1], 12)
+    """Padded texels (h+2, w+2, 3) -> (h+1, w+1, 12) u8 quads.  Each texel is quantised once (x * 255 in float32,
+    truncated) and the quads gather the quantised corners."""
+    u8 = (pad.astype(np.float32) * np.float32(255.0)).astype(np.float32).astype(np.uint8)
+    h1, w1 = u8.shape[0] - 1, u8.shape[1] - 1
+    q = np.empty((h1, w1, 3, 4), np.uint8)             # R{TL,TR,BL,BR} G{..} B{..}
+    q[..., 0] = u8[:-1, :-1]
+    q[..., 1] = u8[:-1, 1:]
+    q[..., 2] = u8[1:, :-1]
+    q[..., 3] = u8[1:, 1:]
+    return q.reshape(h1, w1, 12)
 
 
 def mip_chain(texels: np.ndarray):

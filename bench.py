@@ -13,9 +13,12 @@ Reported:
   ms_per_step  time per frame between two HIP events around --steps back-to-back frames (synchronised before
                and after)
   roofline     HBM roofline of the render kernel: algorithmic bytes per frame / mean kernel time (HIP events on
-               the launch stream); bytes per unit: 64 per sibling-pair visit, 64 per instance visit, 64 per
-               triangle test, 64 per mesh closest-hit attribute fetch, 12 per texel-quad tap, 20 per sample out
-  cpu_baseline the reference's own CPU renderer (compiled from its sources, oracle/_ref/slimref) on this host
+               the launch stream); bytes per unit, counted on the reference's sibling-pair walk (the same for
+               every form): 64 per sibling-pair visit, 64 per instance visit, 64 per triangle test, 64 per mesh
+               closest-hit attribute fetch, 12 per texel-quad tap, 20 per sample out; traffic_frac = measured
+               HBM bytes (committed PMC profile of the form) / kernel time / peak
+  cpu_baseline the reference's own CPU renderer (compiled from its sources, oracle/_ref/slimref) on this host:
+               one full frame with the threads this process may use (at most 16) and with one thread, CPU model
   neural_cache the plain CPU-runnable scene (seed 0) at 1920x1080 with bounce depth 8, rendered over a seeded
                orbit of the camera (300 frames) twice per frame: the plain full-depth trace (the cache's reference) and the
                neural-cache form (cache after the second bounce; 1 in 8 pixels, 2^18 at 1080p, trace full depth
@@ -27,7 +30,7 @@ Reported:
                BVH rebuild, every view's tiles in one launch, launch shape from the autotuner) replayed as one
                HIP graph; baseline: the host rebuild + upload and one launch per view.  Per-frame GPU latency
                p50/p99 (HIP events) and aggregate Mrays/s (rays counted on the first frames' transforms).
-  large        (with --large) the combined 4K workload: 16 instances of a 262,144-triangle displaced sphere,
+  large        (skip with --no-large) the combined 4K workload: 16 instances of a 262,144-triangle displaced sphere,
                32 implicit primitives, 4 emissive quads, 8 + 8 textures at 2048^2, 512^2 cube IBL, depth 8; one
                4K frame in every form, 8 uneven views (4K down to 640x360) in one launch vs one launch per view,
                and the neural cache at the primary hit against the plain trace.
@@ -74,26 +77,49 @@ def algorithmic_bytes(cnt: dict, wide: bool = False) -> int:
     return sum(cnt[k] * b for k, b in per.items()) + cnt["pixels"] * OUT_BYTES_PER_SAMPLE
 
 
-def cpu_baseline(spec, rays_per_frame: int, threads: int) -> dict:
-    """Time the compiled reference renderer on a full frame (best of 2) with `threads` host threads."""
+def cpu_model() -> str:
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
+def time_reference(spec, threads: int, reps: int) -> tuple:
+    """The compiled reference renderer (oracle/_ref/slimref) on one full frame: best-of-`reps` ms and process wall."""
     import oracle
     from slimtracincpp_amd.specfile import write_spec, read_container
-    if not os.path.exists(oracle.REF_PATH):
-        return {"value": None, "unit": "Mrays/s", "cores": 0, "kind": "reference",
-                "sample": "oracle/_ref/slimref not built"}
     with tempfile.TemporaryDirectory() as td:
         sp = write_spec(spec, os.path.join(td, "scene.spec"))
         out = os.path.join(td, "out.bin")
         t0 = time.time()
-        subprocess.run([oracle.REF_PATH, sp, out, "--threads", str(threads), "--reps", "2"], check=True,
+        subprocess.run([oracle.REF_PATH, sp, out, "--threads", str(threads), "--reps", str(reps)], check=True,
                        capture_output=True, timeout=900)
-        wall = time.time() - t0
-        ms = float(read_container(out)["timing"][0])
-    return {"value": rays_per_frame / (ms * 1e-3) / 1e6, "unit": "Mrays/s", "cores": threads, "kind": "reference",
-            "ms_per_frame": ms,
-            "sample": f"one full {spec.width}x{spec.height} frame of the same scene, best of 2, "
-                      f"{threads} threads (rows split), reference renderPixel + closed-form primary rays; "
-                      f"process wall {wall:.1f}s incl. its mesh BVH build"}
+        return float(read_container(out)["timing"][0]), time.time() - t0
+
+
+def cpu_baseline(spec, rays_per_frame: int) -> dict:
+    """The reference's own CPU renderer on this host: one full frame of the same scene, (i) with one thread, as the
+    reference ships it (renderOnCPU is a serial loop over pixels), and (ii) with the host threads this process may
+    use, capped at 16 (the host share of one GPU on these machines), rows split evenly.  value = mode (ii)."""
+    import oracle
+    if not os.path.exists(oracle.REF_PATH):
+        return {"value": None, "unit": "Mrays/s", "cores": 0, "kind": "reference",
+                "sample": "oracle/_ref/slimref not built"}
+    nproc = os.cpu_count() or 1
+    usable = len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else nproc
+    threads = max(1, min(16, usable))
+    ms_n, wall_n = time_reference(spec, threads, 2)
+    ms_1, wall_1 = time_reference(spec, 1, 1)
+    return {"value": rays_per_frame / (ms_n * 1e-3) / 1e6, "unit": "Mrays/s", "cores": threads, "kind": "reference",
+            "ms_per_frame": ms_n, "cpu_model": cpu_model(), "nproc": nproc, "usable_cpus": usable,
+            "single_thread": {"value": rays_per_frame / (ms_1 * 1e-3) / 1e6, "unit": "Mrays/s", "cores": 1,
+                              "ms_per_frame": ms_1},
+            "sample": f"one full {spec.width}x{spec.height} frame of the same scene with the reference's renderPixel "
+                      f"and closed-form primary rays: {threads} threads (rows split, best of 2; process wall "
+                      f"{wall_n:.1f}s incl. its mesh BVH build) and 1 thread (one frame; wall {wall_1:.1f}s)"}
 
 
 def orbit_camera(frame: int, frames: int, radius: float = 8.246, height: float = 2.0, target=(0.0, 0.0, 0.0)):
@@ -314,7 +340,8 @@ def main():
     ap.add_argument("--no-neural-cache", action="store_true")
     ap.add_argument("--nrc-frames", type=int, default=300)
     ap.add_argument("--no-views", action="store_true")
-    ap.add_argument("--large", action="store_true", help="also run the combined 4K workload (about 2 minutes)")
+    ap.add_argument("--no-large", action="store_true", help="skip the combined 4K workload (about 1 minute)")
+    ap.add_argument("--large", action="store_true", help=argparse.SUPPRESS)   # the 4K workload runs by default
     ap.add_argument("--view-frames", type=int, default=20)
     ap.add_argument("--form", choices=FORM_NAMES, default=None,
                     help="time only this frame form (for profiler runs); by default every form, the fastest reported")
@@ -400,8 +427,12 @@ def main():
     wide = form.endswith("_bvh4")
     rays = cnt["rays"]
     mrays = rays / (ms_step * 1e-3) / 1e6
-    bytes_frame = algorithmic_bytes(cnt4 if wide else cnt, wide)
+    # Algorithmic bytes on the reference's own traversal (sibling-pair walk, counts equal to the oracle's): the
+    # same work for every frame form.  The 4-wide walk's own node bytes (128 per node visit) are reported beside.
+    bytes_frame = algorithmic_bytes(cnt)
+    bytes_form = algorithmic_bytes(cnt4, True) if wide else bytes_frame
     achieved = bytes_frame / (k_ms * 1e-3) / 1e9
+    traffic = measured_traffic(args.width, args.height, form)
     result = {
         "metric": "Mrays/s + ms/frame at 1080p, plain vs neural-cache at PSNR>=40 dB vs plain",
         "value": mrays,
@@ -423,7 +454,12 @@ def main():
                    "wall_ms_per_step": wall_ms, "mode": "plain (full-depth trace)", "form": form,
                    "forms_ms_per_step": {k: v["ms_per_step"] for k, v in timings.items()}},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "traffic": measured_traffic(args.width, args.height, form),
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
+                     "traffic_gbs": traffic / (k_ms * 1e-3) / 1e9 if traffic else None,
+                     "traffic_frac": traffic / (k_ms * 1e-3) / 1e9 / HBM_PEAK_GBS if traffic else None,
+                     "basis": "algorithmic bytes of the reference's sibling-pair walk (64 B per pair visit); "
+                              "traffic = measured HBM bytes per frame (PMC FETCH_SIZE + WRITE_SIZE, calibrated)",
+                     "achieved_form_bytes": bytes_form / (k_ms * 1e-3) / 1e9, "form_bytes": bytes_form,
                      "kernel": ("4-wide mesh BVHs; " if wide else "") + {
                                 "megakernel": "render_kernel<false> (tiles dispatched costliest first; their order, from "
                                               "the tile durations of two frames before, is computed by "
@@ -442,7 +478,7 @@ def main():
                                 "wavefront_sorted_split": "the frame's launches: wf_trace_kernel + wf_primary_kernel + "
                                                           "octant sort + wf_bounce_kernel<true>"}[form.replace("_bvh4", "")],
                      "kernel_ms": k_ms, "algorithmic_bytes": bytes_frame,
-                     "counters": cnt4 if wide else cnt},
+                     "counters": cnt, "counters_form": cnt4 if wide else cnt},
         "neural_cache": None,
     }
     if form in ("megakernel", "megakernel_bvh4"):
@@ -453,15 +489,20 @@ def main():
         costs, order = r.tile_costs()
         if costs.size:
             c = costs.astype(np.int64)
-            ideal_us = float(c.sum()) / r.tile_resident / 100.0   # summed durations over the resident waves
+            # the ideal frame: the summed wave durations over the resident waves (a split tile records twice the
+            # duration of its later half, which stands for its two half waves)
+            ideal_us = float(c.sum()) / r.tile_resident / 100.0
+            wave = c.copy()   # the duration of each tile's longest wave
+            ordered = not np.array_equal(order, np.arange(order.size))
+            if ordered:
+                wave[order[:min(c.size // 32, 2048)]] //= 2
             result["tile_costs"] = {
-                "tiles": int(c.size), "resident_waves": r.tile_resident, "median_us": float(np.median(c)) / 100.0,
-                "p99_us": float(np.percentile(c, 99)) / 100.0, "longest_us": float(c.max()) / 100.0,
-                "ideal_frame_us": ideal_us,
-                "order": "image" if np.array_equal(order, np.arange(order.size)) else "longest first"}
+                "tiles": int(c.size), "resident_waves": r.tile_resident, "median_us": float(np.median(wave)) / 100.0,
+                "p99_us": float(np.percentile(wave, 99)) / 100.0, "longest_us": float(wave.max()) / 100.0,
+                "ideal_frame_us": ideal_us, "order": "longest first" if ordered else "image"}
     if not args.no_views:
         result["batched_views"] = batched_views_leg(args.view_frames, stream, spec)
-    if args.large:
+    if not args.no_large:
         result["large"] = large_leg(stream)
     if not args.no_neural_cache:
         legs = [neural_cache_leg(args.nrc_frames, args.width, args.height, stream, d) for d in (2, 0)]
@@ -480,8 +521,7 @@ def main():
                                             "around (0, 1, 0.4)",
                                 "after_second_bounce": mesh_legs[0], "at_primary_hit": mesh_legs[1]}}
     if not args.no_cpu_baseline:
-        threads = min(16, os.cpu_count() or 1)
-        result["cpu_baseline"] = cpu_baseline(spec, rays, threads)
+        result["cpu_baseline"] = cpu_baseline(spec, rays)
     print(json.dumps(result))
 
 

@@ -215,7 +215,9 @@ __global__ __launch_bounds__(kOrderThreads) void tile_order_kernel(const uint32_
 // is set, the first `split` tiles of the order (the costliest) are each rendered by two workgroups, lanes 0-31
 // and lanes 32-63: a costly tile's 64 paths diverge, and half as many lanes per wave shorten the frame's longest
 // waves (0.77 -> 0.65 ms on the mesh + PBR frame; quarter waves gain nothing and lose fast as their count grows).
-template <bool kCount, bool WIDE>
+// LIGHTS false (scenes without lights or emissive quads) leaves the shadow rays and the glow out: no VGPR spills
+// at three waves per SIMD.
+template <bool kCount, bool WIDE, bool LIGHTS = true>
 __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(WIDE && !kCount ? 3 : 1)))
 void render_kernel(SceneView s, RenderParams rp, float4 *__restrict__ pixels, float *__restrict__ depths,
                    uint32_t scene_depth, uint32_t mesh_depth, uint32_t tiles_x, uint32_t tiles,
@@ -246,7 +248,7 @@ void render_kernel(SceneView s, RenderParams rp, float4 *__restrict__ pixels, fl
     tr.cnt = kCount ? &cnt : nullptr;
     if ((half == 2u || (lane >> 5) == half) && x < ws && y < hs) {
         float px[4], depth;
-        render_sample(s, rp, x, y, tr, px, depth);
+        render_sample<TracerT<WIDE>, LIGHTS>(s, rp, x, y, tr, px, depth);
         const uint32_t o = canvas_offset(rp, x, y);
         pixels[o] = make_float4(px[0], px[1], px[2], px[3]);
         depths[o] = depth;
@@ -448,12 +450,13 @@ int prepare_tile_order(slimrt_context *c, uint32_t tiles) {
 
 // Waves of the frame kernel the device holds at once (its workgroups are single waves), for this node width and
 // stack size; cached per configuration.
-int resident_waves(slimrt_context *c, bool wide, size_t lds, uint32_t &out) {
-    const uint64_t key = ((uint64_t)lds << 1) | (wide ? 1u : 0u);
+int resident_waves(slimrt_context *c, bool wide, bool lights, size_t lds, uint32_t &out) {
+    const uint64_t key = ((uint64_t)lds << 2) | (lights ? 2u : 0u) | (wide ? 1u : 0u);
     if (c->resident_key != key || !c->resident) {
         int per_cu = 0, cus = 0;
-        HIP_TRY(c, wide ? hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, render_kernel<false, true>, 64, lds)
-                        : hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, render_kernel<false, false>, 64, lds));
+        auto occupancy = [&](auto kernel) { return hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kernel, 64, lds); };
+        HIP_TRY(c, wide ? (lights ? occupancy(render_kernel<false, true, true>) : occupancy(render_kernel<false, true, false>))
+                        : (lights ? occupancy(render_kernel<false, false, true>) : occupancy(render_kernel<false, false, false>)));
         HIP_TRY(c, hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, c->device));
         c->resident = (uint32_t)std::max(1, per_cu * cus);
         c->resident_key = key;
@@ -495,15 +498,17 @@ int launch(slimrt_context *c, const slimrt_render_params *p, float4 *pixels, flo
             cost = c->tile_cost[par];
         }
         const uint32_t split = order ? std::min(wtiles / kSplitShare, kSplitMax) : 0u;
-#define RENDER_LAUNCH(COUNT, WIDE)                                                                              \
-    hipLaunchKernelGGL((render_kernel<COUNT, WIDE>), dim3(wtiles + split), dim3(64), wlds, c->stream, v, rp,        \
-                       pixels, depths, scene_depth, mesh_depth, wtiles_x, wtiles, c->d_status, c->d_counters, order,  \
-                       cost, split, nullptr)
+#define RENDER_LAUNCH(COUNT, WIDE, LIGHTS)                                                                      \
+    hipLaunchKernelGGL((render_kernel<COUNT, WIDE, LIGHTS>), dim3(wtiles + split), dim3(64), wlds, c->stream, v,    \
+                       rp, pixels, depths, scene_depth, mesh_depth, wtiles_x, wtiles, c->d_status, c->d_counters,     \
+                       order, cost, split, nullptr)
         const bool wide = wide_mesh_bvh(rp);
         if (count) {
-            if (wide) RENDER_LAUNCH(true, true); else RENDER_LAUNCH(true, false);
+            if (wide) RENDER_LAUNCH(true, true, true); else RENDER_LAUNCH(true, false, true);
+        } else if (scene_has_lights(v)) {
+            if (wide) RENDER_LAUNCH(false, true, true); else RENDER_LAUNCH(false, false, true);
         } else {
-            if (wide) RENDER_LAUNCH(false, true); else RENDER_LAUNCH(false, false);
+            if (wide) RENDER_LAUNCH(false, true, false); else RENDER_LAUNCH(false, false, false);
         }
 #undef RENDER_LAUNCH
         HIP_TRY(c, hipGetLastError());
@@ -517,7 +522,7 @@ int launch(slimrt_context *c, const slimrt_render_params *p, float4 *pixels, flo
             if (c->cost_tiles[nxt] == wtiles) {
                 HIP_TRY(c, hipStreamWaitEvent(c->side_stream, c->tile_rendered[nxt], 0));
                 uint32_t resident = 0;
-                if (int rc = resident_waves(c, wide_mesh_bvh(rp), wlds, resident)) return rc;
+                if (int rc = resident_waves(c, wide_mesh_bvh(rp), scene_has_lights(v), wlds, resident)) return rc;
                 hipLaunchKernelGGL(tile_order_kernel, dim3(1), dim3(kOrderThreads), 0, c->side_stream,
                                    c->tile_cost[nxt], c->tile_order[nxt], wtiles, resident, nullptr);
                 HIP_TRY(c, hipGetLastError());
@@ -997,7 +1002,7 @@ int slimrt_render_device_noalloc(slimrt_context *c, const slimrt_render_params *
     if (by_cost) {
         cost = c->noalloc_scratch + kNoAllocCost;
         order = c->noalloc_scratch + kNoAllocOrder;
-        if ((rc = resident_waves(c, wide, wlds, resident))) return rc;
+        if ((rc = resident_waves(c, wide, scene_has_lights(device_view(c)), wlds, resident))) return rc;
     }
     const SceneView v = device_view(c);
     c->timed = true;
@@ -1010,14 +1015,16 @@ int slimrt_render_device_noalloc(slimrt_context *c, const slimrt_render_params *
         HIP_TRY(c, hipGetLastError());
     }
     const uint32_t split = by_cost ? std::min(wtiles / kSplitShare, kSplitMax) : 0u;
-    if (wide)
-        hipLaunchKernelGGL((render_kernel<false, true>), dim3(wtiles + split), dim3(64), wlds, c->stream, v, rp,
-                           (float4 *)d_pixels, (float *)d_depths, scene_depth, mesh_depth, wtiles_x, wtiles,
-                           c->d_status, nullptr, order, cost, split, by_cost ? cost_tiles : nullptr);
-    else
-        hipLaunchKernelGGL((render_kernel<false, false>), dim3(wtiles + split), dim3(64), wlds, c->stream, v, rp,
-                           (float4 *)d_pixels, (float *)d_depths, scene_depth, mesh_depth, wtiles_x, wtiles,
-                           c->d_status, nullptr, order, cost, split, by_cost ? cost_tiles : nullptr);
+#define NOALLOC_LAUNCH(WIDE, LIGHTS)                                                                              \
+    hipLaunchKernelGGL((render_kernel<false, WIDE, LIGHTS>), dim3(wtiles + split), dim3(64), wlds, c->stream, v, rp, \
+                       (float4 *)d_pixels, (float *)d_depths, scene_depth, mesh_depth, wtiles_x, wtiles, c->d_status,  \
+                       nullptr, order, cost, split, by_cost ? cost_tiles : nullptr)
+    if (scene_has_lights(v)) {
+        if (wide) NOALLOC_LAUNCH(true, true); else NOALLOC_LAUNCH(false, true);
+    } else {
+        if (wide) NOALLOC_LAUNCH(true, false); else NOALLOC_LAUNCH(false, false);
+    }
+#undef NOALLOC_LAUNCH
     HIP_TRY(c, hipGetLastError());
     HIP_TRY(c, hipEventRecord(c->ev1, c->stream));
     return SLIMRT_OK;

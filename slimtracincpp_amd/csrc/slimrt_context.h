@@ -234,6 +234,9 @@ inline SceneView device_view(const slimrt_context *c) {
     return v;
 }
 
+// Whether the integrator's light loops run at all: point or directional lights, or emissive quads.
+inline bool scene_has_lights(const SceneView &v) { return v.L > 0 || (v.flags & 1u) != 0; }
+
 inline void update_scene_flags(slimrt_context *c) {   // Scene::Scene (scene.h:239-242)
     c->scene_flags = 0;
     for (const Geo &g : c->geos)

@@ -564,8 +564,10 @@ SLIM_HD void primary_ray(const Projection &p, uint32_t x, uint32_t y, v3 &dir, f
           v3{p.right[0], p.right[1], p.right[2]} * (float)(int32_t)x;
 }
 
-// renderPixelBeauty (src/slim/renderer/ray_tracer.h:19-108)
-template <class TR>
+// renderPixelBeauty (src/slim/renderer/ray_tracer.h:19-108).  LIGHTS false: for scenes without lights and without
+// emissive quads, where the light loops run zero times; leaving them out spares a kernel the registers of shadow
+// rays and glow.
+template <class TR, bool LIGHTS = true>
 SLIM_HD void render_beauty(const SceneView &s, const RenderParams &rp, v3 direction, float scaling, TR &tr,
                            color &out, float &depth) {
     Surface sf;
@@ -590,8 +592,10 @@ SLIM_HD void render_beauty(const SceneView &s, const RenderParams &rp, v3 direct
             } else {
                 prepare_for_shading(sf, s, ray, hit, tr);
                 if (depth_left == rp.max_depth) depth = depth_at(p, hit.position);
-                for (uint32_t i = 0; i < s.L; i++) shade_from_light(sf, s.lights[i], s, tr, cur);
-                if (s.flags & 1u) shade_from_emissive_quads(sf, s, cur);
+                if constexpr (LIGHTS) {
+                    for (uint32_t i = 0; i < s.L; i++) shade_from_light(sf, s.lights[i], s, tr, cur);
+                    if (s.flags & 1u) shade_from_emissive_quads(sf, s, cur);
+                }
                 if (rp.sky_irradiance >= 0 && rp.sky_radiance >= 0) {
                     sf.L = sf.N;
                     sf.NdotL = 1.0f;
@@ -612,10 +616,12 @@ SLIM_HD void render_beauty(const SceneView &s, const RenderParams &rp, v3 direct
             if (rp.sky_color >= 0) SLIM_COUNT(tr, taps);
             if (rp.sky_color >= 0) cur = tex_sample_cube(s, (uint32_t)rp.sky_color, ray.direction.x, ray.direction.y, ray.direction.z);
         }
-        for (uint32_t i = 0; i < s.L; i++) {
-            const Light l = s.lights[i];
-            float density;
-            if (light_glow(l, ray, hit.distance, density)) cur = cscale_add(color{l.r, l.g, l.b}, glow_factor(density), cur);
+        if constexpr (LIGHTS) {
+            for (uint32_t i = 0; i < s.L; i++) {
+                const Light l = s.lights[i];
+                float density;
+                if (light_glow(l, ray, hit.distance, density)) cur = cscale_add(color{l.r, l.g, l.b}, glow_factor(density), cur);
+            }
         }
         col = cmul_add(cur, tp, col);
         tp = tp * next_tp;
@@ -681,7 +687,7 @@ SLIM_HD void output_value(const RenderParams &rp, color c, float out_px[4]) {
 }
 
 // One sample: primary ray, integrator, and the value the canvas or the radiance buffer receives.
-template <class TR>
+template <class TR, bool LIGHTS = true>
 SLIM_HD void render_sample(const SceneView &s, const RenderParams &rp, uint32_t x, uint32_t y, TR &tr,
                            float out_px[4], float &out_depth) {
     v3 dir;
@@ -703,7 +709,7 @@ SLIM_HD void render_sample(const SceneView &s, const RenderParams &rp, uint32_t 
         return;
     }
     color c;
-    if (rp.render_mode == RM_Beauty) render_beauty(s, rp, dir, scaling, tr, c, out_depth);
+    if (rp.render_mode == RM_Beauty) render_beauty<TR, LIGHTS>(s, rp, dir, scaling, tr, c, out_depth);
     else render_debug(s, rp, dir, scaling, tr, c, out_depth);
     if (tr.cnt) tr.cnt->pixels++;
     output_value(rp, c, out_px);

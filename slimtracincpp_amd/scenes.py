@@ -717,6 +717,17 @@ def instance_transforms(spec: SceneSpec, rng: PCG32) -> np.ndarray:
     return geo_f.astype(np.float32)
 
 
+def primitive_transforms(spec: SceneSpec, rng: PCG32, indices=(2, 3, 4)) -> np.ndarray:
+    """New transforms for the implicit primitives `indices` of `spec` (the plain scene's spheres 1-3 by default;
+    the others keep theirs): position U([-4, 4] x [0.5, 3] x [-4, 4]) and Euler angles U[0, 2pi)^3, as the plain
+    scene places them; scales are kept.  Draw order per primitive: position xyz, euler xyz."""
+    geo_f = spec.geo_f.copy()
+    for i in indices:
+        geo_f[i, 3:6] = (rng.uniform(-4, 4), rng.uniform(0.5, 3), rng.uniform(-4, 4))
+        geo_f[i, 0:3] = (rng.uniform(0, 2 * math.pi), rng.uniform(0, 2 * math.pi), rng.uniform(0, 2 * math.pi))
+    return geo_f.astype(np.float32)
+
+
 def batched_views(count: int = 48, seed: int = 0, resolutions=VIEW_RESOLUTIONS):
     """`count` cameras on spheres of radius U[6, 12] around the origin (uniform direction, upper hemisphere
     kept above the floor), looking at the origin with a vertical FOV U[40, 90] degrees, at resolutions drawn

@@ -23,7 +23,7 @@ extern "C" {
 #define SLIMRT_OK 0
 #define SLIMRT_E_ARG (-1)        /* invalid argument */
 #define SLIMRT_E_DEVICE (-2)     /* no usable GPU, or a HIP runtime error */
-#define SLIMRT_E_MEMORY (-3)     /* device allocation failed */
+#define SLIMRT_E_MEMORY (-3)     /* device allocation failed, or beyond the context's memory limit */
 #define SLIMRT_E_STATE (-4)      /* call out of order (e.g. render before a scene upload) */
 #define SLIMRT_E_OVERFLOW (-5)   /* a traversal stack would have overflowed; results of that frame are invalid */
 
@@ -228,6 +228,22 @@ int slimrt_render_device_noalloc(slimrt_context *ctx, const slimrt_render_params
 /* Waits for the context's stream and reports the status of the frames since the last call
  * (SLIMRT_E_OVERFLOW if a traversal stack overflowed). */
 int slimrt_synchronize(slimrt_context *ctx);
+/* Status flags of the frames since the last call, then cleared (no reference counterpart; the reference has no
+ * device-side checks, src/slim/core/base.h:20-26): SLIMRT_FLAG_STACK_OVERFLOW and SLIMRT_FLAG_QUEUE_OVERFLOW (also
+ * reported as SLIMRT_E_OVERFLOW by the next slimrt_synchronize or synchronous call), SLIMRT_FLAG_NAN_SAMPLE: some
+ * sample's radiance, canvas value or depth came out NaN (for example from NaN material or light inputs; the
+ * reference passes such values through silently, and so do the samples: this flag only reports it).
+ * Synchronizes the context's stream. */
+#define SLIMRT_FLAG_STACK_OVERFLOW 1u
+#define SLIMRT_FLAG_QUEUE_OVERFLOW 2u
+#define SLIMRT_FLAG_NAN_SAMPLE 4u
+int slimrt_frame_flags(slimrt_context *ctx, uint32_t *flags);
+/* A limit on the device memory the context holds (no reference counterpart): every allocation it makes from
+ * then on (scene arrays, canvases, queues, cache buffers) that would take its total past `bytes` fails with
+ * SLIMRT_E_MEMORY, leaving what it already holds in place; 0 removes the limit.  slimrt_memory_usage reports the
+ * bytes held and the limit (either pointer may be NULL). */
+int slimrt_set_memory_limit(slimrt_context *ctx, uint64_t bytes);
+int slimrt_memory_usage(slimrt_context *ctx, uint64_t *used, uint64_t *limit);
 /* Renders one frame with work counters and returns them: rays (SceneTracer::trace calls), sibling-pair
  * visits, root-box tests, instance (leaf geometry) visits, triangle tests, mesh closest-hit attribute
  * fetches, texel-quad taps, samples.  Slower than slimrt_render; for measurement only. */

@@ -252,7 +252,8 @@ void render_kernel(SceneView s, RenderParams rp, float4 *__restrict__ pixels, fl
         const uint32_t o = canvas_offset(rp, x, y);
         pixels[o] = make_float4(px[0], px[1], px[2], px[3]);
         depths[o] = depth;
-        if (tr.overflow) atomicOr(status, 1u);
+        tr.status |= nan_status(rp, px, depth);
+        if (tr.status) atomicOr(status, tr.status);
     }
     if (cost && lane == 0) {
         // A split tile records twice the duration of its half that ends last: a half wave runs about half its
@@ -306,9 +307,11 @@ __global__ __launch_bounds__(64) void nrc_frame_kernel(SceneView s, RenderParams
             float px[4];
             output_value(rp, color{tone_map(col.r), tone_map(col.g), tone_map(col.b)}, px);
             pixels[o] = make_float4(px[0], px[1], px[2], px[3]);
+            tr.status |= nan_status(rp, px, depth);
         }
         depths[o] = depth;
-        if (tr.overflow) atomicOr(status, 1u);
+        if (depth != depth) tr.status |= kStatusNaN;
+        if (tr.status) atomicOr(status, tr.status);
         if (training) (void)wave_append(true, nf.counters + NC_TRAIN_PIXELS);
     }
 }
@@ -317,7 +320,7 @@ __global__ __launch_bounds__(64) void nrc_frame_kernel(SceneView s, RenderParams
 __global__ void nrc_composite_kernel(RenderParams rp, const uint32_t *__restrict__ q_count, uint32_t q_cap,
                                      const uint32_t *__restrict__ q_pixel, const float *__restrict__ q_prefix,
                                      const float *__restrict__ q_tp, const float *__restrict__ radiance,
-                                     float4 *__restrict__ pixels) {
+                                     float4 *__restrict__ pixels, uint32_t *__restrict__ status) {
     const uint32_t n = min(*q_count, q_cap);
     for (uint32_t q = blockIdx.x * blockDim.x + threadIdx.x; q < n; q += gridDim.x * blockDim.x) {
         const color L{fmaxf(radiance[3 * q], 0.0f), fmaxf(radiance[3 * q + 1], 0.0f), fmaxf(radiance[3 * q + 2], 0.0f)};
@@ -326,6 +329,7 @@ __global__ void nrc_composite_kernel(RenderParams rp, const uint32_t *__restrict
         float px[4];
         output_value(rp, color{tone_map(col.r), tone_map(col.g), tone_map(col.b)}, px);
         pixels[q_pixel[q]] = make_float4(px[0], px[1], px[2], px[3]);
+        if (nan_status(rp, px, 0.0f)) atomicOr(status, kStatusNaN);
     }
 }
 
@@ -351,7 +355,7 @@ __global__ __launch_bounds__(kBlock) void pick_kernel(SceneView s, RenderParams 
     pick(s, rp, x, y, tr, oi, of);
     for (int k = 0; k < 3; k++) out[k] = oi[k];
     for (int k = 0; k < 4; k++) out[3 + k] = __builtin_bit_cast(int32_t, of[k]);
-    if (tr.overflow) atomicOr(status, 1u);
+    if (tr.status) atomicOr(status, tr.status);
 }
 
 // Clears the frame counters and step losses (a kernel rather than a memset, so the frame can be captured).
@@ -433,7 +437,7 @@ int prepare_tile_order(slimrt_context *c, uint32_t tiles) {
     HIP_TRY(c, hipStreamSynchronize(c->stream));
     for (int p = 0; p < 2; p++) {
         for (uint32_t **b : {&c->tile_cost[p], &c->tile_order[p]}) {
-            if (*b) HIP_TRY(c, hipFree(*b));
+            if (*b) HIP_TRY(c, ctx_free(c, *b));
             *b = nullptr;
         }
         c->cost_tiles[p] = c->order_tiles[p] = 0;
@@ -441,8 +445,8 @@ int prepare_tile_order(slimrt_context *c, uint32_t tiles) {
     }
     c->tile_cap = 0;
     for (int p = 0; p < 2; p++) {
-        HIP_TRY(c, hipMalloc(&c->tile_cost[p], tiles * sizeof(uint32_t)));
-        HIP_TRY(c, hipMalloc(&c->tile_order[p], (tiles + 1) * sizeof(uint32_t)));   // + the longest-first flag
+        HIP_TRY(c, ctx_malloc(c, &c->tile_cost[p], tiles * sizeof(uint32_t)));
+        HIP_TRY(c, ctx_malloc(c, &c->tile_order[p], (tiles + 1) * sizeof(uint32_t)));   // + the longest-first flag
     }
     c->tile_cap = tiles;
     return SLIMRT_OK;
@@ -554,7 +558,7 @@ int cpu_render(slimrt_context *c, const slimrt_render_params *p, slimrt_pixel *p
         tr.mesh_stack = Stack{stack.data() + sd, 1};
         tr.scene_depth = sd;
         tr.mesh_depth = md;
-        tr.overflow = 0;
+        tr.status = 0;
         per[ti] = Counters{};
         tr.cnt = counters ? &per[ti] : nullptr;   // the host context runs the reference's sibling-pair walk
         for (uint32_t y = ti; y < hs; y += nthreads)
@@ -565,9 +569,10 @@ int cpu_render(slimrt_context *c, const slimrt_render_params *p, slimrt_pixel *p
                     const uint32_t o = canvas_offset(rp, x, y);
                     pixels[o] = slimrt_pixel{px[0], px[1], px[2], px[3]};
                     depths[o] = depth;
+                    tr.status |= nan_status(rp, px, depth);
                 }
             }
-        overflow[ti] = tr.overflow;
+        overflow[ti] = tr.status;
     };
     std::vector<std::thread> pool;
     for (unsigned i = 0; i < nthreads; i++) pool.emplace_back(work, i);
@@ -579,8 +584,10 @@ int cpu_render(slimrt_context *c, const slimrt_render_params *p, slimrt_pixel *p
             for (int i = 0; i < 8; i++) counters[i] += s[i];
         }
     }
-    for (uint32_t o : overflow)
-        if (o) return fail(c, SLIMRT_E_OVERFLOW, "traversal stack overflow");
+    for (uint32_t o : overflow) {
+        c->host_status |= o & kStatusNaN;
+        if (o & kStatusStack) return fail(c, SLIMRT_E_OVERFLOW, "traversal stack overflow");
+    }
     return SLIMRT_OK;
 }
 
@@ -668,11 +675,11 @@ uint32_t collapse_mesh_bvh4(const Node *n2, std::vector<Node4> &out, uint32_t &s
 
 int ensure_outputs(slimrt_context *c, size_t n) {
     if (c->d_pixels.cap < n) {
-        dev_free(c->d_pixels);
-        dev_free(c->d_depths);
-        HIP_TRY(c, hipMalloc(&c->d_pixels.ptr, n * sizeof(float4)));
+        dev_free(c, c->d_pixels);
+        dev_free(c, c->d_depths);
+        HIP_TRY(c, ctx_malloc(c, &c->d_pixels.ptr, n * sizeof(float4)));
         c->d_pixels.cap = n;
-        HIP_TRY(c, hipMalloc(&c->d_depths.ptr, n * sizeof(float)));
+        HIP_TRY(c, ctx_malloc(c, &c->d_depths.ptr, n * sizeof(float)));
         c->d_depths.cap = n;
     }
     return SLIMRT_OK;
@@ -684,7 +691,7 @@ void free_nrc(slimrt_context *c) {
     for (void *p : {(void *)c->q_raw, (void *)c->q_prefix, (void *)c->q_tp, (void *)c->q_out, (void *)c->q_pixel,
                     (void *)c->t_raw, (void *)c->t_target, (void *)c->t_next, (void *)c->t_link,
                     (void *)c->nrc_counters, (void *)c->nrc_losses, (void *)c->cells})
-        if (p) (void)hipFree(p);
+        if (p) (void)ctx_free(c, p);
     c->q_raw = c->q_prefix = c->q_tp = c->q_out = c->t_raw = c->t_target = c->t_next = c->nrc_losses = c->cells = nullptr;
     c->q_pixel = c->nrc_counters = c->t_link = nullptr;
     c->q_cap = c->t_cap = c->cells_log2 = 0;
@@ -693,39 +700,39 @@ void free_nrc(slimrt_context *c) {
 int ensure_nrc(slimrt_context *c, uint32_t q_cap, uint32_t t_cap, uint32_t cells_log2, bool &cells_new) {
     cells_new = false;
     if (!c->nrc_counters) {
-        HIP_TRY(c, hipMalloc(&c->nrc_counters, NC_COUNT * sizeof(uint32_t)));
-        HIP_TRY(c, hipMalloc(&c->nrc_losses, kMaxTrainSteps * sizeof(float)));
+        HIP_TRY(c, ctx_malloc(c, &c->nrc_counters, NC_COUNT * sizeof(uint32_t)));
+        HIP_TRY(c, ctx_malloc(c, &c->nrc_losses, kMaxTrainSteps * sizeof(float)));
     }
     if (c->q_cap < q_cap) {
         for (void *p : {(void *)c->q_raw, (void *)c->q_prefix, (void *)c->q_tp, (void *)c->q_out, (void *)c->q_pixel})
-            if (p) HIP_TRY(c, hipFree(p));
+            if (p) HIP_TRY(c, ctx_free(c, p));
         c->q_raw = c->q_prefix = c->q_tp = c->q_out = nullptr;
         c->q_pixel = nullptr;
         c->q_cap = 0;
-        HIP_TRY(c, hipMalloc(&c->q_raw, (size_t)q_cap * kNrcRaw * sizeof(float)));
-        HIP_TRY(c, hipMalloc(&c->q_prefix, (size_t)q_cap * 3 * sizeof(float)));
-        HIP_TRY(c, hipMalloc(&c->q_tp, (size_t)q_cap * 3 * sizeof(float)));
-        HIP_TRY(c, hipMalloc(&c->q_out, (size_t)q_cap * 3 * sizeof(float)));
-        HIP_TRY(c, hipMalloc(&c->q_pixel, (size_t)q_cap * sizeof(uint32_t)));
+        HIP_TRY(c, ctx_malloc(c, &c->q_raw, (size_t)q_cap * kNrcRaw * sizeof(float)));
+        HIP_TRY(c, ctx_malloc(c, &c->q_prefix, (size_t)q_cap * 3 * sizeof(float)));
+        HIP_TRY(c, ctx_malloc(c, &c->q_tp, (size_t)q_cap * 3 * sizeof(float)));
+        HIP_TRY(c, ctx_malloc(c, &c->q_out, (size_t)q_cap * 3 * sizeof(float)));
+        HIP_TRY(c, ctx_malloc(c, &c->q_pixel, (size_t)q_cap * sizeof(uint32_t)));
         c->q_cap = q_cap;
     }
     if (c->t_cap < t_cap) {
         for (void *p : {(void *)c->t_raw, (void *)c->t_target, (void *)c->t_next, (void *)c->t_link})
-            if (p) HIP_TRY(c, hipFree(p));
+            if (p) HIP_TRY(c, ctx_free(c, p));
         c->t_raw = c->t_target = c->t_next = nullptr;
         c->t_link = nullptr;
         c->t_cap = 0;
-        HIP_TRY(c, hipMalloc(&c->t_raw, (size_t)t_cap * kNrcRaw * sizeof(float)));
-        HIP_TRY(c, hipMalloc(&c->t_target, (size_t)t_cap * 3 * sizeof(float)));
-        HIP_TRY(c, hipMalloc(&c->t_next, (size_t)t_cap * 3 * sizeof(float)));
-        HIP_TRY(c, hipMalloc(&c->t_link, (size_t)t_cap * sizeof(uint32_t)));
+        HIP_TRY(c, ctx_malloc(c, &c->t_raw, (size_t)t_cap * kNrcRaw * sizeof(float)));
+        HIP_TRY(c, ctx_malloc(c, &c->t_target, (size_t)t_cap * 3 * sizeof(float)));
+        HIP_TRY(c, ctx_malloc(c, &c->t_next, (size_t)t_cap * 3 * sizeof(float)));
+        HIP_TRY(c, ctx_malloc(c, &c->t_link, (size_t)t_cap * sizeof(uint32_t)));
         c->t_cap = t_cap;
     }
     if (c->cells_log2 != cells_log2) {
-        if (c->cells) HIP_TRY(c, hipFree(c->cells));
+        if (c->cells) HIP_TRY(c, ctx_free(c, c->cells));
         c->cells = nullptr;
         c->cells_log2 = 0;
-        HIP_TRY(c, hipMalloc(&c->cells, ((size_t)kGateFloats << cells_log2) * sizeof(float)));
+        HIP_TRY(c, ctx_malloc(c, &c->cells, ((size_t)kGateFloats << cells_log2) * sizeof(float)));
         c->cells_log2 = cells_log2;
         cells_new = true;
     }
@@ -753,8 +760,8 @@ int slimrt_create(slimrt_context **out, uint32_t flags) {
         (void)hipGetDevice(&c->device);
         if (hipStreamCreateWithFlags(&c->own_stream, hipStreamNonBlocking) != hipSuccess ||
             hipEventCreate(&c->ev0) != hipSuccess || hipEventCreate(&c->ev1) != hipSuccess ||
-            hipMalloc(&c->d_status, sizeof(uint32_t)) != hipSuccess ||
-            hipMalloc(&c->d_counters, 8 * sizeof(unsigned long long)) != hipSuccess ||
+            ctx_malloc(c, &c->d_status, sizeof(uint32_t)) != hipSuccess ||
+            ctx_malloc(c, &c->d_counters, 8 * sizeof(unsigned long long)) != hipSuccess ||
             hipMemset(c->d_status, 0, sizeof(uint32_t)) != hipSuccess ||
             noalloc_scratch_address(c->device, &c->noalloc_scratch) != SLIMRT_OK) {
             c->stream = c->own_stream;
@@ -774,26 +781,26 @@ int slimrt_destroy(slimrt_context *c) {
         // tile_cost and writes tile_order there, and nothing joins it back) both finish before anything is freed
         (void)hipStreamSynchronize(c->stream);
         if (c->side_stream) (void)hipStreamSynchronize(c->side_stream);
-        dev_free(c->d_geos); dev_free(c->d_mats); dev_free(c->d_lights); dev_free(c->d_snodes); dev_free(c->d_mnodes);
-        dev_free(c->d_mnodes4);
-        dev_free(c->d_leaf); dev_free(c->d_meshes); dev_free(c->d_tri_hit); dev_free(c->d_tri_attr); dev_free(c->d_tex);
-        dev_free(c->d_mips); dev_free(c->d_quads); dev_free(c->d_pixels); dev_free(c->d_depths);
+        dev_free(c, c->d_geos); dev_free(c, c->d_mats); dev_free(c, c->d_lights); dev_free(c, c->d_snodes); dev_free(c, c->d_mnodes);
+        dev_free(c, c->d_mnodes4);
+        dev_free(c, c->d_leaf); dev_free(c, c->d_meshes); dev_free(c, c->d_tri_hit); dev_free(c, c->d_tri_attr); dev_free(c, c->d_tex);
+        dev_free(c, c->d_mips); dev_free(c, c->d_quads); dev_free(c, c->d_pixels); dev_free(c, c->d_depths);
         free_nrc(c);
-        dev_free(c->d_geo_stage);
-        dev_free(c->ov_lines);
-        dev_free(c->ov_samples);
+        dev_free(c, c->d_geo_stage);
+        dev_free(c, c->ov_lines);
+        dev_free(c, c->ov_samples);
         for (void *q : {c->wf_queue_a, c->wf_queue_b, c->wf_accum, c->wf_hits, (void *)c->wf_counters, (void *)c->wf_stats,
                         (void *)c->d_pick})
-            if (q) (void)hipFree(q);
+            if (q) (void)ctx_free(c, q);
         for (hipEvent_t e : c->wf_band)
             if (e) (void)hipEventDestroy(e);
         if (c->wf_fork) (void)hipEventDestroy(c->wf_fork);
         if (c->wf_join) (void)hipEventDestroy(c->wf_join);
-        if (c->d_status) (void)hipFree(c->d_status);
-        if (c->d_counters) (void)hipFree(c->d_counters);
+        if (c->d_status) (void)ctx_free(c, c->d_status);
+        if (c->d_counters) (void)ctx_free(c, c->d_counters);
         for (int p = 0; p < 2; p++) {
-            if (c->tile_cost[p]) (void)hipFree(c->tile_cost[p]);
-            if (c->tile_order[p]) (void)hipFree(c->tile_order[p]);
+            if (c->tile_cost[p]) (void)ctx_free(c, c->tile_cost[p]);
+            if (c->tile_order[p]) (void)ctx_free(c, c->tile_order[p]);
             if (c->tile_rendered[p]) (void)hipEventDestroy(c->tile_rendered[p]);
             if (c->tile_ordered[p]) (void)hipEventDestroy(c->tile_ordered[p]);
         }
@@ -971,6 +978,35 @@ int slimrt_render_device(slimrt_context *c, const slimrt_render_params *p, void 
     return launch(c, p, (float4 *)d_pixels, (float *)d_depths, false);
 }
 
+int slimrt_frame_flags(slimrt_context *c, uint32_t *flags) {
+    if (!c || !flags) return SLIMRT_E_ARG;
+    uint32_t st = 0;
+    if (!c->cpu) {
+        HIP_TRY(c, hipMemcpyAsync(&st, c->d_status, sizeof st, hipMemcpyDeviceToHost, c->stream));
+        HIP_TRY(c, hipStreamSynchronize(c->stream));
+        if (st) {
+            HIP_TRY(c, hipMemsetAsync(c->d_status, 0, sizeof st, c->stream));
+            HIP_TRY(c, hipStreamSynchronize(c->stream));
+        }
+    }
+    *flags = st | c->host_status;
+    c->host_status = 0;
+    return SLIMRT_OK;
+}
+
+int slimrt_set_memory_limit(slimrt_context *c, uint64_t bytes) {
+    if (!c) return SLIMRT_E_ARG;
+    c->mem_limit = bytes;
+    return SLIMRT_OK;
+}
+
+int slimrt_memory_usage(slimrt_context *c, uint64_t *used, uint64_t *limit) {
+    if (!c) return SLIMRT_E_ARG;
+    if (used) *used = c->mem_used;
+    if (limit) *limit = c->mem_limit;
+    return SLIMRT_OK;
+}
+
 int slimrt_render_device_noalloc(slimrt_context *c, const slimrt_render_params *p, void *d_pixels, void *d_depths) {
     if (!c) return SLIMRT_E_ARG;
     int rc = validate_params(c, p);
@@ -1123,14 +1159,14 @@ int slimrt_pick(slimrt_context *c, const slimrt_render_params *p, int32_t x, int
         tr.mesh_stack = Stack{stack.data() + c->scene_height + 1, 1};
         tr.scene_depth = c->scene_height + 1;
         tr.mesh_depth = c->mesh_height + 1;
-        tr.overflow = 0;
+        tr.status = 0;
         tr.cnt = nullptr;
         float of[4];
         pick(v, rp, x, y, tr, res, of);
         std::memcpy(res + 3, of, sizeof of);
-        if (tr.overflow) return fail(c, SLIMRT_E_OVERFLOW, "traversal stack overflow");
+        if (tr.status & kStatusStack) return fail(c, SLIMRT_E_OVERFLOW, "traversal stack overflow");
     } else {
-        if (!c->d_pick) HIP_TRY(c, hipMalloc(&c->d_pick, 8 * sizeof(int32_t)));
+        if (!c->d_pick) HIP_TRY(c, ctx_malloc(c, &c->d_pick, 8 * sizeof(int32_t)));
         const uint32_t scene_depth = c->scene_height + 1, mesh_depth = mesh_stack_depth(c, false);
         const size_t lds = stack_lds_bytes(scene_depth, mesh_depth, false);
         if (lds > 160 * 1024) return fail(c, SLIMRT_E_ARG, "BVH too deep for the LDS traversal stacks");
@@ -1263,7 +1299,8 @@ int slimrt_render_nrc(slimrt_context *c, const slimrt_render_params *p, slimrt_n
         const uint32_t blocks = std::min<uint32_t>((samples + 255) / 256, 4096);
         hipLaunchKernelGGL(nrc_composite_kernel, dim3(blocks), dim3(256), 0, c->stream, rp,
                            (const uint32_t *)(c->nrc_counters + NC_QUERIES), samples, (const uint32_t *)c->q_pixel,
-                           (const float *)c->q_prefix, (const float *)c->q_tp, (const float *)c->q_out, (float4 *)d_pixels);
+                           (const float *)c->q_prefix, (const float *)c->q_tp, (const float *)c->q_out, (float4 *)d_pixels,
+                           c->d_status);
         HIP_TRY(c, hipGetLastError());
     }
     c->nrc_last_steps = 0;

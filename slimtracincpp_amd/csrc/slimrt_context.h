@@ -4,6 +4,7 @@
 #include <hip/hip_runtime.h>
 #include <string>
 #include <vector>
+#include <unordered_map>
 #include <cstring>
 
 #include "slimrt_nrc_trace.h"
@@ -34,6 +35,10 @@ struct slimrt_context {
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
     bool timed = false;
     std::string err;
+    // device memory this context holds (every allocation, by pointer) and its limit (0: none)
+    uint64_t mem_used = 0, mem_limit = 0;
+    std::unordered_map<void *, size_t> allocs;
+    uint32_t host_status = 0;   // kStatusNaN seen since the last slimrt_frame_flags (device word folded in)
 
     // host mirror of the device arrays (the CPU path renders from these)
     std::vector<Geo> geos;
@@ -124,14 +129,37 @@ inline int fail(slimrt_context *c, int code, const std::string &msg) {
                         std::string(#expr) + ": " + hipGetErrorString(e_));                        \
     } while (0)
 
+// Every device allocation a context owns goes through these two, which keep its running total: with a limit set
+// (slimrt_set_memory_limit), an allocation that would take the total past it fails as out of memory.
+template <typename T>
+inline hipError_t ctx_malloc(slimrt_context *c, T **p, size_t bytes) {
+    if (c->mem_limit && (bytes > c->mem_limit || c->mem_used > c->mem_limit - bytes)) return hipErrorOutOfMemory;
+    const hipError_t e = hipMalloc(p, bytes);
+    if (e == hipSuccess) {
+        c->allocs[(void *)*p] = bytes;
+        c->mem_used += bytes;
+    }
+    return e;
+}
+
+inline hipError_t ctx_free(slimrt_context *c, void *p) {
+    if (!p) return hipSuccess;
+    const auto it = c->allocs.find(p);
+    if (it != c->allocs.end()) {
+        c->mem_used -= it->second;
+        c->allocs.erase(it);
+    }
+    return hipFree(p);
+}
+
 template <typename T>
 inline int dev_upload(slimrt_context *c, DevBuf<T> &b, const T *src, size_t n) {
     if (n == 0) n = 1;   // keep a valid pointer for empty arrays
     if (b.cap < n) {
-        if (b.ptr) HIP_TRY(c, hipFree(b.ptr));
+        if (b.ptr) HIP_TRY(c, ctx_free(c, b.ptr));
         b.ptr = nullptr;
         b.cap = 0;
-        HIP_TRY(c, hipMalloc(&b.ptr, n * sizeof(T)));
+        HIP_TRY(c, ctx_malloc(c, &b.ptr, n * sizeof(T)));
         b.cap = n;
     }
     if (src) HIP_TRY(c, hipMemcpyAsync(b.ptr, src, n * sizeof(T), hipMemcpyHostToDevice, c->stream));
@@ -139,8 +167,8 @@ inline int dev_upload(slimrt_context *c, DevBuf<T> &b, const T *src, size_t n) {
 }
 
 template <typename T>
-inline void dev_free(DevBuf<T> &b) {
-    if (b.ptr) (void)hipFree(b.ptr);
+inline void dev_free(slimrt_context *c, DevBuf<T> &b) {
+    if (b.ptr) (void)ctx_free(c, b.ptr);
     b.ptr = nullptr;
     b.cap = 0;
 }
@@ -214,7 +242,7 @@ __device__ __forceinline__ void bind_lds_stacks(TracerT<WIDE> &tr, uint32_t *lds
         reinterpret_cast<typename TracerT<WIDE>::MeshEntry *>(lds + scene_depth * THREADS) + t, THREADS};
     tr.scene_depth = scene_depth;
     tr.mesh_depth = mesh_depth;
-    tr.overflow = 0;
+    tr.status = 0;
 }
 
 // Checks that the 4-wide walk can run on the uploaded meshes (16-bit stack entries).
@@ -263,6 +291,8 @@ constexpr uint32_t kWfIterations = 32, kWfSpawned = 32, kWfFinished = 33, kWfSta
 int launch_wavefront(slimrt_context *c, const RenderParams &rp, float4 *pixels, float *depths, bool per_bounce, bool sort,
                      uint32_t bands);
 
+// Reads and clears the device status word of the frames since the last check: a stack or queue overflow fails
+// with SLIMRT_E_OVERFLOW; NaN samples are kept for slimrt_frame_flags.
 inline int check_status(slimrt_context *c) {
     uint32_t st = 0;
     HIP_TRY(c, hipMemcpyAsync(&st, c->d_status, sizeof st, hipMemcpyDeviceToHost, c->stream));
@@ -270,7 +300,9 @@ inline int check_status(slimrt_context *c) {
     if (st) {
         HIP_TRY(c, hipMemsetAsync(c->d_status, 0, sizeof st, c->stream));
         HIP_TRY(c, hipStreamSynchronize(c->stream));
-        return fail(c, SLIMRT_E_OVERFLOW, "traversal stack overflow");
+        c->host_status |= st & kStatusNaN;
+        if (st & kStatusStack) return fail(c, SLIMRT_E_OVERFLOW, "traversal stack overflow");
+        if (st & kStatusQueue) return fail(c, SLIMRT_E_OVERFLOW, "wavefront queue overflow");
     }
     return SLIMRT_OK;
 }

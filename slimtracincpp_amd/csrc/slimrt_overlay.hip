@@ -207,10 +207,10 @@ size_t align256(size_t n) { return (n + 255) & ~(size_t)255; }
 int reserve(slimrt_context *c, DevBuf<uint8_t> &b, size_t bytes) {
     if (b.cap >= bytes) return SLIMRT_OK;
     HIP_TRY(c, hipStreamSynchronize(c->stream));
-    if (b.ptr) HIP_TRY(c, hipFree(b.ptr));
+    if (b.ptr) HIP_TRY(c, ctx_free(c, b.ptr));
     b.ptr = nullptr;
     b.cap = 0;
-    HIP_TRY(c, hipMalloc(&b.ptr, bytes));
+    HIP_TRY(c, ctx_malloc(c, &b.ptr, bytes));
     b.cap = bytes;
     return SLIMRT_OK;
 }

@@ -13,6 +13,11 @@
 
 namespace slimrt {
 
+// Bits of a traversal's status (TracerT::status) and of a context's device status word.
+constexpr uint32_t kStatusStack = 1u;   // a traversal stack push would have exceeded its capacity
+constexpr uint32_t kStatusQueue = 2u;   // a wavefront queue append would have exceeded its capacity
+constexpr uint32_t kStatusNaN = 4u;     // a sample's radiance or depth came out NaN
+
 template <class T>
 struct StackT {
     T *base;
@@ -35,7 +40,7 @@ struct TracerT {
     Stack scene_stack;
     StackT<MeshEntry> mesh_stack;
     uint32_t scene_depth, mesh_depth;   // capacities
-    uint32_t overflow;                  // set when a push would exceed a capacity
+    uint32_t status;                    // kStatus* bits: a push beyond a capacity, a NaN sample
     Ray aux_ray;                        // last localized ray (SceneTracer::aux_ray)
     Hit aux_hit;                        // SceneTracer::aux_hit
     Counters *cnt;                      // null: no counting
@@ -136,7 +141,7 @@ SLIM_HD bool mesh_trace(const SceneView &s, uint32_t mesh_id, const Ray &r, Hit 
             if (go_r) {
                 uint32_t near_pair = L.first_index, far_pair = R.first_index;
                 if (!any_hit && ln > rn) { near_pair = R.first_index; far_pair = L.first_index; }
-                if (top < tr.mesh_depth) tr.mesh_stack[top++] = far_pair; else tr.overflow = 1;
+                if (top < tr.mesh_depth) tr.mesh_stack[top++] = far_pair; else tr.status |= kStatusStack;
                 pair = near_pair;
             } else pair = L.first_index;
         } else if (go_r) {
@@ -202,7 +207,7 @@ SLIM_HD bool mesh_trace4(const SceneView &s, const MeshRec &m, const Ray &r, Hit
             const uint32_t count = i == 0 ? n.count[0] : i == 1 ? n.count[1] : i == 2 ? n.count[2] : n.count[3];
             if (!(key[k] < hit.distance) || count != 0) continue;
             const uint32_t child = i == 0 ? n.child[0] : i == 1 ? n.child[1] : i == 2 ? n.child[2] : n.child[3];
-            if (top < tr.mesh_depth) tr.mesh_stack[top++] = (typename TR::MeshEntry)child; else tr.overflow = 1;
+            if (top < tr.mesh_depth) tr.mesh_stack[top++] = (typename TR::MeshEntry)child; else tr.status |= kStatusStack;
         }
         if (top == 0) break;
         cur = tr.mesh_stack[--top];
@@ -299,7 +304,7 @@ SLIM_HD int scene_trace(const SceneView &s, Ray &r, Hit &hit, bool any_hit, floa
             if (go_r) {
                 uint32_t near_pair = L.first_index, far_pair = R.first_index;
                 if (!any_hit && ln > rn) { near_pair = R.first_index; far_pair = L.first_index; }
-                if (top < tr.scene_depth) tr.scene_stack[top++] = far_pair; else tr.overflow = 1;
+                if (top < tr.scene_depth) tr.scene_stack[top++] = far_pair; else tr.status |= kStatusStack;
                 pair = near_pair;
             } else pair = L.first_index;
         } else if (go_r) {
@@ -773,6 +778,12 @@ SLIM_HD uint32_t pixel_content(const float4 *p, bool ssaa) {
     }
     auto channel = [](float v) { return v > 1.0f ? 255u : (uint32_t)(uint8_t)(255.0f * sqrtf(v)); };
     return channel(c.x) << 16 | channel(c.y) << 8 | channel(c.z);
+}
+
+// kStatusNaN when a sample's value (radiance or canvas output; not the hit records of OUTPUT_HITS) or depth is NaN.
+SLIM_HD uint32_t nan_status(const RenderParams &rp, const float px[4], float depth) {
+    const bool nan = (rp.output != 2 && (px[0] != px[0] || px[1] != px[1] || px[2] != px[2])) || depth != depth;
+    return nan ? kStatusNaN : 0u;
 }
 
 // Canvas offsets: samples are stored row-major, or as 2x2 quads per pixel under SSAA (canvas.h:147).

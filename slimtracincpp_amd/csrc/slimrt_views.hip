@@ -173,6 +173,7 @@ __device__ __forceinline__ void trace_view_tile(const SceneView &s, const ViewRe
         const uint32_t o = canvas_offset(rp, x, y);
         pixels[o] = make_float4(px[0], px[1], px[2], px[3]);
         depths[o] = depth;
+        tr.status |= nan_status(rp, px, depth);
     }
 }
 
@@ -208,7 +209,7 @@ void views_kernel(SceneView s, const ViewRec *__restrict__ views, uint32_t nview
         if (tile >= total_tiles) return;
         trace_view_tile<WX, WY>(s, views, nviews, tile, tr, lane);
     }
-    if (tr.overflow) atomicOr(status, 1u);
+    if (tr.status) atomicOr(status, tr.status);
 }
 
 int check_geometries(slimrt_context *c, const slimrt_geometry *g, uint32_t count) {
@@ -397,8 +398,8 @@ int slimrt_update_scene_device(slimrt_context *c, const slimrt_geometry *g, uint
     int rc;
     if ((rc = check_geometries(c, g, count)) || (rc = check_update_mode(c, mode))) return rc;
     if (c->d_geo_stage.cap < count) {
-        dev_free(c->d_geo_stage);
-        HIP_TRY(c, hipMalloc(&c->d_geo_stage.ptr, count * sizeof(slimrt_geometry)));
+        dev_free(c, c->d_geo_stage);
+        HIP_TRY(c, ctx_malloc(c, &c->d_geo_stage.ptr, count * sizeof(slimrt_geometry)));
         c->d_geo_stage.cap = count;
     }
     HIP_TRY(c, hipMemcpyAsync(c->d_geo_stage.ptr, g, count * sizeof(slimrt_geometry), hipMemcpyHostToDevice, c->stream));

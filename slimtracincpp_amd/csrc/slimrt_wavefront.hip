@@ -127,10 +127,12 @@ __device__ __forceinline__ int load_hit(const HitSlots &h, size_t i, Hit &hit) {
     return g == ~0u ? -1 : (int)(g & 0x7FFFFFFFu);
 }
 
-__device__ __forceinline__ void finish_sample(const RenderParams &rp, color col, float4 *pixels, uint32_t o) {
+__device__ __forceinline__ void finish_sample(const RenderParams &rp, color col, float4 *pixels, uint32_t o,
+                                              uint32_t &status) {
     float px[4];
     output_value(rp, color{tone_map(col.r), tone_map(col.g), tone_map(col.b)}, px);
     pixels[o] = make_float4(px[0], px[1], px[2], px[3]);
+    status |= nan_status(rp, px, 0.0f);
 }
 
 __device__ __forceinline__ void push_path(PathState *q, uint32_t *count, uint32_t cap, bool cont, const Ray &ray,
@@ -139,7 +141,7 @@ __device__ __forceinline__ void push_path(PathState *q, uint32_t *count, uint32_
     const uint32_t slot = wave_append(cont, count);
     if (!cont) return;
     if (slot >= cap) {   // cannot happen: one entry per sample at most
-        atomicOr(status, 2u);
+        atomicOr(status, kStatusQueue);
         return;
     }
     PathState st;
@@ -214,8 +216,9 @@ __global__ __launch_bounds__(kBlock) void wf_primary_kernel(SceneView s, RenderP
         count_end(stats, cont);
         o = canvas_offset(rp, x, y);
         depths[o] = depth;
-        if (!cont) finish_sample(rp, col, pixels, o);
-        if (tr.overflow) atomicOr(status, 1u);
+        if (depth != depth) tr.status |= kStatusNaN;
+        if (!cont) finish_sample(rp, col, pixels, o, tr.status);
+        if (tr.status) atomicOr(status, tr.status);
     }
     push_path(queue, counters + WF_COUNT_A, cap, inside && cont, ray, tp, scaling, depth_left, o, col, accum, status);
 }
@@ -248,7 +251,7 @@ void wf_trace_kernel(SceneView s, RenderParams rp, HitSlots hits, uint32_t scene
         hit.scaling_factor = scaling;
         const int geo = scene_trace(s, ray, hit, false, INFINITY, tr);
         store_hit(hits, (size_t)tile * kBlock + t, hit, geo);
-        if (tr.overflow) atomicOr(status, 1u);
+        if (tr.status) atomicOr(status, tr.status);
     }
 }
 
@@ -293,11 +296,11 @@ void wf_bounce_kernel(SceneView s, RenderParams rp, float4 *__restrict__ pixels,
                 count_end(stats, true);
             }
             count_end(stats, cont);
-            if (!cont) finish_sample(rp, col, pixels, pixel);
+            if (!cont) finish_sample(rp, col, pixels, pixel, tr.status);
         }
         if (!TO_END) push_path(out, out_count, cap, live && cont, ray, tp, scaling, depth_left, pixel, col, accum, status);
     }
-    if (tr.overflow) atomicOr(status, 1u);
+    if (tr.status) atomicOr(status, tr.status);
 }
 
 __device__ __forceinline__ uint32_t octant(const PathState &st) {
@@ -406,24 +409,24 @@ int launch_wavefront(slimrt_context *c, const RenderParams &rp, float4 *pixels, 
     const uint32_t slots = tiles_x * tiles_y * kBlock;   // queue entries: one per sample slot of every tile
     if (c->wf_cap < slots) {
         for (void *p : {(void *)c->wf_queue_a, (void *)c->wf_queue_b, (void *)c->wf_accum})
-            if (p) HIP_TRY(c, hipFree(p));
+            if (p) HIP_TRY(c, ctx_free(c, p));
         c->wf_queue_a = c->wf_queue_b = nullptr;
         c->wf_accum = nullptr;
         c->wf_cap = 0;
-        HIP_TRY(c, hipMalloc(&c->wf_queue_a, (size_t)slots * sizeof(PathState)));
-        HIP_TRY(c, hipMalloc(&c->wf_queue_b, (size_t)slots * sizeof(PathState)));
-        HIP_TRY(c, hipMalloc(&c->wf_accum, (size_t)slots * sizeof(float4)));   // canvas offsets < ws * hs <= slots
+        HIP_TRY(c, ctx_malloc(c, &c->wf_queue_a, (size_t)slots * sizeof(PathState)));
+        HIP_TRY(c, ctx_malloc(c, &c->wf_queue_b, (size_t)slots * sizeof(PathState)));
+        HIP_TRY(c, ctx_malloc(c, &c->wf_accum, (size_t)slots * sizeof(float4)));   // canvas offsets < ws * hs <= slots
         c->wf_cap = slots;
     }
     const bool split = (rp.flags & SLIMRT_WAVEFRONT_SPLIT) != 0;
     if (split && c->wf_hits_cap < slots) {
-        if (c->wf_hits) HIP_TRY(c, hipFree(c->wf_hits));
+        if (c->wf_hits) HIP_TRY(c, ctx_free(c, c->wf_hits));
         c->wf_hits = nullptr;
         c->wf_hits_cap = 0;
-        HIP_TRY(c, hipMalloc(&c->wf_hits, (size_t)slots * kHitSlotBytes));
+        HIP_TRY(c, ctx_malloc(c, &c->wf_hits, (size_t)slots * kHitSlotBytes));
         c->wf_hits_cap = slots;
     }
-    if (!c->wf_counters) HIP_TRY(c, hipMalloc(&c->wf_counters, 8 * WF_COUNTERS * sizeof(uint32_t)));
+    if (!c->wf_counters) HIP_TRY(c, ctx_malloc(c, &c->wf_counters, 8 * WF_COUNTERS * sizeof(uint32_t)));
     const uint32_t scene_depth = c->scene_height + 1, mesh_depth = mesh_stack_depth(c, wide_mesh_bvh(rp));
     const size_t lds = stack_lds_bytes(scene_depth, mesh_depth, wide_mesh_bvh(rp));
     if (lds > 160 * 1024) return fail(c, SLIMRT_E_ARG, "BVH too deep for the LDS traversal stacks");
@@ -438,7 +441,7 @@ int launch_wavefront(slimrt_context *c, const RenderParams &rp, float4 *pixels, 
     const uint32_t blocks = (uint32_t)cus * 2;
     uint32_t *stats = nullptr;
     if (rp.flags & SLIMRT_WAVEFRONT_COUNT) {
-        if (!c->wf_stats) HIP_TRY(c, hipMalloc(&c->wf_stats, kWfStats * sizeof(uint32_t)));
+        if (!c->wf_stats) HIP_TRY(c, ctx_malloc(c, &c->wf_stats, kWfStats * sizeof(uint32_t)));
         stats = c->wf_stats;
     }
     hipLaunchKernelGGL(wf_reset_kernel, dim3(1), dim3(256), 0, c->stream, cnt, bands * WF_COUNTERS, stats);

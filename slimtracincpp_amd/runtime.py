@@ -20,6 +20,7 @@ LIB_PATH = os.path.join(HERE, "libslimrt.so")
 u32, i32, f32, vp = C.c_uint32, C.c_int32, C.c_float, C.c_void_p
 
 SLIMRT_DEVICE_GPU, SLIMRT_DEVICE_CPU = 0, 1
+FLAG_STACK_OVERFLOW, FLAG_QUEUE_OVERFLOW, FLAG_NAN_SAMPLE = 1, 2, 4   # slimrt_frame_flags bits
 OUTPUT_CANVAS, OUTPUT_RADIANCE, OUTPUT_HITS = 0, 1, 2
 RAY_CONES = 1
 ERRORS = {-1: "SLIMRT_E_ARG", -2: "SLIMRT_E_DEVICE", -3: "SLIMRT_E_MEMORY", -4: "SLIMRT_E_STATE", -5: "SLIMRT_E_OVERFLOW"}
@@ -137,6 +138,8 @@ def lib():
             "slimrt_render": [vp, P(RenderParams), vp, vp], "slimrt_render_device": [vp, P(RenderParams), vp, vp],
             "slimrt_render_device_noalloc": [vp, P(RenderParams), vp, vp],
             "slimrt_synchronize": [vp], "slimrt_count_work": [vp, P(RenderParams), vp],
+            "slimrt_frame_flags": [vp, P(u32)], "slimrt_set_memory_limit": [vp, C.c_uint64],
+            "slimrt_memory_usage": [vp, P(C.c_uint64), P(C.c_uint64)],
             "slimrt_last_kernel_ms": [vp, P(f32)],
             "slimrt_host_orientation": [vp, vp],
             "slimrt_host_projection": [vp, vp, f32, u32, u32, u32, P(RenderParams)],
@@ -442,6 +445,21 @@ class Renderer:
 
     def synchronize(self):
         _check(lib().slimrt_synchronize(self.ctx), self.ctx)
+
+    def frame_flags(self) -> int:
+        """slimrt_frame_flags: FLAG_* bits of the frames since the last call (then cleared)."""
+        f = u32()
+        _check(lib().slimrt_frame_flags(self.ctx, C.byref(f)), self.ctx)
+        return f.value
+
+    def set_memory_limit(self, nbytes: int):
+        _check(lib().slimrt_set_memory_limit(self.ctx, nbytes), self.ctx)
+
+    def memory_usage(self):
+        """(bytes of device memory the context holds, its limit or 0)"""
+        used, limit = C.c_uint64(), C.c_uint64()
+        _check(lib().slimrt_memory_usage(self.ctx, C.byref(used), C.byref(limit)), self.ctx)
+        return used.value, limit.value
 
     def set_stream(self, stream_ptr: int | None):
         _check(lib().slimrt_set_stream(self.ctx, C.c_void_p(stream_ptr) if stream_ptr else None), self.ctx)

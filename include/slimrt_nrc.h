@@ -33,6 +33,12 @@ int slimrt_nrc_set_stream(slimrt_nrc *nrc, void *hip_stream);
 /* Float32 master parameters (host arrays of SLIMRT_NRC_PARAMS).  set resets the Adam state. */
 int slimrt_nrc_get_params(slimrt_nrc *nrc, float *params);
 int slimrt_nrc_set_params(slimrt_nrc *nrc, const float *params);
+/* The whole training state, for checkpoints (no reference counterpart): the float32 master parameters, Adam's
+ * first and second moments (host arrays of SLIMRT_NRC_PARAMS floats each) and its step count.  A cache whose state
+ * was set from another's get_state takes the same next Adam steps, bit for bit.  Any pointer of get_state may be
+ * NULL; set_state needs all three arrays. */
+int slimrt_nrc_get_state(slimrt_nrc *nrc, float *params, float *m, float *v, uint32_t *step);
+int slimrt_nrc_set_state(slimrt_nrc *nrc, const float *params, const float *m, const float *v, uint32_t step);
 /* out[i] (3 floats) = cache(raw[i]) for i < n. */
 int slimrt_nrc_infer(slimrt_nrc *nrc, const float *d_raw, uint32_t n, float *d_out);
 /* Loss and gradient of the relative L2 loss over n samples (targets: 3 floats each), without an update;

@@ -589,6 +589,28 @@ int slimrt_nrc_set_params(slimrt_nrc *c, const float *params) {
     return 0;
 }
 
+int slimrt_nrc_get_state(slimrt_nrc *c, float *params, float *m, float *v, uint32_t *step) {
+    if (!c) return -1;
+    if (params) NRC_TRY(c, hipMemcpyAsync(params, c->d_p, NPARAMS * 4, hipMemcpyDeviceToHost, c->stream));
+    if (m) NRC_TRY(c, hipMemcpyAsync(m, c->d_m, NPARAMS * 4, hipMemcpyDeviceToHost, c->stream));
+    if (v) NRC_TRY(c, hipMemcpyAsync(v, c->d_v, NPARAMS * 4, hipMemcpyDeviceToHost, c->stream));
+    NRC_TRY(c, hipStreamSynchronize(c->stream));
+    if (step) *step = (uint32_t)c->step;
+    return 0;
+}
+
+int slimrt_nrc_set_state(slimrt_nrc *c, const float *params, const float *m, const float *v, uint32_t step) {
+    if (!c || !params || !m || !v || step > 0x7FFFFFFFu) return -1;
+    NRC_TRY(c, hipMemcpyAsync(c->d_p, params, NPARAMS * 4, hipMemcpyHostToDevice, c->stream));
+    NRC_TRY(c, hipMemcpyAsync(c->d_m, m, NPARAMS * 4, hipMemcpyHostToDevice, c->stream));
+    NRC_TRY(c, hipMemcpyAsync(c->d_v, v, NPARAMS * 4, hipMemcpyHostToDevice, c->stream));
+    c->step = (int)step;
+    int rc = pack(c);
+    if (rc) return rc;
+    NRC_TRY(c, hipStreamSynchronize(c->stream));
+    return 0;
+}
+
 int slimrt_nrc_infer(slimrt_nrc *c, const float *raw, uint32_t n, float *out) {
     if (!c || (!raw && n) || (!out && n)) return -1;
     if (n == 0) return 0;

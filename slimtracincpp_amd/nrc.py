@@ -34,6 +34,7 @@ def _lib():
         vp, u32, f32, P = C.c_void_p, C.c_uint32, C.c_float, C.POINTER
         sig = {"slimrt_nrc_create": [P(vp), u32, vp, vp], "slimrt_nrc_destroy": [vp], "slimrt_nrc_set_stream": [vp, vp],
                "slimrt_nrc_get_params": [vp, vp], "slimrt_nrc_set_params": [vp, vp],
+               "slimrt_nrc_get_state": [vp, vp, vp, vp, P(u32)], "slimrt_nrc_set_state": [vp, vp, vp, vp, u32],
                "slimrt_nrc_infer": [vp, vp, u32, vp], "slimrt_nrc_gradients": [vp, vp, vp, u32, vp, vp],
                "slimrt_nrc_train_step": [vp, vp, vp, u32, f32, vp],
                "slimrt_nrc_infer_queue": [vp, vp, vp, u32, vp],
@@ -87,6 +88,20 @@ class NeuralCache:
         p = np.ascontiguousarray(p, np.float32)
         assert p.size == N_PARAMS
         self._check(_lib().slimrt_nrc_set_params(self.h, p.ctypes.data_as(C.c_void_p)))
+
+    def state(self) -> dict:
+        """The training state for a checkpoint: params, Adam moments m and v, step count."""
+        p, m, v = (np.zeros(N_PARAMS, np.float32) for _ in range(3))
+        step = C.c_uint32()
+        self._check(_lib().slimrt_nrc_get_state(self.h, p.ctypes.data_as(C.c_void_p), m.ctypes.data_as(C.c_void_p),
+                                                v.ctypes.data_as(C.c_void_p), C.byref(step)))
+        return {"params": p, "m": m, "v": v, "step": step.value}
+
+    def set_state(self, st: dict):
+        p, m, v = (np.ascontiguousarray(st[k], np.float32) for k in ("params", "m", "v"))
+        assert p.size == m.size == v.size == N_PARAMS
+        self._check(_lib().slimrt_nrc_set_state(self.h, p.ctypes.data_as(C.c_void_p), m.ctypes.data_as(C.c_void_p),
+                                                v.ctypes.data_as(C.c_void_p), int(st["step"])))
 
     def infer(self, raw, out):
         """raw: (n, 14) float32 cuda tensor; out: (n, 3) float32 cuda tensor."""

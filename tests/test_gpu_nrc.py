@@ -96,3 +96,28 @@ def test_training_converges(cache):
     last = cache.train_step(r, tt, lr=5e-3, want_loss=True)
     print(f"loss {first:.4f} -> {last:.5f}")
     assert last < 0.1 * first
+
+
+def test_training_state_round_trip(cache):
+    """A checkpoint (params, Adam moments, step) restored into another cache continues training bit for bit."""
+    from slimtracincpp_amd.nrc import NeuralCache
+    raw, t = _data(4096, 11)
+    raw_d, t_d = _cuda(raw), _cuda(t)
+    for _ in range(3):
+        cache.train_step(raw_d, t_d, lr=5e-3)
+    saved = cache.state()
+    assert saved["step"] == 3 and np.any(saved["m"] != 0) and np.any(saved["v"] != 0)
+    for _ in range(2):
+        cache.train_step(raw_d, t_d, lr=5e-3)
+    after = cache.state()
+    other = NeuralCache(CENTER, INV_HALF, seed=99)
+    other.set_state(saved)
+    back = other.state()
+    assert all(np.array_equal(back[k].view(np.uint32), saved[k].view(np.uint32)) for k in ("params", "m", "v"))
+    for _ in range(2):
+        other.train_step(raw_d, t_d, lr=5e-3)
+    got = other.state()
+    assert got["step"] == after["step"] == 5
+    for k in ("params", "m", "v"):
+        assert np.array_equal(got[k].view(np.uint32), after[k].view(np.uint32)), k
+    other.close()

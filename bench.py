@@ -186,6 +186,62 @@ def neural_cache_leg(frames: int, width: int, height: int, stream, cache_depth: 
             "psnr_db_per_frame": [round(float(v), 2) for v in psnr]}
 
 
+def cache_bound(spec, stream, reps: int = 10) -> dict:
+    """The shortest frame any cache of the path's radiance can give on this scene: it must still find each sample's
+    primary hit (a frame that traces the camera rays and writes hit records only, SLIMRT_OUTPUT_HITS, in the
+    fastest per-sample form) and evaluate the network once per sample with a hit (slimrt_nrc_infer).  Against the
+    plain full-depth frame in its fastest form, that bounds the speed-up of the cache frame from above."""
+    import numpy as np
+    import torch
+    from slimtracincpp_amd import runtime
+    from slimtracincpp_amd.nrc import NeuralCache, scene_box
+    r = runtime.Renderer(spec, "gpu")
+    r.set_stream(stream.cuda_stream)
+    n = spec.width * spec.height
+    px = torch.empty((n, 4), dtype=torch.float32, device="cuda")
+    dp = torch.empty((n,), dtype=torch.float32, device="cuda")
+
+    def timed(p):
+        for _ in range(3):
+            r.render_device(p, px.data_ptr(), dp.data_ptr())
+        ts = []
+        for _ in range(reps):
+            r.render_device(p, px.data_ptr(), dp.data_ptr())
+            ts.append(r.last_kernel_ms())
+        r.synchronize()
+        return float(np.median(ts))
+
+    p = r.params(output=runtime.OUTPUT_RADIANCE)
+    flags, _ = r.autotune_form(p, px.data_ptr(), dp.data_ptr(), reps=2)
+    p.flags = flags
+    plain_ms = timed(p)
+    h = r.params(output=runtime.OUTPUT_HITS)
+    trace = {}
+    for name, f in (("megakernel", 0), ("megakernel_bvh4", runtime.BVH4)):
+        h.flags = f
+        trace[name] = timed(h)
+    hits = int(np.sum(px.view(torch.int32)[:, 0].cpu().numpy() >= 0))
+    center, inv_half = scene_box(r.scene)
+    cache = NeuralCache(center, inv_half, 1)
+    cache.set_stream(stream.cuda_stream)
+    raw = torch.rand((max(hits, 1), 14), dtype=torch.float32, device="cuda")
+    out = torch.empty((max(hits, 1), 3), dtype=torch.float32, device="cuda")
+    ev0, ev1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    cache.infer(raw, out)
+    ev0.record(stream)
+    for _ in range(reps):
+        cache.infer(raw, out)
+    ev1.record(stream)
+    torch.cuda.synchronize()
+    infer_ms = ev0.elapsed_time(ev1) / reps
+    cache.close()
+    r.close()
+    trace_ms = min(trace.values())
+    return {"plain_ms": plain_ms, "plain_form_flags": flags, "primary_trace_ms": trace_ms, "primary_trace_forms_ms": trace,
+            "samples_with_hit": hits, "inference_ms": infer_ms, "bound_ms": trace_ms + infer_ms,
+            "max_speedup": plain_ms / (trace_ms + infer_ms)}
+
+
 def batched_views_leg(frames: int, stream, spec, n_views: int = 48) -> dict:
     import numpy as np
     import torch
@@ -519,7 +575,15 @@ def main():
             "after_second_bounce": legs[0], "at_primary_hit": legs[1],
             "mesh_pbr_depth8": {"workload": "the mesh + PBR scene at bounce depth 8, orbit radius 6.6 at height 2.4 "
                                             "around (0, 1, 0.4)",
-                                "after_second_bounce": mesh_legs[0], "at_primary_hit": mesh_legs[1]}}
+                                "after_second_bounce": mesh_legs[0], "at_primary_hit": mesh_legs[1]},
+            "bound": {"what": "the shortest frame any radiance cache can give: the camera rays' closest hits "
+                              "(hit-record frame, fastest per-sample form) plus one network inference per sample "
+                              "with a hit; max_speedup = plain frame / that",
+                      "mesh_pbr_depth4": cache_bound(spec, stream),
+                      "mesh_pbr_depth8": cache_bound(scenes.mesh_pbr_scene(0, args.width, args.height, max_depth=8),
+                                                     stream),
+                      "implicit_depth8": cache_bound(scenes.implicit_scene(0, args.width, args.height, max_depth=8),
+                                                     stream)}}
     if not args.no_cpu_baseline:
         result["cpu_baseline"] = cpu_baseline(spec, rays)
     print(json.dumps(result))

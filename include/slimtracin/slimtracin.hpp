@@ -65,6 +65,10 @@ struct Color {
     Color() = default;
     Color(f32 v) : r{v}, g{v}, b{v} {}
     Color(f32 r, f32 g, f32 b) : r{r}, g{g}, b{b} {}
+    // the arithmetic the example apps use (src/slim/core/base.h:1030-1040; multiply then add, no FMA)
+    Color operator-(const Color &o) const { return {r - o.r, g - o.g, b - o.b}; }
+    Color scaleAdd(f32 factor, const Color &add) const { return {r * factor + add.r, g * factor + add.g, b * factor + add.b}; }
+    Color lerpTo(const Color &to, f32 by) const { return (to - *this).scaleAdd(by, *this); }
     Color(ColorID id) {   // the values of Color(ColorID) (src/slim/core/base.h:805-858)
         switch (id) {
             case White: r = g = b = 1.0f; break;
@@ -106,6 +110,12 @@ struct Color {
         }
     }
 };
+
+inline f32 clampedValue(f32 value, f32 from, f32 to) {   // base.h:222-225
+    f32 mn = value < to ? value : to;
+    return mn > from ? mn : from;
+}
+inline f32 clampedValue(f32 value) { return clampedValue(value, 0.0f, 1.0f); }
 
 struct Pixel {   // Pixel (src/slim/core/base.h:1091-1094): what Canvas::setPixel stores
     Color color;
@@ -160,6 +170,16 @@ struct quat {
     static quat RotationAroundX(f32 r) { return AxisAngle({1, 0, 0}, r); }
     static quat RotationAroundY(f32 r) { return AxisAngle({0, 1, 0}, r); }
     static quat RotationAroundZ(f32 r) { return AxisAngle({0, 0, 1}, r); }
+    f32 length() const { return sqrtf(axis.x * axis.x + axis.y * axis.y + axis.z * axis.z + amount * amount); }
+    quat operator*(f32 s) const { return {axis * s, amount * s}; }
+    quat operator/(f32 s) const { return *this * (1.0f / s); }
+    quat normalized() const { return *this / length(); }
+    quat operator*(const quat &r) const {
+        return {{amount * r.axis.x + axis.x * r.amount + axis.y * r.axis.z - axis.z * r.axis.y,
+                 amount * r.axis.y - axis.x * r.axis.z + axis.y * r.amount + axis.z * r.axis.x,
+                 amount * r.axis.z + axis.x * r.axis.y - axis.y * r.axis.x + axis.z * r.amount},
+                amount * r.amount - axis.x * r.axis.x - axis.y * r.axis.y - axis.z * r.axis.z};
+    }
 };
 
 // Euler-angle orientations (Orientation<T>, src/slim/core/base.h:554-621): rotation z, then x, then y.
@@ -441,15 +461,27 @@ struct Dimensions {
     void update(u16 w, u16 h) { width = w; height = h; width_times_height = (u32)w * h; }
 };
 
-// Canvas over caller-owned buffers: pixels (x4 under SSAA) and depths (Canvas, src/slim/draw/canvas.h:11-39)
+// Canvas (src/slim/draw/canvas.h:11-39): pixels (x4 under SSAA) and depths, in caller-owned buffers or, when
+// none are given (as the reference's platform layer hands its canvases memory), in the canvas's own storage,
+// sized at each render to the current dimensions.
 struct Canvas {
     Dimensions dimensions;
     AntiAliasing antialias = NoAA;
     Pixel *pixels = nullptr;
     f32 *depths = nullptr;
+    std::vector<Pixel> own_pixels;
+    std::vector<f32> own_depths;
     Canvas() = default;
     Canvas(u16 width, u16 height, AntiAliasing aa = NoAA, Pixel *pixels = nullptr, f32 *depths = nullptr)
         : antialias{aa}, pixels{pixels}, depths{depths} { dimensions.update(width, height); }
+    size_t sampleCount() const { return (size_t)dimensions.width_times_height * (antialias == SSAA ? 4 : 1); }
+    void ensureStorage() {
+        if (pixels && pixels != own_pixels.data()) return;   // the caller's buffers
+        own_pixels.resize(sampleCount());
+        own_depths.resize(sampleCount());
+        pixels = own_pixels.data();
+        depths = own_depths.data();
+    }
 };
 
 struct Viewport {
@@ -676,7 +708,8 @@ struct RayTracingRenderer {
     // RayTracingRenderer::render (renderer.h:63-85): use_GPU selects the MI355X kernels, otherwise host threads.
     void render(const Viewport &viewport, bool update_scene = true, bool use_GPU = false) {
         using namespace slimtracin_detail;
-        const Canvas &canvas = viewport.canvas;
+        Canvas &canvas = viewport.canvas;
+        canvas.ensureStorage();
         if (update_scene) {
             scene.updateAABBs();
             scene.updateBVH();
@@ -712,6 +745,15 @@ struct RayTracingRenderer {
             uploadAll(ctx, scene);   // the host context reads the scene's current arrays
         }
         check(slimrt_render(ctx, &p, (slimrt_pixel *)canvas.pixels, canvas.depths), ctx, "render");
+        if (frameHook()) frameHook()(scene, viewport, p, use_GPU);
+    }
+
+    // Called after every render with what was rendered (the scene, the viewport and its canvas, the frame's
+    // parameters); frame recorders for offline checks install one (include/slimtracin/app.hpp).
+    using FrameHook = void (*)(const Scene &, const Viewport &, const slimrt_render_params &, bool use_GPU);
+    static FrameHook &frameHook() {
+        static FrameHook hook = nullptr;
+        return hook;
     }
 };
 

@@ -1428,10 +1428,18 @@ void *oracle_scene_load(const char *path, int ray_cones) {
     s->ray_cones = ray_cones;
     const float *gf = (const float *)spec_get(&s->sf, "geo_f", NULL);
     const uint32_t *gu = (const uint32_t *)spec_get(&s->sf, "geo_u", NULL);
+    /* geo_q: the orientations as quaternions (axis, amount), as an app holds them after composing rotations
+     * (Transform::orientation *= rot, quat.h:56-65); without it they come from the Euler angles */
+    const float *gq = (const float *)spec_get(&s->sf, "geo_q", NULL);
     s->geos = (Geo *)calloc(s->G, sizeof(Geo));
     for (uint32_t i = 0; i < s->G; i++) {
         const float *f = gf + 9 * i;
-        s->geos[i].q = quat_from_euler(f[0], f[1], f[2]);
+        if (gq) {
+            s->geos[i].q.axis = V3(gq[4 * i], gq[4 * i + 1], gq[4 * i + 2]);
+            s->geos[i].q.w = gq[4 * i + 3];
+        } else {
+            s->geos[i].q = quat_from_euler(f[0], f[1], f[2]);
+        }
         s->geos[i].position = V3(f[3], f[4], f[5]);
         s->geos[i].scale = V3(f[6], f[7], f[8]);
         s->geos[i].type = gu[4 * i]; s->geos[i].material_id = gu[4 * i + 1];

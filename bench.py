@@ -71,9 +71,9 @@ def measured_traffic(width: int, height: int, form: str):
 
 
 def algorithmic_bytes(cnt: dict, wide: bool = False) -> int:
-    """Bytes the frame's work reads and writes.  With 4-wide mesh BVHs the 'pairs' counter counts 128-byte node
-    visits instead of 64-byte sibling-pair visits."""
-    per = dict(BYTES_PER, pairs=128) if wide else BYTES_PER
+    """Bytes the frame's work reads and writes on the reference's units, or (wide) on the 4-wide form's own records:
+    128-byte node visits counted by 'pairs' and 48-byte triangle records."""
+    per = dict(BYTES_PER, pairs=128, triangles=48) if wide else BYTES_PER
     return sum(cnt[k] * b for k, b in per.items()) + cnt["pixels"] * OUT_BYTES_PER_SAMPLE
 
 

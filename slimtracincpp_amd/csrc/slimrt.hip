@@ -916,10 +916,11 @@ int slimrt_upload_scene(slimrt_context *c, const slimrt_scene_desc *d) {
         for (uint32_t i = 0; i < m.triangle_count; i++) {
             const slimrt_triangle &t = m.triangles[i];
             const float *L = t.local_to_tangent;
-            c->tri_hit.push_back(TriHit{L[0], L[1], L[2], t.position[0], L[3], L[4], L[5], t.position[1],
-                                        L[6], L[7], L[8], t.position[2], t.normal[0], t.normal[1], t.normal[2], t.uv_coverage});
+            c->tri_hit.push_back(TriHit{L[0], L[1], L[3], L[4], L[6], L[7], t.position[0], t.position[1],
+                                        t.position[2], t.normal[0], t.normal[1], t.normal[2]});
             c->tri_attr.push_back(TriAttr{t.n1[0], t.n1[1], t.n1[2], t.uv1[0], t.n2[0], t.n2[1], t.n2[2], t.uv1[1],
-                                          t.n3[0], t.n3[1], t.n3[2], t.uv2[0], t.uv2[1], t.uv3[0], t.uv3[1], 0.0f});
+                                          t.n3[0], t.n3[1], t.n3[2], t.uv2[0], t.uv2[1], t.uv3[0], t.uv3[1],
+                                          t.uv_coverage});
         }
         c->mesh_height = std::max(c->mesh_height, m.height);
     }

@@ -82,19 +82,20 @@ struct alignas(16) Node4 {
     float maxx[4], maxy[4], maxz[4];
     uint32_t child[4], count[4];
 };
-// Triangle split into the 64 bytes every intersection test reads and the 64 bytes only the closest
-// hit reads (Triangle is 128 bytes, src/slim/scene/mesh.h:19-24).
+// Triangle split into the 48 bytes every intersection test reads and the 64 bytes only the closest hit reads
+// (Triangle is 128 bytes, src/slim/scene/mesh.h:19-24).  The test uses the first two columns of the
+// local-to-tangent matrix (the barycentric coordinates), the position and the plane normal; the third column is
+// read by nothing.
 struct alignas(16) TriHit {
-    float xx, xy, xz, px;        // local_to_tangent row X | position.x
-    float yx, yy, yz, py;        // row Y | position.y
-    float zx, zy, zz, pz;        // row Z | position.z
-    float nx, ny, nz, uv_coverage;
+    float xx, xy, yx, yy;        // local_to_tangent rows X and Y, first two columns
+    float zx, zy, px, py;        // row Z, first two columns | position.x, position.y
+    float pz, nx, ny, nz;        // position.z | plane normal
 };
 struct alignas(16) TriAttr {
     float n1x, n1y, n1z, uv1u;
     float n2x, n2y, n2z, uv1v;
     float n3x, n3y, n3z, uv2u;
-    float uv2v, uv3u, uv3v, pad;
+    float uv2v, uv3u, uv3v, uv_coverage;
 };
 struct alignas(16) TexRec { uint32_t width, height, flags, mip_count; uint32_t mip_base, pad0, pad1, pad2; };
 struct alignas(16) MipRec { uint32_t w, h; uint32_t quad_base, pad; };   // quads: 12 bytes, R{TL,TR,BL,BR} G B

@@ -51,39 +51,53 @@ using Tracer = TracerT<false>;
 
 SLIM_HD Node load_node(const Node *nodes, uint32_t i) { return nodes[i]; }
 
-// MeshTracer::hitTriangles (mesh_tracer.h:23-49); triangle indices are relative to the mesh.
+// MeshTracer::hitTriangles (mesh_tracer.h:23-49); triangle indices are relative to the mesh.  The records are
+// fetched two at a time, so a leaf costs half as many dependent round trips to memory; they are tested in order
+// as before.
 template <class TR>
 SLIM_HD bool hit_triangles(const SceneView &s, const MeshRec &m, uint32_t first, uint32_t n, float closest,
                            const Ray &r, Hit &hit, bool any_hit, TR &tr) {
+    constexpr uint32_t kBatch = 2;
     bool found = false;
     closest = fmin2(closest, hit.distance);
     Hit th = hit;
-    for (uint32_t i = 0; i < n; i++) {
-        const TriHit t = s.tri_hit[m.tri_base + first + i];
-        SLIM_COUNT(tr, triangles);
-        v3 tp{t.px, t.py, t.pz};
-        if (hits_plane(r, tp, v3{t.nx, t.ny, t.nz}, th)) {
-            v3 d = th.position - tp;
-            float ux = t.xx * d.x + t.yx * d.y + t.zx * d.z;
-            float uy = t.xy * d.x + t.yy * d.y + t.zy * d.z;
-            if (ux < 0 || uy < 0 || (ux + uy) > 1 || th.distance >= closest) continue;
-            closest = th.distance;
-            hit = th;
-            hit.uv = v2{ux, uy};
-            hit.uv_coverage = t.uv_coverage;
-            hit.id = first + i;
-            found = true;
-            if (any_hit) break;
+    const TriHit *tris = s.tri_hit + m.tri_base + first;
+    for (uint32_t i0 = 0; i0 < n; i0 += kBatch) {
+        TriHit tb[kBatch];
+#pragma unroll
+        for (uint32_t k = 0; k < kBatch; k++)
+            if (i0 + k < n) tb[k] = tris[i0 + k];
+        bool stop = false;
+#pragma unroll
+        for (uint32_t k = 0; k < kBatch; k++) {
+            if (i0 + k >= n || stop) continue;
+            const TriHit &t = tb[k];
+            SLIM_COUNT(tr, triangles);
+            v3 tp{t.px, t.py, t.pz};
+            if (hits_plane(r, tp, v3{t.nx, t.ny, t.nz}, th)) {
+                v3 d = th.position - tp;
+                float ux = t.xx * d.x + t.yx * d.y + t.zx * d.z;
+                float uy = t.xy * d.x + t.yy * d.y + t.zy * d.z;
+                if (ux < 0 || uy < 0 || (ux + uy) > 1 || th.distance >= closest) continue;
+                closest = th.distance;
+                hit = th;
+                hit.uv = v2{ux, uy};
+                hit.id = first + i0 + k;
+                found = true;
+                if (any_hit) stop = true;
+            }
         }
+        if (stop) break;
     }
     return found;
 }
 
-// The closest hit's interpolated UV and normal (mesh_tracer.h:117-131).
+// The closest hit's UV coverage, interpolated UV and normal (mesh_tracer.h:36, 117-131).
 template <class TR>
 SLIM_HD void mesh_hit_attributes(const SceneView &s, const MeshRec &m, Hit &hit, TR &tr) {
-    if (!(m.normals_count | m.uvs_count)) return;
     const TriAttr t = s.tri_attr[m.tri_base + hit.id];
+    hit.uv_coverage = t.uv_coverage;
+    if (!(m.normals_count | m.uvs_count)) return;
     float a = hit.uv.u, b = hit.uv.v, c = 1 - a - b;
     SLIM_COUNT(tr, mesh_hits);
     if (m.uvs_count) {

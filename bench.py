@@ -30,6 +30,8 @@ Reported:
                BVH rebuild, every view's tiles in one launch, launch shape from the autotuner) replayed as one
                HIP graph; baseline: the host rebuild + upload and one launch per view.  Per-frame GPU latency
                p50/p99 (HIP events) and aggregate Mrays/s (rays counted on the first frames' transforms).
+  texel_storage (skip with --no-texel-storage) the same frame with the configuration's texture formats: RGBA8
+               albedo texels, RGBA16F cube faces of an HDR sky (include/slimrt.h SLIMRT_TEXELS_*)
   large        (skip with --no-large) the combined 4K workload: 16 instances of a 262,144-triangle displaced sphere,
                32 implicit primitives, 4 emissive quads, 8 + 8 textures at 2048^2, 512^2 cube IBL, depth 8; one
                4K frame in every form, 8 uneven views (4K down to 640x360) in one launch vs one launch per view,
@@ -298,6 +300,46 @@ def batched_views_leg(frames: int, stream, spec, n_views: int = 48) -> dict:
             "per_view_mrays_per_s": rays_frame / (float(np.mean(b)) * 1e-3) / 1e6}
 
 
+def texel_storage_leg(width: int, height: int, stream, steps: int, warmup: int) -> dict:
+    """The headline workload with the configuration's texture formats: the 1024^2 albedo (and its mips) as RGBA8
+    texels, the 6 x 256^2 sky and 6 x 32^2 irradiance faces as RGBA16F texels of an HDR sky (the same scene
+    otherwise); the 4-wide megakernel in both tile orders, the faster reported, beside the texel-quad frame."""
+    import numpy as np
+    import torch
+    from slimtracincpp_amd import scenes, runtime
+    spec = scenes.mesh_pbr_scene(0, width, height, texel_storage=True)
+    quads = scenes.mesh_pbr_scene(0, width, height)
+    nbytes = lambda sp: int(sum(m.nbytes for t in sp.textures for m in t.mips))
+    r = runtime.Renderer(spec, "gpu")
+    r.set_stream(stream.cuda_stream)
+    p = r.params()
+    n = width * height
+    px = torch.empty((n, 4), dtype=torch.float32, device="cuda")
+    dp = torch.empty(n, dtype=torch.float32, device="cuda")
+    cnt = r.count_work(p)
+    forms = {}
+    for name, flags in (("megakernel_bvh4", runtime.BVH4),
+                        ("megakernel_image_order_bvh4", runtime.BVH4 | runtime.TILE_ORDER_FIXED)):
+        p.flags = flags
+        for _ in range(warmup):
+            r.render_device(p, px.data_ptr(), dp.data_ptr())
+        r.synchronize()
+        ev0, ev1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        ev0.record(stream)
+        for _ in range(steps):
+            r.render_device(p, px.data_ptr(), dp.data_ptr())
+        ev1.record(stream)
+        torch.cuda.synchronize()
+        forms[name] = ev0.elapsed_time(ev1) / steps
+    r.close()
+    best = min(forms, key=forms.get)
+    return {"workload": "mesh_pbr as configured: 1024^2 RGBA8 albedo + mips, cube IBL 6x256^2 + 6x32^2 RGBA16F "
+                        "faces (HDR sky), otherwise the headline scene",
+            "texture_bytes": nbytes(spec), "texture_bytes_quads": nbytes(quads), "rays_per_frame": cnt["rays"],
+            "forms_ms": forms, "best_form": best, "ms_per_frame": forms[best],
+            "mrays_per_s": cnt["rays"] / (forms[best] * 1e-3) / 1e6}
+
+
 def large_leg(stream, frames: int = 5) -> dict:
     import numpy as np
     import torch
@@ -399,6 +441,7 @@ def main():
     ap.add_argument("--no-large", action="store_true", help="skip the combined 4K workload (about 1 minute)")
     ap.add_argument("--large", action="store_true", help=argparse.SUPPRESS)   # the 4K workload runs by default
     ap.add_argument("--view-frames", type=int, default=20)
+    ap.add_argument("--no-texel-storage", action="store_true", help="skip the RGBA8 / RGBA16F texture-format leg")
     ap.add_argument("--form", choices=FORM_NAMES, default=None,
                     help="time only this frame form (for profiler runs); by default every form, the fastest reported")
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
@@ -556,6 +599,8 @@ def main():
                 "tiles": int(c.size), "resident_waves": r.tile_resident, "median_us": float(np.median(wave)) / 100.0,
                 "p99_us": float(np.percentile(wave, 99)) / 100.0, "longest_us": float(wave.max()) / 100.0,
                 "ideal_frame_us": ideal_us, "order": "longest first" if ordered else "image"}
+    if not args.no_texel_storage:
+        result["texel_storage"] = texel_storage_leg(args.width, args.height, stream, args.steps, args.warmup)
     if not args.no_views:
         result["batched_views"] = batched_views_leg(args.view_frames, stream, spec)
     if not args.no_large:

@@ -141,13 +141,26 @@ typedef struct {
     uint32_t triangle_count, normals_count, uvs_count;
 } slimrt_mesh;
 
-/* TextureMip (src/slim/core/texture.h:13-16): (width+1)*(height+1) texel quads of 12 bytes each. */
+/* TextureMip (src/slim/core/texture.h:13-16): (width+1)*(height+1) texel quads of 12 bytes each, or, when the
+ * texture's flags select texel storage (SLIMRT_TEXELS_*), (width+2)*(height+2) texels of that format. */
 typedef struct {
     uint32_t width, height;
     const uint8_t *texel_quads;
 } slimrt_texture_mip;
 
-/* Texture (src/slim/core/texture.h:44-46): mip 0 size, ImageFlags bits (base.h:1283-1295), mips. */
+/* Texel storage, bits 24-25 of a texture's flags (no reference counterpart: the reference stores texel quads).
+ * The texel formats hold a mip as a grid of (width+2)*(height+2) texels, rows top to bottom, with a one-texel
+ * border: texel (x, y) of that grid is the top-left corner of quad (x, y), so the border holds what the quads'
+ * outer corners hold (wrapped or clamped texels, or for cube maps the neighbouring faces' texels).
+ *   SLIMRT_TEXELS_RGBA8:   4 bytes per texel (R, G, B, A; A unused), sampled as u8 / 255 like the quads, so a
+ *                          texture converted from consistent quads samples to the same bits (3x less memory).
+ *   SLIMRT_TEXELS_RGBA16F: 8 bytes per texel, four IEEE half floats, sampled as they are (HDR values allowed). */
+#define SLIMRT_TEXELS_RGBA8   (1u << 24)
+#define SLIMRT_TEXELS_RGBA16F (2u << 24)
+#define SLIMRT_TEXELS_MASK    (3u << 24)
+
+/* Texture (src/slim/core/texture.h:44-46): mip 0 size, ImageFlags bits (base.h:1283-1295) | texel storage,
+ * mips. */
 typedef struct {
     uint32_t width, height, flags, mip_count;
     const slimrt_texture_mip *mips;
@@ -335,6 +348,16 @@ int slimrt_tile_costs(slimrt_context *ctx, uint32_t *costs, uint32_t *order, uin
 int slimrt_host_bake_texture(const uint8_t *rgb, uint32_t width, uint32_t height, uint32_t flags, uint8_t *quads_out,
                              size_t quad_cap, size_t *quad_count_out, uint32_t *mip_count_out, uint32_t *mip_dims_out,
                              uint32_t *flags_out);
+
+/* A mip's texel quads (width * height texels, (width+1)*(height+1) quads) as the bordered texel grid of the
+ * SLIMRT_TEXELS_* storage (no reference counterpart): texels_out receives (width+2)*(height+2) texels, RGBA8 (A =
+ * 255) or RGBA16F (each u8 component c as the half nearest to the float c / 255, A = 1).  *mismatches_out (may be
+ * NULL) receives the number of quads whose corners differ from the grid's texels: 0 means the RGBA8 texels sample
+ * exactly as the quads do. */
+int slimrt_host_texels_from_quads(const uint8_t *quads, uint32_t width, uint32_t height, uint32_t format,
+                                  void *texels_out, uint64_t *mismatches_out);
+/* IEEE half-float bits of f, rounded to nearest even (no reference counterpart; for building RGBA16F texels). */
+uint16_t slimrt_host_float_to_half(float f);
 
 /* BVHBuilder::buildMesh (src/slim/scene/bvh_builder.h:265-341): mesh BVH and leaf-ordered triangle records.
  * normals/uvs may be NULL (counts 0).  nodes_out: 2T - 1 nodes; triangles_out: T records; aabb_out: 6 floats

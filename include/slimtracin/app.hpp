@@ -217,7 +217,7 @@ inline SpecWriter sceneSpec(const Scene &s, const Viewport &v, const slimrt_rend
         w.u32s(nm, dims);
         for (u32 i = 0; i < t.mip_count; i++) {
             snprintf(nm, sizeof nm, "tex%u_mip%u", k, i);
-            w.section(nm, 3, t.mips[i].texel_quads, 12 * (size_t)(t.mips[i].width + 1) * (t.mips[i].height + 1), 1);
+            w.section(nm, 3, t.mips[i].texel_quads, mipBytes(t, i), 1);
         }
     }
     return w;

@@ -388,6 +388,7 @@ inline bool save(const Mesh &m, const char *path) {
     return fclose(f) == 0 && ok;
 }
 inline bool save(const Texture &t, const char *path) {
+    if (t.flags & SLIMRT_TEXELS_MASK) return false;   // the .texture format holds texel quads only
     FILE *f = fopen(path, "wb");
     if (!f) return false;
     const u32 info[8] = {t.width, t.height, t.width * t.height, t.width, 0, 0, t.mip_count, t.flags};
@@ -447,6 +448,41 @@ inline Texture bakeTexture(const u8 *rgb, u32 width, u32 height, u32 flags) {
         quads += 12 * (size_t)(dims[2 * i] + 1) * (dims[2 * i + 1] + 1);
     }
     return t;
+}
+
+// Bytes of one mip in the texture's storage: the reference's texel quads, or a bordered grid of RGBA8 / RGBA16F
+// texels (SLIMRT_TEXELS_* in the flags; include/slimrt.h).
+inline size_t mipBytes(const Texture &t, u32 mip) {
+    const size_t w = t.mips[mip].width, h = t.mips[mip].height;
+    switch (t.flags & SLIMRT_TEXELS_MASK) {
+        case SLIMRT_TEXELS_RGBA8: return 4 * (w + 2) * (h + 2);
+        case SLIMRT_TEXELS_RGBA16F: return 8 * (w + 2) * (h + 2);
+        default: return 12 * (w + 1) * (h + 1);
+    }
+}
+
+// A texel-quad texture in texel storage (no reference counterpart): RGBA8 samples exactly as the quads whenever
+// no quad's corners differ from the texel grid (*mismatches, summed over the mips, may be null); RGBA16F holds each
+// component as the nearest half to u8 / 255.  The texels point into memory the API keeps, as for loaded assets.
+inline Texture toTexelStorage(const Texture &t, u32 storage, u64 *mismatches = nullptr) {
+    using namespace slimtracin_detail;
+    if ((t.flags & SLIMRT_TEXELS_MASK) || (storage != SLIMRT_TEXELS_RGBA8 && storage != SLIMRT_TEXELS_RGBA16F))
+        throw std::runtime_error("toTexelStorage: a texel-quad texture and SLIMRT_TEXELS_RGBA8 or _RGBA16F expected");
+    Texture out = t;
+    out.flags = t.flags | storage;
+    out.mips = alloc<TextureMip>(t.mip_count);
+    if (mismatches) *mismatches = 0;
+    for (u32 i = 0; i < t.mip_count; i++) {
+        out.mips[i].width = t.mips[i].width;
+        out.mips[i].height = t.mips[i].height;
+        u8 *texels = alloc<u8>(mipBytes(out, i));
+        uint64_t bad = 0;
+        check(slimrt_host_texels_from_quads((const uint8_t *)t.mips[i].texel_quads, t.mips[i].width, t.mips[i].height,
+                                            storage, texels, &bad), nullptr, "toTexelStorage");
+        if (mismatches) *mismatches += bad;
+        out.mips[i].texel_quads = reinterpret_cast<TexelQuad *>(texels);
+    }
+    return out;
 }
 
 struct Camera {

@@ -53,6 +53,9 @@ def lib():
         L.oracle_mip_level.argtypes = [f, u32]
         L.oracle_quat_from_euler.argtypes = [f, f, f, vp]
         L.oracle_mip_sample.argtypes = [u32, u32, vp, f, f, vp]
+        L.oracle_mip_sample_fmt.argtypes = [u32, u32, vp, u32, f, f, vp]
+        L.oracle_half_to_float.argtypes = [ctypes.c_uint16]
+        L.oracle_half_to_float.restype = f
         L.oracle_hits_aabb.restype = i
         L.oracle_hits_aabb.argtypes = [vp, vp, vp, vp]
         L.oracle_intersect.restype = i

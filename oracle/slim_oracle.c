@@ -251,7 +251,9 @@ typedef struct {
     uint32_t tri_count, normals_count, uvs_count;
 } MeshD;
 
-typedef struct { uint32_t w, h; const uint8_t *quads; } Mip;
+/* fmt 0: texel quads (the reference's layout); 1: bordered RGBA8 texels; 2: bordered RGBA16F texels (the
+ * product's texel storage, include/slimrt.h SLIMRT_TEXELS_*: texture flags bits 24-25). */
+typedef struct { uint32_t w, h; const uint8_t *quads; uint32_t fmt; } Mip;
 typedef struct { uint32_t width, height, flags, mip_count; Mip *mips; } Tex;
 
 typedef struct {
@@ -959,6 +961,24 @@ static float integrate_density(const Tracer *tr) {                              
 }
 
 /* ---------------------------------------------------------------- textures (src/slim/core/texture.h) */
+/* IEEE binary16 bits -> the float of the same value (every half is a float exactly). */
+static float half_to_float(uint16_t h) {
+    const uint32_t sign = (uint32_t)(h & 0x8000u) << 16, e = (h >> 10) & 0x1Fu, m = h & 0x3FFu;
+    uint32_t bits;
+    if (e == 31) bits = sign | 0x7F800000u | (m << 13);                      /* infinity, NaN */
+    else if (e) bits = sign | ((e + 112u) << 23) | (m << 13);                  /* normal */
+    else if (!m) bits = sign;                                                 /* zero */
+    else {                                                                    /* subnormal: m * 2^-24 */
+        float f = (float)m * 5.9604644775390625e-8f;
+        memcpy(&bits, &f, 4);
+        bits |= sign;
+    }
+    float f;
+    memcpy(&f, &bits, 4);
+    return f;
+}
+float oracle_half_to_float(uint16_t h) { return half_to_float(h); }
+
 static color mip_sample(const Mip *m, float u, float v, Counters *cnt) {                                    /* texture.h:17-41 */
     cnt->taps++;
     if (u > 1) u -= (float)((uint64_t)u);
@@ -966,8 +986,27 @@ static color mip_sample(const Mip *m, float u, float v, Counters *cnt) {        
     const float U = u * (float)m->w + 0.5f, Vv = v * (float)m->h + 0.5f;
     const uint64_t x = (uint64_t)U, y = (uint64_t)Vv;
     const float r = U - (float)x, b = Vv - (float)y, l = 1 - r, t = 1 - b;
+    if (m->fmt == 2) {   /* half texels: the same bilinear weights without the 1/255 scale, the same order */
+        const float tl = t * l, tr = t * r, bl = b * l, br = b * r;
+        const uint64_t row = (uint64_t)m->w + 2;
+        const uint16_t *p = (const uint16_t *)m->quads + 4 * (y * row + x), *pb = p + 4 * row;
+        color c;
+        c.r = half_to_float(pb[4]) * br + (half_to_float(pb[0]) * bl + (half_to_float(p[4]) * tr + half_to_float(p[0]) * tl));
+        c.g = half_to_float(pb[5]) * br + (half_to_float(pb[1]) * bl + (half_to_float(p[5]) * tr + half_to_float(p[1]) * tl));
+        c.b = half_to_float(pb[6]) * br + (half_to_float(pb[2]) * bl + (half_to_float(p[6]) * tr + half_to_float(p[2]) * tl));
+        return c;
+    }
     const float tl = t * l * COLOR_COMPONENT_TO_FLOAT, tr = t * r * COLOR_COMPONENT_TO_FLOAT;
     const float bl = b * l * COLOR_COMPONENT_TO_FLOAT, br = b * r * COLOR_COMPONENT_TO_FLOAT;
+    if (m->fmt == 1) {   /* RGBA8 texels: grid texel (x, y) is quad (x, y)'s top-left corner */
+        const uint64_t row = (uint64_t)m->w + 2;
+        const uint8_t *p = m->quads + 4 * (y * row + x), *pb = p + 4 * row;
+        color c;
+        c.r = (float)pb[4] * br + ((float)pb[0] * bl + ((float)p[4] * tr + (float)p[0] * tl));
+        c.g = (float)pb[5] * br + ((float)pb[1] * bl + ((float)p[5] * tr + (float)p[1] * tl));
+        c.b = (float)pb[6] * br + ((float)pb[2] * bl + ((float)p[6] * tr + (float)p[2] * tl));
+        return c;
+    }
     const uint8_t *q = m->quads + 12 * (y * (m->w + 1) + x);
     color c;
     c.r = (float)q[3] * br + ((float)q[2] * bl + ((float)q[1] * tr + (float)q[0] * tl));
@@ -1512,6 +1551,7 @@ void *oracle_scene_load(const char *path, int ray_cones) {
             t->mips[m].w = dims[2 * m];
             t->mips[m].h = dims[2 * m + 1];
             t->mips[m].quads = (const uint8_t *)spec_get(&s->sf, nm, NULL);
+            t->mips[m].fmt = (t->flags >> 24) & 3u;
         }
     }
     for (uint32_t i = 0; i < s->G; i++)
@@ -1770,7 +1810,14 @@ void oracle_quat_from_euler(float x, float y, float z, float *out) {
 }
 void oracle_mip_sample(uint32_t w, uint32_t h, const uint8_t *quads, float u, float v, float *out) {
     Counters cnt;
-    Mip m = {w, h, quads};
+    Mip m = {w, h, quads, 0};
+    color c = mip_sample(&m, u, v, &cnt);
+    out[0] = c.r; out[1] = c.g; out[2] = c.b;
+}
+/* The same for a mip in texel storage fmt (0 quads, 1 RGBA8, 2 RGBA16F). */
+void oracle_mip_sample_fmt(uint32_t w, uint32_t h, const uint8_t *data, uint32_t fmt, float u, float v, float *out) {
+    Counters cnt;
+    Mip m = {w, h, data, fmt};
     color c = mip_sample(&m, u, v, &cnt);
     out[0] = c.r; out[1] = c.g; out[2] = c.b;
 }

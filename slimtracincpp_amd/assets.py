@@ -96,6 +96,8 @@ def read_texture(path: str) -> dict:
 def write_texture(path: str, t: dict):
     """Texture save (src/slim/core/base.h:1544-1546, serialization/texture.h:61-68); size and stride default to
     width * height and width, tiles to 0, as the converter leaves them."""
+    if t["flags"] & (3 << 24):
+        raise ValueError("the .texture format holds texel quads only; this texture is in texel storage")
     w, h = t["width"], t["height"]
     info = [w, h, t.get("size", w * h), t.get("stride", w), t.get("tile_width", 0), t.get("tile_height", 0),
             len(t["mips"]), t["flags"]]

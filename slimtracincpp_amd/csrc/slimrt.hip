@@ -924,18 +924,25 @@ int slimrt_upload_scene(slimrt_context *c, const slimrt_scene_desc *d) {
         }
         c->mesh_height = std::max(c->mesh_height, m.height);
     }
-    // textures: flatten mips and texel quads
+    // textures: flatten the mips (texel quads, or bordered RGBA8 / RGBA16F texels) into one byte array, each mip
+    // starting on a 16-byte boundary
     c->textures.clear(); c->mips.clear(); c->quads.clear();
     for (uint32_t k = 0; k < d->texture_count; k++) {
         const slimrt_texture &t = d->textures[k];
         if (!t.mips || t.mip_count == 0) return fail(c, SLIMRT_E_ARG, "texture without mips");
+        const uint32_t fmt = (t.flags & kTexelsMask) >> kTexelsShift;
+        if (fmt > kTexelsRGBA16F) return fail(c, SLIMRT_E_ARG, "unknown texel storage in texture flags");
         c->textures.push_back(TexRec{t.width, t.height, t.flags, t.mip_count, (uint32_t)c->mips.size(), 0, 0, 0});
         for (uint32_t m = 0; m < t.mip_count; m++) {
             const slimrt_texture_mip &mm = t.mips[m];
             if (!mm.texel_quads || mm.width == 0 || mm.height == 0) return fail(c, SLIMRT_E_ARG, "empty texture mip");
-            const size_t nq = (size_t)(mm.width + 1) * (mm.height + 1);
-            c->mips.push_back(MipRec{mm.width, mm.height, (uint32_t)(c->quads.size() / 12), 0});
-            c->quads.insert(c->quads.end(), mm.texel_quads, mm.texel_quads + 12 * nq);
+            const size_t bytes = fmt == kTexelQuads ? 12 * (size_t)(mm.width + 1) * (mm.height + 1)
+                                                    : (fmt == kTexelsRGBA8 ? 4 : 8) * (size_t)(mm.width + 2) * (mm.height + 2);
+            const size_t base = (c->quads.size() + 15) / 16;
+            if (base > 0xFFFFFFFFull) return fail(c, SLIMRT_E_ARG, "textures beyond 64 GiB");
+            c->quads.resize(16 * base, 0);
+            c->mips.push_back(MipRec{mm.width, mm.height, (uint32_t)base, fmt});
+            c->quads.insert(c->quads.end(), mm.texel_quads, mm.texel_quads + bytes);
         }
     }
     for (uint32_t i = 0; i < d->material_count; i++) {

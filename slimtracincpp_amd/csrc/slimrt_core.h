@@ -37,6 +37,9 @@ constexpr uint32_t MAT_REFLECTIVE = 1, MAT_REFRACTIVE = 2, MAT_EMISSIVE = 4, MAT
 constexpr uint32_t GEO_VISIBLE = 1, GEO_SHADOWING = 2, GEO_TRANSPARENT = 4;
 constexpr uint32_t LIGHT_DIRECTIONAL = 1, LIGHT_SHADOWING = 2;
 constexpr uint32_t IMG_MIPMAP = 1u << 4;
+// texel storage of a texture: bits 24-25 of its flags (no reference counterpart; the reference stores quads)
+constexpr uint32_t kTexelsShift = 24, kTexelsMask = 3u << kTexelsShift;
+constexpr uint32_t kTexelQuads = 0, kTexelsRGBA8 = 1, kTexelsRGBA16F = 2;
 
 // ------------------------------------------------------------------ device data layout (all 16-byte aligned)
 // BVH node, 32 bytes; siblings are adjacent so a pair is one 64-byte fetch (BVHNode, src/slim/scene/bvh.h:5-15).
@@ -98,7 +101,10 @@ struct alignas(16) TriAttr {
     float uv2v, uv3u, uv3v, uv_coverage;
 };
 struct alignas(16) TexRec { uint32_t width, height, flags, mip_count; uint32_t mip_base, pad0, pad1, pad2; };
-struct alignas(16) MipRec { uint32_t w, h; uint32_t quad_base, pad; };   // quads: 12 bytes, R{TL,TR,BL,BR} G B
+// A mip's storage starts at byte 16 * base of the texture array.  fmt kTexelQuads: (w + 1) * (h + 1) texel quads
+// of 12 bytes, R{TL,TR,BL,BR} G B (the reference's layout); kTexelsRGBA8 / kTexelsRGBA16F: (w + 2) * (h + 2)
+// texels of 4 / 8 bytes, rows top to bottom, with a one-texel border holding what the quads' outer corners hold.
+struct alignas(16) MipRec { uint32_t w, h; uint32_t base, fmt; };
 
 struct SceneView {
     const Geo *geos;
@@ -419,9 +425,39 @@ SLIM_HD color mip_sample(const SceneView &s, const MipRec &m, float u, float v) 
     // reference's conversion is undefined, so that no input can index outside the mip.
     const uint32_t x = (uint32_t)fminf(fmaxf(U, 0.0f), (float)m.w), y = (uint32_t)fminf(fmaxf(V, 0.0f), (float)m.h);
     const float rr = U - (float)x, b = V - (float)y, l = 1 - rr, t = 1 - b;
+    if (m.fmt == kTexelsRGBA16F) {
+        // half-float texels: the same bilinear weights without the 1/255 scale, the same order of terms
+        const float tl = t * l, tr = t * rr, bl = b * l, br = b * rr;
+        const uint64_t row = (uint64_t)m.w + 2;
+        const uint8_t *src = s.quads + 16ull * m.base + 8 * ((uint64_t)y * row + x);
+        _Float16 p[8], pb[8];   // TL, TR and BL, BR: one 16-byte load per row
+        __builtin_memcpy(p, src, 16);
+        __builtin_memcpy(pb, src + 8 * row, 16);
+        color c;
+        c.r = (float)pb[4] * br + ((float)pb[0] * bl + ((float)p[4] * tr + (float)p[0] * tl));
+        c.g = (float)pb[5] * br + ((float)pb[1] * bl + ((float)p[5] * tr + (float)p[1] * tl));
+        c.b = (float)pb[6] * br + ((float)pb[2] * bl + ((float)p[6] * tr + (float)p[2] * tl));
+        return c;
+    }
     const float tl = t * l * kColorToFloat, tr = t * rr * kColorToFloat;
     const float bl = b * l * kColorToFloat, br = b * rr * kColorToFloat;
-    const uint8_t *q = s.quads + 12ull * ((uint64_t)m.quad_base + (uint64_t)y * (m.w + 1) + x);
+    if (m.fmt == kTexelsRGBA8) {
+        // texel (x, y) of the bordered grid is the quad's top-left corner; its right and lower neighbours the rest
+        const uint64_t row = (uint64_t)m.w + 2;
+        const uint8_t *src = s.quads + 16ull * m.base + 4 * ((uint64_t)y * row + x);
+        uint32_t t2[2], b2[2];   // TL, TR and BL, BR: one 8-byte load per row
+        __builtin_memcpy(t2, src, 8);
+        __builtin_memcpy(b2, src + 4 * row, 8);
+        const uint32_t TL = t2[0], TR = t2[1], BL = b2[0], BR = b2[1];
+        color c;
+        c.r = (float)(BR & 255) * br + ((float)(BL & 255) * bl + ((float)(TR & 255) * tr + (float)(TL & 255) * tl));
+        c.g = (float)((BR >> 8) & 255) * br + ((float)((BL >> 8) & 255) * bl +
+                                               ((float)((TR >> 8) & 255) * tr + (float)((TL >> 8) & 255) * tl));
+        c.b = (float)((BR >> 16) & 255) * br + ((float)((BL >> 16) & 255) * bl +
+                                                ((float)((TR >> 16) & 255) * tr + (float)((TL >> 16) & 255) * tl));
+        return c;
+    }
+    const uint8_t *q = s.quads + 16ull * m.base + 12ull * ((uint64_t)y * (m.w + 1) + x);
     uint32_t R, G, B;
     __builtin_memcpy(&R, q, 4);
     __builtin_memcpy(&G, q + 4, 4);

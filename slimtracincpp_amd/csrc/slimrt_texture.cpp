@@ -20,6 +20,28 @@
 namespace {
 
 struct rgb { float r, g, b; };
+
+// IEEE binary32 -> binary16 bits, rounded to nearest, ties to even; overflow to infinity, NaN kept a (quiet) NaN.
+uint16_t half_bits(float f) {
+    uint32_t x;
+    memcpy(&x, &f, 4);
+    const uint32_t sign = (x >> 16) & 0x8000u, mag = x & 0x7FFFFFFFu;
+    if (mag > 0x7F800000u) return (uint16_t)(sign | 0x7E00u | ((mag >> 13) & 0x3FFu));   // NaN
+    if (mag >= 0x477FF000u) return (uint16_t)(sign | 0x7C00u);   // rounds to >= 65520: infinity
+    if (mag < 0x38800000u) {   // below the smallest normal half (2^-14): subnormal or zero
+        if (mag < 0x33000000u) return (uint16_t)sign;   // <= 2^-25: rounds to zero
+        const uint32_t e = mag >> 23, m = (mag & 0x7FFFFFu) | 0x800000u;
+        const uint32_t shift = 126 - e;   // the value is m * 2^(e - 150); a half subnormal counts 2^-24
+        uint32_t h = m >> shift;
+        const uint32_t rem = m & ((1u << shift) - 1), halfway = 1u << (shift - 1);
+        if (rem > halfway || (rem == halfway && (h & 1u))) h++;
+        return (uint16_t)(sign | h);
+    }
+    uint32_t h = ((mag >> 13) - (112u << 10));   // rebias the exponent 127 -> 15
+    const uint32_t rem = mag & 0x1FFFu;
+    if (rem > 0x1000u || (rem == 0x1000u && (h & 1u))) h++;
+    return (uint16_t)(sign | h);
+}
 struct quad { rgb tl, tr, bl, br; };
 
 constexpr float kByteToFloat = 0.00392156862f;   // COLOR_COMPONENT_TO_FLOAT (src/slim/core/base.h:139)
@@ -273,5 +295,49 @@ int slimrt_host_bake_texture(const uint8_t *rgb_bytes, uint32_t width, uint32_t 
     }
     return SLIMRT_OK;
 }
+
+// The bordered texel grid of a mip's quads (SLIMRT_TEXELS_* storage, include/slimrt.h): grid texel (x, y) is quad
+// (x, y)'s top-left corner for x <= w, y <= h; the last column and row come from the last quads' right and bottom
+// corners.  Quads whose four corners are not the grid's (a converter that wrote a corner of one quad differently
+// from the same texel in its neighbours) are counted: with none, RGBA8 texels sample exactly as the quads do.
+int slimrt_host_texels_from_quads(const uint8_t *quads, uint32_t width, uint32_t height, uint32_t format,
+                                  void *texels_out, uint64_t *mismatches_out) {
+    if (!quads || !texels_out || width == 0 || height == 0) return SLIMRT_E_ARG;
+    if (format != SLIMRT_TEXELS_RGBA8 && format != SLIMRT_TEXELS_RGBA16F) return SLIMRT_E_ARG;
+    const size_t W = (size_t)width + 2, H = (size_t)height + 2, QW = (size_t)width + 1;
+    std::vector<uint8_t> grid(W * H * 3);
+    auto corner = [&](size_t qx, size_t qy, int k, int ch) { return quads[12 * (qy * QW + qx) + 4 * ch + k]; };
+    for (size_t y = 0; y < H; y++)
+        for (size_t x = 0; x < W; x++) {
+            const size_t qx = x <= width ? x : width, qy = y <= height ? y : height;
+            const int k = (x > width ? 1 : 0) + (y > height ? 2 : 0);   // TL, TR, BL, BR
+            for (int ch = 0; ch < 3; ch++) grid[3 * (y * W + x) + ch] = corner(qx, qy, k, ch);
+        }
+    uint64_t bad = 0;
+    for (size_t qy = 0; qy <= height; qy++)
+        for (size_t qx = 0; qx <= width; qx++)
+            for (int k = 0; k < 4; k++) {
+                const size_t gx = qx + (k & 1), gy = qy + (k >> 1);
+                bool same = true;
+                for (int ch = 0; ch < 3; ch++) same = same && grid[3 * (gy * W + gx) + ch] == corner(qx, qy, k, ch);
+                if (!same) { bad++; break; }
+            }
+    if (mismatches_out) *mismatches_out = bad;
+    if (format == SLIMRT_TEXELS_RGBA8) {
+        uint8_t *o = (uint8_t *)texels_out;
+        for (size_t i = 0; i < W * H; i++) {
+            o[4 * i] = grid[3 * i]; o[4 * i + 1] = grid[3 * i + 1]; o[4 * i + 2] = grid[3 * i + 2]; o[4 * i + 3] = 255;
+        }
+    } else {
+        uint16_t *o = (uint16_t *)texels_out;
+        for (size_t i = 0; i < W * H; i++) {
+            for (int ch = 0; ch < 3; ch++) o[4 * i + ch] = half_bits((float)grid[3 * i + ch] / 255.0f);
+            o[4 * i + 3] = half_bits(1.0f);
+        }
+    }
+    return SLIMRT_OK;
+}
+
+uint16_t slimrt_host_float_to_half(float f) { return half_bits(f); }
 
 }  // extern "C"

@@ -159,6 +159,7 @@ def lib():
             "slimrt_pick": [vp, P(RenderParams), i32, i32, P(PickResult)],
             "slimrt_canvas_content": [vp, vp, u32, u32, u32, vp],
             "slimrt_host_bake_texture": [vp, u32, u32, u32, vp, C.c_size_t, P(C.c_size_t), P(u32), vp, P(u32)],
+            "slimrt_host_texels_from_quads": [vp, u32, u32, u32, vp, P(C.c_uint64)],
             "slimrt_draw_boxes": [vp, P(OverlayView), vp, u32, vp, vp],
             "slimrt_draw_mesh_wireframe": [vp, P(OverlayView), vp, u32, vp, u32, vp, vp, f32, u32, vp, vp],
             "slimrt_host_bvh_boxes": [vp, u32, vp, u32, u32, f32, u32, vp, P(u32)],
@@ -168,6 +169,8 @@ def lib():
             fn = getattr(L, name)
             fn.argtypes = args
             fn.restype = C.c_int
+        L.slimrt_host_float_to_half.argtypes = [f32]
+        L.slimrt_host_float_to_half.restype = C.c_uint16
         L.slimrt_last_error.argtypes = [vp]
         L.slimrt_last_error.restype = C.c_char_p
         L.slimrt_views_last_error.argtypes = [vp]
@@ -180,6 +183,31 @@ class SlimrtError(RuntimeError):
     def __init__(self, code, msg):
         super().__init__(f"{ERRORS.get(code, code)}: {msg}")
         self.code = code
+
+
+def host_texels_from_quads(q: np.ndarray, width: int, height: int, texel_storage: int):
+    """slimrt_host_texels_from_quads: (bordered texel grid (h+2, w+2, 4) u8 or float16, mismatching quads)."""
+    q = np.ascontiguousarray(q, np.uint8)
+    assert q.size == 12 * (width + 1) * (height + 1)
+    out = np.zeros((height + 2, width + 2, 4), np.uint8 if texel_storage == scenes.TEXELS_RGBA8 else np.float16)
+    bad = C.c_uint64(0)
+    _check(lib().slimrt_host_texels_from_quads(_ptr(q), width, height, texel_storage, _ptr(out), C.byref(bad)),
+           None, "slimrt_host_texels_from_quads")
+    return out, int(bad.value)
+
+
+def host_float_to_half(x: float) -> int:
+    """slimrt_host_float_to_half: the IEEE half bits nearest to the float x (ties to even)."""
+    return int(lib().slimrt_host_float_to_half(float(x)))
+
+
+def mip_bytes(q: np.ndarray) -> np.ndarray:
+    """A mip's storage as contiguous bytes: u8 quads or RGBA8 texels as they are, float16 texels reinterpreted."""
+    q = np.ascontiguousarray(q)
+    if q.dtype == np.float16:
+        return q.view(np.uint8)
+    assert q.dtype == np.uint8, q.dtype
+    return q
 
 
 def _ptr(a: np.ndarray):
@@ -327,7 +355,7 @@ class SceneBuild:
         self.materials = materials_from_spec(spec)
         self.lights = lights_from_spec(spec)
         self.mesh_descs = (Mesh * max(len(self.meshes), 1))(*[m.desc() for m in self.meshes])
-        self.quads = [[np.ascontiguousarray(q, np.uint8) for q in t.mips] for t in spec.textures]
+        self.quads = [[mip_bytes(q) for q in t.mips] for t in spec.textures]
         self.mip_descs = [(TextureMip * len(t.mips))(*[TextureMip(w, h, _ptr(q)) for (w, h), q in zip(t.mip_dims, qs)])
                           for t, qs in zip(spec.textures, self.quads)]
         self.tex_descs = (Texture * max(len(spec.textures), 1))(

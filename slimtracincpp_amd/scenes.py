@@ -50,6 +50,12 @@ IMG_MIPMAP = 1 << 4
 IMG_WRAP = 1 << 6
 IMG_NORMAL = 1 << 7
 IMG_CUBEMAP = 1 << 8
+# texel storage (texture flags bits 24-25; include/slimrt.h SLIMRT_TEXELS_*): the reference's texel quads, or a
+# bordered grid of RGBA8 or RGBA16F texels
+TEXELS_QUADS = 0
+TEXELS_RGBA8 = 1 << 24
+TEXELS_RGBA16F = 2 << 24
+TEXELS_MASK = 3 << 24
 
 IOR_AIR = np.float32(1.0003)
 
@@ -133,8 +139,13 @@ class TextureSpec:
     width: int
     height: int
     flags: int
-    mips: List[np.ndarray]         # each (h+1, w+1, 12) u8 texel quads: R{TL,TR,BL,BR} G{..} B{..}
+    mips: List[np.ndarray]         # each (h+1, w+1, 12) u8 texel quads: R{TL,TR,BL,BR} G{..} B{..}, or with
+                                   # TEXELS_RGBA8 / TEXELS_RGBA16F in flags (h+2, w+2, 4) u8 / float16 texels
     mip_dims: List[tuple]          # (w, h) per mip
+
+    @property
+    def texel_storage(self) -> int:
+        return self.flags & TEXELS_MASK
 
 
 @dataclass
@@ -414,13 +425,7 @@ def bake_texel_quads(texels: np.ndarray, wrap: bool) -> np.ndarray:
     same quads except, with wrap, the right border column's TR and BR corners, which its later right-edge writes
     leave at the last column's texels.  Both renderers read the same quads, so this is a choice of input, not a
     parity difference.  Quantisation truncates x * 255 (src/bmp2texture.cpp:327-357)."""
-    h, w, _ = texels.shape
-    xs = np.arange(-1, w + 1)
-    ys = np.arange(-1, h + 1)
-    xs = xs % w if wrap else np.clip(xs, 0, w - 1)
-    ys = ys % h if wrap else np.clip(ys, 0, h - 1)
-    pad = texels[ys][:, xs].astype(np.float32)
-    return quantize_quads(pad)
+    return quantize_quads(pad_texels(texels, wrap))
 
 
 def quantize_quads(pad: np.ndarray) -> np.ndarray:
@@ -436,6 +441,55 @@ def quantize_quads(pad: np.ndarray) -> np.ndarray:
     return q.reshape(h1, w1, 12)
 
 
+def quantize_texels(pad: np.ndarray) -> np.ndarray:
+    """Padded texels (h+2, w+2, 3) -> (h+2, w+2, 4) RGBA8 (A = 255), quantised as quantize_quads does, so the
+    grid holds exactly the corners of the quads quantize_quads makes from the same texels."""
+    u8 = (pad.astype(np.float32) * np.float32(255.0)).astype(np.float32).astype(np.uint8)
+    return np.concatenate([u8, np.full(u8.shape[:2] + (1,), 255, np.uint8)], -1)
+
+
+def half_texels(pad: np.ndarray) -> np.ndarray:
+    """Padded float texels (h+2, w+2, 3) -> (h+2, w+2, 4) float16 (A = 1), each value rounded to the nearest
+    half (through float32)."""
+    f = pad.astype(np.float32)
+    return np.concatenate([f, np.ones(f.shape[:2] + (1,), np.float32)], -1).astype(np.float16)
+
+
+def _store(pad: np.ndarray, texel_storage: int) -> np.ndarray:
+    if texel_storage == TEXELS_RGBA8:
+        return quantize_texels(np.clip(pad, 0, 1))
+    if texel_storage == TEXELS_RGBA16F:
+        return half_texels(pad)
+    return quantize_quads(np.clip(pad, 0, 1))
+
+
+def texels_from_quads(q: np.ndarray, texel_storage: int):
+    """(h+1, w+1, 12) quads -> (bordered texel grid, number of quads whose corners differ from it); the numpy
+    form of slimrt_host_texels_from_quads.  Grid texel (x, y) is quad (x, y)'s top-left corner; the last column
+    and row are the last quads' right and bottom corners."""
+    c = q.reshape(q.shape[0], q.shape[1], 3, 4)        # [y, x, channel, corner TL TR BL BR]
+    h1, w1 = c.shape[:2]
+    g = np.empty((h1 + 1, w1 + 1, 3), np.uint8)
+    g[:h1, :w1] = c[..., 0]
+    g[:h1, w1] = c[:, -1, :, 1]
+    g[h1, :w1] = c[-1, :, :, 2]
+    g[h1, w1] = c[-1, -1, :, 3]
+    back = np.stack([g[:-1, :-1], g[:-1, 1:], g[1:, :-1], g[1:, 1:]], -1)
+    mismatches = int(np.count_nonzero(np.any(back != c, axis=(2, 3))))
+    if texel_storage == TEXELS_RGBA8:
+        return np.concatenate([g, np.full(g.shape[:2] + (1,), 255, np.uint8)], -1), mismatches
+    f = g.astype(np.float32) / np.float32(255.0)
+    return np.concatenate([f, np.ones(f.shape[:2] + (1,), np.float32)], -1).astype(np.float16), mismatches
+
+
+def convert_texture(t: TextureSpec, texel_storage: int) -> TextureSpec:
+    """A quad texture in texel storage (RGBA8: the same samples when no quad mismatches its grid)."""
+    assert t.texel_storage == TEXELS_QUADS
+    mips = [texels_from_quads(q, texel_storage)[0] for q in t.mips]
+    return TextureSpec(width=t.width, height=t.height, flags=t.flags | texel_storage, mips=mips,
+                       mip_dims=list(t.mip_dims))
+
+
 def mip_chain(texels: np.ndarray):
     """Box-filtered chain down to 4 px as TextureMipLoader/loadMips do (src/bmp2texture.cpp:201-237, 289-299)."""
     mips = [texels.astype(np.float32)]
@@ -446,12 +500,26 @@ def mip_chain(texels: np.ndarray):
     return mips
 
 
-def texture_from_texels(texels: np.ndarray, mipmap: bool = True, wrap: bool = True) -> TextureSpec:
+def pad_texels(texels: np.ndarray, wrap: bool) -> np.ndarray:
+    """(h, w, 3) -> (h+2, w+2, 3): a one-texel border from the opposite edge (wrap, as a torus) or the edge."""
+    h, w, _ = texels.shape
+    xs = np.arange(-1, w + 1)
+    ys = np.arange(-1, h + 1)
+    xs = xs % w if wrap else np.clip(xs, 0, w - 1)
+    ys = ys % h if wrap else np.clip(ys, 0, h - 1)
+    return texels[ys][:, xs].astype(np.float32)
+
+
+def texture_from_texels(texels: np.ndarray, mipmap: bool = True, wrap: bool = True,
+                        texel_storage: int = TEXELS_QUADS) -> TextureSpec:
     chain = mip_chain(texels) if mipmap else [texels.astype(np.float32)]
     h, w, _ = texels.shape
-    flags = (IMG_MIPMAP if mipmap else 0) | (IMG_WRAP if wrap else 0)
-    return TextureSpec(width=w, height=h, flags=flags, mips=[bake_texel_quads(m, wrap) for m in chain],
-                       mip_dims=[(m.shape[1], m.shape[0]) for m in chain])
+    flags = (IMG_MIPMAP if mipmap else 0) | (IMG_WRAP if wrap else 0) | texel_storage
+    if texel_storage == TEXELS_QUADS:
+        mips = [bake_texel_quads(m, wrap) for m in chain]
+    else:
+        mips = [_store(pad_texels(m, wrap), texel_storage) for m in chain]
+    return TextureSpec(width=w, height=h, flags=flags, mips=mips, mip_dims=[(m.shape[1], m.shape[0]) for m in chain])
 
 
 def checker_noise_texels(size: int = 1024, squares: int = 8, seed: int = 0) -> np.ndarray:
@@ -488,10 +556,11 @@ def _face_dirs(face: str, n: int, pad: int = 1):
     return d / np.linalg.norm(d, axis=-1, keepdims=True)
 
 
-def cube_texture(fn, n: int) -> TextureSpec:
+def cube_texture(fn, n: int, texel_storage: int = TEXELS_QUADS) -> TextureSpec:
     """Cube map of face size n from a direction -> RGB function, as 3 mips: strip (4n x n), top, bottom.
     Border quads take the function's value one texel beyond each face edge, which is the seamless result that
-    the reference converter's border stitching approximates (src/bmp2texture.cpp:97-198)."""
+    the reference converter's border stitching approximates (src/bmp2texture.cpp:97-198).  In RGBA16F storage
+    the values are kept as they are (above 1 for an HDR function); quads and RGBA8 clip them to [0, 1]."""
     # strip: rows -1..n, columns -1..4n (one padding column each side, taken from the L and B extensions)
     left_pad = fn(_face_dirs("L", n))[:, :1]
     body = np.concatenate([fn(_face_dirs(f, n))[:, 1:-1] for f in "LFRB"], axis=1)
@@ -499,14 +568,14 @@ def cube_texture(fn, n: int) -> TextureSpec:
     strip = np.concatenate([left_pad, body, right_pad], axis=1).astype(np.float32)
     top = fn(_face_dirs("T", n)).astype(np.float32)
     bottom = fn(_face_dirs("D", n)).astype(np.float32)
-    mips = [quantize_quads(np.clip(strip, 0, 1)), quantize_quads(np.clip(top, 0, 1)),
-            quantize_quads(np.clip(bottom, 0, 1))]
-    return TextureSpec(width=6 * n, height=n, flags=IMG_CUBEMAP | IMG_LINEAR, mips=mips,
+    mips = [_store(strip, texel_storage), _store(top, texel_storage), _store(bottom, texel_storage)]
+    return TextureSpec(width=6 * n, height=n, flags=IMG_CUBEMAP | IMG_LINEAR | texel_storage, mips=mips,
                        mip_dims=[(4 * n, n), (n, n), (n, n)])
 
 
-def sky_function(seed: int = 0, spots: int = 3):
-    """Seeded sky: vertical gradient (ground, horizon, zenith) plus `spots` Gaussian hot spots."""
+def sky_function(seed: int = 0, spots: int = 3, hdr: bool = False):
+    """Seeded sky: vertical gradient (ground, horizon, zenith) plus `spots` Gaussian hot spots; clipped to [0, 1]
+    unless hdr (for half-float storage: the hot spots then keep their peaks above 1)."""
     rng = PCG32(seed)
     centers, colors, sharp = [], [], []
     for _ in range(spots):
@@ -525,7 +594,7 @@ def sky_function(seed: int = 0, spots: int = 3):
                        ground * (1 - np.clip(1 + 4 * y, 0, 1)))
         for c, k, s in zip(centers, colors, sharp):
             col = col + k * np.exp(s * (np.sum(d * c, -1, keepdims=True) - 1.0))
-        return np.clip(col * 0.8, 0, 1)
+        return col * 0.8 if hdr else np.clip(col * 0.8, 0, 1)
     return fn
 
 
@@ -553,11 +622,13 @@ def irradiance_function(radiance_fn, samples: int = 16):
 # --- scene 2: meshes + PBR + IBL ---------------------------------------------------------
 def mesh_pbr_scene(seed: int = 0, width: int = 1920, height: int = 1080, max_depth: int = 4, level: int = 6,
                    texture_size: int = 1024, sky_size: int = 256, irradiance_size: int = 32,
-                   render_mode: int = MODE_BEAUTY, leaf_size: int = 4) -> SceneSpec:
+                   render_mode: int = MODE_BEAUTY, leaf_size: int = 4, texel_storage: bool = False) -> SceneSpec:
     """Three instances of a displaced icosphere (81,920 triangles at level 6) on a textured floor quad, lit by
     cube-map IBL only.  Instance 0 is glass (IOR 1.5), instance 1 a mirror, instance 2 textured GGX.
     Draw order: per instance (scale, position x z, euler xyz, roughness, metallic coin), then floor roughness.
-    Textures: 0 = checker+noise albedo (mip-mapped, wrapped), 1 = sky radiance cube, 2 = irradiance cube."""
+    Textures: 0 = checker+noise albedo (mip-mapped, wrapped), 1 = sky radiance cube, 2 = irradiance cube; in the
+    reference's texel quads, or with texel_storage the albedo as RGBA8 texels and the cube faces as RGBA16F
+    texels of an HDR sky."""
     rng = PCG32(seed)
     mesh = displaced_sphere_mesh(level, 0.15, seed, leaf_size)
     geos, mats_f, mats_u = [], [], []
@@ -591,12 +662,13 @@ def mesh_pbr_scene(seed: int = 0, width: int = 1920, height: int = 1080, max_dep
                             metalness=metal)
         mats_f.append(f); mats_u.append(u)
         geos.append(_geo(GEO_MESH, eul, (px, s * 1.15, pz), (s, s, s), len(geos), 0, fl))
-    sky = sky_function(seed + 101)
-    tex = [texture_from_texels(checker_noise_texels(texture_size, 8, seed + 202), True, True),
-           cube_texture(sky, sky_size), cube_texture(irradiance_function(sky), irradiance_size)]
+    sky = sky_function(seed + 101, hdr=texel_storage)
+    albedo_storage, cube_storage = (TEXELS_RGBA8, TEXELS_RGBA16F) if texel_storage else (TEXELS_QUADS, TEXELS_QUADS)
+    tex = [texture_from_texels(checker_noise_texels(texture_size, 8, seed + 202), True, True, albedo_storage),
+           cube_texture(sky, sky_size, cube_storage), cube_texture(irradiance_function(sky), irradiance_size, cube_storage)]
     cam_pos = np.array([0.0, 2.4, -6.5], np.float32)
     return SceneSpec(
-        name=f"mesh_pbr_s{seed}_{width}x{height}", width=width, height=height,
+        name=f"mesh_pbr_s{seed}_{width}x{height}" + ("_texels" if texel_storage else ""), width=width, height=height,
         camera_euler=look_at_euler(cam_pos, (0.0, 1.0, 0.4)), camera_position=cam_pos, focal_length=FOV60_FOCAL,
         max_depth=max_depth, render_mode=render_mode, skybox_color=1, skybox_radiance=1, skybox_irradiance=2,
         geo_f=_stack([g[0] for g in geos]), geo_u=_stack([g[1] for g in geos]),

@@ -59,7 +59,8 @@ def _sections(spec: SceneSpec):
         yield f"tex{k}_info", np.array([t.width, t.height, t.flags, len(t.mips)], np.uint32)
         yield f"tex{k}_dims", np.array(t.mip_dims, np.uint32).reshape(-1)
         for mi, q in enumerate(t.mips):
-            yield f"tex{k}_mip{mi}", q
+            q = np.ascontiguousarray(q)
+            yield f"tex{k}_mip{mi}", q.view(np.uint8) if q.dtype == np.float16 else q
 
 
 def spec_bytes(spec: SceneSpec) -> bytes:

@@ -3,6 +3,8 @@
 // BVHBuilder, a Camera, CameraRayProjection::reset, a Canvas + Viewport and RayTracingRenderer::render.
 //
 //   app_api_test spec <scene.spec> <out.bin> <cpu|gpu>   render a scene spec through the API, write the canvas
+//   app_api_test texels8|texels16 <scene.spec> <out.bin> <cpu|gpu>  as spec, with every texel-quad texture turned
+//       into RGBA8 / RGBA16F texel storage by toTexelStorage
 //   app_api_test demo <monkey.mesh> <out.bin> <cpu|gpu>  a hand-built scene with a loaded .mesh asset
 //   app_api_test overlay<LW> <scene.spec> <out.bin> <cpu|gpu>  spec, then the app's wireframe overlays with line
 //       width LW (drawSceneBVH of src/examples/03_BlinnPhong.cpp:194-209, drawBVH + drawMesh per mesh geometry,
@@ -26,7 +28,7 @@ static int write_canvas(const char *path, const std::vector<Pixel> &px, const st
     return 0;
 }
 
-static int run_spec(const char *spec_path, const char *out, bool gpu, int overlay_lw = -1) {
+static int run_spec(const char *spec_path, const char *out, bool gpu, int overlay_lw = -1, u32 texel_storage = 0) {
     SpecFile sf;
     if (spec_open(&sf, spec_path)) return 2;
     const int32_t *fr = (const int32_t *)spec_get(&sf, "frame", nullptr);
@@ -88,6 +90,11 @@ static int run_spec(const char *spec_path, const char *out, bool gpu, int overla
             mips[k][m] = TextureMip{dims[2 * m], dims[2 * m + 1], (TexelQuad *)spec_get(&sf, nm, nullptr)};
         }
         textures[k] = Texture{info[0], info[1], info[3], info[2], mips[k].data()};
+        if (texel_storage && !(textures[k].flags & SLIMRT_TEXELS_MASK)) {
+            u64 mismatches = 0;
+            textures[k] = toTexelStorage(textures[k], texel_storage, &mismatches);
+            fprintf(stderr, "texture %u in texel storage: %llu mismatching quads\n", k, (unsigned long long)mismatches);
+        }
     }
 
     Camera camera{{cam[0], cam[1], cam[2]}, {cam[3], cam[4], cam[5]}, cam[6]};
@@ -217,6 +224,8 @@ int main(int argc, char **argv) {
     const bool gpu = !strcmp(argv[4], "gpu");
     try {
         if (!strcmp(argv[1], "spec")) return run_spec(argv[2], argv[3], gpu);
+        if (!strcmp(argv[1], "texels8")) return run_spec(argv[2], argv[3], gpu, -1, SLIMRT_TEXELS_RGBA8);
+        if (!strcmp(argv[1], "texels16")) return run_spec(argv[2], argv[3], gpu, -1, SLIMRT_TEXELS_RGBA16F);
         if (!strncmp(argv[1], "overlay", 7)) return run_spec(argv[2], argv[3], gpu, atoi(argv[1] + 7));
         if (!strcmp(argv[1], "demo")) return run_demo(argv[2], argv[3], gpu);
         if (!strcmp(argv[1], "assets")) return run_assets(argv[2], argv[3]);

@@ -1,0 +1,277 @@
+"""Texel storage: the bordered RGBA8 and RGBA16F texel grids (include/slimrt.h SLIMRT_TEXELS_*) beside the
+reference's texel quads (src/slim/core/texture.h:5-41).
+
+What pins them:
+  * RGBA8: a grid built from quads whose corners agree with it samples to the same bits as the quads (the sampler
+    keeps the reference's weights and order of terms), so every frame rendered with converted textures equals the
+    quad frame bit for bit - in the oracle, the CPU context, the C++ API and on the GPU.
+  * RGBA16F: the half decoding against numpy for all 65,536 bit patterns; the bilinear filter against the closed
+    form on linear ramps (a swapped corner or a wrong row stride fails it); frames: CPU context, C++ API and GPU
+    against the oracle, which restates the same sampler.
+"""
+import os
+import subprocess
+import tempfile
+
+import numpy as np
+import pytest
+
+import oracle
+from slimtracincpp_amd import assets, runtime, scenes
+from slimtracincpp_amd.specfile import write_spec
+from tests.helpers import GOLDEN
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+EXE = os.path.join(ROOT, "tests", "cpp", "app_api_test")
+STORAGES = [scenes.TEXELS_RGBA8, scenes.TEXELS_RGBA16F]
+FMT = {scenes.TEXELS_QUADS: 0, scenes.TEXELS_RGBA8: 1, scenes.TEXELS_RGBA16F: 2}
+
+
+def _small_mesh(texel_storage=False, **kw):
+    return scenes.mesh_pbr_scene(0, 96, 54, level=3, texture_size=64, sky_size=16, irradiance_size=8,
+                                 texel_storage=texel_storage, **kw)
+
+
+def _converted(spec, storage):
+    out = _small_mesh() if spec is None else spec
+    out.textures = [scenes.convert_texture(t, storage) for t in out.textures]
+    return out
+
+
+def _sample(w, h, data, fmt, u, v):
+    out = np.zeros(3, np.float32)
+    data = np.ascontiguousarray(data)
+    oracle.lib().oracle_mip_sample_fmt(w, h, data.ctypes.data, fmt, float(u), float(v), out.ctypes.data)
+    return out
+
+
+def _bits(a):
+    return np.ascontiguousarray(a, np.float32).view(np.uint32)
+
+
+# ---------------------------------------------------------------------------------------------------- CPU
+def test_half_decoding_all_bit_patterns():
+    bits = np.arange(65536, dtype=np.uint32).astype(np.uint16)
+    want = bits.view(np.float16).astype(np.float32)
+    got = np.array([oracle.lib().oracle_half_to_float(int(b)) for b in bits], np.float32)
+    nan = np.isnan(want)
+    assert np.array_equal(np.isnan(got), nan)
+    assert np.array_equal(_bits(got)[~nan], _bits(want)[~nan])
+
+
+def test_float_to_half_rounds_as_numpy():
+    rng = np.random.default_rng(0)
+    x = (rng.standard_normal(20000) * 10.0 ** rng.integers(-9, 6, 20000)).astype(np.float32)
+    edges = np.array([0.0, -0.0, 65504, 65519.99, 65520, 1e9, -1e9, 2.0 ** -24, 2.0 ** -25, 2.0 ** -25 * 1.0001,
+                      3 * 2.0 ** -26, 2.0 ** -14, 6.1e-5, np.inf, -np.inf, 1 / 3], np.float32)
+    x = np.concatenate([x, edges, np.arange(256, dtype=np.float32) / np.float32(255)])
+    got = np.array([runtime.host_float_to_half(v) for v in x], np.uint16)
+    with np.errstate(over="ignore"):
+        want = x.astype(np.float16).view(np.uint16)
+    assert np.array_equal(got, want)
+
+
+def _textures_to_check():
+    rng = np.random.default_rng(5)
+    img = (rng.random((24, 40, 3)) * 255).astype(np.uint8)
+    yield "generated wrap", scenes.texture_from_texels(scenes.checker_noise_texels(64, 8, 3), True, True), True
+    yield "generated clamp", scenes.texture_from_texels(scenes.checker_noise_texels(32, 4, 4), True, False), True
+    yield "generated cube", scenes.cube_texture(scenes.sky_function(7), 16), True
+    for flags in (0, assets.IMG_WRAP, assets.IMG_WRAP | assets.IMG_MIPMAP):
+        t = assets.bake_texture(img, flags)
+        yield f"converter flags {flags}", scenes.TextureSpec(t["width"], t["height"], t["flags"],
+                                                             [q for _, _, q in t["mips"]],
+                                                             [(w, h) for w, h, _ in t["mips"]]), True
+    t = assets.read_texture(os.path.join(GOLDEN, "bolonga_irradiance.texture"))
+    yield "shipped cube", scenes.TextureSpec(t["width"], t["height"], t["flags"], [q for _, _, q in t["mips"]],
+                                             [(w, h) for w, h, _ in t["mips"]]), False
+
+
+@pytest.mark.parametrize("storage", STORAGES)
+def test_texels_from_quads_abi_equals_numpy(storage):
+    for name, t, consistent in _textures_to_check():
+        total = 0
+        for q, (w, h) in zip(t.mips, t.mip_dims):
+            g, bad = scenes.texels_from_quads(q, storage)
+            g2, bad2 = runtime.host_texels_from_quads(q, w, h, storage)
+            assert g.shape == (h + 2, w + 2, 4) and bad == bad2, name
+            if storage == scenes.TEXELS_RGBA8:
+                assert np.array_equal(g, g2), name
+            else:
+                assert np.array_equal(g.view(np.uint16), g2.view(np.uint16)), name
+            total += bad
+        # the converter's cube seams write some corners differently in neighbouring quads; nothing else does
+        assert (total == 0) == consistent, (name, total)
+
+
+def test_rgba8_samples_equal_quad_samples():
+    rng = np.random.default_rng(2)
+    uv = np.concatenate([rng.random((400, 2)) * 3, rng.random((100, 2)),
+                         [[0, 0], [1, 1], [1, 0], [0, 1], [0.5, 1], [np.float32(0.99999994), 0.25]]]).astype(np.float32)
+    for name, t, consistent in _textures_to_check():
+        if not consistent:
+            continue
+        for q, (w, h) in zip(t.mips, t.mip_dims):
+            g, _ = scenes.texels_from_quads(q, scenes.TEXELS_RGBA8)
+            for u, v in uv:
+                a = _sample(w, h, q, 0, u, v)
+                b = _sample(w, h, g, 1, u, v)
+                assert np.array_equal(_bits(a), _bits(b)), (name, w, h, u, v)
+
+
+@pytest.mark.parametrize("storage", STORAGES)
+def test_texel_bilinear_is_exact_on_ramps(storage):
+    """A texel grid holding f(x, y) = a + bx·x + by·y at texel (x, y) (border texels at x, y = -1 and w, h)
+    samples to f(u·w - 1/2, v·h - 1/2): the bilinear filter reproduces a linear function, with texel centres at
+    half-integers.  Values are multiples of 1/64 (exact in half floats) or integers / 255 (RGBA8)."""
+    w, h = 16, 8
+    ys, xs = np.mgrid[-1:h + 1, -1:w + 1].astype(np.float64)
+    chans = [(10 + 3 * xs + 0 * ys), (40 + 0 * xs + 5 * ys), (100 + 2 * xs - 4 * ys)]   # integers 0..255
+    rng = np.random.default_rng(3)
+    uv = rng.random((300, 2)).astype(np.float32)
+    if storage == scenes.TEXELS_RGBA8:
+        grid = np.stack(chans + [np.full_like(xs, 255)], -1).astype(np.uint8)
+        scale = 1 / 255
+    else:
+        grid = (np.stack(chans + [np.ones_like(xs)], -1) / 64).astype(np.float16)
+        scale = 1 / 64
+    for u, v in uv:
+        got = _sample(w, h, grid, FMT[storage], u, v).astype(np.float64)
+        px, py = np.float64(u) * w - 0.5, np.float64(v) * h - 0.5
+        want = np.array([10 + 3 * px, 40 + 5 * py, 100 + 2 * px - 4 * py]) * scale
+        assert np.allclose(got, want, rtol=2e-6, atol=2e-6), (u, v, got, want)
+
+
+def test_oracle_frames_rgba8_equal_quads():
+    spec = _small_mesh()
+    col, dep, _ = oracle.OracleScene(spec).render()
+    col8, dep8, _ = oracle.OracleScene(_converted(None, scenes.TEXELS_RGBA8)).render()
+    assert np.array_equal(_bits(col), _bits(col8)) and np.array_equal(_bits(dep), _bits(dep8))
+
+
+def test_cpu_context_texel_storage_equals_oracle():
+    spec = _small_mesh(texel_storage=True)
+    assert spec.textures[0].texel_storage == scenes.TEXELS_RGBA8
+    assert spec.textures[1].texel_storage == scenes.TEXELS_RGBA16F
+    assert max(float(m.astype(np.float32)[..., :3].max()) for m in spec.textures[1].mips) > 1.0   # HDR sky
+    col, dep, _ = oracle.OracleScene(spec).render()
+    r = runtime.Renderer(spec, "cpu")
+    px, depth = r.render()
+    r.close()
+    assert np.array_equal(_bits(px[..., :3]), _bits(col)) and np.array_equal(_bits(depth), _bits(dep))
+    # the storage changes the frame (half-float HDR sky, unquantised)
+    quads_col, _, _ = oracle.OracleScene(_small_mesh()).render()
+    assert float(np.abs(quads_col - col).max()) > 1e-3
+    # the converted RGBA16F form too
+    spec16 = _converted(None, scenes.TEXELS_RGBA16F)
+    col16, _, _ = oracle.OracleScene(spec16).render()
+    r = runtime.Renderer(spec16, "cpu")
+    px16, _ = r.render()
+    r.close()
+    assert np.array_equal(_bits(px16[..., :3]), _bits(col16))
+    assert float(np.abs(col16 - quads_col).max()) < 2e-3   # half rounding of u8 / 255 only
+
+
+def test_unknown_texel_storage_is_rejected():
+    spec = _small_mesh()
+    spec.textures[0].flags |= 3 << 24
+    with pytest.raises(runtime.SlimrtError, match="texel storage"):
+        runtime.Renderer(spec, "cpu")
+
+
+def test_texture_files_hold_quads_only():
+    t = scenes.texture_from_texels(scenes.checker_noise_texels(16, 4, 1), False, True, scenes.TEXELS_RGBA8)
+    d = {"width": t.width, "height": t.height, "flags": t.flags,
+         "mips": [(w, h, q) for (w, h), q in zip(t.mip_dims, t.mips)]}
+    with tempfile.TemporaryDirectory() as td, pytest.raises(ValueError):
+        assets.write_texture(os.path.join(td, "t.texture"), d)
+
+
+def _app(kind, spec, device):
+    if not os.path.exists(EXE):
+        subprocess.run(["make", "-s", "-C", os.path.join(ROOT, "slimtracincpp_amd", "csrc"), "app_test"], check=True)
+    n = spec.width * spec.height
+    with tempfile.TemporaryDirectory() as td:
+        path = write_spec(spec, os.path.join(td, "s.spec"))
+        out = os.path.join(td, "canvas.bin")
+        subprocess.run([EXE, kind, path, out, device], check=True, timeout=600, capture_output=True)
+        raw = np.fromfile(out, np.float32)
+    return raw[:4 * n].reshape(n, 4), raw[4 * n:5 * n]
+
+
+def _canvas(col):
+    c = np.clip(col.reshape(-1, 3), 0, 1).astype(np.float32)
+    return (c * (c * np.float32(1.0))).astype(np.float32)
+
+
+def test_app_api_texel_storage_cpu():
+    """toTexelStorage in the C++ API: RGBA8 = the quad frame, RGBA16F = the oracle on numpy's conversion."""
+    spec = _small_mesh()
+    px, dep = _app("texels8", spec, "cpu")
+    col, odep, _ = oracle.OracleScene(spec).render()
+    assert np.array_equal(_bits(px[:, :3]), _bits(_canvas(col)))
+    px, dep = _app("texels16", spec, "cpu")
+    col16, _, _ = oracle.OracleScene(_converted(None, scenes.TEXELS_RGBA16F)).render()
+    assert np.array_equal(_bits(px[:, :3]), _bits(_canvas(col16)))
+
+
+# ---------------------------------------------------------------------------------------------------- GPU
+FORMS = {"megakernel": 0, "megakernel_bvh4": runtime.BVH4, "wavefront_bvh4": runtime.WAVEFRONT | runtime.BVH4}
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("form", list(FORMS))
+def test_gpu_rgba8_frames_equal_quad_frames(gpu_available, form):
+    spec = scenes.mesh_pbr_scene(0, 320, 180, level=4, texture_size=128, sky_size=32, irradiance_size=8)
+    spec8 = scenes.mesh_pbr_scene(0, 320, 180, level=4, texture_size=128, sky_size=32, irradiance_size=8)
+    spec8.textures = [scenes.convert_texture(t, scenes.TEXELS_RGBA8) for t in spec8.textures]
+    frames = []
+    for s in (spec, spec8):
+        r = runtime.Renderer(s, "gpu")
+        p = r.params()
+        p.flags = FORMS[form]
+        frames.append(r.render(p))
+        r.close()
+    (a, da), (b, db) = frames
+    assert np.array_equal(_bits(a), _bits(b)) and np.array_equal(_bits(da), _bits(db))
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("form", list(FORMS))
+def test_gpu_texel_storage_scene_equals_oracle(gpu_available, form):
+    spec = scenes.mesh_pbr_scene(0, 320, 180, level=4, texture_size=128, sky_size=32, irradiance_size=8,
+                                 texel_storage=True)
+    col, dep, _ = oracle.OracleScene(spec).render()
+    r = runtime.Renderer(spec, "gpu")
+    p = r.params()
+    p.flags = FORMS[form]
+    for _ in range(3):   # the cost-ordered forms change their tile order from the third frame
+        px, depth = r.render(p)
+    r.close()
+    err = float(np.nanmax(np.abs(px[..., :3] - col)))
+    print(f"{form}: max |gpu - oracle| {err:.3g}, bit-exact {np.mean(_bits(px[..., :3]) == _bits(col)):.6f}")
+    assert err <= 1e-4
+    assert np.array_equal(np.isinf(depth), np.isinf(dep))
+    fin = np.isfinite(dep)
+    assert np.allclose(depth[fin], dep[fin], rtol=1e-5, atol=0)
+
+
+@pytest.mark.gpu
+def test_gpu_texel_storage_at_full_size(gpu_available):
+    """The mesh + PBR workload as configured (1920x1080, 81,920-triangle meshes, 1024^2 RGBA8 albedo with its
+    mips, 6 x 256^2 RGBA16F sky faces of an HDR sky, 6 x 32^2 irradiance faces) in the benchmark's form: row
+    bands against the oracle, every sample finite."""
+    spec = scenes.mesh_pbr_scene(0, 1920, 1080, texel_storage=True)
+    r = runtime.Renderer(spec, "gpu")
+    p = r.params()
+    p.flags = runtime.BVH4
+    for _ in range(3):
+        px, depth = r.render(p)
+    r.close()
+    assert np.all(np.isfinite(px))
+    o = oracle.OracleScene(spec)
+    for y0 in (300, 540, 760):
+        col, dep, _ = o.render(y0, y0 + 16)
+        err = float(np.nanmax(np.abs(px[y0:y0 + 16, :, :3] - col)))
+        assert err <= 1e-4, (y0, err)
+        assert np.array_equal(np.isinf(depth[y0:y0 + 16]), np.isinf(dep))

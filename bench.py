@@ -3,9 +3,9 @@
 Workload: the 1080p mesh + PBR scene at bounce depth 4 (three 81,920-triangle displaced icospheres on a textured
 floor, Cook-Torrance GGX, 1024^2 mip-mapped checker+noise albedo, cube-map IBL, one glass and one mirror
 instance).  A step is one frame over all 2,073,600 primary rays, with the scene resident in HBM and the canvas
-(pixels + depths) written to HBM, in the fastest of sixteen frame forms (normally one launch of the per-sample
-render kernel, its tiles dispatched costliest first by an order computed on a second stream during the previous
-frame).
+(pixels + depths) written to HBM, in the fastest of eighteen frame forms (normally one launch of the per-sample
+render kernel over the meshes' 8-wide BVHs, its tiles dispatched costliest first by an order computed on a second
+stream during the previous frame).
 
 Reported:
   value        whole-frame Mrays/s, rays = every SceneTracer::trace call (primary + bounce + shadow), counted
@@ -57,7 +57,7 @@ OUT_BYTES_PER_SAMPLE = 20
 
 FORM_NAMES = ("megakernel", "megakernel_image_order", "wavefront", "wavefront_sorted", "wavefront_per_bounce", "wavefront_per_bounce_sorted",
               "wavefront_split", "wavefront_sorted_split")
-FORM_NAMES = FORM_NAMES + tuple(f + "_bvh4" for f in FORM_NAMES)
+FORM_NAMES = FORM_NAMES + tuple(f + "_bvh4" for f in FORM_NAMES) + ("megakernel_bvh8", "megakernel_image_order_bvh8")
 
 
 def measured_traffic(width: int, height: int, form: str):
@@ -219,7 +219,7 @@ def cache_bound(spec, stream, reps: int = 10) -> dict:
     plain_ms = timed(p)
     h = r.params(output=runtime.OUTPUT_HITS)
     trace = {}
-    for name, f in (("megakernel", 0), ("megakernel_bvh4", runtime.BVH4)):
+    for name, f in (("megakernel", 0), ("megakernel_bvh4", runtime.BVH4), ("megakernel_bvh8", runtime.BVH8)):
         h.flags = f
         trace[name] = timed(h)
     hits = int(np.sum(px.view(torch.int32)[:, 0].cpu().numpy() >= 0))
@@ -319,7 +319,9 @@ def texel_storage_leg(width: int, height: int, stream, steps: int, warmup: int) 
     cnt = r.count_work(p)
     forms = {}
     for name, flags in (("megakernel_bvh4", runtime.BVH4),
-                        ("megakernel_image_order_bvh4", runtime.BVH4 | runtime.TILE_ORDER_FIXED)):
+                        ("megakernel_image_order_bvh4", runtime.BVH4 | runtime.TILE_ORDER_FIXED),
+                        ("megakernel_bvh8", runtime.BVH8),
+                        ("megakernel_image_order_bvh8", runtime.BVH8 | runtime.TILE_ORDER_FIXED)):
         p.flags = flags
         for _ in range(warmup):
             r.render_device(p, px.data_ptr(), dp.data_ptr())
@@ -361,7 +363,8 @@ def large_leg(stream, frames: int = 5) -> dict:
     for name, flags in (("megakernel", 0), ("wavefront", runtime.WAVEFRONT),
                         ("wavefront_sorted", runtime.WAVEFRONT | runtime.WAVEFRONT_SORT),
                         ("wavefront_per_bounce", runtime.WAVEFRONT | runtime.WAVEFRONT_PER_BOUNCE),
-                        ("megakernel_bvh4", runtime.BVH4), ("wavefront_bvh4", runtime.WAVEFRONT | runtime.BVH4),
+                        ("megakernel_bvh4", runtime.BVH4), ("megakernel_bvh8", runtime.BVH8),
+                        ("wavefront_bvh4", runtime.WAVEFRONT | runtime.BVH4),
                         ("wavefront_sorted_bvh4", runtime.WAVEFRONT | runtime.WAVEFRONT_SORT | runtime.BVH4)):
         p.flags = flags
         ts = []
@@ -476,14 +479,17 @@ def main():
     cnt = r.count_work(p)                       # sibling-pair walk (equal to the oracle's counts)
     p.flags = runtime.BVH4
     cnt4 = r.count_work(p)                      # 4-wide mesh walk: 'pairs' counts node visits
+    p.flags = runtime.BVH8
+    cnt8 = r.count_work(p)                      # 8-wide mesh walk (128-byte quantised nodes)
     p.flags = 0
 
-    # Sixteen forms of the same frame (tests/test_wavefront.py, tests/test_bvh4.py, tests/test_tile_order.py): the
-    # per-sample megakernel (tiles costliest first by the previous frame's durations, or in image order) and the
-    # bounces as a wavefront over a compacted ray queue (run to the end in one launch, or re-compacted after
-    # every bounce; sorted by direction octant or not; first hits traced by the shading kernel or by a separate
-    # traversal kernel), each with the meshes' sibling-pair BVHs or their 4-wide collapse.  Each is timed over K back-to-back frames between synchronisations; the fastest is the reported
-    # path.
+    # Eighteen forms of the same frame (tests/test_wavefront.py, tests/test_bvh4.py, tests/test_bvh8.py,
+    # tests/test_tile_order.py): the per-sample megakernel (tiles costliest first by the previous frame's durations,
+    # or in image order) and the bounces as a wavefront over a compacted ray queue (run to the end in one launch, or
+    # re-compacted after every bounce; sorted by direction octant or not; first hits traced by the shading kernel or
+    # by a separate traversal kernel), each with the meshes' sibling-pair BVHs or their 4-wide collapse; and the
+    # megakernel over the 8-wide collapse in both tile orders.  Each is timed over K back-to-back frames between
+    # synchronisations; the fastest is the reported path.
     base_forms = {"megakernel": 0, "megakernel_image_order": runtime.TILE_ORDER_FIXED, "wavefront": runtime.WAVEFRONT,
                   "wavefront_sorted": runtime.WAVEFRONT | runtime.WAVEFRONT_SORT,
                   "wavefront_per_bounce": runtime.WAVEFRONT | runtime.WAVEFRONT_PER_BOUNCE,
@@ -492,6 +498,7 @@ def main():
                   "wavefront_sorted_split": runtime.WAVEFRONT | runtime.WAVEFRONT_SORT | runtime.WAVEFRONT_SPLIT}
     forms = dict(base_forms)
     forms.update({k + "_bvh4": v | runtime.BVH4 for k, v in base_forms.items()})
+    forms.update({"megakernel_bvh8": runtime.BVH8, "megakernel_image_order_bvh8": runtime.BVH8 | runtime.TILE_ORDER_FIXED})
     assert tuple(forms) == FORM_NAMES
     if args.form:
         forms = {args.form: forms[args.form]}
@@ -523,13 +530,14 @@ def main():
     ms_step, wall_ms, k_ms = (timings[form][k] for k in ("ms_per_step", "wall_ms_per_step", "kernel_ms"))
     p.flags = forms[form]
 
-    wide = form.endswith("_bvh4")
+    wide = form.endswith("_bvh4") or form.endswith("_bvh8")
+    cnt_form = cnt8 if form.endswith("_bvh8") else cnt4
     rays = cnt["rays"]
     mrays = rays / (ms_step * 1e-3) / 1e6
     # Algorithmic bytes on the reference's own traversal (sibling-pair walk, counts equal to the oracle's): the
     # same work for every frame form.  The 4-wide walk's own node bytes (128 per node visit) are reported beside.
     bytes_frame = algorithmic_bytes(cnt)
-    bytes_form = algorithmic_bytes(cnt4, True) if wide else bytes_frame
+    bytes_form = algorithmic_bytes(cnt_form, True) if wide else bytes_frame
     achieved = bytes_frame / (k_ms * 1e-3) / 1e9
     traffic = measured_traffic(args.width, args.height, form)
     result = {
@@ -559,7 +567,8 @@ def main():
                      "basis": "algorithmic bytes of the reference's sibling-pair walk (64 B per pair visit); "
                               "traffic = measured HBM bytes per frame (PMC FETCH_SIZE + WRITE_SIZE, calibrated)",
                      "achieved_form_bytes": bytes_form / (k_ms * 1e-3) / 1e9, "form_bytes": bytes_form,
-                     "kernel": ("4-wide mesh BVHs; " if wide else "") + {
+                     "kernel": ({"_bvh4": "4-wide mesh BVHs; ", "_bvh8": "8-wide mesh BVHs (quantised child boxes); "}
+                                .get(form[-5:], "")) + {
                                 "megakernel": "render_kernel<false> (tiles dispatched costliest first; their order, from "
                                               "the tile durations of two frames before, is computed by "
                                               "tile_order_kernel on a second stream while the previous frame renders)",
@@ -575,12 +584,12 @@ def main():
                                 "wavefront_split": "the frame's launches: wf_trace_kernel (first hits) + "
                                                    "wf_primary_kernel (their shading) + one wf_bounce_kernel<true>",
                                 "wavefront_sorted_split": "the frame's launches: wf_trace_kernel + wf_primary_kernel + "
-                                                          "octant sort + wf_bounce_kernel<true>"}[form.replace("_bvh4", "")],
+                                                          "octant sort + wf_bounce_kernel<true>"}[form.replace("_bvh4", "").replace("_bvh8", "")],
                      "kernel_ms": k_ms, "algorithmic_bytes": bytes_frame,
-                     "counters": cnt, "counters_form": cnt4 if wide else cnt},
+                     "counters": cnt, "counters_form": cnt_form if wide else cnt},
         "neural_cache": None,
     }
-    if form in ("megakernel", "megakernel_bvh4"):
+    if form in ("megakernel", "megakernel_bvh4", "megakernel_bvh8"):
         # a frame's tile durations (slimrt_tile_costs): how far the frame is from its ideal length
         import numpy as np
         for _ in range(3):   # this form's frames (the order of a frame comes from the durations two frames before)
@@ -588,13 +597,16 @@ def main():
         costs, order = r.tile_costs()
         if costs.size:
             c = costs.astype(np.int64)
-            # the ideal frame: the summed wave durations over the resident waves (a split tile records twice the
-            # duration of its later half, which stands for its two half waves)
+            # the ideal frame: the summed wave durations over the resident waves (a split tile records its last
+            # part's duration times its parts, which stands for its part waves)
             ideal_us = float(c.sum()) / r.tile_resident / 100.0
             wave = c.copy()   # the duration of each tile's longest wave
             ordered = not np.array_equal(order, np.arange(order.size))
             if ordered:
-                wave[order[:min(c.size // 32, 2048)]] //= 2
+                split = min(c.size // 32, 2048)
+                quarter = min(c.size // 1024, 128, split) if form.endswith("_bvh8") else 0
+                wave[order[:quarter]] //= 4
+                wave[order[quarter:split]] //= 2
             result["tile_costs"] = {
                 "tiles": int(c.size), "resident_waves": r.tile_resident, "median_us": float(np.median(wave)) / 100.0,
                 "p99_us": float(np.percentile(wave, 99)) / 100.0, "longest_us": float(wave.max()) / 100.0,

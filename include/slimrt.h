@@ -65,6 +65,12 @@ extern "C" {
 #define SLIMRT_WAVEFRONT_SPLIT 256u /* with SLIMRT_WAVEFRONT: the first hits are found by a traversal-only kernel
                                        (fewer registers, more waves in flight) that hands hit records to the
                                        shading kernel through device memory */
+#define SLIMRT_BVH8 1024u          /* per-sample frames (no SLIMRT_WAVEFRONT) and the CPU context: trace meshes through
+                                      8-wide BVHs collapsed from the uploaded ones, with 8-bit child boxes on a grid
+                                      per node (128-byte nodes) that contain the exact boxes; every leaf repeats the
+                                      exact leaf-box test before its triangles, so results are those of the
+                                      sibling-pair walk up to exact ties between triangles.  Meshes need at most 65536
+                                      such nodes and leaves of at most 254 triangles */
 #define SLIMRT_TILE_ORDER_FIXED 512u /* per-sample frames (no SLIMRT_WAVEFRONT): dispatch the sample tiles in image
                                         order.  Without it a frame whose longest tiles bound its length (the longest
                                         tile two frames before lasted at least 3/4 of that frame's summed tile
@@ -297,6 +303,35 @@ typedef struct slimrt_bvh4_node {
 int slimrt_host_collapse_bvh4(const slimrt_bvh_node *nodes, uint32_t node_count, uint32_t triangle_count,
                               slimrt_bvh4_node *nodes_out, uint32_t node_cap, uint32_t *node_count_out,
                               uint32_t *height_out, uint32_t *stack_out);
+
+/* 8-wide BVH node as the SLIMRT_BVH8 walk reads it (128 bytes; no reference counterpart).  Child box bounds are
+ * grid points origin + q * 2^(e - 127) per axis, e the axis' byte of exps (bits 0-7 x, 8-15 y, 16-23 z) and q the
+ * slot's byte of qlo / qhi (slot k: byte k & 3 of word k >> 2).  Per slot, count (same byte layout): 0 an inner
+ * child (child = node index), 1-254 a leaf of that many triangles (child = its leaf record, in 16-byte units),
+ * 255 an empty slot. */
+typedef struct slimrt_bvh8_node {
+    float origin[3];
+    uint32_t exps;
+    uint32_t child[8];
+    uint32_t count[2];
+    uint32_t qlo[3][2], qhi[3][2];
+    uint32_t pad[6];
+} slimrt_bvh8_node;
+
+/* The 8-wide collapse slimrt_upload_scene applies to every mesh BVH with leaves of at most 254 triangles (no
+ * counterpart in the reference): a node's children are its BVH2 node's two children with, while fewer than eight,
+ * the inner child of largest surface area replaced by its own two; child boxes are quantised outward to 8 bits on
+ * a per-node grid, each decoded bound at least two ulps beyond the exact one; slots are ordered for the walk's
+ * octant order.  Leaf records (leaves_out, 16-byte units): the BVH2 leaf's exact box with its first triangle and
+ * triangle count (min xyz, first, max xyz, count), then its triangles' 48-byte test records (the first two
+ * columns of local_to_tangent's rows, position, normal), followed by a final record's worth of padding.  nodes
+ * receive at most node_cap nodes, root first, and leaves_out at most leaf_cap units; *stack_out the mesh stack
+ * entries the walk can hold at most.  SLIMRT_E_OVERFLOW when a capacity is too small (the counts are set; with
+ * both capacities 0 and NULL outputs this is the size query). */
+int slimrt_host_collapse_bvh8(const slimrt_bvh_node *nodes, uint32_t node_count, const slimrt_triangle *triangles,
+                              uint32_t triangle_count, slimrt_bvh8_node *nodes_out, uint32_t node_cap,
+                              uint32_t *node_count_out, uint32_t *leaves_out, uint32_t leaf_cap,
+                              uint32_t *leaf_count_out, uint32_t *stack_out);
 
 /* Presentation: Canvas::drawToWindow (src/slim/draw/canvas.h:119-126) for a canvas written with
  * SLIMRT_OUTPUT_CANVAS: the window content, one 0xRRGGBB word per pixel (getPixelContent / Pixel::asContent,

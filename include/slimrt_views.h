@@ -73,10 +73,11 @@ const char *slimrt_views_last_error(const slimrt_views *v);
 
 /* Times the frame forms of one view on eager frames into the given device buffers (reps each after one warm-up,
  * median) and returns the fastest form's flags (params->flags with the form bits replaced) and its median time.
- * Forms: node width 2 (the uploaded BVHs) or 4 (SLIMRT_BVH4) x (the per-sample megakernel, or the
- * SLIMRT_WAVEFRONT variants: run to the end or one launch per bounce, octant-sorted or not, in 1 or 4 bands, first
- * hits traced by the shading kernel or by a separate traversal kernel).
- * The wavefront forms give the megakernel's samples bit for bit; node width 4 differs only on exact ties. */
+ * Forms: the per-sample megakernel over 8-wide nodes (SLIMRT_BVH8, when the meshes fit it), and node width 2 (the
+ * uploaded BVHs) or 4 (SLIMRT_BVH4) x (the per-sample megakernel, or the SLIMRT_WAVEFRONT variants: run to the end
+ * or one launch per bounce, octant-sorted or not, in 1 or 4 bands, first hits traced by the shading kernel or by a
+ * separate traversal kernel).
+ * The wavefront forms give the megakernel's samples bit for bit; node widths 4 and 8 differ only on exact ties. */
 int slimrt_render_autotune(slimrt_context *ctx, const slimrt_render_params *params, void *d_pixels, void *d_depths,
                            uint32_t reps, uint32_t *best_flags, float *best_ms);
 

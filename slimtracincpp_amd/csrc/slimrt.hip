@@ -20,6 +20,7 @@
 #include <cstring>
 #include <mutex>
 #include <cstdio>
+#include <cmath>
 
 #include "slimrt_context.h"
 #include "../../include/slimrt_nrc.h"
@@ -57,6 +58,15 @@ constexpr uint32_t kOrderCached = 48 * 1024;  // tiles whose rank stays in LDS b
 constexpr uint32_t kOrderBatch = 16;          // rounds of loads in flight together
 constexpr uint32_t kMaxOrderedTiles = 1u << 30;
 constexpr uint32_t kSplitShare = 32, kSplitMax = 2048;   // costliest tiles rendered as two half waves: 1 in 32
+// With 8-wide nodes the first 1 in 1024 tiles of the order (at most 128) are rendered as four quarter waves: the
+// 8-wide walk leaves less work in the frame (its ideal length, summed tile durations over the resident waves, fell
+// 464 -> 421 us on the 1080p mesh + PBR frame) but not in the frame's longest paths, so the longest waves must be
+// narrower to approach it (0.565 -> 0.518 ms).  With 4-wide nodes quarter waves only add waves (0.553 -> 0.56 ms
+// for the costliest 32), and eight or sixteen parts are slower with either width.
+constexpr uint32_t kQuarterShare = 1024, kQuarterMax = 128, kTopParts = 4;
+uint32_t quarter_tiles(uint32_t tiles, uint32_t width) {
+    return width == 8 ? std::min(tiles / kQuarterShare, kQuarterMax) : 0u;
+}
 
 __device__ __forceinline__ uint32_t cost_rank(uint32_t ticks) {
     if (ticks < 4u) return kCostClasses - 1u;
@@ -217,23 +227,30 @@ __global__ __launch_bounds__(kOrderThreads) void tile_order_kernel(const uint32_
 // waves (0.77 -> 0.65 ms on the mesh + PBR frame; quarter waves gain nothing and lose fast as their count grows).
 // LIGHTS false (scenes without lights or emissive quads) leaves the shadow rays and the glow out: no VGPR spills
 // at three waves per SIMD.
-template <bool kCount, bool WIDE, bool LIGHTS = true>
-__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(WIDE && !kCount ? 3 : 1)))
+template <bool kCount, uint32_t NW, bool LIGHTS = true>
+__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(NW >= 4 && !kCount ? 3 : 1)))
 void render_kernel(SceneView s, RenderParams rp, float4 *__restrict__ pixels, float *__restrict__ depths,
                    uint32_t scene_depth, uint32_t mesh_depth, uint32_t tiles_x, uint32_t tiles,
                    uint32_t *__restrict__ status, unsigned long long *__restrict__ counters,
                    const uint32_t *__restrict__ order, uint32_t *__restrict__ cost, uint32_t split,
-                   uint32_t *__restrict__ cost_tiles) {
+                   uint32_t quarter, uint32_t top_parts, uint32_t *__restrict__ cost_tiles) {
     extern __shared__ uint32_t lds[];
     if (cost_tiles && blockIdx.x == 0 && threadIdx.x == 0) *cost_tiles = tiles;   // what cost will describe
-    uint32_t tile = blockIdx.x, half = 2;   // half 0 / 1: lanes 0-31 / 32-63 only; 2: all lanes
+    uint32_t tile = blockIdx.x, parts = 1, part = 0;   // the wave renders lanes [part, part + 1) * 64 / parts
     if (order) {
-        const uint32_t sp = order[tiles] ? split : 0u;   // order[tiles] = 0: the order kernel chose image order
-        if (blockIdx.x >= tiles + sp) return;
-        if (blockIdx.x < 2 * sp) {
-            tile = order[blockIdx.x >> 1];
-            half = blockIdx.x & 1u;
-        } else tile = order[blockIdx.x - sp];
+        // order[tiles] = 0: the order kernel chose image order, no split
+        const uint32_t sp = order[tiles] ? split : 0u, q = order[tiles] ? min(quarter, split) : 0u;
+        const uint32_t b = blockIdx.x, P = top_parts;
+        if (b >= tiles + sp + (P - 2) * q) return;
+        if (b < P * q) {   // the first q tiles of the order as P part waves
+            tile = order[b / P];
+            parts = P;
+            part = b % P;
+        } else if (b < (P - 2) * q + 2 * sp) {   // the next sp - q as two half waves
+            tile = order[q + ((b - P * q) >> 1)];
+            parts = 2;
+            part = (b - P * q) & 1u;
+        } else tile = order[b - (P - 2) * q - sp];
     }
     if (tile >= tiles) return;
     const uint64_t start = cost ? (uint64_t)wall_clock64() : 0u;
@@ -243,12 +260,12 @@ void render_kernel(SceneView s, RenderParams rp, float4 *__restrict__ pixels, fl
     const uint32_t ws = rp.width << rp.ssaa, hs = rp.height << rp.ssaa;
 
     Counters cnt{};
-    TracerT<WIDE> tr;
-    bind_lds_stacks<WIDE, 64>(tr, lds, lane, scene_depth, mesh_depth);
+    TracerT<NW> tr;
+    bind_lds_stacks<NW, 64>(tr, lds, lane, scene_depth, mesh_depth);
     tr.cnt = kCount ? &cnt : nullptr;
-    if ((half == 2u || (lane >> 5) == half) && x < ws && y < hs) {
+    if (lane * parts / 64u == part && x < ws && y < hs) {
         float px[4], depth;
-        render_sample<TracerT<WIDE>, LIGHTS>(s, rp, x, y, tr, px, depth);
+        render_sample<TracerT<NW>, LIGHTS>(s, rp, x, y, tr, px, depth);
         const uint32_t o = canvas_offset(rp, x, y);
         pixels[o] = make_float4(px[0], px[1], px[2], px[3]);
         depths[o] = depth;
@@ -256,11 +273,11 @@ void render_kernel(SceneView s, RenderParams rp, float4 *__restrict__ pixels, fl
         if (tr.status) atomicOr(status, tr.status);
     }
     if (cost && lane == 0) {
-        // A split tile records twice the duration of its half that ends last: a half wave runs about half its
-        // tile's diverging paths, and the doubled figure keeps the tile in the costliest classes two frames later
-        // (its own half-wave duration would rank it lower, so it would run unsplit and late, then split again).
-        uint64_t d = (uint64_t)wall_clock64() - start;
-        if (half != 2u) d *= 2u;
+        // A split tile records its part's duration times the parts (the part that ends last writes last): a part
+        // runs about its share of the tile's diverging paths, and the scaled figure keeps the tile in the costliest
+        // classes two frames later (its own part's duration would rank it lower, so it would run unsplit and late,
+        // then split again).
+        uint64_t d = ((uint64_t)wall_clock64() - start) * parts;
         cost[tile] = d > 0xFFFFFFFFull ? 0xFFFFFFFFu : (uint32_t)d;
     }
     if (kCount) {
@@ -294,8 +311,8 @@ __global__ __launch_bounds__(64) void nrc_frame_kernel(SceneView s, RenderParams
     h ^= h >> 12;
     const bool training = nf.train_one_in && (h % nf.train_one_in) == 0;
 
-    TracerT<WIDE> tr;
-    bind_lds_stacks<WIDE, 64>(tr, lds, t, scene_depth, mesh_depth);
+    TracerT<WIDE ? 4u : 2u> tr;
+    bind_lds_stacks<WIDE ? 4u : 2u, 64>(tr, lds, t, scene_depth, mesh_depth);
     tr.cnt = nullptr;
     if (x < ws && y < hs) {
         v3 dir;
@@ -347,7 +364,7 @@ __global__ __launch_bounds__(kBlock) void pick_kernel(SceneView s, RenderParams 
     extern __shared__ uint32_t lds[];
     const uint32_t t = threadIdx.x;
     if (t != 0) return;
-    TracerT<false> tr;
+    TracerT<2> tr;
     bind_lds_stacks(tr, lds, t, scene_depth, mesh_depth);
     tr.cnt = nullptr;
     int32_t oi[3];
@@ -454,13 +471,17 @@ int prepare_tile_order(slimrt_context *c, uint32_t tiles) {
 
 // Waves of the frame kernel the device holds at once (its workgroups are single waves), for this node width and
 // stack size; cached per configuration.
-int resident_waves(slimrt_context *c, bool wide, bool lights, size_t lds, uint32_t &out) {
-    const uint64_t key = ((uint64_t)lds << 2) | (lights ? 2u : 0u) | (wide ? 1u : 0u);
+int resident_waves(slimrt_context *c, uint32_t width, bool lights, size_t lds, uint32_t &out) {
+    const uint64_t key = ((uint64_t)lds << 5) | (lights ? 16u : 0u) | width;
     if (c->resident_key != key || !c->resident) {
         int per_cu = 0, cus = 0;
         auto occupancy = [&](auto kernel) { return hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kernel, 64, lds); };
-        HIP_TRY(c, wide ? (lights ? occupancy(render_kernel<false, true, true>) : occupancy(render_kernel<false, true, false>))
-                        : (lights ? occupancy(render_kernel<false, false, true>) : occupancy(render_kernel<false, false, false>)));
+        if (width == 8)
+            HIP_TRY(c, lights ? occupancy(render_kernel<false, 8, true>) : occupancy(render_kernel<false, 8, false>));
+        else if (width == 4)
+            HIP_TRY(c, lights ? occupancy(render_kernel<false, 4, true>) : occupancy(render_kernel<false, 4, false>));
+        else
+            HIP_TRY(c, lights ? occupancy(render_kernel<false, 2, true>) : occupancy(render_kernel<false, 2, false>));
         HIP_TRY(c, hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, c->device));
         c->resident = (uint32_t)std::max(1, per_cu * cus);
         c->resident_key = key;
@@ -474,21 +495,23 @@ int launch(slimrt_context *c, const slimrt_render_params *p, float4 *pixels, flo
     std::memcpy(&rp, p, sizeof rp);
     rp.ssaa = rp.ssaa ? 1u : 0u;
     const uint32_t ws = rp.width << rp.ssaa, hs = rp.height << rp.ssaa;
-    const uint32_t scene_depth = c->scene_height + 1, mesh_depth = mesh_stack_depth(c, wide_mesh_bvh(rp));
+    const uint32_t width = mesh_node_width(rp);
+    const uint32_t scene_depth = c->scene_height + 1, mesh_depth = mesh_stack_depth(c, width);
     // sized for the 256-thread workgroups of the wavefront kernels (the frame kernel needs a quarter)
-    const size_t lds = stack_lds_bytes(scene_depth, mesh_depth, wide_mesh_bvh(rp));
+    const size_t lds = stack_lds_bytes(scene_depth, mesh_depth, width);
     if (lds > 160 * 1024) return fail(c, SLIMRT_E_ARG, "BVH too deep for the LDS traversal stacks");
-    if (wide_mesh_bvh(rp) && check_wide(c)) return SLIMRT_E_ARG;
+    if (check_width(c, width)) return SLIMRT_E_ARG;
     SceneView v = device_view(c);
     if (c->timed) HIP_TRY(c, hipEventRecord(c->ev0, c->stream));
     if (!count && (rp.flags & SLIMRT_WAVEFRONT) && rp.render_mode == RM_Beauty && rp.output != SLIMRT_OUTPUT_HITS) {
+        if (width == 8) return fail(c, SLIMRT_E_ARG, "the wavefront frame forms take node width 2 or 4, not SLIMRT_BVH8");
         int rc = launch_wavefront(c, rp, pixels, depths, (rp.flags & SLIMRT_WAVEFRONT_PER_BOUNCE) != 0,
                                   (rp.flags & SLIMRT_WAVEFRONT_SORT) != 0, 1u << ((rp.flags >> 4) & 3u));
         if (rc) return rc;
     } else {
         const uint32_t wtiles_x = (ws + kWaveTileX - 1) / kWaveTileX;
         const uint32_t wtiles = wtiles_x * ((hs + kWaveTileY - 1) / kWaveTileY);
-        const size_t wlds = stack_lds_bytes(scene_depth, mesh_depth, wide_mesh_bvh(rp), 64);
+        const size_t wlds = stack_lds_bytes(scene_depth, mesh_depth, width, 64);
         const bool by_cost = !count && !(rp.flags & SLIMRT_TILE_ORDER_FIXED) && wtiles < kMaxOrderedTiles;
         const uint32_t par = (uint32_t)(c->tile_frame & 1u);
         uint32_t *order = nullptr, *cost = nullptr;
@@ -502,17 +525,20 @@ int launch(slimrt_context *c, const slimrt_render_params *p, float4 *pixels, flo
             cost = c->tile_cost[par];
         }
         const uint32_t split = order ? std::min(wtiles / kSplitShare, kSplitMax) : 0u;
+        const uint32_t quarter = std::min(quarter_tiles(wtiles, width), split), tp = kTopParts;
 #define RENDER_LAUNCH(COUNT, WIDE, LIGHTS)                                                                      \
-    hipLaunchKernelGGL((render_kernel<COUNT, WIDE, LIGHTS>), dim3(wtiles + split), dim3(64), wlds, c->stream, v,    \
-                       rp, pixels, depths, scene_depth, mesh_depth, wtiles_x, wtiles, c->d_status, c->d_counters,     \
-                       order, cost, split, nullptr)
-        const bool wide = wide_mesh_bvh(rp);
+    hipLaunchKernelGGL((render_kernel<COUNT, WIDE, LIGHTS>), dim3(wtiles + split + (tp - 2) * quarter), dim3(64), wlds,    \
+                       c->stream, v, rp, pixels, depths, scene_depth, mesh_depth, wtiles_x, wtiles, c->d_status,      \
+                       c->d_counters, order, cost, split, quarter, tp, nullptr)
         if (count) {
-            if (wide) RENDER_LAUNCH(true, true, true); else RENDER_LAUNCH(true, false, true);
+            if (width == 8) RENDER_LAUNCH(true, 8, true); else if (width == 4) RENDER_LAUNCH(true, 4, true);
+            else RENDER_LAUNCH(true, 2, true);
         } else if (scene_has_lights(v)) {
-            if (wide) RENDER_LAUNCH(false, true, true); else RENDER_LAUNCH(false, false, true);
+            if (width == 8) RENDER_LAUNCH(false, 8, true); else if (width == 4) RENDER_LAUNCH(false, 4, true);
+            else RENDER_LAUNCH(false, 2, true);
         } else {
-            if (wide) RENDER_LAUNCH(false, true, false); else RENDER_LAUNCH(false, false, false);
+            if (width == 8) RENDER_LAUNCH(false, 8, false); else if (width == 4) RENDER_LAUNCH(false, 4, false);
+            else RENDER_LAUNCH(false, 2, false);
         }
 #undef RENDER_LAUNCH
         HIP_TRY(c, hipGetLastError());
@@ -526,7 +552,7 @@ int launch(slimrt_context *c, const slimrt_render_params *p, float4 *pixels, flo
             if (c->cost_tiles[nxt] == wtiles) {
                 HIP_TRY(c, hipStreamWaitEvent(c->side_stream, c->tile_rendered[nxt], 0));
                 uint32_t resident = 0;
-                if (int rc = resident_waves(c, wide_mesh_bvh(rp), scene_has_lights(v), wlds, resident)) return rc;
+                if (int rc = resident_waves(c, width, scene_has_lights(v), wlds, resident)) return rc;
                 hipLaunchKernelGGL(tile_order_kernel, dim3(1), dim3(kOrderThreads), 0, c->side_stream,
                                    c->tile_cost[nxt], c->tile_order[nxt], wtiles, resident, nullptr);
                 HIP_TRY(c, hipGetLastError());
@@ -541,38 +567,25 @@ int launch(slimrt_context *c, const slimrt_render_params *p, float4 *pixels, flo
     return SLIMRT_OK;
 }
 
+template <uint32_t NW>
+void cpu_rows(const SceneView &v, const RenderParams &rp, unsigned ti, unsigned nthreads, uint32_t sd, uint32_t md,
+              Counters *cnt, slimrt_pixel *pixels, float *depths, uint32_t &status);
+
 int cpu_render(slimrt_context *c, const slimrt_render_params *p, slimrt_pixel *pixels, float *depths, uint64_t *counters) {
     RenderParams rp;
     std::memcpy(&rp, p, sizeof rp);
     rp.ssaa = rp.ssaa ? 1u : 0u;
-    const uint32_t ws = rp.width << rp.ssaa, hs = rp.height << rp.ssaa;
     const SceneView v = host_view(c);
-    const uint32_t sd = c->scene_height + 1, md = c->mesh_height + 1;
+    const uint32_t width = mesh_node_width(rp);   // the sibling-pair walk unless a wide one is asked for
+    if (check_width(c, width)) return SLIMRT_E_ARG;
+    const uint32_t sd = c->scene_height + 1, md = mesh_stack_depth(c, width);
     unsigned nthreads = std::max(1u, std::min(64u, std::thread::hardware_concurrency()));
     std::vector<Counters> per(nthreads);
     std::vector<uint32_t> overflow(nthreads, 0);
     auto work = [&](unsigned ti) {
-        std::vector<uint32_t> stack(sd + md);
-        Tracer tr;
-        tr.scene_stack = Stack{stack.data(), 1};
-        tr.mesh_stack = Stack{stack.data() + sd, 1};
-        tr.scene_depth = sd;
-        tr.mesh_depth = md;
-        tr.status = 0;
-        per[ti] = Counters{};
-        tr.cnt = counters ? &per[ti] : nullptr;   // the host context runs the reference's sibling-pair walk
-        for (uint32_t y = ti; y < hs; y += nthreads)
-            for (uint32_t x = 0; x < ws; x++) {
-                float px[4], depth;
-                render_sample(v, rp, x, y, tr, px, depth);
-                if (pixels) {
-                    const uint32_t o = canvas_offset(rp, x, y);
-                    pixels[o] = slimrt_pixel{px[0], px[1], px[2], px[3]};
-                    depths[o] = depth;
-                    tr.status |= nan_status(rp, px, depth);
-                }
-            }
-        overflow[ti] = tr.status;
+        if (width == 8) cpu_rows<8>(v, rp, ti, nthreads, sd, md, counters ? &per[ti] : nullptr, pixels, depths, overflow[ti]);
+        else if (width == 4) cpu_rows<4>(v, rp, ti, nthreads, sd, md, counters ? &per[ti] : nullptr, pixels, depths, overflow[ti]);
+        else cpu_rows<2>(v, rp, ti, nthreads, sd, md, counters ? &per[ti] : nullptr, pixels, depths, overflow[ti]);
     };
     std::vector<std::thread> pool;
     for (unsigned i = 0; i < nthreads; i++) pool.emplace_back(work, i);
@@ -589,6 +602,36 @@ int cpu_render(slimrt_context *c, const slimrt_render_params *p, slimrt_pixel *p
         if (o & kStatusStack) return fail(c, SLIMRT_E_OVERFLOW, "traversal stack overflow");
     }
     return SLIMRT_OK;
+}
+
+// Rows ti, ti + nthreads, ... of a CPU-context frame with mesh node width NW.
+template <uint32_t NW>
+void cpu_rows(const SceneView &v, const RenderParams &rp, unsigned ti, unsigned nthreads, uint32_t sd, uint32_t md,
+              Counters *cnt, slimrt_pixel *pixels, float *depths, uint32_t &status) {
+    const uint32_t ws = rp.width << rp.ssaa, hs = rp.height << rp.ssaa;
+    using Entry = typename TracerT<NW>::MeshEntry;
+    std::vector<uint32_t> scene_stack(sd);
+    std::vector<Entry> mesh_stack(md);
+    TracerT<NW> tr;
+    tr.scene_stack = Stack{scene_stack.data(), 1};
+    tr.mesh_stack = StackT<Entry>{mesh_stack.data(), 1};
+    tr.scene_depth = sd;
+    tr.mesh_depth = md;
+    tr.status = 0;
+    if (cnt) *cnt = Counters{};
+    tr.cnt = cnt;
+    for (uint32_t y = ti; y < hs; y += nthreads)
+        for (uint32_t x = 0; x < ws; x++) {
+            float px[4], depth;
+            render_sample(v, rp, x, y, tr, px, depth);
+            if (pixels) {
+                const uint32_t o = canvas_offset(rp, x, y);
+                pixels[o] = slimrt_pixel{px[0], px[1], px[2], px[3]};
+                depths[o] = depth;
+                tr.status |= nan_status(rp, px, depth);
+            }
+        }
+    status = tr.status;
 }
 
 // Collapses one mesh's sibling-pair BVH (nodes, mesh-relative indices) into 4-wide nodes appended to out: a
@@ -671,6 +714,166 @@ uint32_t collapse_mesh_bvh4(const Node *n2, std::vector<Node4> &out, uint32_t &s
         collapse4(n2, 0, 1, out, h4, stack);
     }
     return h4;
+}
+
+// The 8-wide tree of a validated sibling-pair mesh BVH (Node8, LeafHead).  collapse8 gathers a node's children: its
+// two children with, while fewer than eight, the inner child of largest surface area replaced by its own two;
+// leaves keep their triangle ranges.  emit8 writes the node: child boxes quantised on a per-axis grid o + q * 2^e
+// (q in 0..255) spanning the node's box with two steps to spare on each side, each bound moved outward until the
+// decoded value, computed with the fused multiply-add the walk uses, lies at least two ulps of the exact bound
+// beyond it (if a bound cannot be reached the node's steps double and it is quantised again); slots filled greedily by how well each child's
+// offset from the node's centre points along the octant diagonal of the slot (slot bit set: negative axis), so a
+// ray whose direction lies in octant o meets the slots in the order s ^ (7 ^ o) nearly front to back.  Node
+// indices are assigned depth first, root 0; leaves go to leaf_out (LeafHead, then the leaf's TriHit records, in
+// 16-byte units).  Both return the node index; stack receives the mesh stack entries the walk can hold below the
+// node: a visit pushes its k hit inner children and pops one, so stack = max(k, k - 1 + the largest below).
+uint32_t emit8(const Node *n2, const TriHit *tris, const uint32_t *kids, uint32_t nk, const Node &par,
+               std::vector<Node8> &out, std::vector<uint4> &leaf_out, uint32_t &stack);
+
+double ulp(float x) {   // the spacing of floats at |x|
+    const float a = std::fabs(x);
+    return (double)std::nextafter(a, INFINITY) - (double)a;
+}
+
+uint32_t collapse8(const Node *n2, const TriHit *tris, uint32_t i2, std::vector<Node8> &out, std::vector<uint4> &leaf_out,
+                   uint32_t &stack) {
+    auto is_leaf = [&](uint32_t k) { return (n2[k].leaf_count_depth & 0xFFFFu) != 0; };
+    auto area = [&](uint32_t k) {
+        const float ex = n2[k].maxx - n2[k].minx, ey = n2[k].maxy - n2[k].miny, ez = n2[k].maxz - n2[k].minz;
+        return ex * ey + ey * ez + ez * ex;
+    };
+    uint32_t kids[8], nk = 2;
+    kids[0] = n2[i2].first_index;
+    kids[1] = n2[i2].first_index + 1;
+    while (nk < 8) {
+        int pick = -1;
+        for (uint32_t j = 0; j < nk; j++)
+            if (!is_leaf(kids[j]) && (pick < 0 || area(kids[j]) > area(kids[pick]))) pick = (int)j;
+        if (pick < 0) break;
+        const uint32_t k = kids[pick];
+        kids[pick] = n2[k].first_index;
+        kids[nk++] = n2[k].first_index + 1;
+    }
+    return emit8(n2, tris, kids, nk, n2[i2], out, leaf_out, stack);
+}
+
+uint32_t emit8(const Node *n2, const TriHit *tris, const uint32_t *kids, uint32_t nk, const Node &par,
+               std::vector<Node8> &out, std::vector<uint4> &leaf_out, uint32_t &stack) {
+    // slot assignment: the (child, slot) pairs by decreasing score, each child and slot taken once
+    const float cx = 0.5f * (par.minx + par.maxx), cy = 0.5f * (par.miny + par.maxy), cz = 0.5f * (par.minz + par.maxz);
+    struct Pair { float score; uint32_t child, slot; };
+    std::vector<Pair> pairs;
+    for (uint32_t j = 0; j < nk; j++) {
+        const Node &c = n2[kids[j]];
+        const float dx = 0.5f * (c.minx + c.maxx) - cx, dy = 0.5f * (c.miny + c.maxy) - cy, dz = 0.5f * (c.minz + c.maxz) - cz;
+        for (uint32_t sl = 0; sl < 8; sl++)
+            pairs.push_back({((sl & 1u) ? -dx : dx) + ((sl & 2u) ? -dy : dy) + ((sl & 4u) ? -dz : dz), j, sl});
+    }
+    std::stable_sort(pairs.begin(), pairs.end(), [](const Pair &a, const Pair &b) { return a.score > b.score; });
+    int slot_of[8], child_in[8];
+    for (int j = 0; j < 8; j++) slot_of[j] = child_in[j] = -1;
+    for (const Pair &pr : pairs)
+        if (slot_of[pr.child] < 0 && child_in[pr.slot] < 0) {
+            slot_of[pr.child] = (int)pr.slot;
+            child_in[pr.slot] = (int)pr.child;
+        }
+    auto set_byte = [](uint32_t w[2], uint32_t k, uint32_t v) {
+        w[k >> 2] = (w[k >> 2] & ~(0xFFu << (8 * (k & 3u)))) | (v << (8 * (k & 3u)));
+    };
+    // the grid: per axis a power-of-two step with the node's extent in at most 251 steps, origin two steps below
+    const float pmin[3] = {par.minx, par.miny, par.minz}, pmax[3] = {par.maxx, par.maxy, par.maxz};
+    int e[3];
+    for (int a = 0; a < 3; a++) {
+        const double ext = (double)pmax[a] - (double)pmin[a];
+        e[a] = -126;
+        while (e[a] < 127 && std::ldexp(251.0, e[a]) < ext) e[a]++;
+    }
+    Node8 node;
+    for (;;) {
+        node = Node8{};
+        float o[3], step[3];
+        for (int a = 0; a < 3; a++) {
+            step[a] = std::ldexp(1.0f, e[a]);
+            o[a] = pmin[a] - 2.0f * step[a];
+            node.exps |= (uint32_t)(e[a] + 127) << (8 * a);
+        }
+        node.ox = o[0]; node.oy = o[1]; node.oz = o[2];
+        bool ok = true;
+        for (uint32_t sl = 0; sl < 8; sl++) {
+            if (child_in[sl] < 0) {
+                set_byte(node.count, sl, 0xFFu);
+                for (int a = 0; a < 3; a++) { set_byte(node.qlo[a], sl, 255u); set_byte(node.qhi[a], sl, 0u); }
+                continue;
+            }
+            const Node &c = n2[kids[child_in[sl]]];
+            const float cmin[3] = {c.minx, c.miny, c.minz}, cmax[3] = {c.maxx, c.maxy, c.maxz};
+            for (int a = 0; a < 3; a++) {
+                const double lo = std::floor(((double)cmin[a] - (double)o[a]) / (double)step[a]);
+                const double hi = std::ceil(((double)cmax[a] - (double)o[a]) / (double)step[a]);
+                int ql = (int)std::min(std::max(lo, 0.0), 255.0), qh = (int)std::min(std::max(hi, 0.0), 255.0);
+                // two ulps of the bound to spare: the walk's fused slab test on the decoded bound then stays at or
+                // beyond the reference's rounded slab distances on the exact bound, for every ray
+                const double lim_lo = (double)cmin[a] - 2.0 * ulp(cmin[a]), lim_hi = (double)cmax[a] + 2.0 * ulp(cmax[a]);
+                while (ql > 0 && (double)std::fmaf((float)ql, step[a], o[a]) > lim_lo) ql--;
+                while (qh < 255 && (double)std::fmaf((float)qh, step[a], o[a]) < lim_hi) qh++;
+                ok = ok && (double)std::fmaf((float)ql, step[a], o[a]) <= lim_lo &&
+                     (double)std::fmaf((float)qh, step[a], o[a]) >= lim_hi;
+                set_byte(node.qlo[a], sl, (uint32_t)ql);
+                set_byte(node.qhi[a], sl, (uint32_t)qh);
+            }
+        }
+        if (ok) break;
+        for (int a = 0; a < 3; a++) e[a] = std::min(e[a] + 1, 127);
+    }
+    const uint32_t idx = (uint32_t)out.size();
+    out.push_back(Node8{});
+    uint32_t inner = 0, below = 0;
+    for (uint32_t sl = 0; sl < 8; sl++) {
+        if (child_in[sl] < 0) continue;
+        const uint32_t k = kids[child_in[sl]];
+        const Node &c = n2[k];
+        const uint32_t count = c.leaf_count_depth & 0xFFFFu;
+        if (count) {
+            set_byte(node.count, sl, count);
+            node.child[sl] = (uint32_t)leaf_out.size();
+            const LeafHead h{c.minx, c.miny, c.minz, c.first_index, c.maxx, c.maxy, c.maxz, count};
+            uint4 u[2];
+            std::memcpy(u, &h, sizeof h);
+            leaf_out.insert(leaf_out.end(), u, u + 2);
+            for (uint32_t t = 0; t < count; t++) {
+                uint4 w[3];
+                std::memcpy(w, &tris[c.first_index + t], sizeof(TriHit));
+                leaf_out.insert(leaf_out.end(), w, w + 3);
+            }
+        } else {
+            uint32_t child_stack = 0;
+            set_byte(node.count, sl, 0u);
+            node.child[sl] = collapse8(n2, tris, k, out, leaf_out, child_stack);
+            inner++;
+            below = std::max(below, child_stack);
+        }
+    }
+    stack = inner ? std::max(inner, inner - 1 + below) : 0;
+    out[idx] = node;
+    return idx;
+}
+
+// The 8-wide tree of a validated sibling-pair mesh BVH (children after parents) whose leaves hold at most 254
+// triangles, root first, and its leaf records; returns the mesh stack entries its walk needs.  A root that is a leaf
+// becomes a node holding it.  The leaf records end with a record's worth of padding (the walk reads a leaf's first
+// two triangle records together).
+uint32_t collapse_mesh_bvh8(const Node *n2, const TriHit *tris, std::vector<Node8> &out, std::vector<uint4> &leaf_out) {
+    uint32_t stack = 0;
+    out.clear();
+    leaf_out.clear();
+    if (n2[0].leaf_count_depth & 0xFFFFu) {
+        const uint32_t root = 0;
+        emit8(n2, tris, &root, 1, n2[0], out, leaf_out, stack);
+    } else {
+        collapse8(n2, tris, 0, out, leaf_out, stack);
+    }
+    for (int i = 0; i < 6; i++) leaf_out.push_back(uint4{0, 0, 0, 0});
+    return stack;
 }
 
 int ensure_outputs(slimrt_context *c, size_t n) {
@@ -783,6 +986,8 @@ int slimrt_destroy(slimrt_context *c) {
         if (c->side_stream) (void)hipStreamSynchronize(c->side_stream);
         dev_free(c, c->d_geos); dev_free(c, c->d_mats); dev_free(c, c->d_lights); dev_free(c, c->d_snodes); dev_free(c, c->d_mnodes);
         dev_free(c, c->d_mnodes4);
+        dev_free(c, c->d_mnodes8);
+        dev_free(c, c->d_leaf8);
         dev_free(c, c->d_leaf); dev_free(c, c->d_meshes); dev_free(c, c->d_tri_hit); dev_free(c, c->d_tri_attr); dev_free(c, c->d_tex);
         dev_free(c, c->d_mips); dev_free(c, c->d_quads); dev_free(c, c->d_pixels); dev_free(c, c->d_depths);
         free_nrc(c);
@@ -879,10 +1084,14 @@ int slimrt_upload_scene(slimrt_context *c, const slimrt_scene_desc *d) {
     c->loaded = false;
     // meshes: concatenate nodes and split triangle records
     c->meshes.clear(); c->mnodes.clear(); c->tri_hit.clear(); c->tri_attr.clear(); c->mnodes4.clear();
+    c->mnodes8.clear(); c->leaf8.clear();
     c->mesh_height = 0;
     c->mesh_height4 = 0;
     c->mesh_stack4 = 0;
     c->mesh_nodes4_max = 0;
+    c->mesh_stack8 = 0;
+    c->mesh_nodes8_max = 0;
+    c->mesh_leaf_max = 0;
     for (uint32_t k = 0; k < d->mesh_count; k++) {
         const slimrt_mesh &m = d->meshes[k];
         if (!m.nodes || !m.triangles || m.node_count == 0 || m.triangle_count == 0)
@@ -908,11 +1117,8 @@ int slimrt_upload_scene(slimrt_context *c, const slimrt_scene_desc *d) {
         c->mesh_stack4 = std::max(c->mesh_stack4, stack4);
         c->mesh_nodes4_max = std::max(c->mesh_nodes4_max, (uint32_t)n4.size());
         c->mnodes4.insert(c->mnodes4.end(), n4.begin(), n4.end());
-        r.node4_count = (uint32_t)n4.size();
-        r.height4 = h4;
         r.pad = 0;
         c->mesh_height4 = std::max(c->mesh_height4, h4);
-        c->meshes.push_back(r);
         for (uint32_t i = 0; i < m.triangle_count; i++) {
             const slimrt_triangle &t = m.triangles[i];
             const float *L = t.local_to_tangent;
@@ -922,6 +1128,21 @@ int slimrt_upload_scene(slimrt_context *c, const slimrt_scene_desc *d) {
                                           t.n3[0], t.n3[1], t.n3[2], t.uv2[0], t.uv2[1], t.uv3[0], t.uv3[1],
                                           t.uv_coverage});
         }
+        uint32_t leaf_max = 0;
+        for (uint32_t i = 0; i < m.node_count; i++) leaf_max = std::max<uint32_t>(leaf_max, m.nodes[i].leaf_count);
+        c->mesh_leaf_max = std::max(c->mesh_leaf_max, leaf_max);
+        r.node8_base = (uint32_t)c->mnodes8.size();
+        r.leaf8_base = (uint32_t)c->leaf8.size();
+        if (leaf_max <= 254) {   // larger leaves cannot be described by the 8-wide nodes (check_width refuses them)
+            std::vector<Node8> n8;
+            std::vector<uint4> l8;
+            const uint32_t stack8 = collapse_mesh_bvh8(c->mnodes.data() + r.node_base, c->tri_hit.data() + r.tri_base, n8, l8);
+            c->mesh_stack8 = std::max(c->mesh_stack8, stack8);
+            c->mesh_nodes8_max = std::max(c->mesh_nodes8_max, (uint32_t)n8.size());
+            c->mnodes8.insert(c->mnodes8.end(), n8.begin(), n8.end());
+            c->leaf8.insert(c->leaf8.end(), l8.begin(), l8.end());
+        }
+        c->meshes.push_back(r);
         c->mesh_height = std::max(c->mesh_height, m.height);
     }
     // textures: flatten the mips (texel quads, or bordered RGBA8 / RGBA16F texels) into one byte array, each mip
@@ -965,6 +1186,8 @@ int slimrt_upload_scene(slimrt_context *c, const slimrt_scene_desc *d) {
         if ((rc = dev_upload(c, c->d_meshes, c->meshes.data(), c->meshes.size()))) return rc;
         if ((rc = dev_upload(c, c->d_mnodes, c->mnodes.data(), c->mnodes.size()))) return rc;
         if ((rc = dev_upload(c, c->d_mnodes4, c->mnodes4.data(), c->mnodes4.size()))) return rc;
+        if ((rc = dev_upload(c, c->d_mnodes8, c->mnodes8.data(), c->mnodes8.size()))) return rc;
+        if ((rc = dev_upload(c, c->d_leaf8, c->leaf8.data(), c->leaf8.size()))) return rc;
         if ((rc = dev_upload(c, c->d_tri_hit, c->tri_hit.data(), c->tri_hit.size()))) return rc;
         if ((rc = dev_upload(c, c->d_tri_attr, c->tri_attr.data(), c->tri_attr.size()))) return rc;
         if ((rc = dev_upload(c, c->d_tex, c->textures.data(), c->textures.size()))) return rc;
@@ -1035,18 +1258,18 @@ int slimrt_render_device_noalloc(slimrt_context *c, const slimrt_render_params *
         return fail(c, SLIMRT_E_ARG, "render_device_noalloc: the frame has " + std::to_string(wtiles64) +
                                      " tiles of 32 x 2 samples; at most 131072 (8,388,608 samples) fit its scratch");
     const uint32_t wtiles = (uint32_t)wtiles64;
-    const bool wide = wide_mesh_bvh(rp);
-    const uint32_t scene_depth = c->scene_height + 1, mesh_depth = mesh_stack_depth(c, wide);
-    const size_t wlds = stack_lds_bytes(scene_depth, mesh_depth, wide, 64);
+    const uint32_t width = mesh_node_width(rp);
+    const uint32_t scene_depth = c->scene_height + 1, mesh_depth = mesh_stack_depth(c, width);
+    const size_t wlds = stack_lds_bytes(scene_depth, mesh_depth, width, 64);
     if (wlds > 160 * 1024) return fail(c, SLIMRT_E_ARG, "BVH too deep for the LDS traversal stacks");
-    if (wide && check_wide(c)) return SLIMRT_E_ARG;
+    if (check_width(c, width)) return SLIMRT_E_ARG;
     const bool by_cost = !(rp.flags & SLIMRT_TILE_ORDER_FIXED);
     uint32_t *cost_tiles = c->noalloc_scratch, *cost = nullptr, *order = nullptr;
     uint32_t resident = 0;
     if (by_cost) {
         cost = c->noalloc_scratch + kNoAllocCost;
         order = c->noalloc_scratch + kNoAllocOrder;
-        if ((rc = resident_waves(c, wide, scene_has_lights(device_view(c)), wlds, resident))) return rc;
+        if ((rc = resident_waves(c, width, scene_has_lights(device_view(c)), wlds, resident))) return rc;
     }
     const SceneView v = device_view(c);
     c->timed = true;
@@ -1059,14 +1282,15 @@ int slimrt_render_device_noalloc(slimrt_context *c, const slimrt_render_params *
         HIP_TRY(c, hipGetLastError());
     }
     const uint32_t split = by_cost ? std::min(wtiles / kSplitShare, kSplitMax) : 0u;
-#define NOALLOC_LAUNCH(WIDE, LIGHTS)                                                                              \
-    hipLaunchKernelGGL((render_kernel<false, WIDE, LIGHTS>), dim3(wtiles + split), dim3(64), wlds, c->stream, v, rp, \
-                       (float4 *)d_pixels, (float *)d_depths, scene_depth, mesh_depth, wtiles_x, wtiles, c->d_status,  \
-                       nullptr, order, cost, split, by_cost ? cost_tiles : nullptr)
+    const uint32_t quarter = std::min(quarter_tiles(wtiles, width), split), tp = kTopParts;
+#define NOALLOC_LAUNCH(NW, LIGHTS)                                                                                \
+    hipLaunchKernelGGL((render_kernel<false, NW, LIGHTS>), dim3(wtiles + split + (tp - 2) * quarter), dim3(64), wlds,      \
+                       c->stream, v, rp, (float4 *)d_pixels, (float *)d_depths, scene_depth, mesh_depth, wtiles_x,     \
+                       wtiles, c->d_status, nullptr, order, cost, split, quarter, tp, by_cost ? cost_tiles : nullptr)
     if (scene_has_lights(v)) {
-        if (wide) NOALLOC_LAUNCH(true, true); else NOALLOC_LAUNCH(false, true);
+        if (width == 8) NOALLOC_LAUNCH(8, true); else if (width == 4) NOALLOC_LAUNCH(4, true); else NOALLOC_LAUNCH(2, true);
     } else {
-        if (wide) NOALLOC_LAUNCH(true, false); else NOALLOC_LAUNCH(false, false);
+        if (width == 8) NOALLOC_LAUNCH(8, false); else if (width == 4) NOALLOC_LAUNCH(4, false); else NOALLOC_LAUNCH(2, false);
     }
 #undef NOALLOC_LAUNCH
     HIP_TRY(c, hipGetLastError());
@@ -1133,6 +1357,40 @@ int slimrt_host_collapse_bvh4(const slimrt_bvh_node *nodes, uint32_t node_count,
     return SLIMRT_OK;
 }
 
+int slimrt_host_collapse_bvh8(const slimrt_bvh_node *nodes, uint32_t node_count, const slimrt_triangle *triangles,
+                              uint32_t triangle_count, slimrt_bvh8_node *nodes_out, uint32_t node_cap,
+                              uint32_t *node_count_out, uint32_t *leaves_out, uint32_t leaf_cap,
+                              uint32_t *leaf_count_out, uint32_t *stack_out) {
+    if (!nodes || !node_count || !triangles || (!nodes_out && node_cap) || !node_count_out || (!leaves_out && leaf_cap) ||
+        !leaf_count_out || !stack_out)
+        return SLIMRT_E_ARG;
+    std::vector<Node> n2(node_count);
+    for (uint32_t i = 0; i < node_count; i++) {   // the checks of the upload: the walk below stays in range
+        const slimrt_bvh_node &n = nodes[i];
+        const bool bad = n.leaf_count ? ((uint64_t)n.first_index + n.leaf_count > triangle_count || n.leaf_count > 254)
+                                      : (n.first_index <= i || (uint64_t)n.first_index + 1 >= node_count);
+        if (bad) return SLIMRT_E_ARG;
+        n2[i] = to_node(n);
+    }
+    std::vector<TriHit> tris(triangle_count);
+    for (uint32_t i = 0; i < triangle_count; i++) {
+        const slimrt_triangle &t = triangles[i];
+        const float *L = t.local_to_tangent;
+        tris[i] = TriHit{L[0], L[1], L[3], L[4], L[6], L[7], t.position[0], t.position[1], t.position[2],
+                         t.normal[0], t.normal[1], t.normal[2]};
+    }
+    std::vector<Node8> out;
+    std::vector<uint4> leaves;
+    *stack_out = collapse_mesh_bvh8(n2.data(), tris.data(), out, leaves);
+    *node_count_out = (uint32_t)out.size();
+    *leaf_count_out = (uint32_t)leaves.size();
+    if (out.size() > node_cap || leaves.size() > leaf_cap) return SLIMRT_E_OVERFLOW;
+    static_assert(sizeof(Node8) == sizeof(slimrt_bvh8_node), "8-wide node layout");
+    memcpy(nodes_out, out.data(), out.size() * sizeof(Node8));
+    memcpy(leaves_out, leaves.data(), leaves.size() * sizeof(uint4));
+    return SLIMRT_OK;
+}
+
 int slimrt_canvas_content(slimrt_context *c, const void *pixels, uint32_t width, uint32_t height, uint32_t ssaa,
                           void *content) {
     if (!c) return SLIMRT_E_ARG;
@@ -1175,8 +1433,8 @@ int slimrt_pick(slimrt_context *c, const slimrt_render_params *p, int32_t x, int
         if (tr.status & kStatusStack) return fail(c, SLIMRT_E_OVERFLOW, "traversal stack overflow");
     } else {
         if (!c->d_pick) HIP_TRY(c, ctx_malloc(c, &c->d_pick, 8 * sizeof(int32_t)));
-        const uint32_t scene_depth = c->scene_height + 1, mesh_depth = mesh_stack_depth(c, false);
-        const size_t lds = stack_lds_bytes(scene_depth, mesh_depth, false);
+        const uint32_t scene_depth = c->scene_height + 1, mesh_depth = mesh_stack_depth(c, 2);
+        const size_t lds = stack_lds_bytes(scene_depth, mesh_depth, 2);
         if (lds > 160 * 1024) return fail(c, SLIMRT_E_ARG, "BVH too deep for the LDS traversal stacks");
         hipLaunchKernelGGL(pick_kernel, dim3(1), dim3(kBlock), lds, c->stream, device_view(c), rp, x, y, scene_depth,
                            mesh_depth, c->d_pick, c->d_status);
@@ -1252,10 +1510,12 @@ int slimrt_render_nrc(slimrt_context *c, const slimrt_render_params *p, slimrt_n
     // one-wave workgroups, a 32x2 strip of samples each (the render kernel's tiling)
     const uint32_t tiles_x = (ws + kWaveTileX - 1) / kWaveTileX, tiles_y = (hs + kWaveTileY - 1) / kWaveTileY;
     const uint32_t tiles = tiles_x * tiles_y;
-    const uint32_t scene_depth = c->scene_height + 1, mesh_depth = mesh_stack_depth(c, wide_mesh_bvh(rp));
-    const size_t lds = stack_lds_bytes(scene_depth, mesh_depth, wide_mesh_bvh(rp), 64);
+    if (rp.flags & SLIMRT_BVH8) return fail(c, SLIMRT_E_ARG, "the cache frame takes node width 2 or 4, not SLIMRT_BVH8");
+    const uint32_t width = wide_mesh_bvh(rp) ? 4u : 2u;
+    const uint32_t scene_depth = c->scene_height + 1, mesh_depth = mesh_stack_depth(c, width);
+    const size_t lds = stack_lds_bytes(scene_depth, mesh_depth, width, 64);
     if (lds > 160 * 1024) return fail(c, SLIMRT_E_ARG, "BVH too deep for the LDS traversal stacks");
-    if (wide_mesh_bvh(rp) && check_wide(c)) return SLIMRT_E_ARG;
+    if (check_width(c, width)) return SLIMRT_E_ARG;
     // training waves are a hashed 1-in-N subset: twice the expected share (at least 64 waves) bounds the samples
     uint64_t t_cap = 0;
     if (np->train_one_in) {

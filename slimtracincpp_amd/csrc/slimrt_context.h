@@ -57,6 +57,11 @@ struct slimrt_context {
     uint32_t mesh_height4 = 0;
     uint32_t mesh_stack4 = 0;       // mesh stack entries the 4-wide walk needs (largest over the meshes)
     uint32_t mesh_nodes4_max = 0;   // 4-wide nodes of the largest mesh (16-bit stack entries need <= 65536)
+    std::vector<Node8> mnodes8;             // 8-wide mesh BVHs collapsed from mnodes at upload
+    std::vector<uint4> leaf8;               // their leaf records (LeafHead + TriHit records)
+    uint32_t mesh_stack8 = 0;       // mesh stack entries the 8-wide walk needs (largest over the meshes)
+    uint32_t mesh_nodes8_max = 0;   // 8-wide nodes of the largest mesh
+    uint32_t mesh_leaf_max = 0;     // triangles of the largest leaf (the 8-wide nodes hold counts up to 254)
     bool loaded = false;
 
     DevBuf<Geo> d_geos;
@@ -64,6 +69,8 @@ struct slimrt_context {
     DevBuf<Light> d_lights;
     DevBuf<Node> d_snodes, d_mnodes;
     DevBuf<Node4> d_mnodes4;
+    DevBuf<Node8> d_mnodes8;
+    DevBuf<uint4> d_leaf8;
     DevBuf<uint32_t> d_leaf;
     DevBuf<MeshRec> d_meshes;
     DevBuf<TriHit> d_tri_hit;
@@ -216,39 +223,44 @@ inline SceneView host_view(const slimrt_context *c) {
     v.snodes = c->snodes.data(); v.leaf_ids = c->leaf_ids.data(); v.meshes = c->meshes.data();
     v.mnodes = c->mnodes.data(); v.tri_hit = c->tri_hit.data(); v.tri_attr = c->tri_attr.data();
     v.mnodes4 = c->mnodes4.data();
+    v.mnodes8 = c->mnodes8.data(); v.leaf8 = c->leaf8.data();
     v.textures = c->textures.data(); v.mips = c->mips.data(); v.quads = c->quads.data();
     v.G = (uint32_t)c->geos.size(); v.M = (uint32_t)c->mats.size(); v.L = (uint32_t)c->lights.size();
     v.NM = (uint32_t)c->meshes.size(); v.NT = (uint32_t)c->textures.size(); v.flags = c->scene_flags;
     return v;
 }
 
-// Traversal stack entries for mesh BVHs: the declared height + 1 for the sibling-pair walk; for the 4-wide walk
-// up to three pending siblings per level of the collapsed tree, plus the children of the node being visited.
-inline uint32_t mesh_stack_depth(const slimrt_context *c, bool wide) {
-    return wide ? std::max(c->mesh_stack4, 1u) : c->mesh_height + 1;
+// Traversal stack entries for mesh BVHs: the declared height + 1 for the sibling-pair walk; for the wide walks
+// what their collapse computed (pending siblings per level of the collapsed tree, plus the children of the node
+// being visited).
+inline uint32_t mesh_stack_depth(const slimrt_context *c, uint32_t width) {
+    return width == 8 ? std::max(c->mesh_stack8, 1u) : width == 4 ? std::max(c->mesh_stack4, 1u) : c->mesh_height + 1;
 }
 
 // A workgroup's traversal stacks in LDS: kBlock columns of scene entries (32-bit), then kBlock columns of mesh
-// entries (32-bit pair indices, or 16-bit node indices for the 4-wide walk).
-inline size_t stack_lds_bytes(uint32_t scene_depth, uint32_t mesh_depth, bool wide, uint32_t threads = kBlock) {
-    return ((size_t)scene_depth * 4 + (size_t)mesh_depth * (wide ? 2 : 4)) * threads;
+// entries (32-bit pair indices, or 16-bit node indices for the wide walks).
+inline size_t stack_lds_bytes(uint32_t scene_depth, uint32_t mesh_depth, uint32_t width, uint32_t threads = kBlock) {
+    return ((size_t)scene_depth * 4 + (size_t)mesh_depth * (width >= 4 ? 2 : 4)) * threads;
 }
 // Thread t's stacks in a workgroup of THREADS threads: one column each, entry i at i * THREADS.
-template <bool WIDE, uint32_t THREADS = kBlock>
-__device__ __forceinline__ void bind_lds_stacks(TracerT<WIDE> &tr, uint32_t *lds, uint32_t t, uint32_t scene_depth,
+template <uint32_t NW, uint32_t THREADS = kBlock>
+__device__ __forceinline__ void bind_lds_stacks(TracerT<NW> &tr, uint32_t *lds, uint32_t t, uint32_t scene_depth,
                                                 uint32_t mesh_depth) {
     tr.scene_stack = StackT<uint32_t>{lds + t, THREADS};
-    tr.mesh_stack = StackT<typename TracerT<WIDE>::MeshEntry>{
-        reinterpret_cast<typename TracerT<WIDE>::MeshEntry *>(lds + scene_depth * THREADS) + t, THREADS};
+    tr.mesh_stack = StackT<typename TracerT<NW>::MeshEntry>{
+        reinterpret_cast<typename TracerT<NW>::MeshEntry *>(lds + scene_depth * THREADS) + t, THREADS};
     tr.scene_depth = scene_depth;
     tr.mesh_depth = mesh_depth;
     tr.status = 0;
 }
 
-// Checks that the 4-wide walk can run on the uploaded meshes (16-bit stack entries).
-inline int check_wide(slimrt_context *c) {
-    if (c->mesh_nodes4_max > 65536)
+// Checks that a wide walk can run on the uploaded meshes (16-bit stack entries; 8-bit leaf counts for width 8).
+inline int check_width(slimrt_context *c, uint32_t width) {
+    if (width == 4 && c->mesh_nodes4_max > 65536)
         return fail(c, SLIMRT_E_ARG, "a mesh has more than 65536 4-wide nodes: render it with node width 2");
+    if (width == 8 && (c->mesh_nodes8_max > 65536 || c->mesh_leaf_max > 254))
+        return fail(c, SLIMRT_E_ARG, "a mesh has more than 65536 8-wide nodes or a leaf of more than 254 triangles: "
+                                     "render it with node width 2 or 4");
     return SLIMRT_OK;
 }
 
@@ -258,6 +270,7 @@ inline SceneView device_view(const slimrt_context *c) {
     v.snodes = c->d_snodes.ptr; v.leaf_ids = c->d_leaf.ptr; v.meshes = c->d_meshes.ptr;
     v.mnodes = c->d_mnodes.ptr; v.tri_hit = c->d_tri_hit.ptr; v.tri_attr = c->d_tri_attr.ptr;
     v.mnodes4 = c->d_mnodes4.ptr;
+    v.mnodes8 = c->d_mnodes8.ptr; v.leaf8 = c->d_leaf8.ptr;
     v.textures = c->d_tex.ptr; v.mips = c->d_mips.ptr; v.quads = c->d_quads.ptr;
     return v;
 }

@@ -71,12 +71,12 @@ struct alignas(16) Light {
     float x, y, z; uint32_t flags;
 };
 // Mesh record, 64 bytes: object-space AABB, node/triangle bases in the concatenated arrays, counts, and the
-// base of the mesh's 4-wide nodes.
+// bases of the mesh's 4-wide nodes, 8-wide nodes and 8-wide leaf records (16-byte units).
 struct alignas(16) MeshRec {
     float minx, miny, minz; uint32_t node_base;
     float maxx, maxy, maxz; uint32_t tri_base;
     uint32_t node_count, tri_count, normals_count, uvs_count;
-    uint32_t node4_base, node4_count, height4, pad;
+    uint32_t node4_base, node8_base, leaf8_base, pad;
 };
 // 4-wide mesh BVH node, 128 bytes: the children's boxes as structure of arrays, then per child either a node
 // index (count 0) or a leaf's first triangle and triangle count; unused slots have child = ~0u.
@@ -84,6 +84,29 @@ struct alignas(16) Node4 {
     float minx[4], miny[4], minz[4];
     float maxx[4], maxy[4], maxz[4];
     uint32_t child[4], count[4];
+};
+// 8-wide mesh BVH node, 128 bytes, with quantised child boxes (the compressed wide node of Ylitie, Karras and Laine,
+// "Efficient Incoherent Ray Traversal on GPUs Through Compressed Wide BVHs", HPG 2017, with this build's own layout).
+// Child box bounds are grid points o + q * 2^(e - 127) per axis, q in 0..255, chosen on the host so that every
+// decoded box contains the child's exact box (each bound checked with the same fused multiply-add the walk uses);
+// a slab test on the decoded box with the reference's arithmetic is monotone in the bounds, so it accepts every
+// ray the exact box accepts.  Slots hold the children ordered for the walk's octant order (slot s is visited at
+// position s ^ (7 ^ ray octant)).  Per slot: count 0 = inner child (child = mesh-relative node index), 1..254 = leaf
+// of that many triangles (child = its record in the leaf array, 16-byte units), 255 = empty.
+struct alignas(16) Node8 {
+    float ox, oy, oz;
+    uint32_t exps;               // bytes: biased exponents of the x, y, z grid steps; the fourth byte is unused
+    uint32_t child[8];
+    uint32_t count[2];           // one byte per slot
+    uint32_t qlo[3][2], qhi[3][2];   // per axis, one byte per slot
+    uint32_t pad[6];
+};
+// An 8-wide leaf record: the leaf's exact box (the sibling-pair node's), its first triangle (mesh-relative, leaf
+// order) and triangle count, followed by its triangles' hit records (TriHit).  The walk repeats the reference's
+// exact leaf test before testing the triangles, so quantised boxes change which leaves are fetched, not results.
+struct alignas(16) LeafHead {
+    float minx, miny, minz; uint32_t first;
+    float maxx, maxy, maxz; uint32_t count;
 };
 // Triangle split into the 48 bytes every intersection test reads and the 64 bytes only the closest hit reads
 // (Triangle is 128 bytes, src/slim/scene/mesh.h:19-24).  The test uses the first two columns of the
@@ -115,6 +138,8 @@ struct SceneView {
     const MeshRec *meshes;
     const Node *mnodes;          // all mesh BVH nodes
     const Node4 *mnodes4;        // all 4-wide mesh BVH nodes (collapsed from mnodes)
+    const Node8 *mnodes8;        // all 8-wide mesh BVH nodes (collapsed from mnodes)
+    const uint4 *leaf8;          // all 8-wide leaf records (LeafHead + TriHit records), 16-byte units
     const TriHit *tri_hit;
     const TriAttr *tri_attr;
     const TexRec *textures;
@@ -145,6 +170,8 @@ struct RenderParams {
 
 SLIM_HD bool keeps_ray_cones(const RenderParams &rp) { return (rp.flags & 1u) != 0; }
 SLIM_HD bool wide_mesh_bvh(const RenderParams &rp) { return (rp.flags & 64u) != 0; }
+// The meshes' node width a frame asks for: 8 (SLIMRT_BVH8), 4 (SLIMRT_BVH4) or the sibling-pair walk.
+SLIM_HD uint32_t mesh_node_width(const RenderParams &rp) { return (rp.flags & 1024u) ? 8u : (rp.flags & 64u) ? 4u : 2u; }
 
 // ------------------------------------------------------------------ small algebra
 struct v3 { float x, y, z; };

@@ -30,13 +30,15 @@ struct Counters {   // work counters, accumulated only when the caller asks for 
     uint64_t rays, pairs, roots, instances, triangles, mesh_hits, taps, pixels;
 };
 
-// The traversal state of one sample.  WIDE selects the meshes' 4-wide BVHs at compile time, so a kernel carries
-// only the mesh walk it uses.
-template <bool WIDE>
+// The traversal state of one sample.  NW, the meshes' node width (2: the sibling-pair walk, 4 or 8: the collapsed
+// wide BVHs), is chosen at compile time, so a kernel carries only the mesh walk it uses.
+template <uint32_t NW>
 struct TracerT {
-    static constexpr bool kWide = WIDE;
-    // mesh stack entries: sibling-pair indices, or 4-wide node indices relative to the mesh (at most 65536 nodes)
-    using MeshEntry = typename std::conditional<WIDE, uint16_t, uint32_t>::type;
+    static_assert(NW == 2 || NW == 4 || NW == 8, "node width 2, 4 or 8");
+    static constexpr uint32_t kWidth = NW;
+    static constexpr bool kWide = NW >= 4;
+    // mesh stack entries: sibling-pair indices, or wide node indices relative to the mesh (at most 65536 nodes)
+    using MeshEntry = typename std::conditional<(NW >= 4), uint16_t, uint32_t>::type;
     Stack scene_stack;
     StackT<MeshEntry> mesh_stack;
     uint32_t scene_depth, mesh_depth;   // capacities
@@ -45,49 +47,54 @@ struct TracerT {
     Hit aux_hit;                        // SceneTracer::aux_hit
     Counters *cnt;                      // null: no counting
 };
-using Tracer = TracerT<false>;
+using Tracer = TracerT<2>;
 
 #define SLIM_COUNT(tr, field) do { if ((tr).cnt) (tr).cnt->field++; } while (0)
 
 SLIM_HD Node load_node(const Node *nodes, uint32_t i) { return nodes[i]; }
 
+// The triangle test of MeshTracer::hitTriangles (mesh_tracer.h:23-49) over nb records tb, triangle ids id0, id0 + 1,
+// ...; closest and the scratch hit th carry over between calls.  Returns true when an any-hit walk may stop.
+template <class TR>
+SLIM_HD bool test_triangles(const TriHit *tb, uint32_t nb, uint32_t id0, float &closest, const Ray &r, Hit &hit,
+                            Hit &th, bool any_hit, bool &found, TR &tr) {
+#pragma unroll
+    for (uint32_t k = 0; k < 2; k++) {
+        if (k >= nb) break;
+        const TriHit &t = tb[k];
+        SLIM_COUNT(tr, triangles);
+        v3 tp{t.px, t.py, t.pz};
+        if (hits_plane(r, tp, v3{t.nx, t.ny, t.nz}, th)) {
+            v3 d = th.position - tp;
+            float ux = t.xx * d.x + t.yx * d.y + t.zx * d.z;
+            float uy = t.xy * d.x + t.yy * d.y + t.zy * d.z;
+            if (ux < 0 || uy < 0 || (ux + uy) > 1 || th.distance >= closest) continue;
+            closest = th.distance;
+            hit = th;
+            hit.uv = v2{ux, uy};
+            hit.id = id0 + k;
+            found = true;
+            if (any_hit) return true;
+        }
+    }
+    return false;
+}
+
 // MeshTracer::hitTriangles (mesh_tracer.h:23-49); triangle indices are relative to the mesh.  The records are
 // fetched two at a time, so a leaf costs half as many dependent round trips to memory; they are tested in order
 // as before.
 template <class TR>
-SLIM_HD bool hit_triangles(const SceneView &s, const MeshRec &m, uint32_t first, uint32_t n, float closest,
-                           const Ray &r, Hit &hit, bool any_hit, TR &tr) {
-    constexpr uint32_t kBatch = 2;
+SLIM_HD bool hit_triangles(const TriHit *tris, uint32_t first, uint32_t n, float closest, const Ray &r, Hit &hit,
+                           bool any_hit, TR &tr) {
     bool found = false;
     closest = fmin2(closest, hit.distance);
     Hit th = hit;
-    const TriHit *tris = s.tri_hit + m.tri_base + first;
-    for (uint32_t i0 = 0; i0 < n; i0 += kBatch) {
-        TriHit tb[kBatch];
+    for (uint32_t i0 = 0; i0 < n; i0 += 2) {
+        TriHit tb[2];
 #pragma unroll
-        for (uint32_t k = 0; k < kBatch; k++)
+        for (uint32_t k = 0; k < 2; k++)
             if (i0 + k < n) tb[k] = tris[i0 + k];
-        bool stop = false;
-#pragma unroll
-        for (uint32_t k = 0; k < kBatch; k++) {
-            if (i0 + k >= n || stop) continue;
-            const TriHit &t = tb[k];
-            SLIM_COUNT(tr, triangles);
-            v3 tp{t.px, t.py, t.pz};
-            if (hits_plane(r, tp, v3{t.nx, t.ny, t.nz}, th)) {
-                v3 d = th.position - tp;
-                float ux = t.xx * d.x + t.yx * d.y + t.zx * d.z;
-                float uy = t.xy * d.x + t.yy * d.y + t.zy * d.z;
-                if (ux < 0 || uy < 0 || (ux + uy) > 1 || th.distance >= closest) continue;
-                closest = th.distance;
-                hit = th;
-                hit.uv = v2{ux, uy};
-                hit.id = first + i0 + k;
-                found = true;
-                if (any_hit) stop = true;
-            }
-        }
-        if (stop) break;
+        if (test_triangles(tb, n - i0, first + i0, closest, r, hit, th, any_hit, found, tr)) break;
     }
     return found;
 }
@@ -113,19 +120,22 @@ SLIM_HD void mesh_hit_attributes(const SceneView &s, const MeshRec &m, Hit &hit,
 
 template <class TR>
 SLIM_HD bool mesh_trace4(const SceneView &s, const MeshRec &m, const Ray &r, Hit &hit, bool any_hit, TR &tr);
+template <class TR>
+SLIM_HD bool mesh_trace8(const SceneView &s, const MeshRec &m, const Ray &r, Hit &hit, bool any_hit, TR &tr);
 
 // MeshTracer::trace (mesh_tracer.h:51-134)
 template <class TR>
 SLIM_HD bool mesh_trace(const SceneView &s, uint32_t mesh_id, const Ray &r, Hit &hit, bool any_hit, TR &tr) {
     const MeshRec m = s.meshes[mesh_id];
-    if constexpr (TR::kWide) return mesh_trace4(s, m, r, hit, any_hit, tr);
+    if constexpr (TR::kWidth == 8) return mesh_trace8(s, m, r, hit, any_hit, tr);
+    if constexpr (TR::kWidth == 4) return mesh_trace4(s, m, r, hit, any_hit, tr);
     const Node *nodes = s.mnodes + m.node_base;
     float ln, rn, lf, rf;
     bool found = false;
     SLIM_COUNT(tr, roots);
     const Node root = load_node(nodes, 0);
     if (!(hits_node(r, root, ln, lf) && ln < hit.distance)) return false;
-    if (root.leaf_count_depth & 0xFFFFu) return hit_triangles(s, m, 0, m.tri_count, lf, r, hit, any_hit, tr);
+    if (root.leaf_count_depth & 0xFFFFu) return hit_triangles(s.tri_hit + m.tri_base, 0, m.tri_count, lf, r, hit, any_hit, tr);
     uint32_t pair = root.first_index, top = 0;
     for (;;) {
         SLIM_COUNT(tr, pairs);
@@ -136,7 +146,7 @@ SLIM_HD bool mesh_trace(const SceneView &s, uint32_t mesh_id, const Ray &r, Hit 
         if (hl) {
             uint32_t lc = L.leaf_count_depth & 0xFFFFu;
             if (lc) {
-                if (hit_triangles(s, m, L.first_index, lc, lf, r, hit, any_hit, tr)) {
+                if (hit_triangles(s.tri_hit + m.tri_base + L.first_index, L.first_index, lc, lf, r, hit, any_hit, tr)) {
                     found = true;
                     if (any_hit) break;
                 }
@@ -145,7 +155,7 @@ SLIM_HD bool mesh_trace(const SceneView &s, uint32_t mesh_id, const Ray &r, Hit 
         if (hr) {
             uint32_t rc = R.leaf_count_depth & 0xFFFFu;
             if (rc) {
-                if (hit_triangles(s, m, R.first_index, rc, rf, r, hit, any_hit, tr)) {
+                if (hit_triangles(s.tri_hit + m.tri_base + R.first_index, R.first_index, rc, rf, r, hit, any_hit, tr)) {
                     found = true;
                     if (any_hit) break;
                 }
@@ -209,7 +219,7 @@ SLIM_HD bool mesh_trace4(const SceneView &s, const MeshRec &m, const Ray &r, Hit
             if (!(key[k] < hit.distance) || count == 0 || stop) continue;
             const uint32_t child = i == 0 ? n.child[0] : i == 1 ? n.child[1] : i == 2 ? n.child[2] : n.child[3];
             const float far_d = i == 0 ? fd[0] : i == 1 ? fd[1] : i == 2 ? fd[2] : fd[3];
-            if (hit_triangles(s, m, child, count, far_d, r, hit, any_hit, tr)) {
+            if (hit_triangles(s.tri_hit + m.tri_base + child, child, count, far_d, r, hit, any_hit, tr)) {
                 found = true;
                 if (any_hit) stop = true;
             }
@@ -222,6 +232,108 @@ SLIM_HD bool mesh_trace4(const SceneView &s, const MeshRec &m, const Ray &r, Hit
             if (!(key[k] < hit.distance) || count != 0) continue;
             const uint32_t child = i == 0 ? n.child[0] : i == 1 ? n.child[1] : i == 2 ? n.child[2] : n.child[3];
             if (top < tr.mesh_depth) tr.mesh_stack[top++] = (typename TR::MeshEntry)child; else tr.status |= kStatusStack;
+        }
+        if (top == 0) break;
+        cur = tr.mesh_stack[--top];
+    }
+    if (found && !any_hit) mesh_hit_attributes(s, m, hit, tr);
+    return found;
+}
+
+// Byte k of a pair of words (k = 0..7; the words hold slots 0-3 and 4-7).
+SLIM_HD uint32_t slot_byte(const uint32_t w[2], uint32_t k) { return (w[k >> 2] >> (8 * (k & 3u))) & 0xFFu; }
+// Word k of eight, for a k known only at run time (a select chain instead of indexed registers).
+SLIM_HD uint32_t pick8(const uint32_t v[8], uint32_t k) {
+    const uint32_t a = (k & 1u) ? v[1] : v[0], b = (k & 1u) ? v[3] : v[2], c = (k & 1u) ? v[5] : v[4],
+                   d = (k & 1u) ? v[7] : v[6];
+    const uint32_t ab = (k & 2u) ? b : a, cd = (k & 2u) ? d : c;
+    return (k & 4u) ? cd : ab;
+}
+SLIM_HD float exp_step(uint32_t biased) {   // 2^(biased - 127) for biased in 1..254
+    const uint32_t bits = biased << 23;
+    float f;
+    __builtin_memcpy(&f, &bits, 4);
+    return f;
+}
+
+// The 8-wide walk of a mesh (Node8): every visit decodes and slab-tests the (up to) eight quantised child boxes,
+// takes the hit leaves in the ray's octant order (each repeats the reference's exact leaf-box test from its leaf
+// record, then tests the leaf's triangles, whose records follow it), and pushes the hit inner children so that the
+// first in octant order is popped next.  The octant order stands in for sorting by entry distance (slots were
+// filled at the collapse so that, for each ray octant, the children nearer the ray's entry side come first); the
+// closest hit does not depend on the order, only exact ties between triangles can resolve differently.
+template <class TR>
+SLIM_HD bool mesh_trace8(const SceneView &s, const MeshRec &m, const Ray &r, Hit &hit, bool any_hit, TR &tr) {
+    const Node8 *nodes = s.mnodes8 + m.node8_base;
+    const uint4 *leafs = s.leaf8 + m.leaf8_base;
+    const uint32_t mo = 7u ^ (r.sl | (r.sb << 1) | (r.sk << 2));   // slot s is visited at position s ^ mo
+    bool found = false;
+    uint32_t cur = 0, top = 0;
+    SLIM_COUNT(tr, roots);
+    for (;;) {
+        SLIM_COUNT(tr, pairs);
+        const Node8 n = nodes[cur];
+        const float sx = exp_step(n.exps & 0xFFu), sy = exp_step((n.exps >> 8) & 0xFFu),
+                    sz = exp_step((n.exps >> 16) & 0xFFu);
+        // the near and far planes of every slot for this ray's octant (the slab test's swap, once per node)
+        uint32_t nx[2], ny[2], nz[2], fx[2], fy[2], fz[2];
+#pragma unroll
+        for (uint32_t w = 0; w < 2; w++) {
+            nx[w] = r.sl ? n.qhi[0][w] : n.qlo[0][w]; fx[w] = r.sl ? n.qlo[0][w] : n.qhi[0][w];
+            ny[w] = r.sb ? n.qhi[1][w] : n.qlo[1][w]; fy[w] = r.sb ? n.qlo[1][w] : n.qhi[1][w];
+            nz[w] = r.sk ? n.qhi[2][w] : n.qlo[2][w]; fz[w] = r.sk ? n.qlo[2][w] : n.qhi[2][w];
+        }
+        uint32_t hits = 0, leaves = 0;   // bit per position
+#pragma unroll
+        for (uint32_t k = 0; k < 8; k++) {
+            const uint32_t cnt = slot_byte(n.count, k);
+            if (cnt == 0xFFu) continue;
+            // decoded bounds two ulps beyond the exact ones (checked at the collapse), so the fused slab distances
+            // bracket the reference's rounded ones on the exact box: a ray the exact box accepts is accepted
+            const float tnx = fmaf(fmaf((float)slot_byte(nx, k), sx, n.ox), r.rcp.x, r.scaled_origin.x);
+            const float tny = fmaf(fmaf((float)slot_byte(ny, k), sy, n.oy), r.rcp.y, r.scaled_origin.y);
+            const float tnz = fmaf(fmaf((float)slot_byte(nz, k), sz, n.oz), r.rcp.z, r.scaled_origin.z);
+            const float tfx = fmaf(fmaf((float)slot_byte(fx, k), sx, n.ox), r.rcp.x, r.scaled_origin.x);
+            const float tfy = fmaf(fmaf((float)slot_byte(fy, k), sy, n.oy), r.rcp.y, r.scaled_origin.y);
+            const float tfz = fmaf(fmaf((float)slot_byte(fz, k), sz, n.oz), r.rcp.z, r.scaled_origin.z);
+            const float nd = fmax2(0, fmax2(tnx, fmax2(tny, tnz))), fd = fmin2(tfx, fmin2(tfy, tfz));
+            if (nd <= fd && nd < hit.distance) {
+                hits |= 1u << (k ^ mo);
+                if (cnt) leaves |= 1u << (k ^ mo);
+            }
+        }
+        bool stop = false;
+        for (uint32_t lm = leaves; lm && !stop; lm &= lm - 1u) {   // hit leaves, first position first
+            const uint32_t k = (uint32_t)__builtin_ctz(lm) ^ mo;
+            // the leaf record and its first two triangle records in one round trip (the leaf array ends with a
+            // record's worth of padding, so the second read stays inside it for one-triangle leaves)
+            const uint4 *rec = leafs + pick8(n.child, k);
+            LeafHead h;
+            TriHit tb[2];
+            __builtin_memcpy(&h, rec, sizeof h);
+            __builtin_memcpy(tb, rec + 2, sizeof tb);
+            float nd, fd;
+            if (!(hits_box_bounds(r, h.minx, h.miny, h.minz, h.maxx, h.maxy, h.maxz, nd, fd) && nd < hit.distance))
+                continue;
+            float closest = fmin2(fd, hit.distance);
+            Hit th = hit;
+            bool got = false;
+            stop = test_triangles(tb, h.count, h.first, closest, r, hit, th, any_hit, got, tr);
+            const TriHit *rest = reinterpret_cast<const TriHit *>(rec + 2);
+            for (uint32_t i0 = 2; i0 < h.count && !stop; i0 += 2) {
+#pragma unroll
+                for (uint32_t j = 0; j < 2; j++)
+                    if (i0 + j < h.count) tb[j] = rest[i0 + j];
+                stop = test_triangles(tb, h.count - i0, h.first + i0, closest, r, hit, th, any_hit, got, tr);
+            }
+            found = found || got;
+        }
+        if (stop) break;
+        for (uint32_t im = hits & ~leaves; im;) {   // inner children, last position first: the first ends on top
+            const uint32_t p = 31u - (uint32_t)__builtin_clz(im);
+            im &= ~(1u << p);
+            if (top < tr.mesh_depth) tr.mesh_stack[top++] = (typename TR::MeshEntry)pick8(n.child, p ^ mo);
+            else tr.status |= kStatusStack;
         }
         if (top == 0) break;
         cur = tr.mesh_stack[--top];

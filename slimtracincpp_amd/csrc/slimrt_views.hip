@@ -191,8 +191,8 @@ void views_kernel(SceneView s, const ViewRec *__restrict__ views, uint32_t nview
     extern __shared__ uint32_t lds[];
     __shared__ uint32_t item;
     const uint32_t lane = threadIdx.x;
-    TracerT<WIDE> tr;
-    bind_lds_stacks<WIDE, 64>(tr, lds, lane, scene_depth, mesh_depth);
+    TracerT<WIDE ? 4u : 2u> tr;
+    bind_lds_stacks<WIDE ? 4u : 2u, 64>(tr, lds, lane, scene_depth, mesh_depth);
     tr.cnt = nullptr;
     if constexpr (PERSISTENT) {
         for (;;) {
@@ -331,15 +331,17 @@ int enqueue_views(slimrt_views *v) {
     const uint32_t blocks = persistent
         ? std::max(1u, std::min((uint32_t)cus * 4 * v->cfg.blocks_per_cu, (v->total_tiles + grab - 1) / grab))
         : v->total_tiles;
+    for (const slimrt_render_params &p : v->params)
+        if (p.flags & SLIMRT_BVH8) return vfail(v, SLIMRT_E_ARG, "views take node width 2 or 4, not SLIMRT_BVH8");
     bool wide = v->cfg.node_width == 4;
     if (v->cfg.node_width == 0) {   // as the views ask: the 4-wide mesh walk when every view has SLIMRT_BVH4
         wide = true;
         for (const slimrt_render_params &p : v->params) wide = wide && (p.flags & SLIMRT_BVH4) != 0;
     }
-    const uint32_t scene_depth = v->scene_cap + 1, mesh_depth = mesh_stack_depth(c, wide);
-    const size_t lds = stack_lds_bytes(scene_depth, mesh_depth, wide, 64);
+    const uint32_t scene_depth = v->scene_cap + 1, mesh_depth = mesh_stack_depth(c, wide ? 4u : 2u);
+    const size_t lds = stack_lds_bytes(scene_depth, mesh_depth, wide ? 4u : 2u, 64);
     if (lds + 16 > 64 * 1024) return vfail(v, SLIMRT_E_ARG, "BVH too deep for the LDS traversal stacks");
-    if (wide && check_wide(c)) return vfail(v, SLIMRT_E_ARG, c->err);
+    if (wide && check_width(c, 4)) return vfail(v, SLIMRT_E_ARG, c->err);
     const SceneView sv = device_view(c);
     const uint32_t n = (uint32_t)v->recs.size();
 #define VIEWS_LAUNCH1(WX, WY, P, W)                                                                           \
@@ -599,8 +601,11 @@ int slimrt_render_autotune(slimrt_context *c, const slimrt_render_params *params
     if (!c || !params || !best_flags || reps == 0) return SLIMRT_E_ARG;
     const uint32_t base = params->flags & ~(uint32_t)(SLIMRT_WAVEFRONT | SLIMRT_WAVEFRONT_PER_BOUNCE |
                                                       SLIMRT_WAVEFRONT_SORT | SLIMRT_WAVEFRONT_BANDS(3) |
-                                                      SLIMRT_WAVEFRONT_SPLIT | SLIMRT_WAVEFRONT_COUNT | SLIMRT_BVH4);
+                                                      SLIMRT_WAVEFRONT_SPLIT | SLIMRT_WAVEFRONT_COUNT | SLIMRT_BVH4 |
+                                                      SLIMRT_BVH8);
     std::vector<uint32_t> forms;
+    // node width 8: the per-sample frame only, on meshes the 8-wide walk can hold
+    if (c->mesh_nodes8_max <= 65536 && c->mesh_leaf_max <= 254) forms.push_back(SLIMRT_BVH8);
     for (uint32_t width : {0u, (uint32_t)SLIMRT_BVH4}) {   // node width 2 or 4
         if (width && c->mesh_nodes4_max > 65536) continue;   // meshes too large for the 4-wide walk
         forms.push_back(width);

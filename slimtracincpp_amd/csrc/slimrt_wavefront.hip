@@ -187,7 +187,7 @@ __global__ __launch_bounds__(kBlock) void wf_primary_kernel(SceneView s, RenderP
     const uint32_t x = (tile % tiles_x) * kTile + (wave & 1u) * 8u + (lane & 7u);
     const uint32_t y = (tile / tiles_x) * kTile + (wave >> 1) * 8u + (lane >> 3);
     const uint32_t ws = rp.width << rp.ssaa, hs = rp.height << rp.ssaa;
-    TracerT<WIDE> tr;
+    TracerT<WIDE ? 4u : 2u> tr;
     bind_lds_stacks(tr, lds, t, scene_depth, mesh_depth);
     tr.cnt = nullptr;
     const bool inside = x < ws && y < hs;
@@ -237,7 +237,7 @@ void wf_trace_kernel(SceneView s, RenderParams rp, HitSlots hits, uint32_t scene
     const uint32_t x = (tile % tiles_x) * kTile + (wave & 1u) * 8u + (lane & 7u);
     const uint32_t y = (tile / tiles_x) * kTile + (wave >> 1) * 8u + (lane >> 3);
     const uint32_t ws = rp.width << rp.ssaa, hs = rp.height << rp.ssaa;
-    TracerT<WIDE> tr;
+    TracerT<WIDE ? 4u : 2u> tr;
     bind_lds_stacks(tr, lds, t, scene_depth, mesh_depth);
     tr.cnt = nullptr;
     if (x < ws && y < hs && rp.max_depth) {
@@ -266,7 +266,7 @@ void wf_bounce_kernel(SceneView s, RenderParams rp, float4 *__restrict__ pixels,
                       uint32_t *__restrict__ stats) {
     extern __shared__ uint32_t lds[];
     const uint32_t t = threadIdx.x;
-    TracerT<WIDE> tr;
+    TracerT<WIDE ? 4u : 2u> tr;
     bind_lds_stacks(tr, lds, t, scene_depth, mesh_depth);
     tr.cnt = nullptr;
     const uint32_t n = min(*in_count, cap);
@@ -427,10 +427,10 @@ int launch_wavefront(slimrt_context *c, const RenderParams &rp, float4 *pixels, 
         c->wf_hits_cap = slots;
     }
     if (!c->wf_counters) HIP_TRY(c, ctx_malloc(c, &c->wf_counters, 8 * WF_COUNTERS * sizeof(uint32_t)));
-    const uint32_t scene_depth = c->scene_height + 1, mesh_depth = mesh_stack_depth(c, wide_mesh_bvh(rp));
-    const size_t lds = stack_lds_bytes(scene_depth, mesh_depth, wide_mesh_bvh(rp));
+    const uint32_t scene_depth = c->scene_height + 1, mesh_depth = mesh_stack_depth(c, wide_mesh_bvh(rp) ? 4u : 2u);
+    const size_t lds = stack_lds_bytes(scene_depth, mesh_depth, wide_mesh_bvh(rp) ? 4u : 2u);
     if (lds > 160 * 1024) return fail(c, SLIMRT_E_ARG, "BVH too deep for the LDS traversal stacks");
-    if (wide_mesh_bvh(rp) && check_wide(c)) return SLIMRT_E_ARG;
+    if (wide_mesh_bvh(rp) && check_width(c, 4)) return SLIMRT_E_ARG;
     int cus = 0;
     HIP_TRY(c, hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, c->device));
     const SceneView v = device_view(c);

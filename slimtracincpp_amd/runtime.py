@@ -101,6 +101,7 @@ class ViewsConfig(C.Structure):
 SCENE_REFIT, SCENE_REBUILD = 0, 1
 WAVEFRONT, WAVEFRONT_PER_BOUNCE, WAVEFRONT_SORT, BVH4, WAVEFRONT_COUNT, WAVEFRONT_SPLIT = 2, 4, 8, 64, 128, 256   # flags
 TILE_ORDER_FIXED = 512   # per-sample frames: tiles in image order instead of costliest first
+BVH8 = 1024   # per-sample frames and the CPU context: 8-wide mesh BVHs with quantised child boxes
 
 
 def WAVEFRONT_BANDS(log2n: int) -> int:
@@ -154,6 +155,7 @@ def lib():
             "slimrt_views_autotune": [vp, u32, P(ViewsConfig), P(f32)],
             "slimrt_render_autotune": [vp, P(RenderParams), vp, vp, u32, P(u32), P(f32)],
             "slimrt_host_collapse_bvh4": [vp, u32, u32, vp, u32, P(u32), P(u32), P(u32)],
+            "slimrt_host_collapse_bvh8": [vp, u32, vp, u32, vp, u32, P(u32), vp, u32, P(u32), P(u32)],
             "slimrt_wavefront_counts": [vp, vp, u32],
             "slimrt_tile_costs": [vp, vp, vp, u32, P(u32), P(u32)],
             "slimrt_pick": [vp, P(RenderParams), i32, i32, P(PickResult)],
@@ -274,6 +276,28 @@ def collapse_bvh4(nodes: np.ndarray, triangle_count: int, with_stack: bool = Fal
     _check(lib().slimrt_host_collapse_bvh4(_ptr(a), len(a), triangle_count, _ptr(out), len(out), C.byref(cnt),
                                            C.byref(h), C.byref(st)), what="collapse_bvh4")
     return (out[:cnt.value], h.value, st.value) if with_stack else (out[:cnt.value], h.value)
+
+
+BVH8_NODE = np.dtype([("origin", "<f4", 3), ("exps", "<u4"), ("child", "<u4", 8), ("count", "u1", 8),
+                      ("qlo", "u1", (3, 8)), ("qhi", "u1", (3, 8)), ("pad", "<u4", 6)])
+assert BVH8_NODE.itemsize == 128
+
+
+def collapse_bvh8(nodes: np.ndarray, triangles: np.ndarray):
+    """The 8-wide tree the renderer walks for a mesh BVH under BVH8 (slimrt_host_collapse_bvh8): nodes, leaf records
+    (as uint32 words, 4 per 16-byte unit) and the stack entries its walk can need."""
+    a = np.ascontiguousarray(nodes, BVH_NODE)
+    t = np.ascontiguousarray(triangles, TRIANGLE)
+    ncnt, lcnt, st = u32(), u32(), u32()
+    rc = lib().slimrt_host_collapse_bvh8(_ptr(a), len(a), _ptr(t), len(t), None, 0, C.byref(ncnt), None, 0,
+                                         C.byref(lcnt), C.byref(st))
+    if rc not in (0, -5):
+        _check(rc, what="collapse_bvh8")
+    out = np.zeros(ncnt.value, BVH8_NODE)
+    leaves = np.zeros(4 * lcnt.value, np.uint32)
+    _check(lib().slimrt_host_collapse_bvh8(_ptr(a), len(a), _ptr(t), len(t), _ptr(out), len(out), C.byref(ncnt),
+                                           _ptr(leaves), lcnt.value, C.byref(lcnt), C.byref(st)), what="collapse_bvh8")
+    return out, leaves, st.value
 
 
 class MeshBuild:

@@ -74,9 +74,10 @@ def test_cost_class_is_monotonic():
 
 
 @pytest.mark.gpu
-@pytest.mark.parametrize("flags", [runtime.BVH4, 0])
-def test_cost_order_renders_every_tile_once(gpu_available, flags):
-    spec = scenes.mesh_pbr_scene(0, 200, 120, level=4, texture_size=64, sky_size=16, irradiance_size=8)
+@pytest.mark.parametrize("flags,size", [(runtime.BVH4, (200, 120)), (0, (200, 120)),
+                                        (runtime.BVH8, (640, 360))])   # 8-wide: 3,600 tiles, 3 as quarter waves
+def test_cost_order_renders_every_tile_once(gpu_available, flags, size):
+    spec = scenes.mesh_pbr_scene(0, size[0], size[1], level=4, texture_size=64, sky_size=16, irradiance_size=8)
     r = runtime.Renderer(spec, "gpu")
     p = r.params()
     p.flags = flags | runtime.TILE_ORDER_FIXED

@@ -84,9 +84,10 @@ def test_render_autotune_picks_a_form_with_the_same_frame(gpu_available):
     px = torch.zeros((n, 4), dtype=torch.float32, device="cuda")
     dp = torch.zeros(n, dtype=torch.float32, device="cuda")
     flags, ms = r.autotune_form(p, px.data_ptr(), dp.data_ptr(), reps=2)
-    assert flags & ~(WAVEFRONT | PER_BOUNCE | SORT | (3 << 4) | SPLIT | runtime.BVH4) == 0 and ms > 0
+    assert flags & ~(WAVEFRONT | PER_BOUNCE | SORT | (3 << 4) | SPLIT | runtime.BVH4 | runtime.BVH8) == 0 and ms > 0
     assert flags & (PER_BOUNCE | SORT | (3 << 4) | SPLIT) == 0 or flags & WAVEFRONT
-    p.flags = flags & runtime.BVH4      # the megakernel with the same node width: the wavefront gives its bits
+    assert not (flags & runtime.BVH8 and flags & WAVEFRONT)   # node width 8 is a megakernel form only
+    p.flags = flags & (runtime.BVH4 | runtime.BVH8)   # the megakernel with the same node width: the same bits
     ref = _frame(r, p)
     p.flags = flags
     got = _frame(r, p)

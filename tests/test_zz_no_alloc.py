@@ -1,5 +1,5 @@
 """slimrt_render_device_noalloc (the frame entry point that allocates nothing) against slimrt_render_device,
-through the C ABI: the same samples bit for bit, call after call, in both node widths and both tile orders, when
+through the C ABI: the same samples bit for bit, call after call, in all three node widths and both tile orders, when
 the frame size shrinks between calls (the recorded durations then belong to another tile count), when another
 context renders in between (the scratch array is shared per GPU), and with every sample written (canvases filled
 with NaN before each call).  Frames beyond the scratch's 131072 tiles and wavefront forms are rejected."""
@@ -42,7 +42,7 @@ def _params(r, size, flags):
     return p
 
 
-@pytest.mark.parametrize("flags", [runtime.BVH4, 0, runtime.BVH4 | runtime.TILE_ORDER_FIXED])
+@pytest.mark.parametrize("flags", [runtime.BVH4, 0, runtime.BVH4 | runtime.TILE_ORDER_FIXED, runtime.BVH8])
 def test_noalloc_equals_render_device(gpu_available, flags):
     spec = scenes.mesh_pbr_scene(0, 640, 360, level=5, texture_size=128, sky_size=32, irradiance_size=8)
     r = runtime.Renderer(spec, "gpu")

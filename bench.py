@@ -571,7 +571,8 @@ def main():
                                 .get(form[-5:], "")) + {
                                 "megakernel": "render_kernel<false> (tiles dispatched costliest first; their order, from "
                                               "the tile durations of two frames before, is computed by "
-                                              "tile_order_kernel on a second stream while the previous frame renders)",
+                                              "order_count/plan/place_kernel on a second stream while the previous "
+                                              "frame renders)",
                                 "megakernel_image_order": "render_kernel<false> (tiles in image order)",
                                 "wavefront": "the frame's launches: wf_primary_kernel + one wf_bounce_kernel<true> "
                                              "over the compacted bounce queue",

@@ -10,7 +10,7 @@
 // PBR frame: a wave's canvas writes are two 512-byte rows).  In image order tile i is workgroup i: workgroups
 // are dealt round-robin to the 8 XCDs, so every XCD gets an even share of each image row band (band-per-XCD
 // dealing left whole XCDs on the costly bands and ran 18-20% slower); frames bound by their longest tiles
-// dispatch those first instead (tile_order_kernel).  Traversal stacks live in LDS, one column per thread, sized
+// dispatch those first instead (launch_tile_order).  Traversal stacks live in LDS, one column per thread, sized
 // from the uploaded BVH heights.
 #include <hip/hip_runtime.h>
 #include <string>
@@ -42,20 +42,27 @@ namespace {
 // image order they start a fifth of the way into the frame, so the frame ends with most of the chip idle while
 // they finish.  Dispatched first, they overlap the rest of the frame.  The cost of a tile is its duration two
 // frames earlier (a scene changes little from one frame to the next): the order for frame f + 1 is computed on a
-// second stream while frame f renders, from the durations frame f - 1 recorded, so it costs the frame nothing
-// (one workgroup takes ~80 us, on a CU that frees up during the frame's tail).
+// second stream while frame f renders, from the durations frame f - 1 recorded.
 //
-// One workgroup of 16 waves, each over a contiguous segment of tiles.  A tile's cost class is its duration's
-// binary exponent and the two bits below it (four classes an octave; rank 0 = costliest).  The costly tiles —
-// those within a factor 16 of the costliest class (narrowed 4 classes at a time while more than kHeavyMax), none
-// if even the costliest class alone has more — go first, by rank and within a rank in image order (a counting
-// sort over the at most 17 ranks of the window: one ballot per rank and 64-tile round); the others follow in
-// image order (ballot compaction).  Deterministic.
-constexpr uint32_t kOrderThreads = 1024, kOrderWaves = kOrderThreads / 64, kCostClasses = 128;
+// A tile's cost class is its duration's binary exponent and the two bits below it (four classes an octave; rank
+// 0 = costliest).  The costly tiles - those within a factor 16 of the costliest class (narrowed 4 classes at a
+// time while more than kHeavyMax), none if even the costliest class alone has more - go first, by rank and within
+// a rank in image order (a counting sort over the at most 17 ranks of the window); the others follow in image
+// order.  Three launches over waves that each own a contiguous segment of tiles (so image order within a class is
+// segment order): order_count_kernel ranks the tiles and counts them per class and wave, with each wave's
+// costliest rank, longest and summed durations; order_plan_kernel (one workgroup) picks the order, the window and
+// every wave's first slot per class and for its other tiles; order_place_kernel writes each tile to its slot.
+// About 10 us in all for a 1080p frame, so a stream that shares its hardware queue with the frame's (streams are
+// dealt to a few queues in turn) delays the next frame that little; the single-workgroup form took ~80 us.
+// Deterministic.
+constexpr uint32_t kCostClasses = 128;
 constexpr uint32_t kWindow = 16;              // costly window: ranks top .. top + 16 (a factor 16 in duration)
 constexpr uint32_t kHeavyMax = 4096;          // at most this many costly tiles
-constexpr uint32_t kOrderCached = 48 * 1024;  // tiles whose rank stays in LDS between passes
-constexpr uint32_t kOrderBatch = 16;          // rounds of loads in flight together
+constexpr uint32_t kOrderBlock = 256, kOrderMaxWaves = 256;   // order workgroups (4 waves) and waves at most
+// per-wave words of the order scratch: class counts, then [costliest rank, longest, summed lo, summed hi], then
+// the plan: first slot per window rank and the first slot of the other tiles
+constexpr uint32_t kOrderStats = kCostClasses, kOrderSlots = kCostClasses + 4, kOrderWaveWords = kCostClasses + 4 + kWindow + 2;
+constexpr uint32_t kOrderScratchWords = 8 + kOrderMaxWaves * kOrderWaveWords;   // [0..8): top, window, flag
 constexpr uint32_t kMaxOrderedTiles = 1u << 30;
 constexpr uint32_t kSplitShare = 32, kSplitMax = 2048;   // costliest tiles rendered as two half waves: 1 in 32
 // With 8-wide nodes the first 1 in 1024 tiles of the order (at most 128) are rendered as four quarter waves: the
@@ -74,52 +81,33 @@ __device__ __forceinline__ uint32_t cost_rank(uint32_t ticks) {
     return kCostClasses - 2u - ((e - 2u) * 4u + ((ticks >> (e - 2u)) & 3u));      // 7 .. 126, costliest lowest
 }
 
-// cost_tiles (may be null): the tile count the durations in cost were recorded for; any other count gives the
-// image order (the durations belong to another frame size).
-__global__ __launch_bounds__(kOrderThreads) void tile_order_kernel(const uint32_t *__restrict__ cost,
-                                                                   uint32_t *__restrict__ order, uint32_t n,
-                                                                   uint32_t resident,
-                                                                   const uint32_t *__restrict__ cost_tiles) {
-    __shared__ uint8_t ranks[kOrderCached];
-    __shared__ uint32_t wave_count[kOrderWaves][kWindow + 1];
-    __shared__ uint32_t min_rank, max_cost;
-    __shared__ unsigned long long wave_sum[kOrderWaves];
-    const uint32_t t = threadIdx.x, wave = t >> 6, lane = t & 63u;
-    const uint32_t rounds = (n + kOrderThreads - 1) / kOrderThreads;   // 64-tile rounds per wave segment
-    const uint32_t seg = wave * rounds * 64u;                          // the wave's first tile
-    const uint64_t below = (1ull << lane) - 1ull;
-    if (cost_tiles && *cost_tiles != n) {   // uniform: every thread reads the same word
-        for (uint32_t i = t; i < n; i += kOrderThreads) order[i] = i;
-        if (t == 0) order[n] = 0;
-        return;
-    }
-    auto rank_at = [&](uint32_t i) -> uint32_t { return i < kOrderCached ? (uint32_t)ranks[i] : cost_rank(cost[i]); };
-    if (t == 0) {
-        min_rank = kCostClasses;
-        max_cost = 0;
-    }
-    __syncthreads();
+// Waves over n tiles and the 64-tile rounds of each wave's segment.
+__host__ __device__ inline void order_shape(uint32_t n, uint32_t &waves, uint32_t &rounds) {
+    waves = n < 64u ? 1u : (n + 1023u) / 1024u * 4u;   // a workgroup of 4 waves per 1024 tiles...
+    if (waves > kOrderMaxWaves) waves = kOrderMaxWaves;  // ...at most 64 workgroups
+    rounds = (n + waves * 64u - 1u) / (waves * 64u);
+}
 
-    // pass 1: ranks (kept in LDS), the costliest rank, the longest and the summed durations
+__global__ __launch_bounds__(kOrderBlock) void order_count_kernel(const uint32_t *__restrict__ cost, uint32_t n,
+                                                                  uint32_t *__restrict__ scratch) {
+    __shared__ uint32_t hist[kOrderBlock / 64][kCostClasses];
+    uint32_t waves, rounds;
+    order_shape(n, waves, rounds);
+    const uint32_t w_local = threadIdx.x >> 6, lane = threadIdx.x & 63u, wave = blockIdx.x * (kOrderBlock / 64) + w_local;
+    for (uint32_t k = lane; k < kCostClasses; k += 64) hist[w_local][k] = 0;
+    __syncthreads();
+    const bool active = wave < waves;
     uint32_t lo = kCostClasses, longest = 0;
     uint64_t sum = 0;
-    for (uint32_t r0 = 0; r0 < rounds; r0 += kOrderBatch) {
-        uint32_t v[kOrderBatch];
-#pragma unroll
-        for (uint32_t j = 0; j < kOrderBatch; j++) {
-            const uint32_t i = seg + (r0 + j) * 64u + lane;
-            v[j] = (r0 + j < rounds && i < n) ? cost[i] : 0u;
-        }
-#pragma unroll
-        for (uint32_t j = 0; j < kOrderBatch; j++) {
-            const uint32_t i = seg + (r0 + j) * 64u + lane;
-            if (r0 + j < rounds && i < n) {
-                const uint32_t rk = cost_rank(v[j]);
-                if (i < kOrderCached) ranks[i] = (uint8_t)rk;
-                lo = min(lo, rk);
-                longest = max(longest, v[j]);
-                sum += v[j];
-            }
+    const uint32_t seg = wave * rounds * 64u;
+    for (uint32_t r = 0; r < rounds && active; r++) {
+        const uint32_t i = seg + r * 64u + lane;
+        if (i < n) {
+            const uint32_t v = cost[i], rk = cost_rank(v);
+            atomicAdd(&hist[w_local][rk], 1u);
+            lo = min(lo, rk);
+            longest = max(longest, v);
+            sum += v;
         }
     }
     for (int off = 32; off > 0; off >>= 1) {
@@ -127,51 +115,74 @@ __global__ __launch_bounds__(kOrderThreads) void tile_order_kernel(const uint32_
         longest = max(longest, (uint32_t)__shfl_xor((int)longest, off, 64));
         sum += (uint64_t)__shfl_xor((long long)sum, off, 64);
     }
+    __syncthreads();   // the histogram's atomics are done
+    if (!active) return;
+    uint32_t *out = scratch + 8 + wave * kOrderWaveWords;
+    for (uint32_t k = lane; k < kCostClasses; k += 64) out[k] = hist[w_local][k];
     if (lane == 0) {
-        atomicMin(&min_rank, lo);
-        atomicMax(&max_cost, longest);
-        wave_sum[wave] = sum;
+        out[kOrderStats] = lo;
+        out[kOrderStats + 1] = longest;
+        out[kOrderStats + 2] = (uint32_t)sum;
+        out[kOrderStats + 3] = (uint32_t)(sum >> 32);
+    }
+}
+
+// Sum over the workgroup (kOrderBlock threads), broadcast; red holds kOrderBlock / 64 words.
+__device__ __forceinline__ uint32_t block_sum(uint32_t v, uint32_t *red) {
+    for (int off = 32; off > 0; off >>= 1) v += (uint32_t)__shfl_xor((int)v, off, 64);
+    __syncthreads();
+    if ((threadIdx.x & 63u) == 0) red[threadIdx.x >> 6] = v;
+    __syncthreads();
+    uint32_t s = 0;
+    for (uint32_t w = 0; w < kOrderBlock / 64; w++) s += red[w];
+    return s;
+}
+
+// One workgroup, a thread per wave of the count.  cost_tiles (may be null): the tile count the durations were
+// recorded for; any other count gives the image order (the durations belong to another frame size).  Writes
+// order[n], the longest-first flag, and the plan: every wave's first slot per window rank (ranks in turn, waves in
+// turn within a rank) and for its other tiles (after every costly tile, waves in turn).
+__global__ __launch_bounds__(kOrderBlock) void order_plan_kernel(uint32_t n, uint32_t resident,
+                                                                 const uint32_t *__restrict__ cost_tiles,
+                                                                 uint32_t *__restrict__ scratch,
+                                                                 uint32_t *__restrict__ order) {
+    __shared__ uint32_t red[kOrderBlock / 64], wave_tot[kOrderBlock / 64][kWindow + 2];
+    __shared__ uint32_t top_s, longest_s;
+    __shared__ unsigned long long summed_s;
+    uint32_t waves, rounds;
+    order_shape(n, waves, rounds);
+    const uint32_t t = threadIdx.x, lane = t & 63u, wid = t >> 6;
+    const bool mine = t < waves;   // order_shape keeps waves <= kOrderBlock
+    const uint32_t *st = scratch + 8 + t * kOrderWaveWords;
+    // the costliest rank, the longest and the summed durations
+    uint32_t lo = mine ? st[kOrderStats] : kCostClasses, longest = mine ? st[kOrderStats + 1] : 0u;
+    unsigned long long sum = mine ? ((unsigned long long)st[kOrderStats + 2] | ((unsigned long long)st[kOrderStats + 3] << 32)) : 0ull;
+    for (int off = 32; off > 0; off >>= 1) {
+        lo = min(lo, (uint32_t)__shfl_xor((int)lo, off, 64));
+        longest = max(longest, (uint32_t)__shfl_xor((int)longest, off, 64));
+        sum += (unsigned long long)__shfl_xor((long long)sum, off, 64);
+    }
+    if (t == 0) { top_s = kCostClasses; longest_s = 0; summed_s = 0; }
+    __syncthreads();
+    if (lane == 0) {
+        atomicMin(&top_s, lo);
+        atomicMax(&longest_s, longest);
+        atomicAdd(&summed_s, sum);
     }
     __syncthreads();
-    const uint32_t top = min_rank;
-
+    const uint32_t top = top_s;
     // Longest first only pays when the frame is bound by its longest tiles: when the longest tile lasts at least
     // 3/4 of the frame's ideal length (summed durations over the resident waves).  Otherwise (large frames whose
     // work fills the chip for longer than any one tile) the image order keeps neighbouring tiles together, which
     // keeps what they share in L2 (4K combined workload: 3.81 ms in image order, 4.14 ms in cost order).
-    uint64_t summed = 0;
-    for (uint32_t w = 0; w < kOrderWaves; w++) summed += wave_sum[w];
-    if (4ull * max_cost * resident <= 3ull * summed) {
-        for (uint32_t i = t; i < n; i += kOrderThreads) order[i] = i;
-        if (t == 0) order[n] = 0;   // no split of the first tiles either
-        return;
-    }
-    if (t == 0) order[n] = 1;
-
-    // pass 2: per wave, the tiles of each rank of the window
-    uint32_t cnt[kWindow + 1];
+    const bool image = (cost_tiles && *cost_tiles != n) || 4ull * (unsigned long long)longest_s * resident <= 3ull * summed_s;
+    // this wave's tiles per window rank, the window's totals and width
+    uint32_t c[kWindow + 1], total[kWindow + 1];
 #pragma unroll
-    for (uint32_t k = 0; k <= kWindow; k++) cnt[k] = 0;
-    for (uint32_t r = 0; r < rounds; r++) {
-        const uint32_t i = seg + r * 64u + lane;
-        const uint32_t d = i < n ? rank_at(i) - top : 0xFFu;
+    for (uint32_t k = 0; k <= kWindow; k++) c[k] = (mine && top + k < kCostClasses) ? st[top + k] : 0u;
 #pragma unroll
-        for (uint32_t k = 0; k <= kWindow; k++) cnt[k] += (uint32_t)__popcll(__ballot(d == k));
-    }
-    if (lane == 0) {
-#pragma unroll
-        for (uint32_t k = 0; k <= kWindow; k++) wave_count[wave][k] = cnt[k];
-    }
-    __syncthreads();
-
-    // the window (every thread the same from LDS): widest of 16, 12, 8, 4, 0 ranks with <= kHeavyMax tiles
-    uint32_t total[kWindow + 1];
-#pragma unroll
-    for (uint32_t k = 0; k <= kWindow; k++) {
-        total[k] = 0;
-        for (uint32_t w = 0; w < kOrderWaves; w++) total[k] += wave_count[w][k];
-    }
-    int window = -1;
+    for (uint32_t k = 0; k <= kWindow; k++) total[k] = block_sum(c[k], red);
+    int window = -1;   // the widest of 16, 12, 8, 4, 0 ranks with <= kHeavyMax tiles (the same in every thread)
     uint32_t heavy = 0;
     for (int w = (int)kWindow; w >= 0; w -= 4) {
         uint32_t h = 0;
@@ -182,25 +193,75 @@ __global__ __launch_bounds__(kOrderThreads) void tile_order_kernel(const uint32_
             break;
         }
     }
-    // this wave's first slot per rank (costly) and for its other tiles
-    uint32_t slot[kWindow + 1], run = 0, lb = heavy;
+    if (t == 0) {
+        scratch[0] = top;
+        scratch[1] = (uint32_t)window;
+        scratch[2] = image ? 0u : 1u;
+        order[n] = image ? 0u : 1u;
+    }
+    if (image) return;   // uniform, after the last barrier
+    // exclusive scans over the waves of the window ranks' counts and of the other tiles' counts
+    const uint32_t first = t * rounds * 64u, count = (mine && first < n) ? min(n - first, rounds * 64u) : 0u;
+    uint32_t v[kWindow + 2], hv = 0;
 #pragma unroll
     for (uint32_t k = 0; k <= kWindow; k++) {
-        slot[k] = run;
-        for (uint32_t w = 0; w < wave; w++) slot[k] += wave_count[w][k];
-        run += (int)k <= window ? total[k] : 0u;
+        v[k] = (int)k <= window ? c[k] : 0u;
+        hv += v[k];
     }
-    for (uint32_t w = 0; w < wave; w++) {
-        const uint32_t first = w * rounds * 64u, count = first < n ? min(n - first, rounds * 64u) : 0u;
-        uint32_t hw = 0;
-        for (int k = 0; k <= window; k++) hw += wave_count[w][k];
-        lb += count - hw;
+    v[kWindow + 1] = count - hv;
+#pragma unroll
+    for (uint32_t k = 0; k <= kWindow + 1; k++) {   // inclusive within the wave
+        for (uint32_t off = 1; off < 64; off <<= 1) {
+            const uint32_t u = (uint32_t)__shfl_up((int)v[k], off, 64);
+            if (lane >= off) v[k] += u;
+        }
+        if (lane == 63) wave_tot[wid][k] = v[k];
     }
+    __syncthreads();
+    if (!mine) return;
+    uint32_t *slot = scratch + 8 + t * kOrderWaveWords + kOrderSlots;
+    uint32_t base = 0;
+#pragma unroll
+    for (uint32_t k = 0; k <= kWindow + 1; k++) {
+        uint32_t before = 0;
+        for (uint32_t w = 0; w < wid; w++) before += wave_tot[w][k];
+        const uint32_t own = (k <= kWindow) ? ((int)k <= window ? c[k] : 0u) : count - hv;
+        const uint32_t excl = before + v[k] - own;
+        if (k <= kWindow) {
+            slot[k] = base + excl;
+            base += (int)k <= window ? total[k] : 0u;
+        } else {
+            slot[k] = heavy + excl;
+        }
+    }
+}
 
-    // pass 3: costly tiles by rank, the others in image order
+__global__ __launch_bounds__(kOrderBlock) void order_place_kernel(const uint32_t *__restrict__ cost, uint32_t n,
+                                                                  const uint32_t *__restrict__ scratch,
+                                                                  uint32_t *__restrict__ order) {
+    uint32_t waves, rounds;
+    order_shape(n, waves, rounds);
+    const uint32_t lane = threadIdx.x & 63u, wave = blockIdx.x * (kOrderBlock / 64) + (threadIdx.x >> 6);
+    if (wave >= waves) return;
+    const uint32_t seg = wave * rounds * 64u;
+    if (!scratch[2]) {   // image order
+        for (uint32_t r = 0; r < rounds; r++) {
+            const uint32_t i = seg + r * 64u + lane;
+            if (i < n) order[i] = i;
+        }
+        return;
+    }
+    const uint32_t top = scratch[0];
+    const int window = (int)scratch[1];
+    const uint32_t *sl = scratch + 8 + wave * kOrderWaveWords + kOrderSlots;
+    uint32_t slot[kWindow + 1];
+#pragma unroll
+    for (uint32_t k = 0; k <= kWindow; k++) slot[k] = sl[k];
+    uint32_t lb = sl[kWindow + 1];
+    const uint64_t below = (1ull << lane) - 1ull;
     for (uint32_t r = 0; r < rounds; r++) {
         const uint32_t i = seg + r * 64u + lane;
-        const uint32_t d = i < n ? rank_at(i) - top : 0xFFu;
+        const uint32_t d = i < n ? cost_rank(cost[i]) - top : 0xFFu;
         const bool is_heavy = (int)d <= window, is_light = i < n && !is_heavy;
         uint32_t pos = 0;
 #pragma unroll
@@ -218,10 +279,23 @@ __global__ __launch_bounds__(kOrderThreads) void tile_order_kernel(const uint32_
     }
 }
 
+// The three launches of a frame's order on one stream.
+void launch_tile_order(hipStream_t stream, const uint32_t *cost, uint32_t *order, uint32_t n, uint32_t resident,
+                       const uint32_t *cost_tiles, uint32_t *order_scratch) {
+    uint32_t waves, rounds;
+    order_shape(n, waves, rounds);
+    const uint32_t blocks = (waves + kOrderBlock / 64 - 1) / (kOrderBlock / 64);
+    hipLaunchKernelGGL(order_count_kernel, dim3(blocks), dim3(kOrderBlock), 0, stream, cost, n, order_scratch);
+    hipLaunchKernelGGL(order_plan_kernel, dim3(1), dim3(kOrderBlock), 0, stream, n, resident, cost_tiles,
+                       order_scratch, order);
+    hipLaunchKernelGGL(order_place_kernel, dim3(blocks), dim3(kOrderBlock), 0, stream, cost, n,
+                       (const uint32_t *)order_scratch, order);
+}
+
 // The 4-wide frame at three waves per SIMD (168 VGPRs, 17 spilled): the walk's dependent fetches want the waves
 // more than the registers (6% faster than two on the mesh + PBR frame).  The 2-wide frame would spill 64 dwords a
 // lane for no gain, and the counting pass keeps its registers: both stay at the compiler's two.  With an order
-// (tile_order_kernel) workgroup k renders tile order[k] and records the tile's duration in cost; when order[tiles]
+// (launch_tile_order) workgroup k renders tile order[k] and records the tile's duration in cost; when order[tiles]
 // is set, the first `split` tiles of the order (the costliest) are each rendered by two workgroups, lanes 0-31
 // and lanes 32-63: a costly tile's 64 paths diverge, and half as many lanes per wave shorten the frame's longest
 // waves (0.77 -> 0.65 ms on the mesh + PBR frame; quarter waves gain nothing and lose fast as their count grows).
@@ -416,13 +490,15 @@ namespace {
 // slimrt_render_device_noalloc keeps the tile durations its frames record and the dispatch order it computes from
 // them in one file-scope device array, for frames of up to kNoAllocTiles tiles (32 x 2 samples each).  Words:
 // [0] the tile count the recorded durations belong to, [1, 64) padding, then the durations, then the order
-// (kNoAllocTiles + 1 words: the order and the longest-first flag).  Zero-initialised, so a device's first such
+// (kNoAllocTiles + 1 words: the order and the longest-first flag), then the order computation's per-wave counts
+// and plan.  Zero-initialised, so a device's first such
 // frame takes the image order.  Every GPU holds its own copy until the process ends, shared by every context on
 // that GPU.  In host code the array's name is not a device address: the address is looked up once per device
 // (hipGetSymbolAddress, with that device current) and only that address reaches kernels.
 constexpr uint32_t kNoAllocTiles = 131072;   // 8,388,608 samples: 3840 x 2160, or 1920 x 1080 with SSAA
 constexpr uint32_t kNoAllocCost = 64, kNoAllocOrder = kNoAllocCost + kNoAllocTiles;
-constexpr uint32_t kNoAllocWords = kNoAllocOrder + kNoAllocTiles + 1;   // 1,048,836 bytes
+constexpr uint32_t kNoAllocOrderScratch = (kNoAllocOrder + kNoAllocTiles + 1 + 63) / 64 * 64;
+constexpr uint32_t kNoAllocWords = kNoAllocOrderScratch + kOrderScratchWords;   // 1,126,656 bytes
 static __device__ __align__(256) uint32_t scratch[kNoAllocWords];
 
 constexpr int kMaxDevices = 64;
@@ -449,6 +525,7 @@ int prepare_tile_order(slimrt_context *c, uint32_t tiles) {
         if (!c->tile_rendered[p]) HIP_TRY(c, hipEventCreateWithFlags(&c->tile_rendered[p], hipEventDisableTiming));
         if (!c->tile_ordered[p]) HIP_TRY(c, hipEventCreateWithFlags(&c->tile_ordered[p], hipEventDisableTiming));
     }
+    if (!c->order_scratch) HIP_TRY(c, ctx_malloc(c, &c->order_scratch, kOrderScratchWords * sizeof(uint32_t)));
     if (c->tile_cap >= tiles) return SLIMRT_OK;
     HIP_TRY(c, hipStreamSynchronize(c->side_stream));
     HIP_TRY(c, hipStreamSynchronize(c->stream));
@@ -553,8 +630,8 @@ int launch(slimrt_context *c, const slimrt_render_params *p, float4 *pixels, flo
                 HIP_TRY(c, hipStreamWaitEvent(c->side_stream, c->tile_rendered[nxt], 0));
                 uint32_t resident = 0;
                 if (int rc = resident_waves(c, width, scene_has_lights(v), wlds, resident)) return rc;
-                hipLaunchKernelGGL(tile_order_kernel, dim3(1), dim3(kOrderThreads), 0, c->side_stream,
-                                   c->tile_cost[nxt], c->tile_order[nxt], wtiles, resident, nullptr);
+                launch_tile_order(c->side_stream, c->tile_cost[nxt], c->tile_order[nxt], wtiles, resident, nullptr,
+                                  c->order_scratch);
                 HIP_TRY(c, hipGetLastError());
                 HIP_TRY(c, hipEventRecord(c->tile_ordered[nxt], c->side_stream));
                 c->order_launched[nxt] = true;
@@ -1009,6 +1086,7 @@ int slimrt_destroy(slimrt_context *c) {
             if (c->tile_rendered[p]) (void)hipEventDestroy(c->tile_rendered[p]);
             if (c->tile_ordered[p]) (void)hipEventDestroy(c->tile_ordered[p]);
         }
+        if (c->order_scratch) (void)ctx_free(c, c->order_scratch);
         if (c->ev0) (void)hipEventDestroy(c->ev0);
         if (c->ev1) (void)hipEventDestroy(c->ev1);
         if (c->side_stream) (void)hipStreamDestroy(c->side_stream);
@@ -1277,8 +1355,7 @@ int slimrt_render_device_noalloc(slimrt_context *c, const slimrt_render_params *
     // On the caller's stream, right before the frame: the order from the durations the last such frame on this
     // GPU recorded (the image order when they belong to another tile count); one stream, so a graph can hold it.
     if (by_cost) {
-        hipLaunchKernelGGL(tile_order_kernel, dim3(1), dim3(kOrderThreads), 0, c->stream, cost, order, wtiles,
-                           resident, cost_tiles);
+        launch_tile_order(c->stream, cost, order, wtiles, resident, cost_tiles, c->noalloc_scratch + kNoAllocOrderScratch);
         HIP_TRY(c, hipGetLastError());
     }
     const uint32_t split = by_cost ? std::min(wtiles / kSplitShare, kSplitMax) : 0u;

@@ -96,6 +96,7 @@ struct slimrt_context {
     // tile_order[f & 1] (computed on side_stream during frame f - 1 from the costs of frame f - 2) and writes its
     // tiles' durations (wall-clock ticks) to tile_cost[f & 1].  *_tiles: the tile count a buffer holds (0: none).
     uint32_t *tile_cost[2] = {}, *tile_order[2] = {};
+    uint32_t *order_scratch = nullptr;          // the order computation's per-wave counts and plan (side stream)
     uint32_t tile_cap = 0, cost_tiles[2] = {}, order_tiles[2] = {};
     bool order_launched[2] = {}, order_used[2] = {};
     uint64_t tile_frame = 0;

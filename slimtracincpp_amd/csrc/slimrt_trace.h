@@ -286,10 +286,9 @@ SLIM_HD bool mesh_trace8(const SceneView &s, const MeshRec &m, const Ray &r, Hit
         uint32_t hits = 0, leaves = 0;   // bit per position
 #pragma unroll
         for (uint32_t k = 0; k < 8; k++) {
-            const uint32_t cnt = slot_byte(n.count, k);
-            if (cnt == 0xFFu) continue;
-            // decoded bounds two ulps beyond the exact ones (checked at the collapse), so the fused slab distances
-            // bracket the reference's rounded ones on the exact box: a ray the exact box accepts is accepted
+            // Every slot without a branch (an empty slot, count 255, is masked out).  Decoded bounds lie two ulps
+            // beyond the exact ones (checked at the collapse), so the fused slab distances bracket the reference's
+            // rounded ones on the exact box: a ray the exact box accepts is accepted.
             const float tnx = fmaf(fmaf((float)slot_byte(nx, k), sx, n.ox), r.rcp.x, r.scaled_origin.x);
             const float tny = fmaf(fmaf((float)slot_byte(ny, k), sy, n.oy), r.rcp.y, r.scaled_origin.y);
             const float tnz = fmaf(fmaf((float)slot_byte(nz, k), sz, n.oz), r.rcp.z, r.scaled_origin.z);
@@ -297,10 +296,10 @@ SLIM_HD bool mesh_trace8(const SceneView &s, const MeshRec &m, const Ray &r, Hit
             const float tfy = fmaf(fmaf((float)slot_byte(fy, k), sy, n.oy), r.rcp.y, r.scaled_origin.y);
             const float tfz = fmaf(fmaf((float)slot_byte(fz, k), sz, n.oz), r.rcp.z, r.scaled_origin.z);
             const float nd = fmax2(0, fmax2(tnx, fmax2(tny, tnz))), fd = fmin2(tfx, fmin2(tfy, tfz));
-            if (nd <= fd && nd < hit.distance) {
-                hits |= 1u << (k ^ mo);
-                if (cnt) leaves |= 1u << (k ^ mo);
-            }
+            const uint32_t cnt = slot_byte(n.count, k);
+            const bool h = nd <= fd && nd < hit.distance && cnt != 0xFFu;
+            hits |= (uint32_t)h << (k ^ mo);
+            leaves |= (uint32_t)(h && cnt != 0u) << (k ^ mo);
         }
         bool stop = false;
         for (uint32_t lm = leaves; lm && !stop; lm &= lm - 1u) {   // hit leaves, first position first

@@ -42,8 +42,9 @@ typedef struct slimrt_views slimrt_views;
  * workgroup i); persistent = 1: a grid of 4 * blocks_per_cu (1..4) workgroups per compute unit (blocks_per_cu
  * waves per SIMD) takes tiles_per_grab (1..8) tiles at a time from a device work counter until all views are
  * done.
- * node_width: the mesh BVH walk, 2 (sibling pairs) or 4 (the SLIMRT_BVH4 collapse), or 0 = as the views' flags
- * ask (4 when every view has SLIMRT_BVH4). */
+ * node_width: the mesh BVH walk, 2 (sibling pairs), 4 (the SLIMRT_BVH4 collapse) or 8 (the SLIMRT_BVH8 collapse),
+ * or 0 = as the views' flags ask (8 when every view has SLIMRT_BVH8, else 4 when every view has SLIMRT_BVH4 or
+ * SLIMRT_BVH8). */
 typedef struct slimrt_views_config {
     uint32_t tile_shape;
     uint32_t blocks_per_cu;
@@ -66,7 +67,7 @@ int slimrt_views_get_config(const slimrt_views *v, slimrt_views_config *cfg);
 int slimrt_views_frame(slimrt_views *v, const slimrt_geometry *geometries, uint32_t count, uint32_t scene_mode,
                        uint32_t use_graph);
 int slimrt_views_last_ms(slimrt_views *v, float *ms);
-/* Times every launch configuration (tile shape x grid or persistent form x node width 2 or 4) on eager frames
+/* Times every launch configuration (tile shape x grid or persistent form x node width 2, 4 or 8) on eager frames
  * (reps each, median) and keeps the fastest. */
 int slimrt_views_autotune(slimrt_views *v, uint32_t reps, slimrt_views_config *best, float *best_ms);
 const char *slimrt_views_last_error(const slimrt_views *v);

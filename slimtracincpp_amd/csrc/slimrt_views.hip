@@ -181,9 +181,9 @@ __device__ __forceinline__ void trace_view_tile(const SceneView &s, const ViewRe
 // PERSISTENT: a grid sized to the chip takes tiles from the work counter (grab at a time) until it passes the
 // total; otherwise one workgroup per tile, tile i on workgroup i (workgroups go round-robin to the 8 XCDs, so
 // each XCD gets an even share).
-// The 4-wide walk at three waves per SIMD, as the frame kernel (render_kernel).
-template <int WX, int WY, bool PERSISTENT, bool WIDE>
-__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(WIDE ? 3 : 1)))
+// The wide walks at three waves per SIMD, as the frame kernel (render_kernel).
+template <int WX, int WY, bool PERSISTENT, uint32_t NW>
+__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(NW >= 4 ? 3 : 1)))
 void views_kernel(SceneView s, const ViewRec *__restrict__ views, uint32_t nviews, uint32_t total_tiles, uint32_t grab,
                   uint32_t *__restrict__ counter, uint32_t scene_depth, uint32_t mesh_depth,
                   uint32_t *__restrict__ status) {
@@ -191,8 +191,8 @@ void views_kernel(SceneView s, const ViewRec *__restrict__ views, uint32_t nview
     extern __shared__ uint32_t lds[];
     __shared__ uint32_t item;
     const uint32_t lane = threadIdx.x;
-    TracerT<WIDE ? 4u : 2u> tr;
-    bind_lds_stacks<WIDE ? 4u : 2u, 64>(tr, lds, lane, scene_depth, mesh_depth);
+    TracerT<NW> tr;
+    bind_lds_stacks<NW, 64>(tr, lds, lane, scene_depth, mesh_depth);
     tr.cnt = nullptr;
     if constexpr (PERSISTENT) {
         for (;;) {
@@ -331,17 +331,19 @@ int enqueue_views(slimrt_views *v) {
     const uint32_t blocks = persistent
         ? std::max(1u, std::min((uint32_t)cus * 4 * v->cfg.blocks_per_cu, (v->total_tiles + grab - 1) / grab))
         : v->total_tiles;
-    for (const slimrt_render_params &p : v->params)
-        if (p.flags & SLIMRT_BVH8) return vfail(v, SLIMRT_E_ARG, "views take node width 2 or 4, not SLIMRT_BVH8");
-    bool wide = v->cfg.node_width == 4;
-    if (v->cfg.node_width == 0) {   // as the views ask: the 4-wide mesh walk when every view has SLIMRT_BVH4
-        wide = true;
-        for (const slimrt_render_params &p : v->params) wide = wide && (p.flags & SLIMRT_BVH4) != 0;
+    uint32_t width = v->cfg.node_width;
+    if (width == 0) {   // as the views ask: a wide walk when every view has its flag (SLIMRT_BVH8 before SLIMRT_BVH4)
+        bool all8 = true, all4 = true;
+        for (const slimrt_render_params &p : v->params) {
+            all8 = all8 && (p.flags & SLIMRT_BVH8) != 0;
+            all4 = all4 && (p.flags & (SLIMRT_BVH4 | SLIMRT_BVH8)) != 0;
+        }
+        width = all8 ? 8u : all4 ? 4u : 2u;
     }
-    const uint32_t scene_depth = v->scene_cap + 1, mesh_depth = mesh_stack_depth(c, wide ? 4u : 2u);
-    const size_t lds = stack_lds_bytes(scene_depth, mesh_depth, wide ? 4u : 2u, 64);
+    const uint32_t scene_depth = v->scene_cap + 1, mesh_depth = mesh_stack_depth(c, width);
+    const size_t lds = stack_lds_bytes(scene_depth, mesh_depth, width, 64);
     if (lds + 16 > 64 * 1024) return vfail(v, SLIMRT_E_ARG, "BVH too deep for the LDS traversal stacks");
-    if (wide && check_width(c, 4)) return vfail(v, SLIMRT_E_ARG, c->err);
+    if (check_width(c, width)) return vfail(v, SLIMRT_E_ARG, c->err);
     const SceneView sv = device_view(c);
     const uint32_t n = (uint32_t)v->recs.size();
 #define VIEWS_LAUNCH1(WX, WY, P, W)                                                                           \
@@ -350,9 +352,13 @@ int enqueue_views(slimrt_views *v) {
                        c->d_status)
 #define VIEWS_LAUNCH(WX, WY)                                                                                  \
     if (persistent) {                                                                                         \
-        if (wide) VIEWS_LAUNCH1(WX, WY, true, true); else VIEWS_LAUNCH1(WX, WY, true, false);                 \
+        if (width == 8) VIEWS_LAUNCH1(WX, WY, true, 8u);                                                      \
+        else if (width == 4) VIEWS_LAUNCH1(WX, WY, true, 4u);                                                 \
+        else VIEWS_LAUNCH1(WX, WY, true, 2u);                                                                 \
     } else {                                                                                                  \
-        if (wide) VIEWS_LAUNCH1(WX, WY, false, true); else VIEWS_LAUNCH1(WX, WY, false, false);               \
+        if (width == 8) VIEWS_LAUNCH1(WX, WY, false, 8u);                                                     \
+        else if (width == 4) VIEWS_LAUNCH1(WX, WY, false, 4u);                                                \
+        else VIEWS_LAUNCH1(WX, WY, false, 2u);                                                                \
     }
     switch (v->cfg.tile_shape) {
         case 0: VIEWS_LAUNCH(8, 8); break;
@@ -489,7 +495,7 @@ int slimrt_views_configure(slimrt_views *v, const slimrt_views_config *cfg) {
     if (!v || !cfg) return SLIMRT_E_ARG;
     if (cfg->tile_shape > 3 || cfg->blocks_per_cu < 1 || cfg->blocks_per_cu > 4 || cfg->tiles_per_grab < 1 ||
         cfg->tiles_per_grab > 8 || cfg->persistent > 1 || (cfg->node_width != 0 && cfg->node_width != 2 &&
-                                                            cfg->node_width != 4))
+                                                            cfg->node_width != 4 && cfg->node_width != 8))
         return vfail(v, SLIMRT_E_ARG, "views config out of range");
     VIEWS_TRY(v, hipStreamSynchronize(v->c->stream));
     v->cfg = *cfg;
@@ -561,8 +567,10 @@ int slimrt_views_autotune(slimrt_views *v, uint32_t reps, slimrt_views_config *b
     slimrt_views_config keep = v->cfg, win = v->cfg;
     float win_ms = INFINITY;
     std::vector<slimrt_views_config> cands;
-    for (uint32_t width : {2u, 4u})
-        for (uint32_t shape = 0; shape < 4 && (width == 2 || v->c->mesh_nodes4_max <= 65536); shape++) {
+    const bool fits8 = v->c->mesh_nodes8_max <= 65536 && v->c->mesh_leaf_max <= 254;
+    for (uint32_t width : {2u, 4u, 8u})
+        for (uint32_t shape = 0; shape < 4 && (width == 2 || (width == 4 && v->c->mesh_nodes4_max <= 65536) ||
+                                               (width == 8 && fits8)); shape++) {
             cands.push_back({shape, 1, 1, 0, width});
             for (uint32_t bpc = 1; bpc <= 2; bpc++)
                 for (uint32_t grab : {1u, 4u}) cands.push_back({shape, bpc, grab, 1, width});

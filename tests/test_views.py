@@ -134,22 +134,23 @@ def _views(spec, r, n=7, seed=3):
 
 @pytest.mark.gpu
 def test_views_equal_single_view_renders_in_every_launch_form(gpu_available):
-    """Node width 2 against single-view renders of the sibling-pair walk, width 4 against single-view renders with
-    SLIMRT_BVH4 (the same per-sample code: bit for bit either way)."""
+    """Node width 2 against single-view renders of the sibling-pair walk, widths 4 and 8 against single-view
+    renders with SLIMRT_BVH4 / SLIMRT_BVH8 (the same per-sample code: bit for bit either way)."""
     import torch
     spec = _scene()
     r = runtime.Renderer(spec, "gpu")
     params, bufs = _views(spec, r)
     refs = {2: [_render(r, p) for p in params]}
-    wide = []
-    for p in params:
-        q = runtime.RenderParams.from_buffer_copy(p)
-        q.flags |= runtime.BVH4
-        wide.append(_render(r, q))
-    refs[4] = wide
+    for width, flag in ((4, runtime.BVH4), (8, runtime.BVH8)):
+        wide = []
+        for p in params:
+            q = runtime.RenderParams.from_buffer_copy(p)
+            q.flags |= flag
+            wide.append(_render(r, q))
+        refs[width] = wide
     v = runtime.Views(r, params, [b[0].data_ptr() for b in bufs], [b[1].data_ptr() for b in bufs])
-    forms = [(shape, 1, 1, 0, w) for w in (2, 4) for shape in range(4)] + \
-            [(shape, bpc, grab, 1, w) for w in (2, 4) for shape in range(4) for bpc in (1, 2) for grab in (1, 4)]
+    forms = [(shape, 1, 1, 0, w) for w in (2, 4, 8) for shape in range(4)] + \
+            [(shape, bpc, grab, 1, w) for w in (2, 4, 8) for shape in range(4) for bpc in (1, 2) for grab in (1, 4)]
     for form in forms + [(0, 2, 1, 0, 0)]:               # node width 0: as the views' flags ask (2 here)
         for px, dp in bufs:
             px.zero_()

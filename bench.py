@@ -165,7 +165,7 @@ def neural_cache_leg(frames: int, width: int, height: int, stream, cache_depth: 
         p.flags = plain_flags
         r.render_device(p, plain.data_ptr(), depths.data_ptr())
         ms_plain.append(r.last_kernel_ms())
-        p.flags = runtime.BVH4    # the cached frame traces through the 4-wide mesh BVHs as well
+        p.flags = runtime.BVH8    # the cached frame traces through the 8-wide mesh BVHs as well
         cf.render_device(p, cached.data_ptr(), depths.data_ptr())
         ms_nrc.append(r.last_kernel_ms())
         d = (cached[:, :3] - plain[:, :3]).double()
@@ -400,8 +400,8 @@ def large_leg(stream, frames: int = 5) -> dict:
         vs.append(ev0.elapsed_time(ev1))
     view_rays = sum(r.count_work(vp)["rays"] for vp in params)
     print("large: views done", file=sys.stderr, flush=True)
-    # the neural cache at the primary hit, static camera, trained over the frames
-    p.flags = 0
+    # the neural cache at the primary hit, static camera, trained over the frames (both frames over 8-wide nodes)
+    p.flags = runtime.BVH8
     p.output = runtime.OUTPUT_RADIANCE
     plain = torch.empty_like(px)
     r.render_device(p, plain.data_ptr(), dp.data_ptr())

@@ -367,7 +367,7 @@ void render_kernel(SceneView s, RenderParams rp, float4 *__restrict__ pixels, fl
 // ---------------------------------------------------------------- rendering with the neural radiance cache
 // The frame kernel has the render kernel's tiling; one in train_one_in waves (chosen by a hash of the wave's
 // position and the frame) traces full paths for training, so training and cached pixels never share a wave.
-template <bool WIDE>
+template <uint32_t NW>
 __global__ __launch_bounds__(64) void nrc_frame_kernel(SceneView s, RenderParams rp, NrcFrame nf,
                                                        float4 *__restrict__ pixels, float *__restrict__ depths,
                                                        uint32_t scene_depth, uint32_t mesh_depth, uint32_t tiles_x,
@@ -385,8 +385,8 @@ __global__ __launch_bounds__(64) void nrc_frame_kernel(SceneView s, RenderParams
     h ^= h >> 12;
     const bool training = nf.train_one_in && (h % nf.train_one_in) == 0;
 
-    TracerT<WIDE ? 4u : 2u> tr;
-    bind_lds_stacks<WIDE ? 4u : 2u, 64>(tr, lds, t, scene_depth, mesh_depth);
+    TracerT<NW> tr;
+    bind_lds_stacks<NW, 64>(tr, lds, t, scene_depth, mesh_depth);
     tr.cnt = nullptr;
     if (x < ws && y < hs) {
         v3 dir;
@@ -1587,8 +1587,7 @@ int slimrt_render_nrc(slimrt_context *c, const slimrt_render_params *p, slimrt_n
     // one-wave workgroups, a 32x2 strip of samples each (the render kernel's tiling)
     const uint32_t tiles_x = (ws + kWaveTileX - 1) / kWaveTileX, tiles_y = (hs + kWaveTileY - 1) / kWaveTileY;
     const uint32_t tiles = tiles_x * tiles_y;
-    if (rp.flags & SLIMRT_BVH8) return fail(c, SLIMRT_E_ARG, "the cache frame takes node width 2 or 4, not SLIMRT_BVH8");
-    const uint32_t width = wide_mesh_bvh(rp) ? 4u : 2u;
+    const uint32_t width = mesh_node_width(rp);
     const uint32_t scene_depth = c->scene_height + 1, mesh_depth = mesh_stack_depth(c, width);
     const size_t lds = stack_lds_bytes(scene_depth, mesh_depth, width, 64);
     if (lds > 160 * 1024) return fail(c, SLIMRT_E_ARG, "BVH too deep for the LDS traversal stacks");
@@ -1629,14 +1628,11 @@ int slimrt_render_nrc(slimrt_context *c, const slimrt_render_params *p, slimrt_n
     }
     hipLaunchKernelGGL(nrc_reset_kernel, dim3(1), dim3(64), 0, c->stream, c->nrc_counters, c->nrc_losses, kMaxTrainSteps);
     HIP_TRY(c, hipGetLastError());
-    if (wide_mesh_bvh(rp))
-        hipLaunchKernelGGL(nrc_frame_kernel<true>, dim3(tiles), dim3(64), lds, c->stream, device_view(c), rp,
-                           nf, (float4 *)d_pixels, (float *)d_depths, scene_depth, mesh_depth, tiles_x, tiles,
-                           c->d_status);
-    else
-        hipLaunchKernelGGL(nrc_frame_kernel<false>, dim3(tiles), dim3(64), lds, c->stream, device_view(c), rp,
-                           nf, (float4 *)d_pixels, (float *)d_depths, scene_depth, mesh_depth, tiles_x, tiles,
-                           c->d_status);
+#define NRC_FRAME_LAUNCH(NW)                                                                                    \
+    hipLaunchKernelGGL(nrc_frame_kernel<NW>, dim3(tiles), dim3(64), lds, c->stream, device_view(c), rp, nf,          \
+                       (float4 *)d_pixels, (float *)d_depths, scene_depth, mesh_depth, tiles_x, tiles, c->d_status)
+    if (width == 8) NRC_FRAME_LAUNCH(8u); else if (width == 4) NRC_FRAME_LAUNCH(4u); else NRC_FRAME_LAUNCH(2u);
+#undef NRC_FRAME_LAUNCH
     HIP_TRY(c, hipGetLastError());
     if (np->cache_depth + 1 <= p->max_depth) {
         if (slimrt_nrc_infer_queue(nrc, c->q_raw, c->nrc_counters + NC_QUERIES, samples, c->q_out))

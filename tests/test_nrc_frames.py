@@ -100,10 +100,12 @@ def test_frame_without_cache_vertex_equals_render(gpu_available, name):
 
 
 @pytest.mark.gpu
-def test_frame_with_gate_closed_equals_render(gpu_available):
+@pytest.mark.parametrize("flags", [0, runtime.BVH4, runtime.BVH8])   # node width 2, 4, 8
+def test_frame_with_gate_closed_equals_render(gpu_available, flags):
     from slimtracincpp_amd.nrc import CachedFrames
     spec = _scenes()["mesh_pbr"]
     r, s, p, px, dp = _frame_setup(spec)
+    p.flags = flags
     ref_px, ref_dp = _plain(r, s, p, px, dp)
     px.zero_()
     frames = CachedFrames(r, cache_depth=0, train_one_in=0, gate_threshold=1.0, gate_min_frames=1 << 30)

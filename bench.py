@@ -303,43 +303,51 @@ def batched_views_leg(frames: int, stream, spec, n_views: int = 48) -> dict:
 def texel_storage_leg(width: int, height: int, stream, steps: int, warmup: int) -> dict:
     """The headline workload with the configuration's texture formats: the 1024^2 albedo (and its mips) as RGBA8
     texels, the 6 x 256^2 sky and 6 x 32^2 irradiance faces as RGBA16F texels of an HDR sky (the same scene
-    otherwise); the 4-wide megakernel in both tile orders, the faster reported, beside the texel-quad frame."""
-    import numpy as np
+    otherwise); the 4- and 8-wide megakernels in both tile orders, the faster reported, beside the same forms on
+    the texel-quad scene rendered the same way (a context of its own next to the headline's)."""
     import torch
     from slimtracincpp_amd import scenes, runtime
     spec = scenes.mesh_pbr_scene(0, width, height, texel_storage=True)
     quads = scenes.mesh_pbr_scene(0, width, height)
     nbytes = lambda sp: int(sum(m.nbytes for t in sp.textures for m in t.mips))
-    r = runtime.Renderer(spec, "gpu")
-    r.set_stream(stream.cuda_stream)
-    p = r.params()
     n = width * height
     px = torch.empty((n, 4), dtype=torch.float32, device="cuda")
     dp = torch.empty(n, dtype=torch.float32, device="cuda")
-    cnt = r.count_work(p)
-    forms = {}
-    for name, flags in (("megakernel_bvh4", runtime.BVH4),
-                        ("megakernel_image_order_bvh4", runtime.BVH4 | runtime.TILE_ORDER_FIXED),
-                        ("megakernel_bvh8", runtime.BVH8),
-                        ("megakernel_image_order_bvh8", runtime.BVH8 | runtime.TILE_ORDER_FIXED)):
-        p.flags = flags
-        for _ in range(warmup):
-            r.render_device(p, px.data_ptr(), dp.data_ptr())
-        r.synchronize()
-        ev0, ev1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
-        ev0.record(stream)
-        for _ in range(steps):
-            r.render_device(p, px.data_ptr(), dp.data_ptr())
-        ev1.record(stream)
-        torch.cuda.synchronize()
-        forms[name] = ev0.elapsed_time(ev1) / steps
-    r.close()
+    form_flags = (("megakernel_bvh4", runtime.BVH4),
+                  ("megakernel_image_order_bvh4", runtime.BVH4 | runtime.TILE_ORDER_FIXED),
+                  ("megakernel_bvh8", runtime.BVH8),
+                  ("megakernel_image_order_bvh8", runtime.BVH8 | runtime.TILE_ORDER_FIXED))
+
+    def time_forms(sp):
+        r = runtime.Renderer(sp, "gpu")
+        r.set_stream(stream.cuda_stream)
+        p = r.params()
+        cnt = r.count_work(p)
+        forms = {}
+        for name, flags in form_flags:
+            p.flags = flags
+            for _ in range(warmup):
+                r.render_device(p, px.data_ptr(), dp.data_ptr())
+            r.synchronize()
+            ev0, ev1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+            ev0.record(stream)
+            for _ in range(steps):
+                r.render_device(p, px.data_ptr(), dp.data_ptr())
+            ev1.record(stream)
+            torch.cuda.synchronize()
+            forms[name] = ev0.elapsed_time(ev1) / steps
+        r.close()
+        return forms, cnt["rays"]
+
+    forms, rays = time_forms(spec)
+    quad_forms, _ = time_forms(quads)
     best = min(forms, key=forms.get)
     return {"workload": "mesh_pbr as configured: 1024^2 RGBA8 albedo + mips, cube IBL 6x256^2 + 6x32^2 RGBA16F "
                         "faces (HDR sky), otherwise the headline scene",
-            "texture_bytes": nbytes(spec), "texture_bytes_quads": nbytes(quads), "rays_per_frame": cnt["rays"],
+            "texture_bytes": nbytes(spec), "texture_bytes_quads": nbytes(quads), "rays_per_frame": rays,
             "forms_ms": forms, "best_form": best, "ms_per_frame": forms[best],
-            "mrays_per_s": cnt["rays"] / (forms[best] * 1e-3) / 1e6}
+            "mrays_per_s": rays / (forms[best] * 1e-3) / 1e6,
+            "quads_forms_ms": quad_forms, "quads_ms_per_frame": min(quad_forms.values())}
 
 
 def large_leg(stream, frames: int = 5) -> dict:

@@ -85,8 +85,7 @@ def test_48_batched_views_full_size(gpu_available, name):
         checked_rows = 0
         for i, (p, (px, dp)) in enumerate(zip(params, bufs)):
             q = runtime.RenderParams.from_buffer_copy(p)
-            if cfg["node_width"] == 4:
-                q.flags |= runtime.BVH4
+            q.flags |= {2: 0, 4: runtime.BVH4, 8: runtime.BVH8}[cfg["node_width"]]
             ref_px, ref_dp = _single(r, q)
             got_px, got_dp = px.cpu().numpy(), dp.cpu().numpy()
             assert np.array_equal(got_px.view(np.uint32), ref_px.view(np.uint32)), (frame, i)

@@ -579,8 +579,8 @@ def main():
                                 .get(form[-5:], "")) + {
                                 "megakernel": "render_kernel<false> (tiles dispatched costliest first; their order, from "
                                               "the tile durations of two frames before, is computed by "
-                                              "order_count/plan/place_kernel on a second stream while the previous "
-                                              "frame renders)",
+                                              "the previous frame's first workgroup while its other workgroups "
+                                              "render)",
                                 "megakernel_image_order": "render_kernel<false> (tiles in image order)",
                                 "wavefront": "the frame's launches: wf_primary_kernel + one wf_bounce_kernel<true> "
                                              "over the compacted bounce queue",

@@ -78,7 +78,8 @@ extern "C" {
                                         renders the first 1 in 32 of them as two half waves; other frames, and the
                                         first two at a tile count, keep the image order.  Results are the same
                                         either way.  The cost order allocates its buffers on a context's first frame
-                                        at a larger tile count and orders on a second stream: callers that capture
+                                        at a larger tile count and alternates two sets of them from frame to frame
+                                        (each frame's first workgroup orders the next frame): callers that capture
                                         frames in a HIP graph pass this flag */
 
 typedef struct slimrt_context slimrt_context;

@@ -41,20 +41,21 @@ namespace {
 // the waves whose samples see the glass and mirror instances run 50-70x longer than the median wave, and in
 // image order they start a fifth of the way into the frame, so the frame ends with most of the chip idle while
 // they finish.  Dispatched first, they overlap the rest of the frame.  The cost of a tile is its duration two
-// frames earlier (a scene changes little from one frame to the next): the order for frame f + 1 is computed on a
-// second stream while frame f renders, from the durations frame f - 1 recorded.
+// frames earlier (a scene changes little from one frame to the next): the order for frame f + 1 is computed while
+// frame f renders, from the durations frame f - 1 recorded, by one wave of frame f's kernel (order_wave, its first
+// workgroup), so a frame is one launch on one stream.
 //
 // A tile's cost class is its duration's binary exponent and the two bits below it (four classes an octave; rank
 // 0 = costliest).  The costly tiles - those within a factor 16 of the costliest class (narrowed 4 classes at a
 // time while more than kHeavyMax), none if even the costliest class alone has more - go first, by rank and within
 // a rank in image order (a counting sort over the at most 17 ranks of the window); the others follow in image
-// order.  Three launches over waves that each own a contiguous segment of tiles (so image order within a class is
-// segment order): order_count_kernel ranks the tiles and counts them per class and wave, with each wave's
-// costliest rank, longest and summed durations; order_plan_kernel (one workgroup) picks the order, the window and
-// every wave's first slot per class and for its other tiles; order_place_kernel writes each tile to its slot.
-// About 10 us in all for a 1080p frame, so a stream that shares its hardware queue with the frame's (streams are
-// dealt to a few queues in turn) delays the next frame that little; the single-workgroup form took ~80 us.
-// Deterministic.
+// order.  The frames that allocate nothing compute the same order in three launches right before the frame (their
+// scratch holds one set of durations, so the order cannot be computed while the next durations are written), over
+// waves that each own a contiguous segment of tiles (so image order within a class is segment order):
+// order_count_kernel ranks the tiles and counts them per class and wave, with each wave's costliest rank, longest
+// and summed durations; order_plan_kernel (one workgroup) picks the order, the window and every wave's first slot
+// per class and for its other tiles; order_place_kernel writes each tile to its slot (about 25 us in all for a
+// 1080p frame, on the frame's path).  Deterministic: both forms give the same order.
 constexpr uint32_t kCostClasses = 128;
 constexpr uint32_t kWindow = 16;              // costly window: ranks top .. top + 16 (a factor 16 in duration)
 constexpr uint32_t kHeavyMax = 4096;          // at most this many costly tiles
@@ -279,6 +280,107 @@ __global__ __launch_bounds__(kOrderBlock) void order_place_kernel(const uint32_t
     }
 }
 
+// The order of launch_tile_order's three launches, computed by one wave (64 lanes, a workgroup of its own) over
+// n tiles in image order: a counting pass (a histogram of the cost classes in LDS, kCostClasses words at hist, with
+// the costliest rank and the longest and summed durations), the plan, and a placing pass (the tiles of each window
+// rank and the others in image order, by ballots).  The frame kernel's first workgroup runs it for the next frame
+// while the other workgroups render (about 0.1 ms for the 32,400 tiles of a 1080p frame, on one of its ~3,000 wave
+// slots).  The caller checks that the durations belong to n tiles.
+constexpr uint32_t kOrderWaveBatch = 8;   // 64-tile rounds whose durations are loaded together
+constexpr size_t kOrderWaveLds = kCostClasses * sizeof(uint32_t);
+__device__ __forceinline__ void order_wave(const uint32_t *__restrict__ cost, uint32_t n, uint32_t resident,
+                                           uint32_t *__restrict__ order, uint32_t *hist) {
+    const uint32_t lane = threadIdx.x;
+    for (uint32_t k = lane; k < kCostClasses; k += 64) hist[k] = 0;
+    __syncthreads();
+    uint32_t lo = kCostClasses, longest = 0;
+    uint64_t sum = 0;
+    for (uint32_t base = 0; base < n; base += 64u * kOrderWaveBatch) {
+        uint32_t v[kOrderWaveBatch];
+#pragma unroll
+        for (uint32_t j = 0; j < kOrderWaveBatch; j++) {
+            const uint32_t i = base + j * 64u + lane;
+            v[j] = i < n ? cost[i] : 0u;
+        }
+#pragma unroll
+        for (uint32_t j = 0; j < kOrderWaveBatch; j++) {
+            if (base + j * 64u + lane < n) {
+                const uint32_t rk = cost_rank(v[j]);
+                atomicAdd(&hist[rk], 1u);
+                lo = min(lo, rk);
+                longest = max(longest, v[j]);
+                sum += v[j];
+            }
+        }
+    }
+    for (int off = 32; off > 0; off >>= 1) {
+        lo = min(lo, (uint32_t)__shfl_xor((int)lo, off, 64));
+        longest = max(longest, (uint32_t)__shfl_xor((int)longest, off, 64));
+        sum += (uint64_t)__shfl_xor((long long)sum, off, 64);
+    }
+    __syncthreads();   // the histogram's atomics are done
+    const uint32_t top = lo;
+    // as order_plan_kernel: longest first only when the longest tile lasts at least 3/4 of the frame's ideal length
+    const bool image = 4ull * (unsigned long long)longest * resident <= 3ull * sum;
+    uint32_t total[kWindow + 1];
+#pragma unroll
+    for (uint32_t k = 0; k <= kWindow; k++) total[k] = top + k < kCostClasses ? hist[top + k] : 0u;
+    int window = -1;   // the widest of 16, 12, 8, 4, 0 ranks with <= kHeavyMax tiles
+    uint32_t heavy = 0;
+    for (int w = (int)kWindow; w >= 0; w -= 4) {
+        uint32_t h = 0;
+        for (int k = 0; k <= w; k++) h += total[k];
+        if (h <= kHeavyMax) {
+            window = w;
+            heavy = h;
+            break;
+        }
+    }
+    if (image) {
+        for (uint32_t i = lane; i < n; i += 64u) order[i] = i;
+        if (lane == 0) order[n] = 0u;
+        return;
+    }
+    uint32_t slot[kWindow + 1];
+    uint32_t acc = 0;
+#pragma unroll
+    for (uint32_t k = 0; k <= kWindow; k++) {
+        slot[k] = acc;
+        acc += (int)k <= window ? total[k] : 0u;
+    }
+    uint32_t lb = heavy;
+    const uint64_t below = (1ull << lane) - 1ull;
+    for (uint32_t base = 0; base < n; base += 64u * kOrderWaveBatch) {
+        uint32_t d[kOrderWaveBatch];
+#pragma unroll
+        for (uint32_t j = 0; j < kOrderWaveBatch; j++) {
+            const uint32_t i = base + j * 64u + lane;
+            d[j] = i < n ? cost_rank(cost[i]) - top : 0xFFu;
+        }
+#pragma unroll
+        for (uint32_t j = 0; j < kOrderWaveBatch; j++) {
+            const uint32_t i = base + j * 64u + lane;
+            const bool is_heavy = (int)d[j] <= window, is_light = i < n && !is_heavy;
+            if (__ballot(is_heavy)) {   // uniform
+                uint32_t pos = 0;
+#pragma unroll
+                for (uint32_t k = 0; k <= kWindow; k++) {
+                    if ((int)k <= window) {   // uniform
+                        const uint64_t m = __ballot(d[j] == k);
+                        if (d[j] == k) pos = slot[k] + (uint32_t)__popcll(m & below);
+                        slot[k] += (uint32_t)__popcll(m);
+                    }
+                }
+                if (is_heavy) order[pos] = i;
+            }
+            const uint64_t ml = __ballot(is_light);
+            if (is_light) order[lb + (uint32_t)__popcll(ml & below)] = i;
+            lb += (uint32_t)__popcll(ml);
+        }
+    }
+    if (lane == 0) order[n] = 1u;
+}
+
 // The three launches of a frame's order on one stream.
 void launch_tile_order(hipStream_t stream, const uint32_t *cost, uint32_t *order, uint32_t n, uint32_t resident,
                        const uint32_t *cost_tiles, uint32_t *order_scratch) {
@@ -295,7 +397,8 @@ void launch_tile_order(hipStream_t stream, const uint32_t *cost, uint32_t *order
 // The 4-wide frame at three waves per SIMD (168 VGPRs, 17 spilled): the walk's dependent fetches want the waves
 // more than the registers (6% faster than two on the mesh + PBR frame).  The 2-wide frame would spill 64 dwords a
 // lane for no gain, and the counting pass keeps its registers: both stay at the compiler's two.  With an order
-// (launch_tile_order) workgroup k renders tile order[k] and records the tile's duration in cost; when order[tiles]
+// (order_wave or launch_tile_order) workgroup k renders tile order[k] and records the tile's duration in cost (with
+// next_order, workgroup 0 computes the next frame's order and workgroup k + 1 takes k's place); when order[tiles]
 // is set, the first `split` tiles of the order (the costliest) are each rendered by two workgroups, lanes 0-31
 // and lanes 32-63: a costly tile's 64 paths diverge, and half as many lanes per wave shorten the frame's longest
 // waves (0.77 -> 0.65 ms on the mesh + PBR frame; quarter waves gain nothing and lose fast as their count grows).
@@ -307,14 +410,23 @@ void render_kernel(SceneView s, RenderParams rp, float4 *__restrict__ pixels, fl
                    uint32_t scene_depth, uint32_t mesh_depth, uint32_t tiles_x, uint32_t tiles,
                    uint32_t *__restrict__ status, unsigned long long *__restrict__ counters,
                    const uint32_t *__restrict__ order, uint32_t *__restrict__ cost, uint32_t split,
-                   uint32_t quarter, uint32_t top_parts, uint32_t *__restrict__ cost_tiles) {
+                   uint32_t quarter, uint32_t top_parts, uint32_t *__restrict__ cost_tiles,
+                   const uint32_t *__restrict__ next_cost, uint32_t *__restrict__ next_order, uint32_t resident) {
     extern __shared__ uint32_t lds[];
-    if (cost_tiles && blockIdx.x == 0 && threadIdx.x == 0) *cost_tiles = tiles;   // what cost will describe
-    uint32_t tile = blockIdx.x, parts = 1, part = 0;   // the wave renders lanes [part, part + 1) * 64 / parts
+    uint32_t b = blockIdx.x;
+    if (next_order) {   // workgroup 0 computes the next frame's order from next_cost; the tiles follow
+        if (b == 0) {
+            order_wave(next_cost, tiles, resident, next_order, lds);
+            return;
+        }
+        b--;
+    }
+    if (cost_tiles && b == 0 && threadIdx.x == 0) *cost_tiles = tiles;   // what cost will describe
+    uint32_t tile = b, parts = 1, part = 0;   // the wave renders lanes [part, part + 1) * 64 / parts
     if (order) {
-        // order[tiles] = 0: the order kernel chose image order, no split
+        // order[tiles] = 0: the order computation chose image order, no split
         const uint32_t sp = order[tiles] ? split : 0u, q = order[tiles] ? min(quarter, split) : 0u;
-        const uint32_t b = blockIdx.x, P = top_parts;
+        const uint32_t P = top_parts;
         if (b >= tiles + sp + (P - 2) * q) return;
         if (b < P * q) {   // the first q tiles of the order as P part waves
             tile = order[b / P];
@@ -518,16 +630,11 @@ int noalloc_scratch_address(int device, uint32_t **out) {
     return SLIMRT_OK;
 }
 
-// Buffers, events and the second stream of cost-ordered frames (created on first use, grown with the tile count).
+// Buffers and the event of cost-ordered frames (created on first use, grown with the tile count).
 int prepare_tile_order(slimrt_context *c, uint32_t tiles) {
-    if (!c->side_stream) HIP_TRY(c, hipStreamCreateWithFlags(&c->side_stream, hipStreamNonBlocking));
-    for (int p = 0; p < 2; p++) {
-        if (!c->tile_rendered[p]) HIP_TRY(c, hipEventCreateWithFlags(&c->tile_rendered[p], hipEventDisableTiming));
-        if (!c->tile_ordered[p]) HIP_TRY(c, hipEventCreateWithFlags(&c->tile_ordered[p], hipEventDisableTiming));
-    }
-    if (!c->order_scratch) HIP_TRY(c, ctx_malloc(c, &c->order_scratch, kOrderScratchWords * sizeof(uint32_t)));
+    if (!c->tile_rendered) HIP_TRY(c, hipEventCreateWithFlags(&c->tile_rendered, hipEventDisableTiming));
     if (c->tile_cap >= tiles) return SLIMRT_OK;
-    HIP_TRY(c, hipStreamSynchronize(c->side_stream));
+    if (c->tile_frame) HIP_TRY(c, hipEventSynchronize(c->tile_rendered));   // the last such frame is done
     HIP_TRY(c, hipStreamSynchronize(c->stream));
     for (int p = 0; p < 2; p++) {
         for (uint32_t **b : {&c->tile_cost[p], &c->tile_order[p]}) {
@@ -535,7 +642,7 @@ int prepare_tile_order(slimrt_context *c, uint32_t tiles) {
             *b = nullptr;
         }
         c->cost_tiles[p] = c->order_tiles[p] = 0;
-        c->order_launched[p] = c->order_used[p] = false;
+        c->order_used[p] = false;
     }
     c->tile_cap = 0;
     for (int p = 0; p < 2; p++) {
@@ -590,23 +697,33 @@ int launch(slimrt_context *c, const slimrt_render_params *p, float4 *pixels, flo
         const uint32_t wtiles = wtiles_x * ((hs + kWaveTileY - 1) / kWaveTileY);
         const size_t wlds = stack_lds_bytes(scene_depth, mesh_depth, width, 64);
         const bool by_cost = !count && !(rp.flags & SLIMRT_TILE_ORDER_FIXED) && wtiles < kMaxOrderedTiles;
-        const uint32_t par = (uint32_t)(c->tile_frame & 1u);
-        uint32_t *order = nullptr, *cost = nullptr;
+        const uint32_t par = (uint32_t)(c->tile_frame & 1u), nxt = par ^ 1u;
+        uint32_t *order = nullptr, *cost = nullptr, *next_order = nullptr, resident = 0;
+        const uint32_t *next_cost = nullptr;
+        size_t klds = wlds;
         if (by_cost) {
             if (int rc = prepare_tile_order(c, wtiles)) return rc;
-            // an order kernel still reading this parity's costs must finish before they are overwritten
-            if (c->order_launched[par]) HIP_TRY(c, hipStreamWaitEvent(c->stream, c->tile_ordered[par], 0));
-            c->order_launched[par] = false;
+            // the last cost-ordered frame ran on another stream: this frame reads what it wrote
+            if (c->tile_frame && c->order_stream != c->stream) HIP_TRY(c, hipStreamWaitEvent(c->stream, c->tile_rendered, 0));
             c->order_used[par] = c->order_tiles[par] == wtiles;
             order = c->order_used[par] ? c->tile_order[par] : nullptr;
             cost = c->tile_cost[par];
+            // the next frame's order, from the previous frame's durations, computed by this frame's first workgroup
+            c->order_tiles[nxt] = 0;
+            if (c->cost_tiles[nxt] == wtiles) {
+                if (int rc = resident_waves(c, width, scene_has_lights(v), wlds, resident)) return rc;
+                next_cost = c->tile_cost[nxt];
+                next_order = c->tile_order[nxt];
+                klds = std::max(wlds, kOrderWaveLds);
+            }
         }
         const uint32_t split = order ? std::min(wtiles / kSplitShare, kSplitMax) : 0u;
         const uint32_t quarter = std::min(quarter_tiles(wtiles, width), split), tp = kTopParts;
+        const uint32_t blocks = wtiles + split + (tp - 2) * quarter + (next_order ? 1u : 0u);
 #define RENDER_LAUNCH(COUNT, WIDE, LIGHTS)                                                                      \
-    hipLaunchKernelGGL((render_kernel<COUNT, WIDE, LIGHTS>), dim3(wtiles + split + (tp - 2) * quarter), dim3(64), wlds,    \
-                       c->stream, v, rp, pixels, depths, scene_depth, mesh_depth, wtiles_x, wtiles, c->d_status,      \
-                       c->d_counters, order, cost, split, quarter, tp, nullptr)
+    hipLaunchKernelGGL((render_kernel<COUNT, WIDE, LIGHTS>), dim3(blocks), dim3(64), klds, c->stream, v, rp, pixels,    \
+                       depths, scene_depth, mesh_depth, wtiles_x, wtiles, c->d_status, c->d_counters, order, cost,  \
+                       split, quarter, tp, nullptr, next_cost, next_order, resident)
         if (count) {
             if (width == 8) RENDER_LAUNCH(true, 8, true); else if (width == 4) RENDER_LAUNCH(true, 4, true);
             else RENDER_LAUNCH(true, 2, true);
@@ -620,23 +737,11 @@ int launch(slimrt_context *c, const slimrt_render_params *p, float4 *pixels, flo
 #undef RENDER_LAUNCH
         HIP_TRY(c, hipGetLastError());
         if (by_cost) {
-            HIP_TRY(c, hipEventRecord(c->tile_rendered[par], c->stream));
+            HIP_TRY(c, hipEventRecord(c->tile_rendered, c->stream));
+            c->order_stream = c->stream;
             c->cost_tiles[par] = wtiles;
+            if (next_order) c->order_tiles[nxt] = wtiles;
             c->tile_frame++;
-            // the next frame's order, from the previous frame's costs, overlapping this frame
-            const uint32_t nxt = par ^ 1u;
-            c->order_tiles[nxt] = 0;
-            if (c->cost_tiles[nxt] == wtiles) {
-                HIP_TRY(c, hipStreamWaitEvent(c->side_stream, c->tile_rendered[nxt], 0));
-                uint32_t resident = 0;
-                if (int rc = resident_waves(c, width, scene_has_lights(v), wlds, resident)) return rc;
-                launch_tile_order(c->side_stream, c->tile_cost[nxt], c->tile_order[nxt], wtiles, resident, nullptr,
-                                  c->order_scratch);
-                HIP_TRY(c, hipGetLastError());
-                HIP_TRY(c, hipEventRecord(c->tile_ordered[nxt], c->side_stream));
-                c->order_launched[nxt] = true;
-                c->order_tiles[nxt] = wtiles;
-            }
         }
     }
     HIP_TRY(c, hipGetLastError());
@@ -1057,9 +1162,10 @@ int slimrt_create(slimrt_context **out, uint32_t flags) {
 int slimrt_destroy(slimrt_context *c) {
     if (!c) return SLIMRT_E_ARG;
     if (!c->cpu) {
-        // the caller's stream and the side stream (the tile-order kernel of the last cost-ordered frame reads
-        // tile_cost and writes tile_order there, and nothing joins it back) both finish before anything is freed
+        // the caller's stream, the stream of the last cost-ordered frame (it reads and writes the tile buffers) and
+        // the side stream of banded wavefront frames all finish before anything is freed
         (void)hipStreamSynchronize(c->stream);
+        if (c->tile_frame) (void)hipEventSynchronize(c->tile_rendered);
         if (c->side_stream) (void)hipStreamSynchronize(c->side_stream);
         dev_free(c, c->d_geos); dev_free(c, c->d_mats); dev_free(c, c->d_lights); dev_free(c, c->d_snodes); dev_free(c, c->d_mnodes);
         dev_free(c, c->d_mnodes4);
@@ -1083,10 +1189,8 @@ int slimrt_destroy(slimrt_context *c) {
         for (int p = 0; p < 2; p++) {
             if (c->tile_cost[p]) (void)ctx_free(c, c->tile_cost[p]);
             if (c->tile_order[p]) (void)ctx_free(c, c->tile_order[p]);
-            if (c->tile_rendered[p]) (void)hipEventDestroy(c->tile_rendered[p]);
-            if (c->tile_ordered[p]) (void)hipEventDestroy(c->tile_ordered[p]);
         }
-        if (c->order_scratch) (void)ctx_free(c, c->order_scratch);
+        if (c->tile_rendered) (void)hipEventDestroy(c->tile_rendered);
         if (c->ev0) (void)hipEventDestroy(c->ev0);
         if (c->ev1) (void)hipEventDestroy(c->ev1);
         if (c->side_stream) (void)hipStreamDestroy(c->side_stream);
@@ -1363,7 +1467,8 @@ int slimrt_render_device_noalloc(slimrt_context *c, const slimrt_render_params *
 #define NOALLOC_LAUNCH(NW, LIGHTS)                                                                                \
     hipLaunchKernelGGL((render_kernel<false, NW, LIGHTS>), dim3(wtiles + split + (tp - 2) * quarter), dim3(64), wlds,      \
                        c->stream, v, rp, (float4 *)d_pixels, (float *)d_depths, scene_depth, mesh_depth, wtiles_x,     \
-                       wtiles, c->d_status, nullptr, order, cost, split, quarter, tp, by_cost ? cost_tiles : nullptr)
+                       wtiles, c->d_status, nullptr, order, cost, split, quarter, tp, by_cost ? cost_tiles : nullptr, \
+                       nullptr, nullptr, 0u)
     if (scene_has_lights(v)) {
         if (width == 8) NOALLOC_LAUNCH(8, true); else if (width == 4) NOALLOC_LAUNCH(4, true); else NOALLOC_LAUNCH(2, true);
     } else {

@@ -93,14 +93,15 @@ struct slimrt_context {
     uint32_t *d_status = nullptr;
     unsigned long long *d_counters = nullptr;
     // per-sample frames dispatched by cost, double-buffered by frame parity: frame f renders in order
-    // tile_order[f & 1] (computed on side_stream during frame f - 1 from the costs of frame f - 2) and writes its
+    // tile_order[f & 1] (computed by frame f - 1's first workgroup from the costs of frame f - 2) and writes its
     // tiles' durations (wall-clock ticks) to tile_cost[f & 1].  *_tiles: the tile count a buffer holds (0: none).
+    // tile_rendered follows the last such frame, launched on order_stream.
     uint32_t *tile_cost[2] = {}, *tile_order[2] = {};
-    uint32_t *order_scratch = nullptr;          // the order computation's per-wave counts and plan (side stream)
     uint32_t tile_cap = 0, cost_tiles[2] = {}, order_tiles[2] = {};
-    bool order_launched[2] = {}, order_used[2] = {};
+    bool order_used[2] = {};
     uint64_t tile_frame = 0;
-    hipEvent_t tile_rendered[2] = {}, tile_ordered[2] = {};
+    hipEvent_t tile_rendered = nullptr;
+    hipStream_t order_stream = nullptr;
     // this device's copy of the no-allocation frames' scratch array (its device address, looked up at creation)
     uint32_t *noalloc_scratch = nullptr;
     uint32_t resident = 0;           // frame-kernel waves the device holds at once, for resident_key's configuration

@@ -1,6 +1,6 @@
 """Per-sample frames dispatch their tiles costliest first, by the tile durations the context recorded two frames
-earlier at the same tile count (the order is computed on a second stream while the frame before renders; render
-flag SLIMRT_TILE_ORDER_FIXED keeps the image order).  Every tile must be rendered exactly once whatever the order
+earlier at the same tile count (the order is computed by the first workgroup of the frame before while its other
+workgroups render; render flag SLIMRT_TILE_ORDER_FIXED keeps the image order).  Every tile must be rendered exactly once whatever the order
 (canvases filled with NaN before each frame), frames must equal the image-order frame bit for bit, the first two
 frames at a tile count must use the image order, and later orders must equal the order a numpy model derives from
 the durations of two frames before (costly tiles sorted first, the rest in image order)."""
@@ -129,6 +129,45 @@ def test_cost_order_follows_tile_count_changes(gpu_available):
         got = _frame(r, p)
         assert all(np.array_equal(a, b) for a, b in zip(got, ref))
     assert np.array_equal(np.sort(r.tile_costs()[1]), np.arange(_tiles(p)))
+
+
+@pytest.mark.gpu
+def test_cost_order_across_a_stream_change(gpu_available):
+    """The context's stream changes between two cost-ordered frames launched back to back: the frame on the new
+    stream waits for the one on the old stream (it reads the durations and the order that frame wrote), and the
+    orders go on as before.  Frames 2k on alternating streams, frames 2k + 1 on a third; frame 2k + 1 is ordered by
+    the durations of frame 2k - 1."""
+    import torch
+    spec = scenes.implicit_scene(1, 160, 96, max_depth=5, transparent=True)
+    r = runtime.Renderer(spec, "gpu")
+    p = r.params(output=runtime.OUTPUT_RADIANCE)
+    p.flags = runtime.TILE_ORDER_FIXED
+    ref = _frame(r, p)
+    p.flags = 0
+    n = _tiles(p)
+    streams = [torch.cuda.Stream(), torch.cuda.Stream()]
+    m = p.width * p.height
+    bufs = [(torch.empty((m, 4), dtype=torch.float32, device="cuda"), torch.empty((m,), dtype=torch.float32, device="cuda"))
+            for _ in range(2)]
+    history = []
+    for k in range(3):
+        for px, dp in bufs:
+            px.fill_(float("nan"))
+            dp.fill_(float("nan"))
+        torch.cuda.synchronize()
+        r.set_stream(streams[0].cuda_stream if k % 2 == 0 else None)
+        r.render_device(p, bufs[0][0].data_ptr(), bufs[0][1].data_ptr())
+        r.set_stream(streams[1].cuda_stream)
+        r.render_device(p, bufs[1][0].data_ptr(), bufs[1][1].data_ptr())
+        r.synchronize()
+        torch.cuda.synchronize()
+        for px, dp in bufs:
+            got = px.cpu().numpy().view(np.uint32), dp.cpu().numpy().view(np.uint32)
+            assert all(np.array_equal(a, b) for a, b in zip(got, ref)), k
+        costs, order = r.tile_costs()
+        assert np.array_equal(order, _expected_order(history[-1], r.tile_resident) if k else np.arange(n)), k
+        history.append(costs)
+    r.set_stream(None)
 
 
 @pytest.mark.gpu

@@ -578,9 +578,9 @@ def main():
                      "kernel": ({"_bvh4": "4-wide mesh BVHs; ", "_bvh8": "8-wide mesh BVHs (quantised child boxes); "}
                                 .get(form[-5:], "")) + {
                                 "megakernel": "render_kernel<false> (tiles dispatched costliest first; their order, from "
-                                              "the tile durations of two frames before, is computed by "
-                                              "the previous frame's first workgroup while its other workgroups "
-                                              "render)",
+                                              "the tile durations of three frames before, is counted and placed "
+                                              "by the first workgroups of the two frames before it while their "
+                                              "other workgroups render)",
                                 "megakernel_image_order": "render_kernel<false> (tiles in image order)",
                                 "wavefront": "the frame's launches: wf_primary_kernel + one wf_bounce_kernel<true> "
                                              "over the compacted bounce queue",
@@ -601,7 +601,7 @@ def main():
     if form in ("megakernel", "megakernel_bvh4", "megakernel_bvh8"):
         # a frame's tile durations (slimrt_tile_costs): how far the frame is from its ideal length
         import numpy as np
-        for _ in range(3):   # this form's frames (the order of a frame comes from the durations two frames before)
+        for _ in range(4):   # this form's frames (the order of a frame comes from the durations three frames before)
             r.render_device(p, pixels.data_ptr(), depths.data_ptr())
         costs, order = r.tile_costs()
         if costs.size:

@@ -73,14 +73,15 @@ extern "C" {
                                       such nodes and leaves of at most 254 triangles */
 #define SLIMRT_TILE_ORDER_FIXED 512u /* per-sample frames (no SLIMRT_WAVEFRONT): dispatch the sample tiles in image
                                         order.  Without it a frame whose longest tiles bound its length (the longest
-                                        tile two frames before lasted at least 3/4 of that frame's summed tile
+                                        tile three frames before lasted at least 3/4 of that frame's summed tile
                                         durations over the resident waves) dispatches its costliest tiles first and
-                                        renders the first 1 in 32 of them as two half waves; other frames, and the
-                                        first two at a tile count, keep the image order.  Results are the same
-                                        either way.  The cost order allocates its buffers on a context's first frame
-                                        at a larger tile count and alternates two sets of them from frame to frame
-                                        (each frame's first workgroup orders the next frame): callers that capture
-                                        frames in a HIP graph pass this flag */
+                                        renders the first 1 in 32 of them as two half waves (with SLIMRT_BVH8 the
+                                        first 1 in 1024 as four quarter waves); other frames, and the first three at
+                                        a tile count, keep the image order.  Results are the same either way.  The
+                                        cost order allocates its buffers on a context's first frame at a larger tile
+                                        count and rotates them from frame to frame (each frame's first workgroups
+                                        count and place for the next ones): callers that capture frames in a HIP
+                                        graph pass this flag */
 
 typedef struct slimrt_context slimrt_context;
 

@@ -72,7 +72,7 @@ int slimrt_views_last_ms(slimrt_views *v, float *ms);
 int slimrt_views_autotune(slimrt_views *v, uint32_t reps, slimrt_views_config *best, float *best_ms);
 const char *slimrt_views_last_error(const slimrt_views *v);
 
-/* Times the frame forms of one view on eager frames into the given device buffers (reps each after two warm-up
+/* Times the frame forms of one view on eager frames into the given device buffers (reps each after three warm-up
  * frames, so that a form that orders its tiles by cost is timed in that order; median) and returns the fastest form's flags (params->flags with the form bits replaced) and its median time.
  * Forms: the per-sample megakernel over 8-wide nodes (SLIMRT_BVH8, when the meshes fit it), and node width 2 (the
  * uploaded BVHs) or 4 (SLIMRT_BVH4) x (the per-sample megakernel, or the SLIMRT_WAVEFRONT variants: run to the end

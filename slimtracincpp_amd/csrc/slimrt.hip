@@ -280,31 +280,57 @@ __global__ __launch_bounds__(kOrderBlock) void order_place_kernel(const uint32_t
     }
 }
 
-// The order of launch_tile_order's three launches, computed by one wave (64 lanes, a workgroup of its own) over
-// n tiles in image order: a counting pass (a histogram of the cost classes in LDS, kCostClasses words at hist, with
-// the costliest rank and the longest and summed durations), the plan, and a placing pass (the tiles of each window
-// rank and the others in image order, by ballots).  The frame kernel's first workgroup runs it for the next frame
-// while the other workgroups render (about 0.1 ms for the 32,400 tiles of a 1080p frame, on one of its ~3,000 wave
-// slots).  The caller checks that the durations belong to n tiles.
-constexpr uint32_t kOrderWaveBatch = 8;   // 64-tile rounds whose durations are loaded together
-constexpr size_t kOrderWaveLds = kCostClasses * sizeof(uint32_t);
-__device__ __forceinline__ void order_wave(const uint32_t *__restrict__ cost, uint32_t n, uint32_t resident,
-                                           uint32_t *__restrict__ order, uint32_t *hist) {
-    const uint32_t lane = threadIdx.x;
+// The order of launch_tile_order's three launches, computed by the first workgroups of later frames (one wave
+// each) in two stages over segments of kOrderSegTiles tiles: a counting wave per segment (class counts by LDS
+// atomics, the costliest rank, the longest and summed durations, kSegWords words per segment) in one frame, then a
+// placing wave per segment in the next (the plan from every segment's counts, then the segment's tiles: those of
+// each window rank by ballots, the others by one ballot).  Segments in image order keep image order within a class.
+// The durations are loaded 64 x kOrderWaveBatch at a time (the waves wait on memory: one wave for all 32,400 tiles
+// of a 1080p frame took ~0.3 ms, longer than a frame of primary hits).
+constexpr uint32_t kOrderWaveBatch = 32, kOrderSegTiles = 4096, kOrderMaxSegs = 64;
+constexpr uint32_t kSegLo = kCostClasses, kSegLongest = kCostClasses + 1, kSegSum = kCostClasses + 2;
+constexpr uint32_t kSegWords = kCostClasses + 4;
+// LDS of a placing wave: the window ranks' counts per segment, their totals and prefixes; then the staged ranks
+constexpr size_t kOrderWaveLds = ((kWindow + 1) * kOrderMaxSegs + 2 * (kWindow + 1)) * sizeof(uint32_t);
+static_assert(kOrderWaveLds >= 64 * kOrderWaveBatch && kOrderWaveLds >= kCostClasses * sizeof(uint32_t), "");
+
+// Segments of the order: segs waves of seg tiles each (a multiple of 64; the last may be shorter).
+__host__ __device__ inline void order_segments(uint32_t n, uint32_t &segs, uint32_t &seg) {
+    segs = (n + kOrderSegTiles - 1) / kOrderSegTiles;
+    segs = segs < 1u ? 1u : (segs > kOrderMaxSegs ? kOrderMaxSegs : segs);
+    seg = ((n + segs - 1) / segs + 63u) / 64u * 64u;
+    segs = (n + seg - 1) / seg;
+}
+
+// The work of a frame's first workgroups: count_waves counting waves over count_cost (into count_hist), then
+// place_waves placing waves over place_cost (counted into place_hist one frame earlier) into place_order.
+struct OrderWork {
+    const uint32_t *count_cost;
+    uint32_t *count_hist;
+    const uint32_t *place_cost;
+    const uint32_t *place_hist;
+    uint32_t *place_order;
+    uint32_t count_waves, place_waves, seg, resident;
+};
+
+__device__ __forceinline__ void order_count_wave(const uint32_t *__restrict__ cost, uint32_t n, uint32_t seg,
+                                                 uint32_t w, uint32_t *__restrict__ out, uint32_t *lds) {
+    const uint32_t lane = threadIdx.x, first = w * seg, last = min(n, first + seg);
+    uint32_t *hist = lds;
     for (uint32_t k = lane; k < kCostClasses; k += 64) hist[k] = 0;
     __syncthreads();
     uint32_t lo = kCostClasses, longest = 0;
     uint64_t sum = 0;
-    for (uint32_t base = 0; base < n; base += 64u * kOrderWaveBatch) {
+    for (uint32_t base = first; base < last; base += 64u * kOrderWaveBatch) {
         uint32_t v[kOrderWaveBatch];
 #pragma unroll
         for (uint32_t j = 0; j < kOrderWaveBatch; j++) {
             const uint32_t i = base + j * 64u + lane;
-            v[j] = i < n ? cost[i] : 0u;
+            v[j] = i < last ? cost[i] : 0u;
         }
 #pragma unroll
         for (uint32_t j = 0; j < kOrderWaveBatch; j++) {
-            if (base + j * 64u + lane < n) {
+            if (base + j * 64u + lane < last) {
                 const uint32_t rk = cost_rank(v[j]);
                 atomicAdd(&hist[rk], 1u);
                 lo = min(lo, rk);
@@ -318,67 +344,123 @@ __device__ __forceinline__ void order_wave(const uint32_t *__restrict__ cost, ui
         longest = max(longest, (uint32_t)__shfl_xor((int)longest, off, 64));
         sum += (uint64_t)__shfl_xor((long long)sum, off, 64);
     }
-    __syncthreads();   // the histogram's atomics are done
-    const uint32_t top = lo;
+    __syncthreads();   // the class counts are complete
+    for (uint32_t k = lane; k < kCostClasses; k += 64) out[k] = hist[k];
+    if (lane == 0) {
+        out[kSegLo] = lo;
+        out[kSegLongest] = longest;
+        out[kSegSum] = (uint32_t)sum;
+        out[kSegSum + 1] = (uint32_t)(sum >> 32);
+    }
+}
+
+__device__ __forceinline__ void order_place_wave(const uint32_t *__restrict__ cost, uint32_t n, uint32_t seg,
+                                                 uint32_t segs, uint32_t w, uint32_t resident,
+                                                 const uint32_t *__restrict__ hist, uint32_t *__restrict__ order,
+                                                 uint32_t *lds) {
+    const uint32_t lane = threadIdx.x, first = w * seg, last = min(n, first + seg);
+    // the frame's costliest rank, longest and summed durations, from every segment's (lane j: segment j)
+    const uint32_t *hj = hist + (size_t)lane * kSegWords;
+    const bool mine = lane < segs;
+    uint32_t top = mine ? hj[kSegLo] : kCostClasses, longest = mine ? hj[kSegLongest] : 0u;
+    uint64_t sum = mine ? ((uint64_t)hj[kSegSum] | ((uint64_t)hj[kSegSum + 1] << 32)) : 0ull;
+    for (int off = 32; off > 0; off >>= 1) {
+        top = min(top, (uint32_t)__shfl_xor((int)top, off, 64));
+        longest = max(longest, (uint32_t)__shfl_xor((int)longest, off, 64));
+        sum += (uint64_t)__shfl_xor((long long)sum, off, 64);
+    }
     // as order_plan_kernel: longest first only when the longest tile lasts at least 3/4 of the frame's ideal length
-    const bool image = 4ull * (unsigned long long)longest * resident <= 3ull * sum;
-    uint32_t total[kWindow + 1];
+    if (4ull * (unsigned long long)longest * resident <= 3ull * sum) {
+        for (uint32_t i = first + lane; i < last; i += 64u) order[i] = i;
+        if (w == 0 && lane == 0) order[n] = 0u;
+        return;
+    }
+    // the window ranks' counts per segment (lds[k * 64 + j]); lane k sums rank k over all segments and over those
+    // before this one
+    uint32_t *grid = lds, *tot = lds + (kWindow + 1) * kOrderMaxSegs, *pre = tot + (kWindow + 1);
 #pragma unroll
-    for (uint32_t k = 0; k <= kWindow; k++) total[k] = top + k < kCostClasses ? hist[top + k] : 0u;
+    for (uint32_t k = 0; k <= kWindow; k++) grid[k * 64u + lane] = (mine && top + k < kCostClasses) ? hj[top + k] : 0u;
+    __syncthreads();
+    if (lane <= kWindow) {
+        uint32_t t = 0, b = 0;
+        for (uint32_t j = 0; j < segs; j++) {
+            const uint32_t c = grid[lane * 64u + j];
+            if (j < w) b += c;
+            t += c;
+        }
+        tot[lane] = t;
+        pre[lane] = b;
+    }
+    __syncthreads();
+    uint32_t total[kWindow + 1], before[kWindow + 1];
+#pragma unroll
+    for (uint32_t k = 0; k <= kWindow; k++) {
+        total[k] = tot[k];
+        before[k] = pre[k];
+    }
     int window = -1;   // the widest of 16, 12, 8, 4, 0 ranks with <= kHeavyMax tiles
     uint32_t heavy = 0;
-    for (int w = (int)kWindow; w >= 0; w -= 4) {
+    for (int wd = (int)kWindow; wd >= 0; wd -= 4) {
         uint32_t h = 0;
-        for (int k = 0; k <= w; k++) h += total[k];
+        for (int k = 0; k <= wd; k++) h += total[k];
         if (h <= kHeavyMax) {
-            window = w;
+            window = wd;
             heavy = h;
             break;
         }
     }
-    if (image) {
-        for (uint32_t i = lane; i < n; i += 64u) order[i] = i;
-        if (lane == 0) order[n] = 0u;
-        return;
-    }
+    // first slots of this segment: per window rank after the costlier ranks and this rank's tiles in earlier
+    // segments; the other tiles after every window tile and the other tiles of earlier segments (full segments)
     uint32_t slot[kWindow + 1];
-    uint32_t acc = 0;
+    uint32_t acc = 0, heavy_before = 0;
 #pragma unroll
     for (uint32_t k = 0; k <= kWindow; k++) {
-        slot[k] = acc;
-        acc += (int)k <= window ? total[k] : 0u;
+        const bool in = (int)k <= window;
+        slot[k] = acc + (in ? before[k] : 0u);
+        acc += in ? total[k] : 0u;
+        heavy_before += in ? before[k] : 0u;
     }
-    uint32_t lb = heavy;
+    uint32_t lb = heavy + first - heavy_before;
     const uint64_t below = (1ull << lane) - 1ull;
-    for (uint32_t base = 0; base < n; base += 64u * kOrderWaveBatch) {
-        uint32_t d[kOrderWaveBatch];
+    uint8_t *stage = reinterpret_cast<uint8_t *>(lds);
+    for (uint32_t base = first; base < last; base += 64u * kOrderWaveBatch) {
+        uint32_t v[kOrderWaveBatch];
 #pragma unroll
         for (uint32_t j = 0; j < kOrderWaveBatch; j++) {
             const uint32_t i = base + j * 64u + lane;
-            d[j] = i < n ? cost_rank(cost[i]) - top : 0xFFu;
+            v[j] = i < last ? cost[i] : 0u;
         }
+        __syncthreads();   // the previous batch's ranks (or the plan's counts) have been read
+        // window rank, 0xFE for the other tiles, 0xFF past the segment
 #pragma unroll
         for (uint32_t j = 0; j < kOrderWaveBatch; j++) {
-            const uint32_t i = base + j * 64u + lane;
-            const bool is_heavy = (int)d[j] <= window, is_light = i < n && !is_heavy;
-            if (__ballot(is_heavy)) {   // uniform
+            const uint32_t i = base + j * 64u + lane, r = cost_rank(v[j]) - top;
+            stage[j * 64u + lane] = (uint8_t)(i >= last ? 0xFFu : (window >= 0 && r <= (uint32_t)window ? r : 0xFEu));
+        }
+        __syncthreads();
+        const uint32_t rounds = min(kOrderWaveBatch, (last - base + 63u) / 64u);
+        for (uint32_t j = 0; j < rounds; j++) {
+            const uint32_t i = base + j * 64u + lane, d = stage[j * 64u + lane];
+            const bool is_heavy = d < 0xFEu, is_light = d == 0xFEu;
+            if (__ballot(is_heavy)) {   // uniform; ballots only (a shuffle waits on LDS, and they would chain)
                 uint32_t pos = 0;
 #pragma unroll
                 for (uint32_t k = 0; k <= kWindow; k++) {
-                    if ((int)k <= window) {   // uniform
-                        const uint64_t m = __ballot(d[j] == k);
-                        if (d[j] == k) pos = slot[k] + (uint32_t)__popcll(m & below);
+                    const uint64_t m = (int)k <= window ? __ballot(d == k) : 0ull;
+                    if (m) {   // uniform
+                        if (d == k) pos = slot[k] + (uint32_t)__popcll(m & below);
                         slot[k] += (uint32_t)__popcll(m);
                     }
                 }
-                if (is_heavy) order[pos] = i;
+                if (is_heavy && pos < n) order[pos] = i;   // always below n: the counts are of these durations
             }
             const uint64_t ml = __ballot(is_light);
-            if (is_light) order[lb + (uint32_t)__popcll(ml & below)] = i;
+            const uint32_t q = lb + (uint32_t)__popcll(ml & below);
+            if (is_light && q < n) order[q] = i;
             lb += (uint32_t)__popcll(ml);
         }
     }
-    if (lane == 0) order[n] = 1u;
+    if (w == 0 && lane == 0) order[n] = 1u;
 }
 
 // The three launches of a frame's order on one stream.
@@ -397,8 +479,9 @@ void launch_tile_order(hipStream_t stream, const uint32_t *cost, uint32_t *order
 // The 4-wide frame at three waves per SIMD (168 VGPRs, 17 spilled): the walk's dependent fetches want the waves
 // more than the registers (6% faster than two on the mesh + PBR frame).  The 2-wide frame would spill 64 dwords a
 // lane for no gain, and the counting pass keeps its registers: both stay at the compiler's two.  With an order
-// (order_wave or launch_tile_order) workgroup k renders tile order[k] and records the tile's duration in cost (with
-// next_order, workgroup 0 computes the next frame's order and workgroup k + 1 takes k's place); when order[tiles]
+// (order_place_wave or launch_tile_order) workgroup k renders tile order[k] and records the tile's duration in
+// cost (the first ow.count_waves + ow.place_waves workgroups work on later frames' orders, and the tiles' workgroups
+// follow them); when order[tiles]
 // is set, the first `split` tiles of the order (the costliest) are each rendered by two workgroups, lanes 0-31
 // and lanes 32-63: a costly tile's 64 paths diverge, and half as many lanes per wave shorten the frame's longest
 // waves (0.77 -> 0.65 ms on the mesh + PBR frame; quarter waves gain nothing and lose fast as their count grows).
@@ -411,16 +494,20 @@ void render_kernel(SceneView s, RenderParams rp, float4 *__restrict__ pixels, fl
                    uint32_t *__restrict__ status, unsigned long long *__restrict__ counters,
                    const uint32_t *__restrict__ order, uint32_t *__restrict__ cost, uint32_t split,
                    uint32_t quarter, uint32_t top_parts, uint32_t *__restrict__ cost_tiles,
-                   const uint32_t *__restrict__ next_cost, uint32_t *__restrict__ next_order, uint32_t resident) {
+                   OrderWork ow) {
     extern __shared__ uint32_t lds[];
     uint32_t b = blockIdx.x;
-    if (next_order) {   // workgroup 0 computes the next frame's order from next_cost; the tiles follow
-        if (b == 0) {
-            order_wave(next_cost, tiles, resident, next_order, lds);
-            return;
-        }
-        b--;
+    // the first workgroups count and place for later frames' orders; the tiles follow
+    if (b < ow.count_waves) {
+        order_count_wave(ow.count_cost, tiles, ow.seg, b, ow.count_hist + (size_t)b * kSegWords, lds);
+        return;
     }
+    b -= ow.count_waves;
+    if (b < ow.place_waves) {
+        order_place_wave(ow.place_cost, tiles, ow.seg, ow.place_waves, b, ow.resident, ow.place_hist, ow.place_order, lds);
+        return;
+    }
+    b -= ow.place_waves;
     if (cost_tiles && b == 0 && threadIdx.x == 0) *cost_tiles = tiles;   // what cost will describe
     uint32_t tile = b, parts = 1, part = 0;   // the wave renders lanes [part, part + 1) * 64 / parts
     if (order) {
@@ -459,11 +546,11 @@ void render_kernel(SceneView s, RenderParams rp, float4 *__restrict__ pixels, fl
         if (tr.status) atomicOr(status, tr.status);
     }
     if (cost && lane == 0) {
-        // A split tile records its part's duration times the parts (the part that ends last writes last): a part
-        // runs about its share of the tile's diverging paths, and the scaled figure keeps the tile in the costliest
-        // classes two frames later (its own part's duration would rank it lower, so it would run unsplit and late,
-        // then split again).
-        uint64_t d = ((uint64_t)wall_clock64() - start) * parts;
+        // A split tile records its part's duration times the parts (the part that ends last writes last, so a
+        // costly half is not hidden by a cheap one): a part runs about its share of the tile's diverging paths, and
+        // the scaled figure keeps the tile in the costliest classes two frames later (its own part's duration
+        // would rank it lower, so it would run unsplit and late, then split again).
+        const uint64_t d = ((uint64_t)wall_clock64() - start) * parts;
         cost[tile] = d > 0xFFFFFFFFull ? 0xFFFFFFFFu : (uint32_t)d;
     }
     if (kCount) {
@@ -636,18 +723,24 @@ int prepare_tile_order(slimrt_context *c, uint32_t tiles) {
     if (c->tile_cap >= tiles) return SLIMRT_OK;
     if (c->tile_frame) HIP_TRY(c, hipEventSynchronize(c->tile_rendered));   // the last such frame is done
     HIP_TRY(c, hipStreamSynchronize(c->stream));
+    for (int p = 0; p < 3; p++) {
+        if (c->tile_cost[p]) HIP_TRY(c, ctx_free(c, c->tile_cost[p]));
+        c->tile_cost[p] = nullptr;
+        c->cost_tiles[p] = 0;
+    }
     for (int p = 0; p < 2; p++) {
-        for (uint32_t **b : {&c->tile_cost[p], &c->tile_order[p]}) {
+        for (uint32_t **b : {&c->tile_order[p], &c->tile_hist[p]}) {
             if (*b) HIP_TRY(c, ctx_free(c, *b));
             *b = nullptr;
         }
-        c->cost_tiles[p] = c->order_tiles[p] = 0;
+        c->hist_tiles[p] = c->order_tiles[p] = 0;
         c->order_used[p] = false;
     }
     c->tile_cap = 0;
+    for (int p = 0; p < 3; p++) HIP_TRY(c, ctx_malloc(c, &c->tile_cost[p], tiles * sizeof(uint32_t)));
     for (int p = 0; p < 2; p++) {
-        HIP_TRY(c, ctx_malloc(c, &c->tile_cost[p], tiles * sizeof(uint32_t)));
         HIP_TRY(c, ctx_malloc(c, &c->tile_order[p], (tiles + 1) * sizeof(uint32_t)));   // + the longest-first flag
+        HIP_TRY(c, ctx_malloc(c, &c->tile_hist[p], kOrderMaxSegs * kSegWords * sizeof(uint32_t)));
     }
     c->tile_cap = tiles;
     return SLIMRT_OK;
@@ -697,33 +790,50 @@ int launch(slimrt_context *c, const slimrt_render_params *p, float4 *pixels, flo
         const uint32_t wtiles = wtiles_x * ((hs + kWaveTileY - 1) / kWaveTileY);
         const size_t wlds = stack_lds_bytes(scene_depth, mesh_depth, width, 64);
         const bool by_cost = !count && !(rp.flags & SLIMRT_TILE_ORDER_FIXED) && wtiles < kMaxOrderedTiles;
-        const uint32_t par = (uint32_t)(c->tile_frame & 1u), nxt = par ^ 1u;
-        uint32_t *order = nullptr, *cost = nullptr, *next_order = nullptr, resident = 0;
-        const uint32_t *next_cost = nullptr;
+        // Frame g renders in order tile_order[g % 2] and writes its durations to tile_cost[g % 3]; its first
+        // workgroups count frame g - 1's durations into tile_hist[(g + 1) % 2] and place frame g - 2's (counted by
+        // frame g - 1) into tile_order[(g + 1) % 2], the order of frame g + 1.
+        const uint64_t g = c->tile_frame;
+        const uint32_t cs = (uint32_t)(g % 3), op = (uint32_t)(g & 1u), on = op ^ 1u;
+        const uint32_t c_prev = (uint32_t)((g + 2) % 3), c_prev2 = (uint32_t)((g + 1) % 3);
+        uint32_t *order = nullptr, *cost = nullptr;
+        OrderWork ow{};
         size_t klds = wlds;
         if (by_cost) {
             if (int rc = prepare_tile_order(c, wtiles)) return rc;
             // the last cost-ordered frame ran on another stream: this frame reads what it wrote
-            if (c->tile_frame && c->order_stream != c->stream) HIP_TRY(c, hipStreamWaitEvent(c->stream, c->tile_rendered, 0));
-            c->order_used[par] = c->order_tiles[par] == wtiles;
-            order = c->order_used[par] ? c->tile_order[par] : nullptr;
-            cost = c->tile_cost[par];
-            // the next frame's order, from the previous frame's durations, computed by this frame's first workgroup
-            c->order_tiles[nxt] = 0;
-            if (c->cost_tiles[nxt] == wtiles) {
-                if (int rc = resident_waves(c, width, scene_has_lights(v), wlds, resident)) return rc;
-                next_cost = c->tile_cost[nxt];
-                next_order = c->tile_order[nxt];
-                klds = std::max(wlds, kOrderWaveLds);
+            if (g && c->order_stream != c->stream) HIP_TRY(c, hipStreamWaitEvent(c->stream, c->tile_rendered, 0));
+            c->order_used[op] = c->order_tiles[op] == wtiles;
+            order = c->order_used[op] ? c->tile_order[op] : nullptr;
+            cost = c->tile_cost[cs];
+            uint32_t segs = 0, seg = 0;
+            order_segments(wtiles, segs, seg);
+            ow.seg = seg;
+            c->order_tiles[on] = 0;
+            if (c->hist_tiles[op] == wtiles && c->cost_tiles[c_prev2] == wtiles) {   // frame g - 2, counted
+                if (int rc = resident_waves(c, width, scene_has_lights(v), wlds, ow.resident)) return rc;
+                ow.place_cost = c->tile_cost[c_prev2];
+                ow.place_hist = c->tile_hist[op];
+                ow.place_order = c->tile_order[on];
+                ow.place_waves = segs;
+                c->order_tiles[on] = wtiles;
             }
+            c->hist_tiles[on] = 0;
+            if (c->cost_tiles[c_prev] == wtiles) {   // frame g - 1
+                ow.count_cost = c->tile_cost[c_prev];
+                ow.count_hist = c->tile_hist[on];
+                ow.count_waves = segs;
+                c->hist_tiles[on] = wtiles;
+            }
+            if (ow.count_waves || ow.place_waves) klds = std::max(wlds, kOrderWaveLds);
         }
         const uint32_t split = order ? std::min(wtiles / kSplitShare, kSplitMax) : 0u;
         const uint32_t quarter = std::min(quarter_tiles(wtiles, width), split), tp = kTopParts;
-        const uint32_t blocks = wtiles + split + (tp - 2) * quarter + (next_order ? 1u : 0u);
+        const uint32_t blocks = ow.count_waves + ow.place_waves + wtiles + split + (tp - 2) * quarter;
 #define RENDER_LAUNCH(COUNT, WIDE, LIGHTS)                                                                      \
     hipLaunchKernelGGL((render_kernel<COUNT, WIDE, LIGHTS>), dim3(blocks), dim3(64), klds, c->stream, v, rp, pixels,    \
                        depths, scene_depth, mesh_depth, wtiles_x, wtiles, c->d_status, c->d_counters, order, cost,  \
-                       split, quarter, tp, nullptr, next_cost, next_order, resident)
+                       split, quarter, tp, nullptr, ow)
         if (count) {
             if (width == 8) RENDER_LAUNCH(true, 8, true); else if (width == 4) RENDER_LAUNCH(true, 4, true);
             else RENDER_LAUNCH(true, 2, true);
@@ -739,8 +849,7 @@ int launch(slimrt_context *c, const slimrt_render_params *p, float4 *pixels, flo
         if (by_cost) {
             HIP_TRY(c, hipEventRecord(c->tile_rendered, c->stream));
             c->order_stream = c->stream;
-            c->cost_tiles[par] = wtiles;
-            if (next_order) c->order_tiles[nxt] = wtiles;
+            c->cost_tiles[cs] = wtiles;
             c->tile_frame++;
         }
     }
@@ -1186,9 +1295,11 @@ int slimrt_destroy(slimrt_context *c) {
         if (c->wf_join) (void)hipEventDestroy(c->wf_join);
         if (c->d_status) (void)ctx_free(c, c->d_status);
         if (c->d_counters) (void)ctx_free(c, c->d_counters);
-        for (int p = 0; p < 2; p++) {
+        for (int p = 0; p < 3; p++)
             if (c->tile_cost[p]) (void)ctx_free(c, c->tile_cost[p]);
+        for (int p = 0; p < 2; p++) {
             if (c->tile_order[p]) (void)ctx_free(c, c->tile_order[p]);
+            if (c->tile_hist[p]) (void)ctx_free(c, c->tile_hist[p]);
         }
         if (c->tile_rendered) (void)hipEventDestroy(c->tile_rendered);
         if (c->ev0) (void)hipEventDestroy(c->ev0);
@@ -1468,7 +1579,7 @@ int slimrt_render_device_noalloc(slimrt_context *c, const slimrt_render_params *
     hipLaunchKernelGGL((render_kernel<false, NW, LIGHTS>), dim3(wtiles + split + (tp - 2) * quarter), dim3(64), wlds,      \
                        c->stream, v, rp, (float4 *)d_pixels, (float *)d_depths, scene_depth, mesh_depth, wtiles_x,     \
                        wtiles, c->d_status, nullptr, order, cost, split, quarter, tp, by_cost ? cost_tiles : nullptr, \
-                       nullptr, nullptr, 0u)
+                       OrderWork{})
     if (scene_has_lights(v)) {
         if (width == 8) NOALLOC_LAUNCH(8, true); else if (width == 4) NOALLOC_LAUNCH(4, true); else NOALLOC_LAUNCH(2, true);
     } else {
@@ -1645,13 +1756,14 @@ int slimrt_tile_costs(slimrt_context *c, uint32_t *costs, uint32_t *order, uint3
                       uint32_t *resident) {
     if (!c || !tile_count) return SLIMRT_E_ARG;
     if (resident) *resident = c->cpu ? 0u : c->resident;
-    const uint32_t last = (uint32_t)((c->tile_frame + 1u) & 1u);
-    const uint32_t n = (c->cpu || c->tile_frame == 0) ? 0u : c->cost_tiles[last];
+    const uint64_t g = c->tile_frame - 1u;   // the last cost-ordered frame
+    const uint32_t last = (uint32_t)(g & 1u), last_cost = (uint32_t)(g % 3);
+    const uint32_t n = (c->cpu || c->tile_frame == 0) ? 0u : c->cost_tiles[last_cost];
     *tile_count = n;
     if (n == 0) return SLIMRT_OK;
     if ((costs || order) && cap < n) return fail(c, SLIMRT_E_ARG, "cap is smaller than the tile count");
     HIP_TRY(c, hipStreamSynchronize(c->stream));
-    if (costs) HIP_TRY(c, hipMemcpy(costs, c->tile_cost[last], n * sizeof(uint32_t), hipMemcpyDeviceToHost));
+    if (costs) HIP_TRY(c, hipMemcpy(costs, c->tile_cost[last_cost], n * sizeof(uint32_t), hipMemcpyDeviceToHost));
     if (order) {
         if (c->order_used[last]) HIP_TRY(c, hipMemcpy(order, c->tile_order[last], n * sizeof(uint32_t), hipMemcpyDeviceToHost));
         else

@@ -92,12 +92,13 @@ struct slimrt_context {
     hipEvent_t wf_fork = nullptr, wf_join = nullptr, wf_band[8] = {};
     uint32_t *d_status = nullptr;
     unsigned long long *d_counters = nullptr;
-    // per-sample frames dispatched by cost, double-buffered by frame parity: frame f renders in order
-    // tile_order[f & 1] (computed by frame f - 1's first workgroup from the costs of frame f - 2) and writes its
-    // tiles' durations (wall-clock ticks) to tile_cost[f & 1].  *_tiles: the tile count a buffer holds (0: none).
-    // tile_rendered follows the last such frame, launched on order_stream.
-    uint32_t *tile_cost[2] = {}, *tile_order[2] = {};
-    uint32_t tile_cap = 0, cost_tiles[2] = {}, order_tiles[2] = {};
+    // per-sample frames dispatched by cost: cost-ordered frame g renders in order tile_order[g % 2] and writes its
+    // tiles' durations (wall-clock ticks) to tile_cost[g % 3]; its first workgroups count frame g - 1's durations
+    // per segment into tile_hist[(g + 1) % 2] and, from the counts frame g - 1 made, place frame g - 2's tiles into
+    // tile_order[(g + 1) % 2].  *_tiles: the tile count a buffer holds (0: none).  tile_rendered follows the last
+    // such frame, launched on order_stream.
+    uint32_t *tile_cost[3] = {}, *tile_order[2] = {}, *tile_hist[2] = {};
+    uint32_t tile_cap = 0, cost_tiles[3] = {}, hist_tiles[2] = {}, order_tiles[2] = {};
     bool order_used[2] = {};
     uint64_t tile_frame = 0;
     hipEvent_t tile_rendered = nullptr;

@@ -629,13 +629,13 @@ int slimrt_render_autotune(slimrt_context *c, const slimrt_render_params *params
     for (uint32_t f : forms) {
         p.flags = base | f;
         std::vector<float> t;
-        // two warm-up frames: a per-sample form renders its tiles costliest first from its third frame on
-        for (uint32_t r = 0; r < reps + 2; r++) {
+        // three warm-up frames: a per-sample form renders its tiles costliest first from its fourth frame on
+        for (uint32_t r = 0; r < reps + 3; r++) {
             int rc = slimrt_render_device(c, &p, d_pixels, d_depths);
             if (rc) return rc;
             float ms;
             if ((rc = slimrt_last_kernel_ms(c, &ms))) return rc;
-            if (r >= 2) t.push_back(ms);
+            if (r >= 3) t.push_back(ms);
         }
         std::sort(t.begin(), t.end());
         if (t[t.size() / 2] < win_ms) {

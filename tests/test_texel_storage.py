@@ -245,7 +245,7 @@ def test_gpu_texel_storage_scene_equals_oracle(gpu_available, form):
     r = runtime.Renderer(spec, "gpu")
     p = r.params()
     p.flags = FORMS[form]
-    for _ in range(3):   # the cost-ordered forms change their tile order from the third frame
+    for _ in range(4):   # the cost-ordered forms change their tile order from the fourth frame
         px, depth = r.render(p)
     r.close()
     err = float(np.nanmax(np.abs(px[..., :3] - col)))

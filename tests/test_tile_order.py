@@ -1,9 +1,10 @@
-"""Per-sample frames dispatch their tiles costliest first, by the tile durations the context recorded two frames
-earlier at the same tile count (the order is computed by the first workgroup of the frame before while its other
-workgroups render; render flag SLIMRT_TILE_ORDER_FIXED keeps the image order).  Every tile must be rendered exactly once whatever the order
+"""Per-sample frames dispatch their tiles costliest first, by the tile durations the context recorded three frames
+earlier at the same tile count (the first workgroups of the frame after those durations count them, those of the
+next frame place the tiles, while their other workgroups render; render flag SLIMRT_TILE_ORDER_FIXED keeps the
+image order).  Every tile must be rendered exactly once whatever the order
 (canvases filled with NaN before each frame), frames must equal the image-order frame bit for bit, the first two
 frames at a tile count must use the image order, and later orders must equal the order a numpy model derives from
-the durations of two frames before (costly tiles sorted first, the rest in image order)."""
+the durations of three frames before (costly tiles sorted first, the rest in image order)."""
 import numpy as np
 import pytest
 
@@ -35,7 +36,7 @@ def _cost_class(ticks: np.ndarray) -> np.ndarray:
 
 
 def _expected_order(costs: np.ndarray, resident: int) -> np.ndarray:
-    """The dispatch order a frame takes from the tile durations of two frames before.  Image order unless the
+    """The dispatch order a frame takes from the tile durations of three frames before.  Image order unless the
     longest tile lasted at least 3/4 of the summed durations over the resident waves; then the tiles within a
     factor 16 of the costliest class (narrowed four classes at a time while more than 4096; none if the costliest
     class alone has more), costliest class first and by tile index within a class, then every other tile in
@@ -86,14 +87,14 @@ def test_cost_order_renders_every_tile_once(gpu_available, flags, size):
     p.flags = flags
     n = _tiles(p)
     history = []
-    for f in range(6):
+    for f in range(7):
         got = _frame(r, p)
         assert all(np.array_equal(a, b) for a, b in zip(got, ref))
         costs, order = r.tile_costs()
         assert costs.size == n and np.all(costs > 0)
         assert np.array_equal(np.sort(order), np.arange(n))           # a permutation of the tiles
-        # frame f is ordered by the durations of frame f - 2 (its order is computed while frame f - 1 renders)
-        expected = _expected_order(history[f - 2], r.tile_resident) if f >= 2 else np.arange(n)
+        # frame f is ordered by the durations of frame f - 3 (counted during frame f - 2, placed during f - 1)
+        expected = _expected_order(history[f - 3], r.tile_resident) if f >= 3 else np.arange(n)
         assert np.array_equal(order, expected), f
         history.append(costs)
 
@@ -109,19 +110,19 @@ def test_cost_order_follows_tile_count_changes(gpu_available):
     for _ in range(3):
         got = _frame(r, p)
         assert all(np.array_equal(a, b) for a, b in zip(got, ref))
-    # SSAA quadruples the tiles: the first two frames at the new count use the image order again
+    # SSAA quadruples the tiles: the first three frames at the new count use the image order again
     p.ssaa = 1
     p.flags = runtime.TILE_ORDER_FIXED
     ref_ssaa = _frame(r, p)
     p.flags = 0
     n = _tiles(p)
     history = []
-    for f in range(4):
+    for f in range(5):
         got = _frame(r, p)
         assert all(np.array_equal(a, b) for a, b in zip(got, ref_ssaa))
         costs, order = r.tile_costs()
         assert order.size == n
-        assert np.array_equal(order, _expected_order(history[f - 2], r.tile_resident) if f >= 2 else np.arange(n)), f
+        assert np.array_equal(order, _expected_order(history[f - 3], r.tile_resident) if f >= 3 else np.arange(n)), f
         history.append(costs)
     # back to the first tile count
     p.ssaa = 0
@@ -133,10 +134,10 @@ def test_cost_order_follows_tile_count_changes(gpu_available):
 
 @pytest.mark.gpu
 def test_cost_order_across_a_stream_change(gpu_available):
-    """The context's stream changes between two cost-ordered frames launched back to back: the frame on the new
-    stream waits for the one on the old stream (it reads the durations and the order that frame wrote), and the
-    orders go on as before.  Frames 2k on alternating streams, frames 2k + 1 on a third; frame 2k + 1 is ordered by
-    the durations of frame 2k - 1."""
+    """The context's stream changes between cost-ordered frames launched back to back: the frame on the new stream
+    waits for the one on the old stream (it reads the durations and the order that frame wrote), and the orders go
+    on as before.  Frames 3k, 3k + 1 and 3k + 2 on changing streams with no synchronization between them; frame
+    3k + 2 is ordered by the durations of frame 3k - 1."""
     import torch
     spec = scenes.implicit_scene(1, 160, 96, max_depth=5, transparent=True)
     r = runtime.Renderer(spec, "gpu")
@@ -148,17 +149,17 @@ def test_cost_order_across_a_stream_change(gpu_available):
     streams = [torch.cuda.Stream(), torch.cuda.Stream()]
     m = p.width * p.height
     bufs = [(torch.empty((m, 4), dtype=torch.float32, device="cuda"), torch.empty((m,), dtype=torch.float32, device="cuda"))
-            for _ in range(2)]
+            for _ in range(3)]
     history = []
     for k in range(3):
         for px, dp in bufs:
             px.fill_(float("nan"))
             dp.fill_(float("nan"))
         torch.cuda.synchronize()
-        r.set_stream(streams[0].cuda_stream if k % 2 == 0 else None)
-        r.render_device(p, bufs[0][0].data_ptr(), bufs[0][1].data_ptr())
-        r.set_stream(streams[1].cuda_stream)
-        r.render_device(p, bufs[1][0].data_ptr(), bufs[1][1].data_ptr())
+        for i, st in enumerate((streams[0].cuda_stream if k % 2 == 0 else None, streams[1].cuda_stream,
+                                streams[0].cuda_stream)):
+            r.set_stream(st)
+            r.render_device(p, bufs[i][0].data_ptr(), bufs[i][1].data_ptr())
         r.synchronize()
         torch.cuda.synchronize()
         for px, dp in bufs:
@@ -183,7 +184,7 @@ def test_counting_pass_ignores_the_order(gpu_available):
 
 @pytest.mark.gpu
 def test_cost_order_at_full_size(gpu_available):
-    """The 1080p mesh + PBR frame of the benchmark: from the third frame its costliest tiles go first and the first
+    """The 1080p mesh + PBR frame of the benchmark: from the fourth frame its costliest tiles go first and the first
     1,012 of them render as two half waves; every frame equals the image-order frame bit for bit."""
     spec = scenes.mesh_pbr_scene(0, 1920, 1080)
     r = runtime.Renderer(spec, "gpu")
@@ -192,11 +193,11 @@ def test_cost_order_at_full_size(gpu_available):
     ref = _frame(r, p)
     p.flags = runtime.BVH4
     history = []
-    for f in range(4):
+    for f in range(5):
         got = _frame(r, p)
         assert all(np.array_equal(a, b) for a, b in zip(got, ref)), f
         costs, order = r.tile_costs()
-        expected = _expected_order(history[f - 2], r.tile_resident) if f >= 2 else np.arange(costs.size)
+        expected = _expected_order(history[f - 3], r.tile_resident) if f >= 3 else np.arange(costs.size)
         assert np.array_equal(order, expected), f
         history.append(costs)
     assert not np.array_equal(order, np.arange(order.size))   # this frame is bound by its longest tiles
@@ -205,9 +206,9 @@ def test_cost_order_at_full_size(gpu_available):
 @pytest.mark.gpu
 def test_split_tiles_stay_split(gpu_available):
     """A split tile records twice the duration of its later half, so it keeps its rank: on the 1080p benchmark
-    frame the tiles split in frame f (the first 1 in 32 of its order) are mostly split again in frame f + 2, whose
+    frame the tiles split in frame f (the first 1 in 32 of its order) are mostly split again in frame f + 3, whose
     order comes from frame f's durations.  Recording the half wave's own duration instead would drop most of them
-    an octave, out of the split share, and they would run unsplit and late in every other frame of a parity."""
+    an octave, out of the split share, and they would run unsplit and late in every other frame of their sequence."""
     spec = scenes.mesh_pbr_scene(0, 1920, 1080)
     r = runtime.Renderer(spec, "gpu")
     p = r.params()
@@ -215,10 +216,10 @@ def test_split_tiles_stay_split(gpu_available):
     n = _tiles(p)
     split = min(n // 32, 2048)
     sets = []
-    for f in range(8):
+    for f in range(10):
         _frame(r, p)
         costs, order = r.tile_costs()
-        sets.append(set(order[:split].tolist()) if f >= 2 else None)
-    overlaps = [len(sets[f] & sets[f - 2]) / split for f in range(4, 8)]
-    print("split-set overlap with the frame two before:", [round(o, 3) for o in overlaps])
+        sets.append(set(order[:split].tolist()) if f >= 3 else None)
+    overlaps = [len(sets[f] & sets[f - 3]) / split for f in range(6, 10)]
+    print("split-set overlap with the frame three before:", [round(o, 3) for o in overlaps])
     assert min(overlaps) >= 0.7

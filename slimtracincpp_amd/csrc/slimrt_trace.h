@@ -295,7 +295,9 @@ SLIM_HD bool mesh_trace8(const SceneView &s, const MeshRec &m, const Ray &r, Hit
             const float tfx = fmaf(fmaf((float)slot_byte(fx, k), sx, n.ox), r.rcp.x, r.scaled_origin.x);
             const float tfy = fmaf(fmaf((float)slot_byte(fy, k), sy, n.oy), r.rcp.y, r.scaled_origin.y);
             const float tfz = fmaf(fmaf((float)slot_byte(fz, k), sz, n.oz), r.rcp.z, r.scaled_origin.z);
-            const float nd = fmax2(0, fmax2(tnx, fmax2(tny, tnz))), fd = fmin2(tfx, fmin2(tfy, tfz));
+            // IEEE max / min (three-operand instructions): a NaN operand is dropped wherever it stands, where the
+            // reference's ternaries keep or drop it by position, so this test accepts every box they accept
+            const float nd = fmaxf(0.0f, fmaxf(tnx, fmaxf(tny, tnz))), fd = fminf(tfx, fminf(tfy, tfz));
             const uint32_t cnt = slot_byte(n.count, k);
             const bool h = nd <= fd && nd < hit.distance && cnt != 0xFFu;
             hits |= (uint32_t)h << (k ^ mo);

@@ -46,9 +46,9 @@ namespace {
 // workgroup), so a frame is one launch on one stream.
 //
 // A tile's cost class is its duration's binary exponent and the two bits below it (four classes an octave; rank
-// 0 = costliest).  The costly tiles - those within a factor 16 of the costliest class (narrowed 4 classes at a
+// 0 = costliest).  The costly tiles - those within a factor 64 of the costliest class (narrowed 4 classes at a
 // time while more than kHeavyMax), none if even the costliest class alone has more - go first, by rank and within
-// a rank in image order (a counting sort over the at most 17 ranks of the window); the others follow in image
+// a rank in image order (a counting sort over the at most 25 ranks of the window); the others follow in image
 // order.  The frames that allocate nothing compute the same order in three launches right before the frame (their
 // scratch holds one set of durations, so the order cannot be computed while the next durations are written), over
 // waves that each own a contiguous segment of tiles (so image order within a class is segment order):
@@ -57,8 +57,8 @@ namespace {
 // per class and for its other tiles; order_place_kernel writes each tile to its slot (about 25 us in all for a
 // 1080p frame, on the frame's path).  Deterministic: both forms give the same order.
 constexpr uint32_t kCostClasses = 128;
-constexpr uint32_t kWindow = 16;              // costly window: ranks top .. top + 16 (a factor 16 in duration)
-constexpr uint32_t kHeavyMax = 4096;          // at most this many costly tiles
+constexpr uint32_t kWindow = 24;              // costly window: ranks top .. top + 24 (a factor 64 in duration)
+constexpr uint32_t kHeavyMax = 8192;          // at most this many costly tiles
 constexpr uint32_t kOrderBlock = 256, kOrderMaxWaves = 256;   // order workgroups (4 waves) and waves at most
 // per-wave words of the order scratch: class counts, then [costliest rank, longest, summed lo, summed hi], then
 // the plan: first slot per window rank and the first slot of the other tiles

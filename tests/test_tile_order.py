@@ -38,7 +38,7 @@ def _cost_class(ticks: np.ndarray) -> np.ndarray:
 def _expected_order(costs: np.ndarray, resident: int) -> np.ndarray:
     """The dispatch order a frame takes from the tile durations of three frames before.  Image order unless the
     longest tile lasted at least 3/4 of the summed durations over the resident waves; then the tiles within a
-    factor 16 of the costliest class (narrowed four classes at a time while more than 4096; none if the costliest
+    factor 64 of the costliest class (narrowed four classes at a time while more than 8192; none if the costliest
     class alone has more), costliest class first and by tile index within a class, then every other tile in
     image order."""
     idx = np.arange(costs.size)
@@ -47,9 +47,9 @@ def _expected_order(costs: np.ndarray, resident: int) -> np.ndarray:
     rank = 127 - _cost_class(costs)
     top = rank.min()
     heavy = np.zeros(costs.size, bool)
-    for window in (16, 12, 8, 4, 0):
+    for window in (24, 20, 16, 12, 8, 4, 0):
         h = rank <= top + window
-        if h.sum() <= 4096:
+        if h.sum() <= 8192:
             heavy = h
             break
     first = idx[heavy][np.lexsort((idx[heavy], rank[heavy]))]
@@ -58,14 +58,15 @@ def _expected_order(costs: np.ndarray, resident: int) -> np.ndarray:
 
 def test_expected_order_model():
     costs = np.array([5, 900, 40, 1000, 7, 64, 950, 3, 0, 33], np.uint32)
-    # 900, 950 and 1000 share the costliest class; 64 lies within a factor 16 below it, 40 and less further
-    assert _expected_order(costs, 3072).tolist() == [1, 3, 6, 5, 0, 2, 4, 7, 8, 9]
+    # 900, 950 and 1000 share the costliest class; 64, 40 and 33 lie within a factor 64 below it (by class), 7 and
+    # less further
+    assert _expected_order(costs, 3072).tolist() == [1, 3, 6, 5, 2, 9, 0, 4, 7, 8]
     # the longest tile (1000) against the summed durations (3002) over the resident waves: 4 * 1000 * 3 > 3 * 3002,
     # costly first; 4 * 1000 * 2 <= 3 * 3002, image order
     assert _expected_order(costs, 3).tolist()[:4] == [1, 3, 6, 5]
     assert np.array_equal(_expected_order(costs, 2), np.arange(10))
-    flat = np.full(5000, 100, np.uint32)              # one class larger than the cap: image order
-    assert np.array_equal(_expected_order(flat, 3072), np.arange(5000))
+    flat = np.full(9000, 100, np.uint32)              # one class larger than the cap: image order
+    assert np.array_equal(_expected_order(flat, 3072), np.arange(9000))
 
 
 def test_cost_class_is_monotonic():

@@ -485,9 +485,9 @@ void launch_tile_order(hipStream_t stream, const uint32_t *cost, uint32_t *order
 // is set, the first `split` tiles of the order (the costliest) are each rendered by two workgroups, lanes 0-31
 // and lanes 32-63: a costly tile's 64 paths diverge, and half as many lanes per wave shorten the frame's longest
 // waves (0.77 -> 0.65 ms on the mesh + PBR frame; quarter waves gain nothing and lose fast as their count grows).
-// LIGHTS false (scenes without lights or emissive quads) leaves the shadow rays and the glow out: no VGPR spills
-// at three waves per SIMD.
-template <bool kCount, uint32_t NW, bool LIGHTS = true>
+// LIGHTS selects the light terms (kLightsNone for scenes without lights or emissive quads: no VGPR spills at three
+// waves per SIMD; kLightsQuads for emissive quads without point lights: no shadow-ray walks).
+template <bool kCount, uint32_t NW, uint32_t LIGHTS = kLightsAll>
 __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(NW >= 4 && !kCount ? 3 : 1)))
 void render_kernel(SceneView s, RenderParams rp, float4 *__restrict__ pixels, float *__restrict__ depths,
                    uint32_t scene_depth, uint32_t mesh_depth, uint32_t tiles_x, uint32_t tiles,
@@ -748,17 +748,19 @@ int prepare_tile_order(slimrt_context *c, uint32_t tiles) {
 
 // Waves of the frame kernel the device holds at once (its workgroups are single waves), for this node width and
 // stack size; cached per configuration.
-int resident_waves(slimrt_context *c, uint32_t width, bool lights, size_t lds, uint32_t &out) {
-    const uint64_t key = ((uint64_t)lds << 5) | (lights ? 16u : 0u) | width;
+int resident_waves(slimrt_context *c, uint32_t width, uint32_t lights, size_t lds, uint32_t &out) {
+    const uint64_t key = ((uint64_t)lds << 6) | ((uint64_t)lights << 4) | width;
     if (c->resident_key != key || !c->resident) {
         int per_cu = 0, cus = 0;
         auto occupancy = [&](auto kernel) { return hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kernel, 64, lds); };
-        if (width == 8)
-            HIP_TRY(c, lights ? occupancy(render_kernel<false, 8, true>) : occupancy(render_kernel<false, 8, false>));
-        else if (width == 4)
-            HIP_TRY(c, lights ? occupancy(render_kernel<false, 4, true>) : occupancy(render_kernel<false, 4, false>));
-        else
-            HIP_TRY(c, lights ? occupancy(render_kernel<false, 2, true>) : occupancy(render_kernel<false, 2, false>));
+#define OCCUPANCY(NW)                                                                                           \
+    HIP_TRY(c, lights == kLightsAll ? occupancy(render_kernel<false, NW, kLightsAll>)                              \
+               : lights == kLightsQuads ? occupancy(render_kernel<false, NW, kLightsQuads>)                        \
+                                        : occupancy(render_kernel<false, NW, kLightsNone>))
+        if (width == 8) OCCUPANCY(8);
+        else if (width == 4) OCCUPANCY(4);
+        else OCCUPANCY(2);
+#undef OCCUPANCY
         HIP_TRY(c, hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, c->device));
         c->resident = (uint32_t)std::max(1, per_cu * cus);
         c->resident_key = key;
@@ -811,7 +813,7 @@ int launch(slimrt_context *c, const slimrt_render_params *p, float4 *pixels, flo
             ow.seg = seg;
             c->order_tiles[on] = 0;
             if (c->hist_tiles[op] == wtiles && c->cost_tiles[c_prev2] == wtiles) {   // frame g - 2, counted
-                if (int rc = resident_waves(c, width, scene_has_lights(v), wlds, ow.resident)) return rc;
+                if (int rc = resident_waves(c, width, scene_lights(v), wlds, ow.resident)) return rc;
                 ow.place_cost = c->tile_cost[c_prev2];
                 ow.place_hist = c->tile_hist[op];
                 ow.place_order = c->tile_order[on];
@@ -834,15 +836,19 @@ int launch(slimrt_context *c, const slimrt_render_params *p, float4 *pixels, flo
     hipLaunchKernelGGL((render_kernel<COUNT, WIDE, LIGHTS>), dim3(blocks), dim3(64), klds, c->stream, v, rp, pixels,    \
                        depths, scene_depth, mesh_depth, wtiles_x, wtiles, c->d_status, c->d_counters, order, cost,  \
                        split, quarter, tp, nullptr, ow)
+        const uint32_t lights = scene_lights(v);
         if (count) {
-            if (width == 8) RENDER_LAUNCH(true, 8, true); else if (width == 4) RENDER_LAUNCH(true, 4, true);
-            else RENDER_LAUNCH(true, 2, true);
-        } else if (scene_has_lights(v)) {
-            if (width == 8) RENDER_LAUNCH(false, 8, true); else if (width == 4) RENDER_LAUNCH(false, 4, true);
-            else RENDER_LAUNCH(false, 2, true);
+            if (width == 8) RENDER_LAUNCH(true, 8, kLightsAll); else if (width == 4) RENDER_LAUNCH(true, 4, kLightsAll);
+            else RENDER_LAUNCH(true, 2, kLightsAll);
+        } else if (lights == kLightsAll) {
+            if (width == 8) RENDER_LAUNCH(false, 8, kLightsAll); else if (width == 4) RENDER_LAUNCH(false, 4, kLightsAll);
+            else RENDER_LAUNCH(false, 2, kLightsAll);
+        } else if (lights == kLightsQuads) {
+            if (width == 8) RENDER_LAUNCH(false, 8, kLightsQuads); else if (width == 4) RENDER_LAUNCH(false, 4, kLightsQuads);
+            else RENDER_LAUNCH(false, 2, kLightsQuads);
         } else {
-            if (width == 8) RENDER_LAUNCH(false, 8, false); else if (width == 4) RENDER_LAUNCH(false, 4, false);
-            else RENDER_LAUNCH(false, 2, false);
+            if (width == 8) RENDER_LAUNCH(false, 8, kLightsNone); else if (width == 4) RENDER_LAUNCH(false, 4, kLightsNone);
+            else RENDER_LAUNCH(false, 2, kLightsNone);
         }
 #undef RENDER_LAUNCH
         HIP_TRY(c, hipGetLastError());
@@ -1562,7 +1568,7 @@ int slimrt_render_device_noalloc(slimrt_context *c, const slimrt_render_params *
     if (by_cost) {
         cost = c->noalloc_scratch + kNoAllocCost;
         order = c->noalloc_scratch + kNoAllocOrder;
-        if ((rc = resident_waves(c, width, scene_has_lights(device_view(c)), wlds, resident))) return rc;
+        if ((rc = resident_waves(c, width, scene_lights(device_view(c)), wlds, resident))) return rc;
     }
     const SceneView v = device_view(c);
     c->timed = true;
@@ -1580,10 +1586,13 @@ int slimrt_render_device_noalloc(slimrt_context *c, const slimrt_render_params *
                        c->stream, v, rp, (float4 *)d_pixels, (float *)d_depths, scene_depth, mesh_depth, wtiles_x,     \
                        wtiles, c->d_status, nullptr, order, cost, split, quarter, tp, by_cost ? cost_tiles : nullptr, \
                        OrderWork{})
-    if (scene_has_lights(v)) {
-        if (width == 8) NOALLOC_LAUNCH(8, true); else if (width == 4) NOALLOC_LAUNCH(4, true); else NOALLOC_LAUNCH(2, true);
+    const uint32_t lights = scene_lights(v);
+    if (lights == kLightsAll) {
+        if (width == 8) NOALLOC_LAUNCH(8, kLightsAll); else if (width == 4) NOALLOC_LAUNCH(4, kLightsAll); else NOALLOC_LAUNCH(2, kLightsAll);
+    } else if (lights == kLightsQuads) {
+        if (width == 8) NOALLOC_LAUNCH(8, kLightsQuads); else if (width == 4) NOALLOC_LAUNCH(4, kLightsQuads); else NOALLOC_LAUNCH(2, kLightsQuads);
     } else {
-        if (width == 8) NOALLOC_LAUNCH(8, false); else if (width == 4) NOALLOC_LAUNCH(4, false); else NOALLOC_LAUNCH(2, false);
+        if (width == 8) NOALLOC_LAUNCH(8, kLightsNone); else if (width == 4) NOALLOC_LAUNCH(4, kLightsNone); else NOALLOC_LAUNCH(2, kLightsNone);
     }
 #undef NOALLOC_LAUNCH
     HIP_TRY(c, hipGetLastError());

@@ -279,7 +279,10 @@ inline SceneView device_view(const slimrt_context *c) {
 }
 
 // Whether the integrator's light loops run at all: point or directional lights, or emissive quads.
-inline bool scene_has_lights(const SceneView &v) { return v.L > 0 || (v.flags & 1u) != 0; }
+// The light terms a frame of this scene needs (kLightsNone, kLightsQuads or kLightsAll).
+inline uint32_t scene_lights(const SceneView &v) {
+    return v.L > 0 ? kLightsAll : ((v.flags & 1u) != 0 ? kLightsQuads : kLightsNone);
+}
 
 inline void update_scene_flags(slimrt_context *c) {   // Scene::Scene (scene.h:239-242)
     c->scene_flags = 0;

@@ -696,10 +696,15 @@ SLIM_HD void primary_ray(const Projection &p, uint32_t x, uint32_t y, v3 &dir, f
           v3{p.right[0], p.right[1], p.right[2]} * (float)(int32_t)x;
 }
 
-// renderPixelBeauty (src/slim/renderer/ray_tracer.h:19-108).  LIGHTS false: for scenes without lights and without
-// emissive quads, where the light loops run zero times; leaving them out spares a kernel the registers of shadow
-// rays and glow.
-template <class TR, bool LIGHTS = true>
+// The light terms a frame kernel instantiation carries: point and directional lights (with their shadow rays and
+// glow) and emissive quads.  A scene without lights and without emissive quads runs kLightsNone, one with emissive
+// quads only kLightsQuads: the loops left out would run zero times, and leaving them out spares a kernel their
+// registers (the shadow rays' walks above all).
+constexpr uint32_t kLightPoint = 1, kLightQuads = 2;
+constexpr uint32_t kLightsNone = 0, kLightsQuads = kLightQuads, kLightsAll = kLightPoint | kLightQuads;
+
+// renderPixelBeauty (src/slim/renderer/ray_tracer.h:19-108), with the light terms LIGHTS selects.
+template <class TR, uint32_t LIGHTS = kLightsAll>
 SLIM_HD void render_beauty(const SceneView &s, const RenderParams &rp, v3 direction, float scaling, TR &tr,
                            color &out, float &depth) {
     Surface sf;
@@ -724,10 +729,10 @@ SLIM_HD void render_beauty(const SceneView &s, const RenderParams &rp, v3 direct
             } else {
                 prepare_for_shading(sf, s, ray, hit, tr);
                 if (depth_left == rp.max_depth) depth = depth_at(p, hit.position);
-                if constexpr (LIGHTS) {
+                if constexpr ((LIGHTS & kLightPoint) != 0)
                     for (uint32_t i = 0; i < s.L; i++) shade_from_light(sf, s.lights[i], s, tr, cur);
+                if constexpr ((LIGHTS & kLightQuads) != 0)
                     if (s.flags & 1u) shade_from_emissive_quads(sf, s, cur);
-                }
                 if (rp.sky_irradiance >= 0 && rp.sky_radiance >= 0) {
                     sf.L = sf.N;
                     sf.NdotL = 1.0f;
@@ -748,7 +753,7 @@ SLIM_HD void render_beauty(const SceneView &s, const RenderParams &rp, v3 direct
             if (rp.sky_color >= 0) SLIM_COUNT(tr, taps);
             if (rp.sky_color >= 0) cur = tex_sample_cube(s, (uint32_t)rp.sky_color, ray.direction.x, ray.direction.y, ray.direction.z);
         }
-        if constexpr (LIGHTS) {
+        if constexpr ((LIGHTS & kLightPoint) != 0) {
             for (uint32_t i = 0; i < s.L; i++) {
                 const Light l = s.lights[i];
                 float density;
@@ -819,7 +824,7 @@ SLIM_HD void output_value(const RenderParams &rp, color c, float out_px[4]) {
 }
 
 // One sample: primary ray, integrator, and the value the canvas or the radiance buffer receives.
-template <class TR, bool LIGHTS = true>
+template <class TR, uint32_t LIGHTS = kLightsAll>
 SLIM_HD void render_sample(const SceneView &s, const RenderParams &rp, uint32_t x, uint32_t y, TR &tr,
                            float out_px[4], float &out_depth) {
     v3 dir;

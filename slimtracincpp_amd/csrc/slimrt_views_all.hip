@@ -1,4 +1,4 @@
-// The views kernel's instantiations for scenes with point or directional lights (slimrt_views_trace.h).
+// The views kernel's instantiations for scenes with lights or emissive quads (slimrt_views_trace.h).
 #include "slimrt_views_trace.h"
 
 namespace slimrt_views_detail {

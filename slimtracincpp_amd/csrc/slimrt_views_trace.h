@@ -1,7 +1,7 @@
 // The multi-view trace kernel (slimrt_views.hip launches it): one-wave workgroups with a WX x WY tile (64
 // samples) of one view each, one workgroup per tile or a persistent grid fed by a device counter.  Its
 // instantiations are split over two files by light terms (slimrt_views_all.hip: kLightsAll,
-// slimrt_views_quads.hip: kLightsQuads), which compile in parallel.
+// slimrt_views_none.hip: kLightsNone), which compile in parallel.
 #ifndef SLIMRT_VIEWS_TRACE_H
 #define SLIMRT_VIEWS_TRACE_H
 
@@ -27,8 +27,8 @@ struct ViewsLaunch {
     bool persistent;
 };
 
-hipError_t launch_views_all(const ViewsLaunch &l);     // scenes with point or directional lights
-hipError_t launch_views_quads(const ViewsLaunch &l);   // scenes without (emissive quads or no lights)
+hipError_t launch_views_all(const ViewsLaunch &l);    // scenes with lights or emissive quads
+hipError_t launch_views_none(const ViewsLaunch &l);   // scenes with neither
 
 }  // namespace slimrt_views_detail
 
@@ -37,7 +37,7 @@ namespace {
 using slimrt_views_detail::ViewRec;
 using slimrt_views_detail::ViewsLaunch;
 using slimrt_views_detail::launch_views_all;
-using slimrt_views_detail::launch_views_quads;
+using slimrt_views_detail::launch_views_none;
 
 
 
@@ -82,7 +82,7 @@ __device__ __forceinline__ void trace_view_tile(const SceneView &s, const ViewRe
 // total; otherwise one workgroup per tile, tile i on workgroup i (workgroups go round-robin to the 8 XCDs, so
 // each XCD gets an even share).
 // The wide walks at three waves per SIMD, as the frame kernel (render_kernel).  LIGHTS: kLightsAll for scenes with
-// point or directional lights, else kLightsQuads (no shadow-ray walks compiled in, no VGPR spills).
+// lights or emissive quads, else kLightsNone (no light terms compiled in, no VGPR spills).
 template <int WX, int WY, bool PERSISTENT, uint32_t NW, uint32_t LIGHTS>
 __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(NW >= 4 ? 3 : 1)))
 void views_kernel(SceneView s, const ViewRec *__restrict__ views, uint32_t nviews, uint32_t total_tiles, uint32_t grab,

@@ -476,8 +476,9 @@ void launch_tile_order(hipStream_t stream, const uint32_t *cost, uint32_t *order
                        (const uint32_t *)order_scratch, order);
 }
 
-// The 4-wide frame at three waves per SIMD (168 VGPRs, 17 spilled): the walk's dependent fetches want the waves
-// more than the registers (6% faster than two on the mesh + PBR frame).  The 2-wide frame would spill 64 dwords a
+// The 4- and 8-wide frames at three waves per SIMD (168 VGPRs; only the instantiations with every light term
+// spill, 32 and 35 VGPRs): the walk's dependent fetches want the waves more than the registers (6% faster than two
+// on the mesh + PBR frame).  The 2-wide frame would spill 64 dwords a
 // lane for no gain, and the counting pass keeps its registers: both stay at the compiler's two.  With an order
 // (order_place_wave or launch_tile_order) workgroup k renders tile order[k] and records the tile's duration in
 // cost (the first ow.count_waves + ow.place_waves workgroups work on later frames' orders, and the tiles' workgroups

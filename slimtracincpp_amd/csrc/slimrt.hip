@@ -485,7 +485,8 @@ void launch_tile_order(hipStream_t stream, const uint32_t *cost, uint32_t *order
 // follow them); when order[tiles]
 // is set, the first `split` tiles of the order (the costliest) are each rendered by two workgroups, lanes 0-31
 // and lanes 32-63: a costly tile's 64 paths diverge, and half as many lanes per wave shorten the frame's longest
-// waves (0.77 -> 0.65 ms on the mesh + PBR frame; quarter waves gain nothing and lose fast as their count grows).
+// waves (0.77 -> 0.65 ms on the mesh + PBR frame); the first `quarter` of them (8-wide nodes only, quarter_tiles)
+// by top_parts workgroups of 64 / top_parts lanes each.
 // LIGHTS selects the light terms (kLightsNone for scenes without lights or emissive quads: no VGPR spills at three
 // waves per SIMD; kLightsQuads for emissive quads without point lights: no shadow-ray walks).
 template <bool kCount, uint32_t NW, uint32_t LIGHTS = kLightsAll>
